@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Throughput of the fire-simulation step on one MI355X.
+
+Workload: 128 x 192 x 128 cells at dx = 5 mm (0.64 x 0.96 x 0.64 m), dt = 1 ms,
+vorticity confinement eps 0.3, the seed-7 campfire emitter, noise 128^3 seed 1.
+Two forms of the same step run back to back from the same warmed state:
+  * plain FP32:  float4 fields, 2 x V(2,2) FP32 V-cycles per step (the reference's solve);
+  * mixed:       rgba16float fields (the reference's storage), FP32 math, FP32 V-cycles
+                 inside FP64 defect correction to ||b - A p|| / ||b|| <= 1e-5.
+`value` is the mixed step's voxel-steps/s; the FP32 step's rate and the ratio are
+reported beside it.  Inputs are resident in HBM before the timed region.
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes / average
+launch time from HIP events on the library's stream), the multigrid's achieved
+bandwidth, and the CPU oracle (reference semantics: fp16 storage, f32 math,
+2 V-cycles) timed on this host's cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "procedural-fire_amd"))
+
+import numpy as np  # noqa: E402
+
+import firesim as F  # noqa: E402
+
+RES = (128, 192, 128)
+DX = 0.005
+SIZE = tuple(n * DX for n in RES)
+DT = 1e-3
+PARAMS = {"vortEps": 0.3}
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per cell of each kernel: every input field read once, every
+# output written once; E = bytes per 4-channel texel (8 fp16/bf16, 16 fp32)
+def kernel_bytes(E):
+    return {"adv1": 6 * E, "adv2": 9 * E, "curl": 2 * E, "forces": 8 * E, "div": E + 4, "project": 3 * E + 4}
+
+
+def mg_bytes_per_cycle(levels, coarse_iters):
+    """V(2,2) traffic per fine cell: 4 sweeps x 12 B, restrict 9 B, prolong 8.5 B per cell of each
+    non-coarsest level; 12 B per sweep on the coarsest (src/sim.js:588-599)."""
+    n0 = float(np.prod(levels[0]))
+    tot = 0.0
+    for i, d in enumerate(levels):
+        w = np.prod(d) / n0
+        tot += w * (12.0 * coarse_iters if i == len(levels) - 1 else 65.5)
+    return tot
+
+
+REFINE_BYTES = 40.0   # per fine cell per refinement iteration: residual (x8 b4 r4 p4) + update (x8 e4 x8)
+
+
+def make_sim(precision, solve, use_graph=True):
+    logs, _ = F.build_campfire(7)
+    s = F.FireSim(res=RES, size=SIZE, logs=logs, vcycles=2, precision=precision, solve=solve, div_tol=1e-5,
+                  max_vcycles=8, noise_n=128, use_graph=use_graph)
+    s.set_params(PARAMS)
+    return s
+
+
+def time_steps(sim, steps):
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(DT, steps)
+    sim.sync()
+    return time.perf_counter() - t0
+
+
+def profile_kernels(sim, steps=5):
+    sim.profile(True)
+    acc = {k: 0.0 for k in F.STAGE_LABELS}
+    for _ in range(steps):
+        sim.step(DT, 1)
+        t = sim.timings()
+        for k in acc:
+            acc[k] += t[k]
+    sim.profile(False)
+    return {k: v / steps for k, v in acc.items()}
+
+
+def cpu_baseline(gpu_sim, steps=3):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import fsoracle as O
+    logs, _ = F.build_campfire(7)
+    orc = O.OracleSim(res=RES, size=SIZE, logs=logs, n_logs=16, precision=O.FP16, vcycles=2, noise_n=128)
+    orc.set_params(**PARAMS)
+    for f in (F.VEL, F.S, F.S2, F.VORT, F.GEO, F.VELX, F.SX, F.S2X, F.VELY, F.SY, F.S2Y, F.NOISE, F.CURL):
+        orc.write(f, gpu_sim.read(f, np.float64))
+    orc.write(O.P, gpu_sim.read(F.P, np.float64))
+    orc.set_clock(gpu_sim.time, gpu_sim.frame)
+    orc.step(DT, 1)   # warm-up
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        orc.step(DT, 1)
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    return {"value": float(np.prod(RES)) / med, "unit": "voxel-steps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{steps} steps (median, after 1 warm-up) of the 128x192x128 step in reference semantics "
+                      f"(fp16 storage, f32 math, 2 V(2,2)) from the warmed GPU state; {med:.2f} s/step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the step runs on a single MI355X (no multi-GPU path is built); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    ncell = float(np.prod(RES))
+    graph = not args.no_graph
+    # ---- plain FP32 step
+    fp32 = make_sim(F.PREC_FP32, F.SOLVE_VCYCLES, graph)
+    fp32.step(DT, args.warmup)
+    t32 = time_steps(fp32, args.steps)
+    prof32 = profile_kernels(fp32)
+    rel32, _ = fp32.residual()
+    levels, coarse = fp32.mg_info()
+    fp32.close()
+    # ---- mixed step
+    mix = make_sim(F.PREC_FP16, F.SOLVE_REFINE64, graph)
+    mix.step(DT, args.warmup)
+    tmix = time_steps(mix, args.steps)
+    # V-cycles per step over a short sample (device counter read after each step)
+    used = []
+    for _ in range(5):
+        mix.step(DT, 1)
+        used.append(mix.residual()[1])
+    relmix = mix.residual()[0]
+    profm = profile_kernels(mix)
+
+    v32 = ncell * args.steps / t32
+    vmix = ncell * args.steps / tmix
+    # dominant single kernel of the mixed step (MG is many launches; reported separately)
+    kb = kernel_bytes(8)
+    dom = max(kb, key=lambda k: profm[k])
+    dom_ms = profm[dom]
+    achieved = kb[dom] * ncell / (dom_ms * 1e-3) / 1e9
+    mgc = mg_bytes_per_cycle(levels, coarse)
+    nv = float(np.mean(used))
+    mg_alg = (mgc + REFINE_BYTES) * nv * ncell + 20.0 * ncell
+    mg_gbs = mg_alg / (profm["mg"] * 1e-3) / 1e9
+    mg32_gbs = mgc * 2 * ncell / (prof32["mg"] * 1e-3) / 1e9
+    out = {
+        "metric": "sim voxel-steps/s @128x192x128 (mixed vs FP32) + MG HBM GB/s % of peak",
+        "value": vmix,
+        "unit": "voxel-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tmix / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16 storage / f32 math / f64 refinement",
+        "data": "synthetic (seed-7 campfire emitter from rest, noise seed 1)",
+        "config": {"workload": "fire step 128x192x128 dx=5mm dt=1ms vortEps=0.3", "grid": list(RES),
+                   "solve_mixed": "FP32 V(2,2) in FP64 defect correction, tol 1e-5",
+                   "solve_fp32": "2 x V(2,2) FP32", "graph": graph},
+        "fp32_value": v32,
+        "fp32_ms_per_step": t32 / args.steps * 1e3,
+        "mixed_vs_fp32": vmix / v32,
+        "mixed_vcycles_per_step": nv,
+        "mixed_rel_residual": relmix,
+        "fp32_rel_residual": rel32,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": kb[dom] * ncell, "avg_launch_ms": dom_ms},
+        "mg": {"achieved_GBs": mg_gbs, "frac": mg_gbs / HBM_PEAK_GBS, "ms_per_step": profm["mg"],
+               "alg_bytes_per_vcycle_per_cell": mgc, "fp32_achieved_GBs": mg32_gbs},
+        "kernel_ms_mixed": profm,
+        "kernel_ms_fp32": prof32,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(mix)
+        out["vs_cpu"] = vmix / out["cpu_baseline"]["value"]
+    mix.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

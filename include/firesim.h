@@ -1,0 +1,191 @@
+/*
+ * firesim.h — C ABI of the MI355X (gfx950) fire-simulation step.
+ *
+ * One handle = one simulation grid on one GPU, the equivalent of the reference's
+ * `FireSim` object (src/sim.js:48-649).  Every entry point returns an int status
+ * (FS_OK = 0) and never throws; the message of the last failure on the calling
+ * thread is returned by fs_last_error().  Handles are not thread-safe: calls on one
+ * handle must be serialised by the caller (the reference is single-threaded JS).
+ *
+ * Work is asynchronous on the handle's HIP stream, like WebGPU's encode/submit:
+ * fs_step() enqueues, fs_sync() waits (the reference awaits
+ * queue.onSubmittedWorkDone(), src/main.js:244).
+ *
+ * Field layout (fs_read_field / fs_write_field / fs_device_ptr): tightly packed,
+ * x fastest, then y, then z; 4 interleaved channels (RGBA) for the vector fields,
+ * 1 channel for pressure and divergence — the texel order of the reference's 3D
+ * textures (src/sim.js:71-84).
+ */
+#ifndef FIRESIM_H
+#define FIRESIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define FS_OK       0
+#define FS_EINVAL  -1   /* bad argument, unknown parameter name (src/gpu.js:65 throws) */
+#define FS_EDEVICE -2   /* HIP runtime error */
+#define FS_ENOMEM  -3   /* device allocation failed */
+#define FS_ESTATE  -4   /* call not valid in the handle's current state */
+
+/* ---- storage precision of the advected fields ---------------------------- */
+#define FS_PREC_FP32        0  /* plain FP32 step: float4 fields, FP32 V-cycles           */
+#define FS_PREC_FP16        1  /* reference format: rgba16float fields, FP32 V-cycles     */
+#define FS_PREC_BF16        2  /* bf16x4 fields, FP32 V-cycles                            */
+
+/* ---- pressure solve ------------------------------------------------------- */
+#define FS_SOLVE_VCYCLES    0  /* `vcycles` V(2,2) cycles on p, warm-started (src/sim.js:642) */
+#define FS_SOLVE_REFINE64   1  /* FP32 V-cycles inside FP64 defect correction to div_tol  */
+
+/* ---- field identifiers ----------------------------------------------------- */
+#define FS_VEL    0   /* xyz velocity (m/s), w = expansion phi              (src/sim.js:3)   */
+#define FS_S      1   /* theta, fuel, soot, q*1e-4                           (src/sim.js:4)   */
+#define FS_S2     2   /* products, age*Z, smoke, CO fraction                 (src/sim.js:5-6) */
+#define FS_VORT   3   /* curl * dx, |curl * dx|                              (src/sim.js:210) */
+#define FS_GEO    4   /* band, bed, solid, sdf                               (src/sim.js:76)  */
+#define FS_VELX   5   /* ping-pong intermediates (src/sim.js:72-74)                            */
+#define FS_SX     6
+#define FS_S2X    7
+#define FS_VELY   8
+#define FS_SY     9
+#define FS_S2Y   10
+#define FS_P     11   /* level-0 pressure, 1 channel (r32float, src/sim.js:84)                */
+#define FS_B     12   /* level-0 divergence / right-hand side, 1 channel                      */
+#define FS_NOISE 13   /* 128^3 rgba16float noise texture   (src/noise.js:6-64)                */
+#define FS_CURL  14   /* 128^3 rgba16float curl texture    (src/noise.js:67-90)               */
+#define FS_NUM_FIELDS 15
+
+/* ---- host-side element type for field transfers -------------------------- */
+#define FS_DT_F32     0   /* converted to/from float                      */
+#define FS_DT_F64     1   /* converted to/from double                     */
+#define FS_DT_NATIVE  2   /* raw storage bytes (half/bf16/float/double)   */
+
+/* ---- stages (the reference's Profiler labels + individual dispatches) ---- */
+#define FS_STAGE_GEO      0   /* src/sim.js:498-523 */
+#define FS_STAGE_ADV1     1   /* src/sim.js:142-157 */
+#define FS_STAGE_ADV2     2   /* src/sim.js:159-197 */
+#define FS_STAGE_CURL     3   /* src/sim.js:200-211 */
+#define FS_STAGE_FORCES   4   /* src/sim.js:214-396 */
+#define FS_STAGE_DIV      5   /* src/sim.js:399-413 */
+#define FS_STAGE_MG       6   /* src/sim.js:642 (all V-cycles of one step, or the FP64 refinement) */
+#define FS_STAGE_PROJECT  7   /* src/sim.js:526-550 */
+#define FS_NUM_STAGES     8
+
+/* Number of named parameters: SIM_FIELDS (src/sim.js:14-23), in that order. */
+#define FS_NUM_PARAMS 60
+
+typedef struct fs_sim fs_sim;
+
+typedef struct fs_config {
+    int    res[3];        /* cells (nx, ny, nz); reference default 80x120x80 (src/sim.js:49)  */
+    float  size[3];       /* metres; dx = size[0]/res[0] (src/sim.js:53)                      */
+    int    vcycles;       /* V-cycles per step in FS_SOLVE_VCYCLES (src/sim.js:49, default 2) */
+    int    max_levels;    /* 0 = the reference's automatic hierarchy (src/sim.js:80-87)        */
+    int    precision;     /* FS_PREC_*                                                         */
+    int    solve;         /* FS_SOLVE_*                                                        */
+    double div_tol;       /* FS_SOLVE_REFINE64: target ||b - A p|| / ||b||                     */
+    int    max_vcycles;   /* FS_SOLVE_REFINE64: cap on V-cycles per step                       */
+    int    noise_n;       /* noise/curl texture edge (reference: 128, src/main.js:36)          */
+    int    noise_seed;    /* reference: 1 (src/main.js:36)                                     */
+    int    device;        /* HIP device ordinal                                                */
+    void*  stream;        /* hipStream_t to run on, or NULL for a stream owned by the handle   */
+    int    use_graph;     /* 1 = replay each step from a captured HIP graph                    */
+    int    reserved[7];
+} fs_config;
+
+/* Fills a config with the reference defaults (res 80x120x80, size 1.2x1.8x1.2,
+ * vcycles 2, fp16 storage, noise 128 seed 1). */
+void        fs_config_default(fs_config* cfg);
+
+const char* fs_last_error(void);
+const char* fs_version(void);
+
+/* Replaces `new FireSim(device, {res, size, logs, vcycles, noiseTex, curlTex})`
+ * (src/sim.js:49-101).  `logs` is packLogs() output: n_logs records of 8 floats
+ * (a.xyz, r, b.xyz, burn) (src/scene.js:92-98); the noise and curl textures are
+ * baked on the device from cfg->noise_seed (src/noise.js:6-90, src/main.js:36-37).
+ * The state starts zeroed, as after reset(). */
+int  fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out);
+int  fs_destroy(fs_sim* h);
+
+/* Parameter block (SIM_FIELDS order, src/sim.js:14-23; defaults src/sim.js:25-44). */
+int         fs_param_count(void);
+const char* fs_param_name(int index);
+int         fs_param_index(const char* name);        /* -1 if unknown */
+int         fs_param_defaults(float out[FS_NUM_PARAMS]);
+
+/* Replaces setParams(obj) (src/sim.js:601-604): unknown names give FS_EINVAL and
+ * change nothing; hotR0/hotR1/hotY1/bedRadius invalidate the emitter geometry.
+ * The grid/clock entries (nx..nLogs) are owned by the handle and are ignored. */
+int  fs_set_params(fs_sim* h, const char* const* names, const float* values, int n);
+int  fs_get_params(fs_sim* h, float out[FS_NUM_PARAMS]);
+
+/* Rewrites the log buffer and invalidates the geometry (src/main.js:147-149). */
+int  fs_set_logs(fs_sim* h, const float* logs, int n_logs);
+
+/* Replaces reset() (src/sim.js:606-618): zero the 10 dynamic fields and every
+ * level's p; b and geo keep their contents; time = 0, frame = 0. */
+int  fs_reset(fs_sim* h);
+
+/* Replaces `nsteps` calls of step(enc, dt) (src/sim.js:627-648), enqueued. */
+int  fs_step(fs_sim* h, double dt, int nsteps);
+int  fs_sync(fs_sim* h);                                /* onSubmittedWorkDone() */
+
+/* Runs one stage of the step on the current state (for tests and profiling);
+ * FS_STAGE_MG runs the whole pressure solve of one step. Does not advance time. */
+int  fs_run_stage(fs_sim* h, int stage, double dt);
+
+int  fs_read_field(fs_sim* h, int field, int dtype, void* host_dst, size_t bytes);
+int  fs_write_field(fs_sim* h, int field, int dtype, const void* host_src, size_t bytes);
+int  fs_field_info(fs_sim* h, int field, int dims[3], int* channels, int* native_bytes);
+void* fs_device_ptr(fs_sim* h, int field);
+
+/* time (s), frame, grid, multigrid hierarchy (src/sim.js:80-92). */
+int  fs_get_clock(fs_sim* h, double* time, int64_t* frame);
+int  fs_grid_info(fs_sim* h, int res[3], double* dx, double origin[3]);
+int  fs_mg_info(fs_sim* h, int* levels, int dims[][3], int* coarse_iters);
+
+/* Relative Poisson residual ||b - A p||/||b|| of the last solve, FP64 on the device,
+ * and the number of V-cycles the last FS_SOLVE_REFINE64 solve used. */
+int  fs_residual(fs_sim* h, double* rel, int* vcycles_used);
+
+/* Per-kernel device time (ms) of the last step while profiling is on (HIP events
+ * on the handle's stream): adv1, adv2, curl, forces, div, mg, project — the
+ * Profiler labels of src/sim.js:629-644 with `adv` split into its two passes.
+ * Profiled steps are launched eagerly, not from the graph. */
+#define FS_NUM_TIMINGS 7
+int  fs_profile_enable(fs_sim* h, int on);
+int  fs_timings(fs_sim* h, float ms[FS_NUM_TIMINGS]);
+
+/* Replaces buildCampfire(seed).logs + packLogs() (src/scene.js:13-42, 92-98):
+ * writes max_logs*8 floats (zero-padded) and returns the number of real logs. */
+int  fs_build_campfire(int seed, float* out, int max_logs);
+
+/* ---- isolated pressure Poisson solve (cell-centred 7-point operator,
+ *      Dirichlet p=0 on the x/z faces and the top, Neumann floor) ------------- */
+typedef struct fs_poisson fs_poisson;
+
+int  fs_poisson_create(const int dims[3], double h, int max_levels, int device, void* stream,
+                       fs_poisson** out);
+int  fs_poisson_destroy(fs_poisson* ps);
+int  fs_poisson_set_rhs(fs_poisson* ps, int dtype, const void* host_b);   /* FS_DT_F32 / F64 */
+int  fs_poisson_set_x(fs_poisson* ps, int dtype, const void* host_x);
+/* Runs FP32 V(2,2) cycles inside FP64 defect correction until ||b-Ax||/||b|| <= tol
+ * or max_cycles.  hist (max_cycles+1 entries, may be NULL): hist[0] = relative
+ * residual of the starting x, hist[k] = after cycle k (FP64, device-reduced). */
+int  fs_poisson_solve(fs_poisson* ps, double tol, int max_cycles, double* hist, int* cycles);
+/* Plain FP32 V-cycles directly on p (the reference's solve, src/sim.js:642);
+ * hist[k] (may be NULL) = relative residual after cycle k+1. */
+int  fs_poisson_vcycles(fs_poisson* ps, int cycles, double* hist);
+int  fs_poisson_read_x(fs_poisson* ps, int dtype, void* host_x);
+int  fs_poisson_info(fs_poisson* ps, int* levels, int dims[][3], int* coarse_iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FIRESIM_H */

@@ -1,0 +1,347 @@
+"""TEST INFRASTRUCTURE ONLY — ctypes binding of the CPU oracle (oracle/_build/libfsoracle.so).
+
+The oracle is a C++ restatement of the reference's FireSim step (src/sim.js:627-648) and
+its inputs (src/noise.js, src/scene.js).  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg import this module, and only as the checker or the timed
+CPU baseline.  The product (procedural-fire_amd/) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libfsoracle.so")
+
+FP32, FP16, BF16, FP64 = 0, 1, 2, 3
+SOLVE_VCYCLES, SOLVE_REFINE64 = 0, 1
+(VEL, S, S2, VORT, GEO, VELX, SX, S2X, VELY, SY, S2Y, P, B, NOISE, CURL) = range(15)
+(ST_GEO, ST_ADV1, ST_ADV2, ST_CURL, ST_FORCES, ST_DIV, ST_MG, ST_PROJECT) = range(8)
+NUM_PARAMS = 60
+
+
+class FsConfig(ctypes.Structure):
+    """Mirror of fs_config in include/firesim.h."""
+    _fields_ = [
+        ("res", ctypes.c_int * 3),
+        ("size", ctypes.c_float * 3),
+        ("vcycles", ctypes.c_int),
+        ("max_levels", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("solve", ctypes.c_int),
+        ("div_tol", ctypes.c_double),
+        ("max_vcycles", ctypes.c_int),
+        ("noise_n", ctypes.c_int),
+        ("noise_seed", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("stream", ctypes.c_void_p),
+        ("use_graph", ctypes.c_int),
+        ("reserved", ctypes.c_int * 7),
+    ]
+
+
+def make_config(res=(80, 120, 80), size=(1.2, 1.8, 1.2), vcycles=2, max_levels=0, precision=FP16,
+                solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1):
+    c = FsConfig()
+    c.res[:] = list(res)
+    c.size[:] = list(size)
+    c.vcycles = vcycles
+    c.max_levels = max_levels
+    c.precision = precision
+    c.solve = solve
+    c.div_tol = div_tol
+    c.max_vcycles = max_vcycles
+    c.noise_n = noise_n
+    c.noise_seed = noise_seed
+    return c
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "-j4"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, dp, fp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [ctypes.POINTER(FsConfig), fp, ctypes.c_int]
+        L.orc_destroy.argtypes = [vp]
+        for n in ("orc_set_params", "orc_get_params"):
+            getattr(L, n).argtypes = [vp, fp]
+        L.orc_param_defaults.argtypes = [fp]
+        L.orc_param_name.restype = ctypes.c_char_p
+        L.orc_param_name.argtypes = [ctypes.c_int]
+        L.orc_set_logs.argtypes = [vp, fp, ctypes.c_int]
+        L.orc_reset.argtypes = [vp]
+        L.orc_step.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+        L.orc_stage.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.orc_read_field.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_write_field.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_field_dims.argtypes = [vp, ctypes.c_int, ip, ip]
+        L.orc_time.restype = ctypes.c_double
+        L.orc_time.argtypes = [vp]
+        L.orc_set_clock.argtypes = [vp, ctypes.c_double, ctypes.c_long]
+        L.orc_frame.restype = ctypes.c_long
+        L.orc_frame.argtypes = [vp]
+        L.orc_mg_info.argtypes = [vp, ip]
+        L.orc_coarse_iters.argtypes = [vp]
+        L.orc_residual.restype = ctypes.c_double
+        L.orc_residual.argtypes = [vp, ip]
+        L.orc_build_campfire.argtypes = [ctypes.c_int, fp, ctypes.c_int]
+        L.orc_bake_noise.argtypes = [ctypes.c_int, ctypes.c_int, fp, fp]
+        L.orc_hash3f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]
+        L.orc_f16_round.argtypes = [fp, fp, ctypes.c_long]
+        L.orc_bf16_round.argtypes = [fp, fp, ctypes.c_long]
+        L.orc_mg_create.restype = vp
+        L.orc_mg_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_mg_destroy.argtypes = [vp]
+        L.orc_mg_levels.argtypes = [vp, ip, ip]
+        L.orc_mg_set.argtypes = [vp, dp, dp]
+        L.orc_mg_get.argtypes = [vp, dp]
+        L.orc_mg_vcycles.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_mg_op.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_mg_level_get.argtypes = [vp, ctypes.c_int, dp, dp]
+        L.orc_mg_level_set.argtypes = [vp, ctypes.c_int, dp, dp]
+        L.orc_mg_refine.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp]
+        L.orc_mg_get_x.argtypes = [vp, dp]
+        L.orc_mg_residual.restype = ctypes.c_double
+        L.orc_mg_residual.argtypes = [vp]
+        L.orc_mjs_study.restype = ctypes.c_double
+        L.orc_mjs_study.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def param_defaults():
+    out = np.zeros(NUM_PARAMS, np.float32)
+    lib().orc_param_defaults(_fp(out))
+    return out
+
+
+def param_names():
+    return [lib().orc_param_name(i).decode() for i in range(NUM_PARAMS)]
+
+
+def build_campfire(seed=7, max_logs=16):
+    out = np.zeros(max_logs * 8, np.float32)
+    n = lib().orc_build_campfire(seed, _fp(out), max_logs)
+    return out, n
+
+
+def bake_noise(n=128, seed=1):
+    noise = np.zeros(n ** 3 * 4, np.float32)
+    curl = np.zeros(n ** 3 * 4, np.float32)
+    lib().orc_bake_noise(n, seed, _fp(noise), _fp(curl))
+    return noise.reshape(n, n, n, 4), curl.reshape(n, n, n, 4)
+
+
+def hash3f(x, y, z):
+    out = np.zeros(3, np.float32)
+    lib().orc_hash3f(int(x), int(y), int(z), _fp(out))
+    return out
+
+
+def f16_round(a):
+    a = np.ascontiguousarray(a, np.float32)
+    out = np.empty_like(a)
+    lib().orc_f16_round(_fp(a), _fp(out), a.size)
+    return out
+
+
+def bf16_round(a):
+    a = np.ascontiguousarray(a, np.float32)
+    out = np.empty_like(a)
+    lib().orc_bf16_round(_fp(a), _fp(out), a.size)
+    return out
+
+
+class OracleSim:
+    """CPU FireSim restatement.  precision: FP32/FP16/BF16 (f32 math) or FP64."""
+
+    def __init__(self, res=(64, 96, 64), size=None, logs=None, n_logs=None, precision=FP16, vcycles=2,
+                 max_levels=0, solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1):
+        if size is None:
+            size = (res[0] * 0.015, res[1] * 0.015, res[2] * 0.015)
+        if logs is None:
+            logs, n = build_campfire(7)
+            n_logs = 16 if n_logs is None else n_logs
+        logs = np.ascontiguousarray(logs, np.float32)
+        if n_logs is None:
+            n_logs = logs.size // 8
+        self.res = tuple(res)
+        self.cfg = make_config(res, size, vcycles, max_levels, precision, solve, div_tol, max_vcycles, noise_n, noise_seed)
+        self.h = lib().orc_create(ctypes.byref(self.cfg), _fp(logs), n_logs)
+        if not self.h:
+            raise RuntimeError("oracle create failed")
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, **kw):
+        names = param_names()
+        p = self.get_params()
+        for k, v in kw.items():
+            p[names.index(k)] = v
+        lib().orc_set_params(self.h, _fp(p))
+
+    def set_param_block(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        lib().orc_set_params(self.h, _fp(p))
+
+    def get_params(self):
+        out = np.zeros(NUM_PARAMS, np.float32)
+        lib().orc_get_params(self.h, _fp(out))
+        return out
+
+    def reset(self):
+        lib().orc_reset(self.h)
+
+    def step(self, dt, n=1):
+        lib().orc_step(self.h, float(dt), int(n))
+
+    def stage(self, st, dt=0.0):
+        lib().orc_stage(self.h, int(st), float(dt))
+
+    def dims(self, field):
+        d = (ctypes.c_int * 3)()
+        ch = ctypes.c_int()
+        lib().orc_field_dims(self.h, field, d, ctypes.byref(ch))
+        return tuple(d), ch.value
+
+    def read(self, field):
+        (nx, ny, nz), ch = self.dims(field)
+        out = np.zeros(nx * ny * nz * ch, np.float64)
+        rc = lib().orc_read_field(self.h, field, _dp(out))
+        if rc:
+            raise ValueError("bad field")
+        return out.reshape((nz, ny, nx, ch) if ch > 1 else (nz, ny, nx))
+
+    def write(self, field, arr):
+        a = np.ascontiguousarray(arr, np.float64).ravel()
+        (nx, ny, nz), ch = self.dims(field)
+        assert a.size == nx * ny * nz * ch
+        rc = lib().orc_write_field(self.h, field, _dp(a))
+        if rc:
+            raise ValueError("bad field")
+
+    @property
+    def time(self):
+        return lib().orc_time(self.h)
+
+    @property
+    def frame(self):
+        return lib().orc_frame(self.h)
+
+    def set_clock(self, time, frame):
+        lib().orc_set_clock(self.h, float(time), int(frame))
+
+    def mg_info(self):
+        d = (ctypes.c_int * 48)()
+        n = lib().orc_mg_info(self.h, d)
+        return [tuple(d[i * 3:i * 3 + 3]) for i in range(n)], lib().orc_coarse_iters(self.h)
+
+    def residual(self):
+        u = ctypes.c_int()
+        r = lib().orc_residual(self.h, ctypes.byref(u))
+        return r, u.value
+
+
+class OracleMG:
+    """Stand-alone sim.js multigrid (src/sim.js:416-495, 588-599) on an nx*ny*nz grid."""
+
+    def __init__(self, dims, h, max_levels=0, fp64=False):
+        self.dims = tuple(dims)
+        self.h = lib().orc_mg_create(dims[0], dims[1], dims[2], float(h), int(max_levels), 1 if fp64 else 0)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_mg_destroy(self.h)
+        except Exception:
+            pass
+
+    def levels(self):
+        d = (ctypes.c_int * 48)()
+        c = ctypes.c_int()
+        n = lib().orc_mg_levels(self.h, d, ctypes.byref(c))
+        return [tuple(d[i * 3:i * 3 + 3]) for i in range(n)], c.value
+
+    def set(self, b=None, p=None):
+        bb = None if b is None else np.ascontiguousarray(b, np.float64).ravel()
+        pp = None if p is None else np.ascontiguousarray(p, np.float64).ravel()
+        lib().orc_mg_set(self.h, _dp(bb) if bb is not None else None, _dp(pp) if pp is not None else None)
+
+    def get(self):
+        out = np.zeros(int(np.prod(self.dims)), np.float64)
+        lib().orc_mg_get(self.h, _dp(out))
+        return out.reshape(self.dims[::-1])
+
+    def get_x(self):
+        out = np.zeros(int(np.prod(self.dims)), np.float64)
+        lib().orc_mg_get_x(self.h, _dp(out))
+        return out.reshape(self.dims[::-1])
+
+    def vcycles(self, n):
+        hist = np.zeros(max(n, 1), np.float64)
+        lib().orc_mg_vcycles(self.h, n, _dp(hist))
+        return hist[:n]
+
+    def refine(self, tol, max_cycles):
+        hist = np.zeros(max_cycles, np.float64)
+        used = lib().orc_mg_refine(self.h, tol, max_cycles, _dp(hist))
+        return hist[:used]
+
+    def op(self, op, level, arg=0):
+        lib().orc_mg_op(self.h, op, level, arg)
+
+    def level_get(self, level):
+        dims, _ = self.levels()
+        nx, ny, nz = dims[level]
+        p = np.zeros(nx * ny * nz)
+        b = np.zeros(nx * ny * nz)
+        lib().orc_mg_level_get(self.h, level, _dp(p), _dp(b))
+        return p.reshape(nz, ny, nx), b.reshape(nz, ny, nx)
+
+    def level_set(self, level, p=None, b=None):
+        pp = None if p is None else np.ascontiguousarray(p, np.float64).ravel()
+        bb = None if b is None else np.ascontiguousarray(b, np.float64).ravel()
+        lib().orc_mg_level_set(self.h, level, _dp(pp) if pp is not None else None, _dp(bb) if bb is not None else None)
+
+    def residual(self):
+        return lib().orc_mg_residual(self.h)
+
+
+def mjs_study(N, face=True, mode=2, omega=1.0, pre=2, post=2):
+    """Re-run research/webgpu-bench/poisson.mjs's study; returns (r0, residual/r0 history)."""
+    hist = np.zeros(8, np.float64)
+    r0 = lib().orc_mjs_study(N, 1 if face else 0, mode, omega, pre, post, _dp(hist))
+    return r0, hist[: (6 if mode in (0, 1) else 8)]

@@ -1,0 +1,754 @@
+// C ABI of the gfx950 fire-simulation step (include/firesim.h).
+//
+// One fs_sim owns its device fields, the multigrid hierarchy, the noise textures
+// and a device-resident parameter block.  A step is a fixed sequence of launches
+// on the handle's stream (no host round trips, no allocation), optionally
+// replayed from a captured HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fs_internal.h"
+
+// ---- errors ------------------------------------------------------------------
+static thread_local std::string g_last_error;
+void fs_set_error(const std::string& msg) { g_last_error = msg; }
+static int fail(int code, const std::string& msg) { g_last_error = msg; return code; }
+
+// ---- SIM_FIELDS / SIM_DEFAULTS (src/sim.js:14-44) ------------------------------
+static const char* const kParamNames[FS_NUM_PARAMS] = {
+    "nx", "ny", "nz", "dx", "dt", "time", "frame", "seed",
+    "ox", "oy", "oz", "nLogs",
+    "fuelSrc", "bedFuel", "srcTemp", "bedTemp", "srcVel", "srcFreq", "srcSpeed", "srcThresh",
+    "ignT", "kMix", "rs", "heatOut", "expansion", "sootForm", "sootOx", "radCool",
+    "coolRate", "sootDissip", "smokeDissip", "vitDecay", "fuelDissip", "buoyancy", "vortEps", "turbAmp",
+    "turbFreq", "turbRise", "turbEvolve", "windX", "windZ", "gust", "intensity", "velDissip",
+    "bedRadius", "logSrc", "burstX", "burstY", "burstZ", "burstAmt", "smolder", "sponge", "hotR0", "hotR1", "hotY1",
+    "visc", "lamAge", "anchorFreq", "srcFloor", "anchorGain",
+};
+static void default_params(float* p) {
+    std::fill(p, p + FS_NUM_PARAMS, 0.f);
+    p[P_fuelSrc] = 0.13f; p[P_bedFuel] = 0.16f; p[P_srcTemp] = 0.75f; p[P_bedTemp] = 0.82f; p[P_srcVel] = 0.25f;
+    p[P_srcFreq] = 35.0f; p[P_srcSpeed] = 0.06f; p[P_srcThresh] = 0.0f;
+    p[P_ignT] = 550.f; p[P_kMix] = 30.f; p[P_rs] = 4.7f; p[P_heatOut] = 7700.f; p[P_expansion] = 0.7f;
+    p[P_sootForm] = 25.f; p[P_sootOx] = 30.f; p[P_radCool] = 0.25f;
+    p[P_coolRate] = 3.0f; p[P_sootDissip] = 0.4f; p[P_smokeDissip] = 0.3f; p[P_vitDecay] = 1.5f; p[P_fuelDissip] = 0.3f;
+    p[P_buoyancy] = 26.8f;
+    p[P_vortEps] = 8.f; p[P_turbAmp] = 0.8f; p[P_turbFreq] = 1.5f; p[P_turbRise] = 1.5f; p[P_turbEvolve] = 0.6f;
+    p[P_windX] = 0.f; p[P_windZ] = 0.f; p[P_gust] = 0.f; p[P_intensity] = 1.f; p[P_velDissip] = 0.05f;
+    p[P_bedRadius] = 0.22f; p[P_hotR0] = 0.12f; p[P_hotR1] = 0.45f; p[P_hotY1] = 0.5f; p[P_logSrc] = 1.f;
+    p[P_burstX] = 0.f; p[P_burstY] = 0.f; p[P_burstZ] = 0.f; p[P_burstAmt] = 0.f; p[P_smolder] = 0.02f; p[P_sponge] = 1.f;
+    p[P_visc] = 0.f; p[P_lamAge] = 0.f; p[P_anchorFreq] = 1.6f; p[P_srcFloor] = 0.25f; p[P_anchorGain] = 0.95f;
+}
+
+// ---- the handle ------------------------------------------------------------------
+namespace {
+enum { F_VEL, F_VELX, F_VELY, F_S, F_SX, F_SY, F_S2, F_S2X, F_S2Y, F_VORT, F_GEO, F_COUNT };
+int field_slot(int field) {
+    switch (field) {
+        case FS_VEL: return F_VEL; case FS_S: return F_S; case FS_S2: return F_S2; case FS_VORT: return F_VORT;
+        case FS_GEO: return F_GEO; case FS_VELX: return F_VELX; case FS_SX: return F_SX; case FS_S2X: return F_S2X;
+        case FS_VELY: return F_VELY; case FS_SY: return F_SY; case FS_S2Y: return F_S2Y;
+    }
+    return -1;
+}
+size_t texel_bytes(int prec) { return prec == FS_PREC_FP32 ? 16 : 8; }
+}  // namespace
+
+struct fs_sim {
+    fs_config cfg;
+    int nx, ny, nz;
+    size_t ncell;
+    double dx;
+    double origin[3];
+    int prec;
+    Dims g;
+    void* f[F_COUNT] = {};
+    half4_t* noise = nullptr;
+    half4_t* curl = nullptr;
+    float* logs_d = nullptr;
+    int logs_cap = 0;
+    std::vector<float> logs_h;
+    DevParams* P_d = nullptr;
+    float prm[FS_NUM_PARAMS];
+    bool params_dirty = true;
+    double dev_dt = -1.0;
+    GpuMG mg;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    bool geo_ready = false;
+    double time = 0.0;
+    int64_t frame = 0;
+    bool profile = false;
+    hipEvent_t ev[8] = {};
+    bool timings_valid = false;
+
+    FieldSet fields() const {
+        FieldSet F;
+        F.vel = f[F_VEL]; F.velX = f[F_VELX]; F.velY = f[F_VELY];
+        F.S = f[F_S]; F.SX = f[F_SX]; F.SY = f[F_SY];
+        F.S2 = f[F_S2]; F.S2X = f[F_S2X]; F.S2Y = f[F_S2Y];
+        F.vort = f[F_VORT]; F.geo = f[F_GEO];
+        F.noise = noise; F.curl = curl; F.noise_n = cfg.noise_n;
+        F.logs = logs_d;
+        return F;
+    }
+    bool refine() const { return cfg.solve == FS_SOLVE_REFINE64; }
+};
+
+namespace {
+
+int set_device(const fs_sim* h) {
+    FS_HIP_CHECK(hipSetDevice(h->cfg.device));
+    return FS_OK;
+}
+
+// host parameter block -> device (stream ordered; the block travels as a kernel argument)
+void push_params(fs_sim* h, double dt) {
+    DevParams d;
+    std::memset(&d, 0, sizeof(d));
+    for (int i = 0; i < FS_NUM_PARAMS; i++) d.f[i] = h->prm[i];
+    // grid and clock entries are owned by the handle (sim.js:620-624)
+    d.f[P_nx] = (float)h->nx; d.f[P_ny] = (float)h->ny; d.f[P_nz] = (float)h->nz;
+    d.f[P_dx] = (float)h->dx; d.f[P_dt] = (float)dt; d.f[P_time] = (float)h->time; d.f[P_frame] = (float)h->frame;
+    d.f[P_ox] = (float)h->origin[0]; d.f[P_oy] = (float)h->origin[1]; d.f[P_oz] = (float)h->origin[2];
+    d.f[P_nLogs] = (float)(h->logs_h.size() / 8);
+    d.time = h->time;
+    d.dt = dt;
+    d.frame = h->frame;
+    d.n_logs = (int)(h->logs_h.size() / 8);
+    launch_set_params(h->stream, h->P_d, d);
+    h->params_dirty = false;
+    h->dev_dt = dt;
+}
+void sync_params(fs_sim* h, double dt) {
+    if (h->params_dirty || dt != h->dev_dt) push_params(h, dt);
+}
+
+// the launches of one step after the geometry (sim.js:633-644)
+void enqueue_step(fs_sim* h, bool prof) {
+    const FieldSet F = h->fields();
+    hipStream_t s = h->stream;
+    if (prof) (void)hipEventRecord(h->ev[0], s);
+    launch_adv1(h->prec, h->g, s, h->P_d, F);
+    if (prof) (void)hipEventRecord(h->ev[1], s);
+    launch_adv2(h->prec, h->g, s, h->P_d, F);
+    if (prof) (void)hipEventRecord(h->ev[2], s);
+    launch_curl(h->prec, h->g, s, F);
+    if (prof) (void)hipEventRecord(h->ev[3], s);
+    launch_forces(h->prec, h->g, s, h->P_d, F);
+    if (prof) (void)hipEventRecord(h->ev[4], s);
+    launch_div(h->prec, h->g, s, h->P_d, F, h->refine() ? h->mg.bkeep : h->mg.b[0], 1);
+    if (prof) (void)hipEventRecord(h->ev[5], s);
+    if (h->refine()) mg_refine(h->mg, s, h->cfg.div_tol, h->cfg.max_vcycles);
+    else for (int i = 0; i < h->cfg.vcycles; i++) mg_vcycle(h->mg, s, nullptr);
+    if (prof) (void)hipEventRecord(h->ev[6], s);
+    launch_project(h->prec, h->g, s, h->P_d, F, h->refine() ? nullptr : h->mg.p[0], h->refine() ? h->mg.x64 : nullptr);
+    if (prof) (void)hipEventRecord(h->ev[7], s);
+}
+
+int ensure_graph(fs_sim* h) {
+    if (h->gexec) return FS_OK;
+    FS_HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(h, false);
+    hipGraph_t gr = nullptr;
+    FS_HIP_CHECK(hipStreamEndCapture(h->stream, &gr));
+    h->graph = gr;
+    FS_HIP_CHECK(hipGraphInstantiate(&h->gexec, gr, nullptr, nullptr, 0));
+    return FS_OK;
+}
+
+void free_sim(fs_sim* h) {
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    for (auto& p : h->f) if (p) (void)hipFree(p);
+    if (h->noise) (void)hipFree(h->noise);
+    if (h->curl) (void)hipFree(h->curl);
+    if (h->logs_d) (void)hipFree(h->logs_d);
+    if (h->P_d) (void)hipFree(h->P_d);
+    mg_free(h->mg);
+    for (auto& e : h->ev) if (e) (void)hipEventDestroy(e);
+    if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int upload_logs(fs_sim* h, const float* logs, int n_logs) {
+    h->logs_h.assign(logs, logs + (size_t)n_logs * 8);
+    const int need = std::max(n_logs, 1) * 8;
+    if (need > h->logs_cap) {
+        if (h->logs_d) { FS_HIP_CHECK(hipStreamSynchronize(h->stream)); (void)hipFree(h->logs_d); h->logs_d = nullptr; }
+        FS_HIP_CHECK(hipMalloc(&h->logs_d, need * sizeof(float)));
+        h->logs_cap = need;
+        if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    }
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (n_logs > 0) FS_HIP_CHECK(hipMemcpy(h->logs_d, logs, (size_t)n_logs * 8 * sizeof(float), hipMemcpyHostToDevice));
+    h->geo_ready = false;
+    h->params_dirty = true;
+    return FS_OK;
+}
+
+// float <-> storage conversion kernels for field transfers
+__global__ void k_pack_f16(const float* __restrict__ src, half4_t* __restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    half4_t h;
+    h.x = (_Float16)src[4 * i]; h.y = (_Float16)src[4 * i + 1]; h.z = (_Float16)src[4 * i + 2]; h.w = (_Float16)src[4 * i + 3];
+    dst[i] = h;
+}
+__global__ void k_unpack_f16(const half4_t* __restrict__ src, float* __restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const half4_t h = src[i];
+    dst[4 * i] = (float)h.x; dst[4 * i + 1] = (float)h.y; dst[4 * i + 2] = (float)h.z; dst[4 * i + 3] = (float)h.w;
+}
+__device__ __forceinline__ unsigned short f2bf_io(float f) {
+    unsigned x = __float_as_uint(f);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((x >> 16) | 0x40u);
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return (unsigned short)(x >> 16);
+}
+__global__ void k_pack_bf16(const float* __restrict__ src, unsigned short* __restrict__ dst, size_t n4) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n4) return;
+    dst[i] = f2bf_io(src[i]);
+}
+__global__ void k_unpack_bf16(const unsigned short* __restrict__ src, float* __restrict__ dst, size_t n4) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n4) return;
+    dst[i] = __uint_as_float((unsigned)src[i] << 16);
+}
+
+struct DevField { void* ptr; size_t count; int channels; int kind; };   // kind: 0 f32, 1 f16, 2 bf16, 3 f64
+bool resolve_field(fs_sim* h, int field, DevField& d) {
+    const size_t n = h->ncell;
+    const int sk = h->prec == FS_PREC_FP32 ? 0 : (h->prec == FS_PREC_FP16 ? 1 : 2);
+    if (field == FS_NOISE || field == FS_CURL) {
+        const size_t nn = (size_t)h->cfg.noise_n * h->cfg.noise_n * h->cfg.noise_n;
+        d = {field == FS_NOISE ? (void*)h->noise : (void*)h->curl, nn, 4, 1};
+        return true;
+    }
+    if (field == FS_P) { d = h->refine() ? DevField{h->mg.x64, n, 1, 3} : DevField{h->mg.p[0], n, 1, 0}; return true; }
+    if (field == FS_B) { d = DevField{h->refine() ? (void*)h->mg.bkeep : (void*)h->mg.b[0], n, 1, 0}; return true; }
+    const int slot = field_slot(field);
+    if (slot < 0) return false;
+    d = DevField{h->f[slot], n, 4, sk};
+    return true;
+}
+size_t kind_bytes(int kind) { return kind == 0 ? 4 : kind == 3 ? 8 : 2; }
+
+}  // namespace
+
+extern "C" {
+
+const char* fs_last_error(void) { return g_last_error.c_str(); }
+const char* fs_version(void) { return "firesim-gfx950 1.0"; }
+
+void fs_config_default(fs_config* c) {
+    std::memset(c, 0, sizeof(*c));
+    c->res[0] = 80; c->res[1] = 120; c->res[2] = 80;          // src/sim.js:49
+    c->size[0] = 1.2f; c->size[1] = 1.8f; c->size[2] = 1.2f;
+    c->vcycles = 2;
+    c->max_levels = 0;
+    c->precision = FS_PREC_FP16;
+    c->solve = FS_SOLVE_VCYCLES;
+    c->div_tol = 1e-5;
+    c->max_vcycles = 8;
+    c->noise_n = 128;                                         // src/main.js:36
+    c->noise_seed = 1;
+    c->device = 0;
+    c->stream = nullptr;
+    c->use_graph = 0;
+}
+
+int fs_param_count(void) { return FS_NUM_PARAMS; }
+const char* fs_param_name(int i) { return (i >= 0 && i < FS_NUM_PARAMS) ? kParamNames[i] : nullptr; }
+int fs_param_index(const char* name) {
+    if (!name) return -1;
+    for (int i = 0; i < FS_NUM_PARAMS; i++) if (std::strcmp(name, kParamNames[i]) == 0) return i;
+    return -1;
+}
+int fs_param_defaults(float* out) {
+    if (!out) return fail(FS_EINVAL, "null output");
+    default_params(out);
+    return FS_OK;
+}
+
+int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out) {
+    if (!cfg || !out) return fail(FS_EINVAL, "null argument");
+    *out = nullptr;
+    const int nx = cfg->res[0], ny = cfg->res[1], nz = cfg->res[2];
+    if (nx < 2 || ny < 2 || nz < 2) return fail(FS_EINVAL, "res must be >= 2 in every dimension");
+    if ((long long)nx * ny * nz * 16 > (1LL << 31) * 8LL) return fail(FS_EINVAL, "grid too large");
+    if ((long long)nx * ny * nz >= (1LL << 31) / 4) return fail(FS_EINVAL, "grid exceeds 32-bit texel indexing");
+    if (cfg->precision < FS_PREC_FP32 || cfg->precision > FS_PREC_BF16) return fail(FS_EINVAL, "bad precision");
+    if (cfg->solve != FS_SOLVE_VCYCLES && cfg->solve != FS_SOLVE_REFINE64) return fail(FS_EINVAL, "bad solve mode");
+    if (cfg->vcycles < 0 || cfg->max_vcycles < 0 || cfg->max_vcycles > MG_MAX_REFINE) return fail(FS_EINVAL, "bad V-cycle count");
+    const int nn = cfg->noise_n;
+    if (nn < 4 || (nn & (nn - 1)) != 0 || nn > 512) return fail(FS_EINVAL, "noise_n must be a power of two in [4, 512]");
+    if (n_logs < 0 || (n_logs > 0 && !logs)) return fail(FS_EINVAL, "bad logs");
+    if (!(cfg->size[0] > 0.f)) return fail(FS_EINVAL, "size must be positive");
+
+    fs_sim* h = new fs_sim();
+    h->cfg = *cfg;
+    h->nx = nx; h->ny = ny; h->nz = nz;
+    h->ncell = (size_t)nx * ny * nz;
+    h->dx = (double)cfg->size[0] / nx;                                  // sim.js:53
+    h->origin[0] = -(double)cfg->size[0] / 2; h->origin[1] = 0; h->origin[2] = -(double)cfg->size[2] / 2;   // sim.js:54
+    h->prec = cfg->precision;
+    h->g = make_dims(nx, ny, nz);
+    default_params(h->prm);
+    auto bail = [&](int code) { std::string e = g_last_error; free_sim(h); g_last_error = e; return code; };
+    if (hipSetDevice(cfg->device) != hipSuccess) return bail(fail(FS_EDEVICE, "hipSetDevice failed"));
+    if (cfg->stream) h->stream = (hipStream_t)cfg->stream;
+    else {
+        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(FS_EDEVICE, "stream create failed"));
+        h->own_stream = true;
+    }
+    const size_t fb = h->ncell * texel_bytes(h->prec);
+    for (int i = 0; i < F_COUNT; i++) {
+        if (hipMalloc(&h->f[i], fb) != hipSuccess) return bail(fail(FS_ENOMEM, "hipMalloc failed for a field"));
+        launch_fill_zero(h->stream, h->f[i], fb);
+    }
+    const size_t nb = (size_t)nn * nn * nn * sizeof(half4_t);
+    if (hipMalloc(&h->noise, nb) != hipSuccess || hipMalloc(&h->curl, nb) != hipSuccess)
+        return bail(fail(FS_ENOMEM, "hipMalloc failed for noise textures"));
+    if (hipMalloc(&h->P_d, sizeof(DevParams)) != hipSuccess) return bail(fail(FS_ENOMEM, "hipMalloc failed"));
+    std::string err;
+    int rc = mg_build(h->mg, nx, ny, nz, (float)h->dx, cfg->max_levels, err);
+    if (rc) return bail(fail(rc, err));
+    for (auto& e : h->ev) if (hipEventCreate(&e) != hipSuccess) return bail(fail(FS_EDEVICE, "event create failed"));
+    launch_noise_bake(h->stream, h->noise, h->curl, nn, cfg->noise_seed);   // src/main.js:36-37
+    std::vector<float> none;
+    rc = upload_logs(h, n_logs > 0 ? logs : none.data(), n_logs);
+    if (rc) return bail(rc);
+    push_params(h, 0.0);
+    if (hipStreamSynchronize(h->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+        return bail(fail(FS_EDEVICE, "device error during create"));
+    *out = h;
+    return FS_OK;
+}
+
+int fs_destroy(fs_sim* h) {
+    if (!h) return FS_OK;
+    (void)hipSetDevice(h->cfg.device);
+    (void)hipStreamSynchronize(h->stream);
+    free_sim(h);
+    return FS_OK;
+}
+
+int fs_set_params(fs_sim* h, const char* const* names, const float* values, int n) {
+    if (!h || (n > 0 && (!names || !values))) return fail(FS_EINVAL, "null argument");
+    int idx[FS_NUM_PARAMS * 4];
+    if (n > (int)(sizeof(idx) / sizeof(idx[0]))) return fail(FS_EINVAL, "too many parameters");
+    for (int i = 0; i < n; i++) {   // validate all first: an unknown key changes nothing (src/gpu.js:65)
+        idx[i] = fs_param_index(names[i]);
+        if (idx[i] < 0) return fail(FS_EINVAL, std::string("unknown uniform SimParams.") + (names[i] ? names[i] : "(null)"));
+    }
+    for (int i = 0; i < n; i++) {
+        const int k = idx[i];
+        if (k == P_hotR0 || k == P_hotR1 || k == P_hotY1 || k == P_bedRadius) h->geo_ready = false;   // sim.js:602
+        if (k <= P_nLogs && k != P_seed) continue;   // grid/clock entries belong to the handle (sim.js:622-623)
+        h->prm[k] = values[i];
+    }
+    h->params_dirty = true;
+    return FS_OK;
+}
+
+int fs_get_params(fs_sim* h, float* out) {
+    if (!h || !out) return fail(FS_EINVAL, "null argument");
+    std::memcpy(out, h->prm, sizeof(h->prm));
+    out[P_nx] = (float)h->nx; out[P_ny] = (float)h->ny; out[P_nz] = (float)h->nz; out[P_dx] = (float)h->dx;
+    out[P_time] = (float)h->time; out[P_frame] = (float)h->frame;
+    out[P_ox] = (float)h->origin[0]; out[P_oy] = (float)h->origin[1]; out[P_oz] = (float)h->origin[2];
+    out[P_nLogs] = (float)(h->logs_h.size() / 8);
+    out[P_dt] = (float)(h->dev_dt < 0 ? 0.0 : h->dev_dt);
+    return FS_OK;
+}
+
+int fs_set_logs(fs_sim* h, const float* logs, int n_logs) {
+    if (!h || n_logs < 0 || (n_logs > 0 && !logs)) return fail(FS_EINVAL, "bad logs");
+    if (set_device(h)) return FS_EDEVICE;
+    return upload_logs(h, logs, n_logs);
+}
+
+int fs_reset(fs_sim* h) {   // sim.js:606-618
+    if (!h) return fail(FS_EINVAL, "null handle");
+    if (set_device(h)) return FS_EDEVICE;
+    const size_t fb = h->ncell * texel_bytes(h->prec);
+    for (int i = 0; i < F_COUNT; i++) if (i != F_GEO) launch_fill_zero(h->stream, h->f[i], fb);
+    for (int l = 0; l < h->mg.nlev; l++)
+        launch_fill_zero(h->stream, h->mg.p[l], (size_t)h->mg.dims[l][0] * h->mg.dims[l][1] * h->mg.dims[l][2] * sizeof(float));
+    launch_fill_zero(h->stream, h->mg.x64, h->ncell * sizeof(double));
+    h->time = 0.0;
+    h->frame = 0;
+    h->params_dirty = true;
+    if (hipGetLastError() != hipSuccess) return fail(FS_EDEVICE, "launch failed in reset");
+    return FS_OK;
+}
+
+int fs_step(fs_sim* h, double dt, int nsteps) {   // sim.js:627-648
+    if (!h) return fail(FS_EINVAL, "null handle");
+    if (nsteps < 0 || !(dt > 0.0) || !std::isfinite(dt)) return fail(FS_EINVAL, "dt must be positive and finite, nsteps >= 0");
+    if (set_device(h)) return FS_EDEVICE;
+    for (int k = 0; k < nsteps; k++) {
+        sync_params(h, dt);
+        if (!h->geo_ready) {   // sim.js:632
+            launch_geo(h->prec, h->g, h->stream, h->P_d, h->fields());
+            h->geo_ready = true;
+        }
+        if (h->cfg.use_graph && !h->profile) {
+            int rc = ensure_graph(h);
+            if (rc) return rc;
+            FS_HIP_CHECK(hipGraphLaunch(h->gexec, h->stream));
+        } else {
+            enqueue_step(h, h->profile);
+        }
+        h->time += dt;   // sim.js:646-647 (the device clock advances inside the step)
+        h->frame += 1;
+        if (h->profile) h->timings_valid = true;
+    }
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int fs_sync(fs_sim* h) {
+    if (!h) return fail(FS_EINVAL, "null handle");
+    if (set_device(h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return FS_OK;
+}
+
+int fs_run_stage(fs_sim* h, int stage, double dt) {
+    if (!h) return fail(FS_EINVAL, "null handle");
+    if (stage < 0 || stage >= FS_NUM_STAGES) return fail(FS_EINVAL, "bad stage");
+    if (set_device(h)) return FS_EDEVICE;
+    sync_params(h, dt);
+    const FieldSet F = h->fields();
+    hipStream_t s = h->stream;
+    switch (stage) {
+        case FS_STAGE_GEO: launch_geo(h->prec, h->g, s, h->P_d, F); h->geo_ready = true; break;
+        case FS_STAGE_ADV1: launch_adv1(h->prec, h->g, s, h->P_d, F); break;
+        case FS_STAGE_ADV2: launch_adv2(h->prec, h->g, s, h->P_d, F); break;
+        case FS_STAGE_CURL: launch_curl(h->prec, h->g, s, F); break;
+        case FS_STAGE_FORCES: launch_forces(h->prec, h->g, s, h->P_d, F); break;
+        case FS_STAGE_DIV: launch_div(h->prec, h->g, s, h->P_d, F, h->refine() ? h->mg.bkeep : h->mg.b[0], 0); break;
+        case FS_STAGE_MG:
+            if (h->refine()) mg_refine(h->mg, s, h->cfg.div_tol, h->cfg.max_vcycles);
+            else for (int i = 0; i < h->cfg.vcycles; i++) mg_vcycle(h->mg, s, nullptr);
+            break;
+        case FS_STAGE_PROJECT:
+            launch_project(h->prec, h->g, s, h->P_d, F, h->refine() ? nullptr : h->mg.p[0], h->refine() ? h->mg.x64 : nullptr);
+            break;
+    }
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int fs_field_info(fs_sim* h, int field, int dims[3], int* channels, int* native_bytes) {
+    if (!h) return fail(FS_EINVAL, "null handle");
+    DevField d;
+    if (!resolve_field(h, field, d)) return fail(FS_EINVAL, "unknown field");
+    const bool tex = field == FS_NOISE || field == FS_CURL;
+    if (dims) {
+        dims[0] = tex ? h->cfg.noise_n : h->nx; dims[1] = tex ? h->cfg.noise_n : h->ny; dims[2] = tex ? h->cfg.noise_n : h->nz;
+    }
+    if (channels) *channels = d.channels;
+    if (native_bytes) *native_bytes = (int)kind_bytes(d.kind);
+    return FS_OK;
+}
+
+void* fs_device_ptr(fs_sim* h, int field) {
+    if (!h) return nullptr;
+    DevField d;
+    if (!resolve_field(h, field, d)) { fail(FS_EINVAL, "unknown field"); return nullptr; }
+    return d.ptr;
+}
+
+int fs_read_field(fs_sim* h, int field, int dtype, void* dst, size_t bytes) {
+    if (!h || !dst) return fail(FS_EINVAL, "null argument");
+    if (set_device(h)) return FS_EDEVICE;
+    DevField d;
+    if (!resolve_field(h, field, d)) return fail(FS_EINVAL, "unknown field");
+    const size_t nval = d.count * d.channels;
+    const size_t want = dtype == FS_DT_F32 ? nval * 4 : dtype == FS_DT_F64 ? nval * 8 : nval * kind_bytes(d.kind);
+    if (dtype != FS_DT_F32 && dtype != FS_DT_F64 && dtype != FS_DT_NATIVE) return fail(FS_EINVAL, "bad dtype");
+    if (bytes != want) return fail(FS_EINVAL, "destination size mismatch: expected " + std::to_string(want) + " bytes");
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (dtype == FS_DT_NATIVE) { FS_HIP_CHECK(hipMemcpy(dst, d.ptr, want, hipMemcpyDeviceToHost)); return FS_OK; }
+    // decode to f32 (or f64) on the device, then copy
+    if (d.kind == 3) {
+        std::vector<double> tmp(nval);
+        FS_HIP_CHECK(hipMemcpy(tmp.data(), d.ptr, nval * 8, hipMemcpyDeviceToHost));
+        if (dtype == FS_DT_F64) std::memcpy(dst, tmp.data(), nval * 8);
+        else for (size_t i = 0; i < nval; i++) ((float*)dst)[i] = (float)tmp[i];
+        return FS_OK;
+    }
+    float* dev = nullptr;
+    if (d.kind == 0) dev = (float*)d.ptr;
+    else {
+        FS_HIP_CHECK(hipMalloc(&dev, nval * 4));
+        if (d.kind == 1) k_unpack_f16<<<(unsigned)((d.count + 255) / 256), 256, 0, h->stream>>>((const half4_t*)d.ptr, dev, d.count);
+        else k_unpack_bf16<<<(unsigned)((nval + 255) / 256), 256, 0, h->stream>>>((const unsigned short*)d.ptr, dev, nval);
+        FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    }
+    std::vector<float> tmp(nval);
+    hipError_t e = hipMemcpy(tmp.data(), dev, nval * 4, hipMemcpyDeviceToHost);
+    if (d.kind != 0) (void)hipFree(dev);
+    if (e != hipSuccess) return fail(FS_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    if (dtype == FS_DT_F32) std::memcpy(dst, tmp.data(), nval * 4);
+    else for (size_t i = 0; i < nval; i++) ((double*)dst)[i] = (double)tmp[i];
+    return FS_OK;
+}
+
+int fs_write_field(fs_sim* h, int field, int dtype, const void* src, size_t bytes) {
+    if (!h || !src) return fail(FS_EINVAL, "null argument");
+    if (set_device(h)) return FS_EDEVICE;
+    DevField d;
+    if (!resolve_field(h, field, d)) return fail(FS_EINVAL, "unknown field");
+    const size_t nval = d.count * d.channels;
+    const size_t want = dtype == FS_DT_F32 ? nval * 4 : dtype == FS_DT_F64 ? nval * 8 : nval * kind_bytes(d.kind);
+    if (dtype != FS_DT_F32 && dtype != FS_DT_F64 && dtype != FS_DT_NATIVE) return fail(FS_EINVAL, "bad dtype");
+    if (bytes != want) return fail(FS_EINVAL, "source size mismatch: expected " + std::to_string(want) + " bytes");
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (dtype == FS_DT_NATIVE) FS_HIP_CHECK(hipMemcpy(d.ptr, src, want, hipMemcpyHostToDevice));
+    else if (d.kind == 3) {
+        std::vector<double> tmp(nval);
+        if (dtype == FS_DT_F64) std::memcpy(tmp.data(), src, nval * 8);
+        else for (size_t i = 0; i < nval; i++) tmp[i] = (double)((const float*)src)[i];
+        FS_HIP_CHECK(hipMemcpy(d.ptr, tmp.data(), nval * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(nval);
+        if (dtype == FS_DT_F32) std::memcpy(tmp.data(), src, nval * 4);
+        else for (size_t i = 0; i < nval; i++) tmp[i] = (float)((const double*)src)[i];
+        if (d.kind == 0) FS_HIP_CHECK(hipMemcpy(d.ptr, tmp.data(), nval * 4, hipMemcpyHostToDevice));
+        else {
+            float* dev = nullptr;
+            FS_HIP_CHECK(hipMalloc(&dev, nval * 4));
+            hipError_t e = hipMemcpy(dev, tmp.data(), nval * 4, hipMemcpyHostToDevice);
+            if (e == hipSuccess) {
+                if (d.kind == 1) k_pack_f16<<<(unsigned)((d.count + 255) / 256), 256, 0, h->stream>>>(dev, (half4_t*)d.ptr, d.count);
+                else k_pack_bf16<<<(unsigned)((nval + 255) / 256), 256, 0, h->stream>>>(dev, (unsigned short*)d.ptr, nval);
+                e = hipStreamSynchronize(h->stream);
+            }
+            (void)hipFree(dev);
+            if (e != hipSuccess) return fail(FS_EDEVICE, std::string("field upload: ") + hipGetErrorString(e));
+        }
+    }
+    // writing the pressure in FS_SOLVE_VCYCLES mode also seeds the FP64 iterate and vice versa
+    if (field == FS_P) {
+        if (h->refine()) launch_convert(h->stream, 0, h->mg.x64, h->mg.p[0], h->ncell);
+        else launch_convert(h->stream, 1, h->mg.p[0], h->mg.x64, h->ncell);
+        FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    }
+    if (field == FS_GEO) h->geo_ready = true;
+    return FS_OK;
+}
+
+int fs_get_clock(fs_sim* h, double* time, int64_t* frame) {
+    if (!h) return fail(FS_EINVAL, "null handle");
+    if (time) *time = h->time;
+    if (frame) *frame = h->frame;
+    return FS_OK;
+}
+
+int fs_grid_info(fs_sim* h, int res[3], double* dx, double origin[3]) {
+    if (!h) return fail(FS_EINVAL, "null handle");
+    if (res) { res[0] = h->nx; res[1] = h->ny; res[2] = h->nz; }
+    if (dx) *dx = h->dx;
+    if (origin) { origin[0] = h->origin[0]; origin[1] = h->origin[1]; origin[2] = h->origin[2]; }
+    return FS_OK;
+}
+
+int fs_mg_info(fs_sim* h, int* levels, int dims[][3], int* coarse_iters) {
+    if (!h) return fail(FS_EINVAL, "null handle");
+    if (levels) *levels = h->mg.nlev;
+    if (dims) for (int l = 0; l < h->mg.nlev; l++) for (int k = 0; k < 3; k++) dims[l][k] = h->mg.dims[l][k];
+    if (coarse_iters) *coarse_iters = h->mg.coarse_iters;
+    return FS_OK;
+}
+
+int fs_residual(fs_sim* h, double* rel, int* used) {
+    if (!h) return fail(FS_EINVAL, "null handle");
+    if (set_device(h)) return FS_EDEVICE;
+    if (!h->refine()) mg_residual_of_p(h->mg, h->stream);
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    RefineState st;
+    FS_HIP_CHECK(hipMemcpy(&st, h->mg.st, sizeof(st), hipMemcpyDeviceToHost));
+    if (rel) *rel = h->refine() ? st.rel[st.used] : st.rel[0];
+    if (used) *used = h->refine() ? st.used : h->cfg.vcycles;
+    return FS_OK;
+}
+
+int fs_profile_enable(fs_sim* h, int on) {
+    if (!h) return fail(FS_EINVAL, "null handle");
+    h->profile = on != 0;
+    h->timings_valid = false;
+    return FS_OK;
+}
+
+int fs_timings(fs_sim* h, float ms[FS_NUM_TIMINGS]) {
+    if (!h || !ms) return fail(FS_EINVAL, "null argument");
+    if (!h->timings_valid) return fail(FS_ESTATE, "no profiled step yet (fs_profile_enable, then fs_step)");
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < FS_NUM_TIMINGS; i++) FS_HIP_CHECK(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
+    return FS_OK;
+}
+
+// buildCampfire(seed).logs + packLogs (src/scene.js:13-42, 92-98)
+int fs_build_campfire(int seed, float* out, int max_logs) {
+    if (!out || max_logs < 0) return fail(FS_EINVAL, "bad arguments");
+    uint32_t s = (uint32_t)seed;
+    auto R = [&]() { s = s * 1664525u + 1013904223u; return (double)s / 4294967296.0; };   // scene.js:13-16
+    std::vector<std::vector<double>> logs;
+    auto add = [&](double ax, double ay, double az, double bx, double by, double bz, double r, double burn) {
+        logs.push_back({ax, ay, az, r, bx, by, bz, burn});
+    };
+    add(-0.40, 0.055, -0.17, 0.38, 0.065, -0.13, 0.062, 0.8);   // two base logs
+    add(-0.36, 0.060, 0.15, 0.41, 0.050, 0.19, 0.058, 0.8);
+    for (int i = 0; i < 5; i++) {                                // teepee of split wood
+        const double ang = ((double)i / 5) * M_PI * 2 + 0.35 + (R() - 0.5) * 0.4;
+        const double rb = 0.30 + R() * 0.06;
+        const double topR = 0.035 + R() * 0.03;
+        const double ta = ang + M_PI + (R() - 0.5) * 0.8;
+        const double ty = 0.36 + R() * 0.1;
+        const double rr = 0.032 + R() * 0.012;
+        add(std::cos(ang) * rb, 0.03, std::sin(ang) * rb, std::cos(ta) * topR, ty, std::sin(ta) * topR, rr, 1.0);
+    }
+    add(-0.12, 0.035, 0.02, 0.10, 0.03, -0.05, 0.035, 1.0);      // charred chunk in the coals
+    std::fill(out, out + (size_t)max_logs * 8, 0.f);
+    const int cnt = std::min((int)logs.size(), max_logs);
+    for (int i = 0; i < cnt; i++) for (int k = 0; k < 8; k++) out[i * 8 + k] = (float)logs[i][k];
+    return cnt;
+}
+
+// ---- isolated Poisson solve ---------------------------------------------------------
+struct fs_poisson {
+    GpuMG mg;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+    int n[3];
+};
+
+int fs_poisson_create(const int dims[3], double h, int max_levels, int device, void* stream, fs_poisson** out) {
+    if (!dims || !out || !(h > 0)) return fail(FS_EINVAL, "bad arguments");
+    *out = nullptr;
+    if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2) return fail(FS_EINVAL, "dims must be >= 2");
+    if ((long long)dims[0] * dims[1] * dims[2] >= (1LL << 31) / 8) return fail(FS_EINVAL, "grid too large");
+    fs_poisson* ps = new fs_poisson();
+    ps->device = device;
+    for (int k = 0; k < 3; k++) ps->n[k] = dims[k];
+    if (hipSetDevice(device) != hipSuccess) { delete ps; return fail(FS_EDEVICE, "hipSetDevice failed"); }
+    if (stream) ps->stream = (hipStream_t)stream;
+    else {
+        if (hipStreamCreateWithFlags(&ps->stream, hipStreamNonBlocking) != hipSuccess) { delete ps; return fail(FS_EDEVICE, "stream"); }
+        ps->own_stream = true;
+    }
+    std::string err;
+    int rc = mg_build(ps->mg, dims[0], dims[1], dims[2], (float)h, max_levels, err);
+    if (rc) { mg_free(ps->mg); if (ps->own_stream) (void)hipStreamDestroy(ps->stream); delete ps; return fail(rc, err); }
+    *out = ps;
+    return FS_OK;
+}
+
+int fs_poisson_destroy(fs_poisson* ps) {
+    if (!ps) return FS_OK;
+    (void)hipSetDevice(ps->device);
+    (void)hipStreamSynchronize(ps->stream);
+    mg_free(ps->mg);
+    if (ps->own_stream) (void)hipStreamDestroy(ps->stream);
+    delete ps;
+    return FS_OK;
+}
+
+static int poisson_upload(fs_poisson* ps, int dtype, const void* src, float* f32dst, double* f64dst) {
+    const size_t n = (size_t)ps->n[0] * ps->n[1] * ps->n[2];
+    std::vector<double> d(n);
+    if (dtype == FS_DT_F64) std::memcpy(d.data(), src, n * 8);
+    else if (dtype == FS_DT_F32) for (size_t i = 0; i < n; i++) d[i] = ((const float*)src)[i];
+    else return fail(FS_EINVAL, "dtype must be FS_DT_F32 or FS_DT_F64");
+    std::vector<float> f(n);
+    for (size_t i = 0; i < n; i++) f[i] = (float)d[i];
+    FS_HIP_CHECK(hipStreamSynchronize(ps->stream));
+    if (f32dst) FS_HIP_CHECK(hipMemcpy(f32dst, f.data(), n * 4, hipMemcpyHostToDevice));
+    if (f64dst) FS_HIP_CHECK(hipMemcpy(f64dst, d.data(), n * 8, hipMemcpyHostToDevice));
+    return FS_OK;
+}
+
+int fs_poisson_set_rhs(fs_poisson* ps, int dtype, const void* b) {
+    if (!ps || !b) return fail(FS_EINVAL, "null argument");
+    if (hipSetDevice(ps->device) != hipSuccess) return fail(FS_EDEVICE, "hipSetDevice failed");
+    int rc = poisson_upload(ps, dtype, b, ps->mg.bkeep, nullptr);
+    if (rc) return rc;
+    FS_HIP_CHECK(hipMemcpy(ps->mg.b[0], ps->mg.bkeep, (size_t)ps->n[0] * ps->n[1] * ps->n[2] * 4, hipMemcpyDeviceToDevice));
+    return FS_OK;
+}
+
+int fs_poisson_set_x(fs_poisson* ps, int dtype, const void* x) {
+    if (!ps || !x) return fail(FS_EINVAL, "null argument");
+    if (hipSetDevice(ps->device) != hipSuccess) return fail(FS_EDEVICE, "hipSetDevice failed");
+    return poisson_upload(ps, dtype, x, ps->mg.p[0], ps->mg.x64);
+}
+
+int fs_poisson_solve(fs_poisson* ps, double tol, int max_cycles, double* hist, int* cycles) {
+    if (!ps || max_cycles < 0 || max_cycles > MG_MAX_REFINE) return fail(FS_EINVAL, "bad arguments");
+    if (hipSetDevice(ps->device) != hipSuccess) return fail(FS_EDEVICE, "hipSetDevice failed");
+    mg_refine(ps->mg, ps->stream, tol, max_cycles);
+    FS_HIP_CHECK(hipGetLastError());
+    FS_HIP_CHECK(hipStreamSynchronize(ps->stream));
+    RefineState st;
+    FS_HIP_CHECK(hipMemcpy(&st, ps->mg.st, sizeof(st), hipMemcpyDeviceToHost));
+    if (hist) for (int k = 0; k <= st.used; k++) hist[k] = st.rel[k];
+    if (cycles) *cycles = st.used;
+    return FS_OK;
+}
+
+int fs_poisson_vcycles(fs_poisson* ps, int cycles, double* hist) {
+    if (!ps || cycles < 0) return fail(FS_EINVAL, "bad arguments");
+    if (hipSetDevice(ps->device) != hipSuccess) return fail(FS_EDEVICE, "hipSetDevice failed");
+    // the V-cycles act on (p, b) of level 0; b is the stored right-hand side
+    FS_HIP_CHECK(hipMemcpyAsync(ps->mg.b[0], ps->mg.bkeep, (size_t)ps->n[0] * ps->n[1] * ps->n[2] * 4,
+                                hipMemcpyDeviceToDevice, ps->stream));
+    for (int k = 0; k < cycles; k++) {
+        mg_vcycle(ps->mg, ps->stream, nullptr);
+        if (hist) {
+            mg_residual_of_p(ps->mg, ps->stream);
+            FS_HIP_CHECK(hipStreamSynchronize(ps->stream));
+            RefineState st;
+            FS_HIP_CHECK(hipMemcpy(&st, ps->mg.st, sizeof(st), hipMemcpyDeviceToHost));
+            hist[k] = st.rel[0];
+        }
+    }
+    FS_HIP_CHECK(hipGetLastError());
+    FS_HIP_CHECK(hipStreamSynchronize(ps->stream));
+    return FS_OK;
+}
+
+int fs_poisson_read_x(fs_poisson* ps, int dtype, void* x) {
+    if (!ps || !x) return fail(FS_EINVAL, "null argument");
+    if (hipSetDevice(ps->device) != hipSuccess) return fail(FS_EDEVICE, "hipSetDevice failed");
+    const size_t n = (size_t)ps->n[0] * ps->n[1] * ps->n[2];
+    FS_HIP_CHECK(hipStreamSynchronize(ps->stream));
+    if (dtype == FS_DT_F64) FS_HIP_CHECK(hipMemcpy(x, ps->mg.x64, n * 8, hipMemcpyDeviceToHost));
+    else if (dtype == FS_DT_F32) FS_HIP_CHECK(hipMemcpy(x, ps->mg.p[0], n * 4, hipMemcpyDeviceToHost));
+    else return fail(FS_EINVAL, "dtype must be FS_DT_F32 (the FP32 iterate p) or FS_DT_F64 (the FP64 iterate x)");
+    return FS_OK;
+}
+
+int fs_poisson_info(fs_poisson* ps, int* levels, int dims[][3], int* coarse_iters) {
+    if (!ps) return fail(FS_EINVAL, "null handle");
+    if (levels) *levels = ps->mg.nlev;
+    if (dims) for (int l = 0; l < ps->mg.nlev; l++) for (int k = 0; k < 3; k++) dims[l][k] = ps->mg.dims[l][k];
+    if (coarse_iters) *coarse_iters = ps->mg.coarse_iters;
+    return FS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// Internal declarations of the gfx950 fire-simulation library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/firesim.h"
+
+// ---- parameter block: SIM_FIELDS order (src/sim.js:14-23) ------------------
+enum {
+    P_nx, P_ny, P_nz, P_dx, P_dt, P_time, P_frame, P_seed, P_ox, P_oy, P_oz, P_nLogs,
+    P_fuelSrc, P_bedFuel, P_srcTemp, P_bedTemp, P_srcVel, P_srcFreq, P_srcSpeed, P_srcThresh,
+    P_ignT, P_kMix, P_rs, P_heatOut, P_expansion, P_sootForm, P_sootOx, P_radCool,
+    P_coolRate, P_sootDissip, P_smokeDissip, P_vitDecay, P_fuelDissip, P_buoyancy, P_vortEps, P_turbAmp,
+    P_turbFreq, P_turbRise, P_turbEvolve, P_windX, P_windZ, P_gust, P_intensity, P_velDissip,
+    P_bedRadius, P_logSrc, P_burstX, P_burstY, P_burstZ, P_burstAmt, P_smolder, P_sponge, P_hotR0, P_hotR1, P_hotY1,
+    P_visc, P_lamAge, P_anchorFreq, P_srcFloor, P_anchorGain,
+};
+
+// Device-resident uniform block.  The f32 entries are what the reference uploads
+// each step (src/sim.js:620-624); `time` and `dt` are kept in double as the host's
+// JS numbers are, and rounded to f32 where a kernel reads them.  Living in device
+// memory (not kernel arguments) keeps every launch of a step static, so the step
+// can be replayed from a captured graph.
+struct DevParams {
+    float f[64];
+    double time;
+    double dt;
+    long long frame;
+    int n_logs;
+    int pad;
+};
+
+// ---- storage of the 4-channel fields ------------------------------------------
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef unsigned short ushort4_t __attribute__((ext_vector_type(4)));
+
+struct StF32  { typedef float4 raw; };
+struct StF16  { typedef half4_t raw; };
+struct StBF16 { typedef ushort4_t raw; };
+
+// grid of a 3D field plus the block tiling of the per-cell kernels
+struct Dims {
+    int nx, ny, nz;
+    int nbx, nby;      // tiles of 64 x 4 cells per block in x, y
+    int nblocks;       // nbx * nby * nz
+};
+constexpr int TILE_X = 64;
+constexpr int TILE_Y = 4;
+
+// ---- multigrid (sim.js variant) --------------------------------------------
+constexpr int MG_MAX_LEVELS = 12;
+constexpr int MG_MAX_REFINE = 64;
+
+struct RefineState {            // device-resident FP64 refinement bookkeeping
+    double rel[MG_MAX_REFINE + 1];
+    double bnorm2;
+    int used;                   // V-cycles applied in the current solve
+    int done;                   // 1 once rel <= tol or used == max
+    int pad[2];
+};
+
+struct GpuMG {
+    int nlev = 0;
+    int dims[MG_MAX_LEVELS][3];
+    float h[MG_MAX_LEVELS];
+    int coarse_iters = 8;
+    float* p[MG_MAX_LEVELS] = {};
+    float* b[MG_MAX_LEVELS] = {};
+    // FP64 defect correction
+    double* x64 = nullptr;       // level-0 FP64 iterate
+    float* bkeep = nullptr;      // level-0 right-hand side (the divergence)
+    double* partials = nullptr;  // per-block (r.r, b.b) pairs
+    int npart = 0;
+    RefineState* st = nullptr;
+    int tail_from = -1;          // first level solved by the single-workgroup tail kernel
+};
+
+int  mg_build(GpuMG& mg, int nx, int ny, int nz, float h, int max_levels, std::string& err);
+void mg_free(GpuMG& mg);
+// FP32 V(2,2) cycle on level-0 p/b; `skip` (device int, may be null) = early exit flag
+void mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip);
+// FP64 refinement: r = bkeep - A x; sets level-0 b = r, p = 0; reduces norms;
+// then up to max_cycles of {V-cycle, x += p, residual}; all device-side.
+void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
+// relative FP64 residual of the FP32 level-0 p against level-0 b -> st->rel[0]
+void mg_residual_of_p(GpuMG& mg, hipStream_t s);
+// single-operator launches (tests)
+void mg_smooth(GpuMG& mg, hipStream_t s, int level, int iters, const int* skip);
+
+// ---- error handling ----------------------------------------------------------
+void fs_set_error(const std::string& msg);
+#define FS_HIP_CHECK(call)                                                         \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            fs_set_error(std::string(#call) + ": " + hipGetErrorString(e_));       \
+            return FS_EDEVICE;                                                     \
+        }                                                                          \
+    } while (0)
+
+// ---- field kernels (fs_fields.hip) -------------------------------------------
+Dims make_dims(int nx, int ny, int nz);
+struct FieldSet {
+    void* vel; void* velX; void* velY;
+    void* S; void* SX; void* SY;
+    void* S2; void* S2X; void* S2Y;
+    void* vort; void* geo;
+    const half4_t* noise; const half4_t* curl; int noise_n;
+    const float* logs;
+};
+void launch_geo(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F);
+void launch_adv1(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F);
+void launch_adv2(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F);
+void launch_curl(int prec, const Dims& g, hipStream_t s, const FieldSet& F);
+void launch_forces(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F);
+void launch_div(int prec, const Dims& g, hipStream_t s, DevParams* P, const FieldSet& F, float* b_out, int advance_clock);
+void launch_project(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F,
+                    const float* p32, const double* p64);
+void launch_noise_bake(hipStream_t s, half4_t* noise, half4_t* curl, int n, int seed);
+void launch_set_params(hipStream_t s, DevParams* dst, const DevParams& src);
+void launch_fill_zero(hipStream_t s, void* ptr, size_t bytes);
+void launch_convert(hipStream_t s, int kind, const void* src, void* dst, size_t n);

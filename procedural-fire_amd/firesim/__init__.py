@@ -1,0 +1,361 @@
+"""Python host binding of the gfx950 fire-simulation library (libfiresim.so, C ABI in include/firesim.h).
+
+Mirrors the reference's `FireSim` surface (src/sim.js:48-649): construction options
+(res, size, logs, vcycles), `set_params` / `setParams` (src/sim.js:601-604), `reset`
+(src/sim.js:606-618) and `step` (src/sim.js:627-648), plus field readback in the
+texel order of the reference's 3D textures.  Unknown parameter names raise, like
+UniformStruct.set (src/gpu.js:65).
+
+There is no CPU fallback: if libfiresim.so is missing or no GPU is present, the
+constructors raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libfiresim.so")
+
+PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
+SOLVE_VCYCLES, SOLVE_REFINE64 = 0, 1
+(VEL, S, S2, VORT, GEO, VELX, SX, S2X, VELY, SY, S2Y, P, B, NOISE, CURL) = range(15)
+FIELD_NAMES = {"vel": VEL, "S": S, "S2": S2, "vort": VORT, "geo": GEO, "velX": VELX, "SX": SX, "S2X": S2X,
+               "velY": VELY, "SY": SY, "S2Y": S2Y, "p": P, "b": B, "noise": NOISE, "curl": CURL}
+DT_F32, DT_F64, DT_NATIVE = 0, 1, 2
+(ST_GEO, ST_ADV1, ST_ADV2, ST_CURL, ST_FORCES, ST_DIV, ST_MG, ST_PROJECT) = range(8)
+STAGE_LABELS = ("adv1", "adv2", "curl", "forces", "div", "mg", "project")
+NUM_PARAMS = 60
+
+
+class FsError(RuntimeError):
+    pass
+
+
+class FsConfig(ctypes.Structure):
+    _fields_ = [
+        ("res", ctypes.c_int * 3),
+        ("size", ctypes.c_float * 3),
+        ("vcycles", ctypes.c_int),
+        ("max_levels", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("solve", ctypes.c_int),
+        ("div_tol", ctypes.c_double),
+        ("max_vcycles", ctypes.c_int),
+        ("noise_n", ctypes.c_int),
+        ("noise_seed", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("stream", ctypes.c_void_p),
+        ("use_graph", ctypes.c_int),
+        ("reserved", ctypes.c_int * 7),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libfiresim.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FsError(f"{LIB_PATH} is missing: build it with `make -C procedural-fire_amd/csrc` "
+                      "(or __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, fp, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)
+    sz = ctypes.c_size_t
+    sig = {
+        "fs_config_default": (None, [ctypes.POINTER(FsConfig)]),
+        "fs_last_error": (ctypes.c_char_p, []),
+        "fs_version": (ctypes.c_char_p, []),
+        "fs_create": (ip, [ctypes.POINTER(FsConfig), fp, ip, ctypes.POINTER(vp)]),
+        "fs_destroy": (ip, [vp]),
+        "fs_param_count": (ip, []),
+        "fs_param_name": (ctypes.c_char_p, [ip]),
+        "fs_param_index": (ip, [ctypes.c_char_p]),
+        "fs_param_defaults": (ip, [fp]),
+        "fs_set_params": (ip, [vp, ctypes.POINTER(ctypes.c_char_p), fp, ip]),
+        "fs_get_params": (ip, [vp, fp]),
+        "fs_set_logs": (ip, [vp, fp, ip]),
+        "fs_reset": (ip, [vp]),
+        "fs_step": (ip, [vp, ctypes.c_double, ip]),
+        "fs_sync": (ip, [vp]),
+        "fs_run_stage": (ip, [vp, ip, ctypes.c_double]),
+        "fs_read_field": (ip, [vp, ip, ip, vp, sz]),
+        "fs_write_field": (ip, [vp, ip, ip, vp, sz]),
+        "fs_field_info": (ip, [vp, ip, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "fs_device_ptr": (vp, [vp, ip]),
+        "fs_get_clock": (ip, [vp, dp, ctypes.POINTER(ctypes.c_int64)]),
+        "fs_grid_info": (ip, [vp, ctypes.POINTER(ctypes.c_int), dp, dp]),
+        "fs_mg_info": (ip, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "fs_residual": (ip, [vp, dp, ctypes.POINTER(ctypes.c_int)]),
+        "fs_profile_enable": (ip, [vp, ip]),
+        "fs_timings": (ip, [vp, fp]),
+        "fs_build_campfire": (ip, [ip, fp, ip]),
+        "fs_poisson_create": (ip, [ctypes.POINTER(ctypes.c_int), ctypes.c_double, ip, ip, vp, ctypes.POINTER(vp)]),
+        "fs_poisson_destroy": (ip, [vp]),
+        "fs_poisson_set_rhs": (ip, [vp, ip, vp]),
+        "fs_poisson_set_x": (ip, [vp, ip, vp]),
+        "fs_poisson_solve": (ip, [vp, ctypes.c_double, ip, dp, ctypes.POINTER(ctypes.c_int)]),
+        "fs_poisson_vcycles": (ip, [vp, ip, dp]),
+        "fs_poisson_read_x": (ip, [vp, ip, vp]),
+        "fs_poisson_info": (ip, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise FsError(f"firesim error {rc}: {lib().fs_last_error().decode()}")
+
+
+def param_names():
+    return [lib().fs_param_name(i).decode() for i in range(lib().fs_param_count())]
+
+
+def param_defaults():
+    out = np.zeros(NUM_PARAMS, np.float32)
+    _check(lib().fs_param_defaults(out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return out
+
+
+def build_campfire(seed=7, max_logs=16):
+    """buildCampfire(seed).logs packed by packLogs (src/scene.js:21-42, 92-98)."""
+    out = np.zeros(max_logs * 8, np.float32)
+    n = lib().fs_build_campfire(seed, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), max_logs)
+    if n < 0:
+        _check(n)
+    return out, n
+
+
+class FireSim:
+    """One simulation grid on one GPU: the C-ABI counterpart of `new FireSim(device, {...})`."""
+
+    def __init__(self, res=(80, 120, 80), size=(1.2, 1.8, 1.2), logs=None, vcycles=2, max_levels=0,
+                 precision=PREC_FP16, solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128,
+                 noise_seed=1, device=0, stream=None, use_graph=False):
+        L = lib()
+        cfg = FsConfig()
+        L.fs_config_default(ctypes.byref(cfg))
+        cfg.res[:] = [int(v) for v in res]
+        cfg.size[:] = [float(v) for v in size]
+        cfg.vcycles = int(vcycles)
+        cfg.max_levels = int(max_levels)
+        cfg.precision = int(precision)
+        cfg.solve = int(solve)
+        cfg.div_tol = float(div_tol)
+        cfg.max_vcycles = int(max_vcycles)
+        cfg.noise_n = int(noise_n)
+        cfg.noise_seed = int(noise_seed)
+        cfg.device = int(device)
+        cfg.stream = stream
+        cfg.use_graph = 1 if use_graph else 0
+        if logs is None:
+            logs, _ = build_campfire(7)   # src/main.js:34-35: buildCampfire(7) packed to 16 slots
+        logs = np.ascontiguousarray(logs, np.float32)
+        self._h = ctypes.c_void_p()
+        _check(L.fs_create(ctypes.byref(cfg), logs.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), logs.size // 8,
+                           ctypes.byref(self._h)))
+        self.res = tuple(int(v) for v in res)
+        self.size = tuple(float(v) for v in size)
+        self.precision = precision
+        self.solve = solve
+        self.vcycles = vcycles
+
+    # ---- lifetime
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().fs_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- parameters (src/sim.js:601-604)
+    def set_params(self, params=None, **kw):
+        d = dict(params or {})
+        d.update(kw)
+        if not d:
+            return
+        names = (ctypes.c_char_p * len(d))(*[k.encode() for k in d])
+        vals = np.array([float(v) for v in d.values()], np.float32)
+        _check(lib().fs_set_params(self._h, names, vals.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(d)))
+
+    setParams = set_params
+
+    @property
+    def params(self):
+        out = np.zeros(NUM_PARAMS, np.float32)
+        _check(lib().fs_get_params(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return dict(zip(param_names(), out.tolist()))
+
+    def set_logs(self, logs):
+        logs = np.ascontiguousarray(logs, np.float32)
+        _check(lib().fs_set_logs(self._h, logs.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), logs.size // 8))
+
+    # ---- stepping
+    def reset(self):
+        _check(lib().fs_reset(self._h))
+
+    def step(self, dt, n=1):
+        _check(lib().fs_step(self._h, float(dt), int(n)))
+
+    def sync(self):
+        _check(lib().fs_sync(self._h))
+
+    def run_stage(self, stage, dt=0.0):
+        _check(lib().fs_run_stage(self._h, int(stage), float(dt)))
+
+    # ---- fields
+    def field_info(self, field):
+        field = FIELD_NAMES.get(field, field)
+        d = (ctypes.c_int * 3)()
+        ch, nb = ctypes.c_int(), ctypes.c_int()
+        _check(lib().fs_field_info(self._h, field, d, ctypes.byref(ch), ctypes.byref(nb)))
+        return tuple(d), ch.value, nb.value
+
+    def read(self, field, dtype=np.float32):
+        """Field as an array of shape (nz, ny, nx[, 4]) (x fastest, like the texture)."""
+        field = FIELD_NAMES.get(field, field)
+        (nx, ny, nz), ch, nb = self.field_info(field)
+        shape = (nz, ny, nx, ch) if ch > 1 else (nz, ny, nx)
+        if dtype == "native":
+            raw = {2: np.uint16, 4: np.float32, 8: np.float64}[nb]
+            out = np.empty(shape, raw)
+            code = DT_NATIVE
+        else:
+            out = np.empty(shape, np.dtype(dtype))
+            code = DT_F64 if out.dtype == np.float64 else DT_F32
+        _check(lib().fs_read_field(self._h, field, code, out.ctypes.data, out.nbytes))
+        return out
+
+    def write(self, field, arr, native=False):
+        field = FIELD_NAMES.get(field, field)
+        a = np.ascontiguousarray(arr)
+        if native:
+            code = DT_NATIVE
+        elif a.dtype == np.float64:
+            code = DT_F64
+        else:
+            a = np.ascontiguousarray(a, np.float32)
+            code = DT_F32
+        _check(lib().fs_write_field(self._h, field, code, a.ctypes.data, a.nbytes))
+
+    def device_ptr(self, field):
+        return lib().fs_device_ptr(self._h, FIELD_NAMES.get(field, field))
+
+    # ---- state / diagnostics
+    @property
+    def time(self):
+        t = ctypes.c_double()
+        _check(lib().fs_get_clock(self._h, ctypes.byref(t), None))
+        return t.value
+
+    @property
+    def frame(self):
+        f = ctypes.c_int64()
+        _check(lib().fs_get_clock(self._h, None, ctypes.byref(f)))
+        return f.value
+
+    def grid_info(self):
+        r = (ctypes.c_int * 3)()
+        dx = ctypes.c_double()
+        o = (ctypes.c_double * 3)()
+        _check(lib().fs_grid_info(self._h, r, ctypes.byref(dx), o))
+        return tuple(r), dx.value, tuple(o)
+
+    @property
+    def dx(self):
+        return self.grid_info()[1]
+
+    def mg_info(self):
+        n = ctypes.c_int()
+        d = (ctypes.c_int * 48)()
+        c = ctypes.c_int()
+        _check(lib().fs_mg_info(self._h, ctypes.byref(n), d, ctypes.byref(c)))
+        return [tuple(d[3 * i:3 * i + 3]) for i in range(n.value)], c.value
+
+    def residual(self):
+        r = ctypes.c_double()
+        u = ctypes.c_int()
+        _check(lib().fs_residual(self._h, ctypes.byref(r), ctypes.byref(u)))
+        return r.value, u.value
+
+    def profile(self, on=True):
+        _check(lib().fs_profile_enable(self._h, 1 if on else 0))
+
+    def timings(self):
+        ms = np.zeros(len(STAGE_LABELS), np.float32)
+        _check(lib().fs_timings(self._h, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return dict(zip(STAGE_LABELS, ms.tolist()))
+
+
+class Poisson:
+    """Isolated pressure solve with the sim's operator and multigrid."""
+
+    def __init__(self, dims, h, max_levels=0, device=0, stream=None):
+        L = lib()
+        d = (ctypes.c_int * 3)(*[int(v) for v in dims])
+        self._h = ctypes.c_void_p()
+        _check(L.fs_poisson_create(d, float(h), int(max_levels), int(device), stream, ctypes.byref(self._h)))
+        self.dims = tuple(int(v) for v in dims)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().fs_poisson_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_rhs(self, b):
+        a = np.ascontiguousarray(b, np.float64)
+        _check(lib().fs_poisson_set_rhs(self._h, DT_F64, a.ctypes.data))
+
+    def set_x(self, x):
+        a = np.ascontiguousarray(x, np.float64)
+        _check(lib().fs_poisson_set_x(self._h, DT_F64, a.ctypes.data))
+
+    def solve(self, tol, max_cycles=32):
+        hist = np.zeros(max_cycles + 1, np.float64)
+        n = ctypes.c_int()
+        _check(lib().fs_poisson_solve(self._h, float(tol), int(max_cycles), hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      ctypes.byref(n)))
+        return hist[: n.value + 1]
+
+    def vcycles(self, cycles, history=True):
+        hist = np.zeros(max(cycles, 1), np.float64)
+        _check(lib().fs_poisson_vcycles(self._h, int(cycles),
+                                        hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if history else None))
+        return hist[:cycles] if history else None
+
+    def read_x(self, fp64=True):
+        nx, ny, nz = self.dims
+        out = np.empty((nz, ny, nx), np.float64 if fp64 else np.float32)
+        _check(lib().fs_poisson_read_x(self._h, DT_F64 if fp64 else DT_F32, out.ctypes.data))
+        return out
+
+    def info(self):
+        n = ctypes.c_int()
+        d = (ctypes.c_int * 48)()
+        c = ctypes.c_int()
+        _check(lib().fs_poisson_info(self._h, ctypes.byref(n), d, ctypes.byref(c)))
+        return [tuple(d[3 * i:3 * i + 3]) for i in range(n.value)], c.value
