@@ -41,6 +41,8 @@ extern "C" {
 /* ---- pressure solve ------------------------------------------------------- */
 #define FS_SOLVE_VCYCLES    0  /* `vcycles` V(2,2) cycles on p, warm-started (src/sim.js:642) */
 #define FS_SOLVE_REFINE64   1  /* FP32 V-cycles inside FP64 defect correction to div_tol  */
+#define FS_SOLVE_TOL32      2  /* FP32 V-cycles on p until the FP64-evaluated ||b - A p||/||b||
+                                  <= div_tol or max_vcycles (no defect correction)        */
 
 /* ---- field identifiers ----------------------------------------------------- */
 #define FS_VEL    0   /* xyz velocity (m/s), w = expansion phi              (src/sim.js:3)   */
@@ -88,14 +90,19 @@ typedef struct fs_config {
     int    max_levels;    /* 0 = the reference's automatic hierarchy (src/sim.js:80-87)        */
     int    precision;     /* FS_PREC_*                                                         */
     int    solve;         /* FS_SOLVE_*                                                        */
-    double div_tol;       /* FS_SOLVE_REFINE64: target ||b - A p|| / ||b||                     */
-    int    max_vcycles;   /* FS_SOLVE_REFINE64: cap on V-cycles per step                       */
+    double div_tol;       /* FS_SOLVE_REFINE64 / TOL32: target ||b - A p|| / ||b||             */
+    int    max_vcycles;   /* FS_SOLVE_REFINE64 / TOL32: cap on V-cycles per step               */
     int    noise_n;       /* noise/curl texture edge (reference: 128, src/main.js:36)          */
     int    noise_seed;    /* reference: 1 (src/main.js:36)                                     */
     int    device;        /* HIP device ordinal                                                */
     void*  stream;        /* hipStream_t to run on, or NULL for a stream owned by the handle   */
     int    use_graph;     /* 1 = replay each step from a captured HIP graph                    */
-    int    reserved[7];
+    int    adaptive;      /* FS_SOLVE_REFINE64 / TOL32: 1 = the host enqueues as many V-cycles
+                             as the previous step needed and checks the device's stopping test
+                             once per step (one stream sync per step, no wasted launches);
+                             0 = fully asynchronous: max_vcycles iterations are enqueued and the
+                             ones after convergence exit at once                              */
+    int    reserved[6];
 } fs_config;
 
 /* Fills a config with the reference defaults (res 80x120x80, size 1.2x1.8x1.2,
@@ -132,7 +139,9 @@ int  fs_set_logs(fs_sim* h, const float* logs, int n_logs);
  * level's p; b and geo keep their contents; time = 0, frame = 0. */
 int  fs_reset(fs_sim* h);
 
-/* Replaces `nsteps` calls of step(enc, dt) (src/sim.js:627-648), enqueued. */
+/* Replaces `nsteps` calls of step(enc, dt) (src/sim.js:627-648), enqueued (with
+ * cfg.adaptive on a tolerance-driven solve, each step waits once for its solve's
+ * stopping test before enqueueing the projection). */
 int  fs_step(fs_sim* h, double dt, int nsteps);
 int  fs_sync(fs_sim* h);                                /* onSubmittedWorkDone() */
 
@@ -143,6 +152,9 @@ int  fs_run_stage(fs_sim* h, int stage, double dt);
 int  fs_read_field(fs_sim* h, int field, int dtype, void* host_dst, size_t bytes);
 int  fs_write_field(fs_sim* h, int field, int dtype, const void* host_src, size_t bytes);
 int  fs_field_info(fs_sim* h, int field, int dims[3], int* channels, int* native_bytes);
+/* Device address of a field.  FS_P and FS_B are stored parity-split on the device
+ * (each x-row holds its even-x cells, then its odd-x cells; row stride 2*ceil(nx/2));
+ * fs_read_field / fs_write_field convert them to and from x-fastest order. */
 void* fs_device_ptr(fs_sim* h, int field);
 
 /* time (s), frame, grid, multigrid hierarchy (src/sim.js:80-92). */
@@ -151,7 +163,7 @@ int  fs_grid_info(fs_sim* h, int res[3], double* dx, double origin[3]);
 int  fs_mg_info(fs_sim* h, int* levels, int dims[][3], int* coarse_iters);
 
 /* Relative Poisson residual ||b - A p||/||b|| of the last solve, FP64 on the device,
- * and the number of V-cycles the last FS_SOLVE_REFINE64 solve used. */
+ * and the number of V-cycles it used. */
 int  fs_residual(fs_sim* h, double* rel, int* vcycles_used);
 
 /* Per-kernel device time (ms) of the last step while profiling is on (HIP events

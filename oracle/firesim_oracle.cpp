@@ -766,6 +766,18 @@ struct Oracle : OracleBase {
             std::vector<M> bk = mg.L[0].b;
             last_used = refine64(mg, x64, bk, div_tol, max_vcycles, &last_rel, nullptr);
             for (size_t i = 0; i < ncell; i++) mg.L[0].p[i] = (M)x64[i];
+        } else if (solve_mode == FS_SOLVE_TOL32) {
+            double rr, bb;
+            mg.residual64(mg.L[0].p.data(), nullptr, &rr, &bb);
+            double rel = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+            int used = 0;
+            while (rel > div_tol && used < max_vcycles) {
+                mg.vcycle();
+                mg.residual64(mg.L[0].p.data(), nullptr, &rr, &bb);
+                rel = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+                used++;
+            }
+            last_used = used; last_rel = rel;
         } else {
             for (int i = 0; i < vcycles; i++) mg.vcycle();
             last_used = vcycles;
@@ -1047,6 +1059,7 @@ void orc_bf16_round(const float* in, float* out, long n) { for (long i = 0; i < 
 struct orc_mg { Multigrid<float>* f; Multigrid<double>* d; std::vector<double> x; };
 orc_mg* orc_mg_create(int nx, int ny, int nz, double h, int max_levels, int fp64) {
     orc_mg* m = new orc_mg{nullptr, nullptr, {}};
+    h = (double)(float)h;   // the level spacing is an f32 uniform (sim.js:96)
     if (fp64) { m->d = new Multigrid<double>(); m->d->build(nx, ny, nz, h, max_levels); }
     else { m->f = new Multigrid<float>(); m->f->build(nx, ny, nz, h, max_levels); }
     m->x.assign((size_t)nx * ny * nz, 0.0);

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libfsoracle.so")
 
 FP32, FP16, BF16, FP64 = 0, 1, 2, 3
-SOLVE_VCYCLES, SOLVE_REFINE64 = 0, 1
+SOLVE_VCYCLES, SOLVE_REFINE64, SOLVE_TOL32 = 0, 1, 2
 (VEL, S, S2, VORT, GEO, VELX, SX, S2X, VELY, SY, S2Y, P, B, NOISE, CURL) = range(15)
 (ST_GEO, ST_ADV1, ST_ADV2, ST_CURL, ST_FORCES, ST_DIV, ST_MG, ST_PROJECT) = range(8)
 NUM_PARAMS = 60
@@ -37,7 +37,8 @@ class FsConfig(ctypes.Structure):
         ("device", ctypes.c_int),
         ("stream", ctypes.c_void_p),
         ("use_graph", ctypes.c_int),
-        ("reserved", ctypes.c_int * 7),
+        ("adaptive", ctypes.c_int),
+        ("reserved", ctypes.c_int * 6),
     ]
 
 

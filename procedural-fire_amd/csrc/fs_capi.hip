@@ -81,8 +81,12 @@ struct fs_sim {
     GpuMG mg;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    // captured graphs: the whole step (fixed solves), or its three parts (adaptive solves)
+    enum { G_FULL, G_PRE, G_ITER, G_POST, G_COUNT };
+    hipGraphExec_t gexec[G_COUNT] = {};
+    hipGraph_t graph[G_COUNT] = {};
+    int predict = 1;                 // V-cycles the last adaptive solve needed
+    int* host_state = nullptr;       // pinned: {used, done} of the last solve
     bool geo_ready = false;
     double time = 0.0;
     int64_t frame = 0;
@@ -101,6 +105,21 @@ struct fs_sim {
         return F;
     }
     bool refine() const { return cfg.solve == FS_SOLVE_REFINE64; }
+    bool tracked() const { return cfg.solve != FS_SOLVE_VCYCLES; }
+    bool adaptive() const { return tracked() && cfg.adaptive; }
+    void solve_begin(hipStream_t s) {
+        if (cfg.solve == FS_SOLVE_REFINE64) mg_refine_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
+        else mg_tol32_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
+    }
+    void solve_iter(hipStream_t s) {
+        if (cfg.solve == FS_SOLVE_REFINE64) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
+        else mg_tol32_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
+    }
+    void solve(hipStream_t s) {   // the pressure solve of one step (sim.js:642)
+        if (cfg.solve == FS_SOLVE_REFINE64) mg_refine(mg, s, cfg.div_tol, cfg.max_vcycles);
+        else if (cfg.solve == FS_SOLVE_TOL32) mg_solve_tol32(mg, s, cfg.div_tol, cfg.max_vcycles);
+        else for (int i = 0; i < cfg.vcycles; i++) mg_vcycle(mg, s, nullptr);
+    }
 };
 
 namespace {
@@ -132,8 +151,10 @@ void sync_params(fs_sim* h, double dt) {
     if (h->params_dirty || dt != h->dev_dt) push_params(h, dt);
 }
 
-// the launches of one step after the geometry (sim.js:633-644)
-void enqueue_step(fs_sim* h, bool prof) {
+// the launches of one step after the geometry (sim.js:633-644), in three parts so a
+// tolerance-driven solve can be continued from the host: pre = adv1 .. div (+ the
+// solve's starting residual), iter = one V-cycle + residual, post = project
+void enqueue_pre(fs_sim* h, bool prof, bool begin_solve) {
     const FieldSet F = h->fields();
     hipStream_t s = h->stream;
     if (prof) (void)hipEventRecord(h->ev[0], s);
@@ -147,27 +168,82 @@ void enqueue_step(fs_sim* h, bool prof) {
     if (prof) (void)hipEventRecord(h->ev[4], s);
     launch_div(h->prec, h->g, s, h->P_d, F, h->refine() ? h->mg.bkeep : h->mg.b[0], 1);
     if (prof) (void)hipEventRecord(h->ev[5], s);
-    if (h->refine()) mg_refine(h->mg, s, h->cfg.div_tol, h->cfg.max_vcycles);
-    else for (int i = 0; i < h->cfg.vcycles; i++) mg_vcycle(h->mg, s, nullptr);
+    if (begin_solve) h->solve_begin(s);
+}
+void enqueue_post(fs_sim* h, bool prof) {
+    const FieldSet F = h->fields();
+    hipStream_t s = h->stream;
     if (prof) (void)hipEventRecord(h->ev[6], s);
     launch_project(h->prec, h->g, s, h->P_d, F, h->refine() ? nullptr : h->mg.p[0], h->refine() ? h->mg.x64 : nullptr);
     if (prof) (void)hipEventRecord(h->ev[7], s);
 }
+void enqueue_step(fs_sim* h, bool prof) {
+    enqueue_pre(h, prof, false);
+    h->solve(h->stream);
+    enqueue_post(h, prof);
+}
 
-int ensure_graph(fs_sim* h) {
-    if (h->gexec) return FS_OK;
+int capture(fs_sim* h, int which) {
+    if (h->gexec[which]) return FS_OK;
     FS_HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(h, false);
+    switch (which) {
+        case fs_sim::G_FULL: enqueue_step(h, false); break;
+        case fs_sim::G_PRE: enqueue_pre(h, false, true); break;
+        case fs_sim::G_ITER: h->solve_iter(h->stream); break;
+        case fs_sim::G_POST: enqueue_post(h, false); break;
+    }
     hipGraph_t gr = nullptr;
     FS_HIP_CHECK(hipStreamEndCapture(h->stream, &gr));
-    h->graph = gr;
-    FS_HIP_CHECK(hipGraphInstantiate(&h->gexec, gr, nullptr, nullptr, 0));
+    h->graph[which] = gr;
+    FS_HIP_CHECK(hipGraphInstantiate(&h->gexec[which], gr, nullptr, nullptr, 0));
     return FS_OK;
+}
+int run_part(fs_sim* h, int which, bool prof) {
+    if (h->cfg.use_graph && !prof) {
+        int rc = capture(h, which);
+        if (rc) return rc;
+        FS_HIP_CHECK(hipGraphLaunch(h->gexec[which], h->stream));
+        return FS_OK;
+    }
+    switch (which) {
+        case fs_sim::G_FULL: enqueue_step(h, prof); break;
+        case fs_sim::G_PRE: enqueue_pre(h, prof, true); break;
+        case fs_sim::G_ITER: h->solve_iter(h->stream); break;
+        case fs_sim::G_POST: enqueue_post(h, prof); break;
+    }
+    return FS_OK;
+}
+void drop_graphs(fs_sim* h) {
+    for (int i = 0; i < fs_sim::G_COUNT; i++) {
+        if (h->gexec[i]) (void)hipGraphExecDestroy(h->gexec[i]);
+        if (h->graph[i]) (void)hipGraphDestroy(h->graph[i]);
+        h->gexec[i] = nullptr;
+        h->graph[i] = nullptr;
+    }
+}
+
+// one step with a tolerance-driven solve: enqueue as many iterations as the last
+// step needed, then read the device's stopping state and continue if needed
+int adaptive_step(fs_sim* h, bool prof) {
+    int rc = run_part(h, fs_sim::G_PRE, prof);
+    if (rc) return rc;
+    const int maxc = std::min(h->cfg.max_vcycles, MG_MAX_REFINE);
+    int k = 0;
+    for (; k < std::min(h->predict, maxc); k++) if ((rc = run_part(h, fs_sim::G_ITER, prof))) return rc;
+    while (true) {
+        FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+        if (h->host_state[1] || k >= maxc) break;
+        if ((rc = run_part(h, fs_sim::G_ITER, prof))) return rc;
+        k++;
+    }
+    h->predict = std::max(1, h->host_state[0]);
+    return run_part(h, fs_sim::G_POST, prof);
 }
 
 void free_sim(fs_sim* h) {
-    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-    if (h->graph) (void)hipGraphDestroy(h->graph);
+    drop_graphs(h);
+    if (h->host_state) (void)hipHostFree(h->host_state);
     for (auto& p : h->f) if (p) (void)hipFree(p);
     if (h->noise) (void)hipFree(h->noise);
     if (h->curl) (void)hipFree(h->curl);
@@ -186,7 +262,7 @@ int upload_logs(fs_sim* h, const float* logs, int n_logs) {
         if (h->logs_d) { FS_HIP_CHECK(hipStreamSynchronize(h->stream)); (void)hipFree(h->logs_d); h->logs_d = nullptr; }
         FS_HIP_CHECK(hipMalloc(&h->logs_d, need * sizeof(float)));
         h->logs_cap = need;
-        if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+        drop_graphs(h);
     }
     FS_HIP_CHECK(hipStreamSynchronize(h->stream));
     if (n_logs > 0) FS_HIP_CHECK(hipMemcpy(h->logs_d, logs, (size_t)n_logs * 8 * sizeof(float), hipMemcpyHostToDevice));
@@ -244,6 +320,19 @@ bool resolve_field(fs_sim* h, int field, DevField& d) {
 }
 size_t kind_bytes(int kind) { return kind == 0 ? 4 : kind == 3 ? 8 : 2; }
 
+// the multigrid's parity-split order (rb_index) <-> x-fastest order
+template <class T> void rb_to_natural(const T* src, T* dst, int nx, int ny, int nz) {
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) dst[((size_t)z * ny + y) * nx + x] = src[rb_index(x, y, z, nx, ny)];
+}
+template <class T> void natural_to_rb(const T* src, T* dst, int nx, int ny, int nz) {
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) dst[rb_index(x, y, z, nx, ny)] = src[((size_t)z * ny + y) * nx + x];
+}
+bool is_mg_field(int field) { return field == FS_P || field == FS_B; }
+
 }  // namespace
 
 extern "C" {
@@ -266,6 +355,7 @@ void fs_config_default(fs_config* c) {
     c->device = 0;
     c->stream = nullptr;
     c->use_graph = 0;
+    c->adaptive = 1;
 }
 
 int fs_param_count(void) { return FS_NUM_PARAMS; }
@@ -289,7 +379,7 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     if ((long long)nx * ny * nz * 16 > (1LL << 31) * 8LL) return fail(FS_EINVAL, "grid too large");
     if ((long long)nx * ny * nz >= (1LL << 31) / 4) return fail(FS_EINVAL, "grid exceeds 32-bit texel indexing");
     if (cfg->precision < FS_PREC_FP32 || cfg->precision > FS_PREC_BF16) return fail(FS_EINVAL, "bad precision");
-    if (cfg->solve != FS_SOLVE_VCYCLES && cfg->solve != FS_SOLVE_REFINE64) return fail(FS_EINVAL, "bad solve mode");
+    if (cfg->solve < FS_SOLVE_VCYCLES || cfg->solve > FS_SOLVE_TOL32) return fail(FS_EINVAL, "bad solve mode");
     if (cfg->vcycles < 0 || cfg->max_vcycles < 0 || cfg->max_vcycles > MG_MAX_REFINE) return fail(FS_EINVAL, "bad V-cycle count");
     const int nn = cfg->noise_n;
     if (nn < 4 || (nn & (nn - 1)) != 0 || nn > 512) return fail(FS_EINVAL, "noise_n must be a power of two in [4, 512]");
@@ -325,6 +415,8 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     int rc = mg_build(h->mg, nx, ny, nz, (float)h->dx, cfg->max_levels, err);
     if (rc) return bail(fail(rc, err));
     for (auto& e : h->ev) if (hipEventCreate(&e) != hipSuccess) return bail(fail(FS_EDEVICE, "event create failed"));
+    if (hipHostMalloc((void**)&h->host_state, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(FS_ENOMEM, "pinned host allocation failed"));
     launch_noise_bake(h->stream, h->noise, h->curl, nn, cfg->noise_seed);   // src/main.js:36-37
     std::vector<float> none;
     rc = upload_logs(h, n_logs > 0 ? logs : none.data(), n_logs);
@@ -404,13 +496,8 @@ int fs_step(fs_sim* h, double dt, int nsteps) {   // sim.js:627-648
             launch_geo(h->prec, h->g, h->stream, h->P_d, h->fields());
             h->geo_ready = true;
         }
-        if (h->cfg.use_graph && !h->profile) {
-            int rc = ensure_graph(h);
-            if (rc) return rc;
-            FS_HIP_CHECK(hipGraphLaunch(h->gexec, h->stream));
-        } else {
-            enqueue_step(h, h->profile);
-        }
+        const int rc = h->adaptive() ? adaptive_step(h, h->profile) : run_part(h, fs_sim::G_FULL, h->profile);
+        if (rc) return rc;
         h->time += dt;   // sim.js:646-647 (the device clock advances inside the step)
         h->frame += 1;
         if (h->profile) h->timings_valid = true;
@@ -440,10 +527,7 @@ int fs_run_stage(fs_sim* h, int stage, double dt) {
         case FS_STAGE_CURL: launch_curl(h->prec, h->g, s, F); break;
         case FS_STAGE_FORCES: launch_forces(h->prec, h->g, s, h->P_d, F); break;
         case FS_STAGE_DIV: launch_div(h->prec, h->g, s, h->P_d, F, h->refine() ? h->mg.bkeep : h->mg.b[0], 0); break;
-        case FS_STAGE_MG:
-            if (h->refine()) mg_refine(h->mg, s, h->cfg.div_tol, h->cfg.max_vcycles);
-            else for (int i = 0; i < h->cfg.vcycles; i++) mg_vcycle(h->mg, s, nullptr);
-            break;
+        case FS_STAGE_MG: h->solve(s); break;
         case FS_STAGE_PROJECT:
             launch_project(h->prec, h->g, s, h->P_d, F, h->refine() ? nullptr : h->mg.p[0], h->refine() ? h->mg.x64 : nullptr);
             break;
@@ -482,6 +566,23 @@ int fs_read_field(fs_sim* h, int field, int dtype, void* dst, size_t bytes) {
     if (dtype != FS_DT_F32 && dtype != FS_DT_F64 && dtype != FS_DT_NATIVE) return fail(FS_EINVAL, "bad dtype");
     if (bytes != want) return fail(FS_EINVAL, "destination size mismatch: expected " + std::to_string(want) + " bytes");
     FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (is_mg_field(field)) {   // parity-split on the device: copy, reorder, convert
+        const size_t nd = mg_level_floats(h->mg, 0);
+        if (d.kind == 3) {
+            std::vector<double> raw(nd), nat(nval);
+            FS_HIP_CHECK(hipMemcpy(raw.data(), d.ptr, nd * 8, hipMemcpyDeviceToHost));
+            rb_to_natural(raw.data(), nat.data(), h->nx, h->ny, h->nz);
+            if (dtype == FS_DT_F32) for (size_t i = 0; i < nval; i++) ((float*)dst)[i] = (float)nat[i];
+            else std::memcpy(dst, nat.data(), nval * 8);
+        } else {
+            std::vector<float> raw(nd), nat(nval);
+            FS_HIP_CHECK(hipMemcpy(raw.data(), d.ptr, nd * 4, hipMemcpyDeviceToHost));
+            rb_to_natural(raw.data(), nat.data(), h->nx, h->ny, h->nz);
+            if (dtype == FS_DT_F64) for (size_t i = 0; i < nval; i++) ((double*)dst)[i] = (double)nat[i];
+            else std::memcpy(dst, nat.data(), nval * 4);
+        }
+        return FS_OK;
+    }
     if (dtype == FS_DT_NATIVE) { FS_HIP_CHECK(hipMemcpy(dst, d.ptr, want, hipMemcpyDeviceToHost)); return FS_OK; }
     // decode to f32 (or f64) on the device, then copy
     if (d.kind == 3) {
@@ -518,7 +619,21 @@ int fs_write_field(fs_sim* h, int field, int dtype, const void* src, size_t byte
     if (dtype != FS_DT_F32 && dtype != FS_DT_F64 && dtype != FS_DT_NATIVE) return fail(FS_EINVAL, "bad dtype");
     if (bytes != want) return fail(FS_EINVAL, "source size mismatch: expected " + std::to_string(want) + " bytes");
     FS_HIP_CHECK(hipStreamSynchronize(h->stream));
-    if (dtype == FS_DT_NATIVE) FS_HIP_CHECK(hipMemcpy(d.ptr, src, want, hipMemcpyHostToDevice));
+    if (is_mg_field(field)) {   // reorder into the device's parity-split layout (padding stays zero)
+        const size_t nd = mg_level_floats(h->mg, 0);
+        const bool src64 = dtype == FS_DT_F64 || (dtype == FS_DT_NATIVE && d.kind == 3);
+        if (d.kind == 3) {
+            std::vector<double> nat(nval), raw(nd, 0.0);
+            for (size_t i = 0; i < nval; i++) nat[i] = src64 ? ((const double*)src)[i] : (double)((const float*)src)[i];
+            natural_to_rb(nat.data(), raw.data(), h->nx, h->ny, h->nz);
+            FS_HIP_CHECK(hipMemcpy(d.ptr, raw.data(), nd * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> nat(nval), raw(nd, 0.f);
+            for (size_t i = 0; i < nval; i++) nat[i] = src64 ? (float)((const double*)src)[i] : ((const float*)src)[i];
+            natural_to_rb(nat.data(), raw.data(), h->nx, h->ny, h->nz);
+            FS_HIP_CHECK(hipMemcpy(d.ptr, raw.data(), nd * 4, hipMemcpyHostToDevice));
+        }
+    } else if (dtype == FS_DT_NATIVE) FS_HIP_CHECK(hipMemcpy(d.ptr, src, want, hipMemcpyHostToDevice));
     else if (d.kind == 3) {
         std::vector<double> tmp(nval);
         if (dtype == FS_DT_F64) std::memcpy(tmp.data(), src, nval * 8);
@@ -544,8 +659,9 @@ int fs_write_field(fs_sim* h, int field, int dtype, const void* src, size_t byte
     }
     // writing the pressure in FS_SOLVE_VCYCLES mode also seeds the FP64 iterate and vice versa
     if (field == FS_P) {
-        if (h->refine()) launch_convert(h->stream, 0, h->mg.x64, h->mg.p[0], h->ncell);
-        else launch_convert(h->stream, 1, h->mg.p[0], h->mg.x64, h->ncell);
+        const size_t nd = mg_level_floats(h->mg, 0);
+        if (h->refine()) launch_convert(h->stream, 0, h->mg.x64, h->mg.p[0], nd);
+        else launch_convert(h->stream, 1, h->mg.p[0], h->mg.x64, nd);
         FS_HIP_CHECK(hipStreamSynchronize(h->stream));
     }
     if (field == FS_GEO) h->geo_ready = true;
@@ -578,12 +694,13 @@ int fs_mg_info(fs_sim* h, int* levels, int dims[][3], int* coarse_iters) {
 int fs_residual(fs_sim* h, double* rel, int* used) {
     if (!h) return fail(FS_EINVAL, "null handle");
     if (set_device(h)) return FS_EDEVICE;
-    if (!h->refine()) mg_residual_of_p(h->mg, h->stream);
+    const bool tracked = h->cfg.solve != FS_SOLVE_VCYCLES;   // the solve itself recorded its residuals
+    if (!tracked) mg_residual_of_p(h->mg, h->stream);
     FS_HIP_CHECK(hipStreamSynchronize(h->stream));
     RefineState st;
     FS_HIP_CHECK(hipMemcpy(&st, h->mg.st, sizeof(st), hipMemcpyDeviceToHost));
-    if (rel) *rel = h->refine() ? st.rel[st.used] : st.rel[0];
-    if (used) *used = h->refine() ? st.used : h->cfg.vcycles;
+    if (rel) *rel = tracked ? st.rel[st.used] : st.rel[0];
+    if (used) *used = tracked ? st.used : h->cfg.vcycles;
     return FS_OK;
 }
 
@@ -670,16 +787,18 @@ int fs_poisson_destroy(fs_poisson* ps) {
 }
 
 static int poisson_upload(fs_poisson* ps, int dtype, const void* src, float* f32dst, double* f64dst) {
-    const size_t n = (size_t)ps->n[0] * ps->n[1] * ps->n[2];
-    std::vector<double> d(n);
+    const int nx = ps->n[0], ny = ps->n[1], nz = ps->n[2];
+    const size_t n = (size_t)nx * ny * nz, nd = mg_level_floats(ps->mg, 0);
+    std::vector<double> d(n), drb(nd, 0.0);
     if (dtype == FS_DT_F64) std::memcpy(d.data(), src, n * 8);
     else if (dtype == FS_DT_F32) for (size_t i = 0; i < n; i++) d[i] = ((const float*)src)[i];
     else return fail(FS_EINVAL, "dtype must be FS_DT_F32 or FS_DT_F64");
-    std::vector<float> f(n);
-    for (size_t i = 0; i < n; i++) f[i] = (float)d[i];
+    natural_to_rb(d.data(), drb.data(), nx, ny, nz);
+    std::vector<float> f(nd);
+    for (size_t i = 0; i < nd; i++) f[i] = (float)drb[i];
     FS_HIP_CHECK(hipStreamSynchronize(ps->stream));
-    if (f32dst) FS_HIP_CHECK(hipMemcpy(f32dst, f.data(), n * 4, hipMemcpyHostToDevice));
-    if (f64dst) FS_HIP_CHECK(hipMemcpy(f64dst, d.data(), n * 8, hipMemcpyHostToDevice));
+    if (f32dst) FS_HIP_CHECK(hipMemcpy(f32dst, f.data(), nd * 4, hipMemcpyHostToDevice));
+    if (f64dst) FS_HIP_CHECK(hipMemcpy(f64dst, drb.data(), nd * 8, hipMemcpyHostToDevice));
     return FS_OK;
 }
 
@@ -688,7 +807,7 @@ int fs_poisson_set_rhs(fs_poisson* ps, int dtype, const void* b) {
     if (hipSetDevice(ps->device) != hipSuccess) return fail(FS_EDEVICE, "hipSetDevice failed");
     int rc = poisson_upload(ps, dtype, b, ps->mg.bkeep, nullptr);
     if (rc) return rc;
-    FS_HIP_CHECK(hipMemcpy(ps->mg.b[0], ps->mg.bkeep, (size_t)ps->n[0] * ps->n[1] * ps->n[2] * 4, hipMemcpyDeviceToDevice));
+    FS_HIP_CHECK(hipMemcpy(ps->mg.b[0], ps->mg.bkeep, mg_level_floats(ps->mg, 0) * 4, hipMemcpyDeviceToDevice));
     return FS_OK;
 }
 
@@ -715,8 +834,8 @@ int fs_poisson_vcycles(fs_poisson* ps, int cycles, double* hist) {
     if (!ps || cycles < 0) return fail(FS_EINVAL, "bad arguments");
     if (hipSetDevice(ps->device) != hipSuccess) return fail(FS_EDEVICE, "hipSetDevice failed");
     // the V-cycles act on (p, b) of level 0; b is the stored right-hand side
-    FS_HIP_CHECK(hipMemcpyAsync(ps->mg.b[0], ps->mg.bkeep, (size_t)ps->n[0] * ps->n[1] * ps->n[2] * 4,
-                                hipMemcpyDeviceToDevice, ps->stream));
+    FS_HIP_CHECK(hipMemcpyAsync(ps->mg.b[0], ps->mg.bkeep, mg_level_floats(ps->mg, 0) * 4, hipMemcpyDeviceToDevice,
+                                ps->stream));
     for (int k = 0; k < cycles; k++) {
         mg_vcycle(ps->mg, ps->stream, nullptr);
         if (hist) {
@@ -735,11 +854,18 @@ int fs_poisson_vcycles(fs_poisson* ps, int cycles, double* hist) {
 int fs_poisson_read_x(fs_poisson* ps, int dtype, void* x) {
     if (!ps || !x) return fail(FS_EINVAL, "null argument");
     if (hipSetDevice(ps->device) != hipSuccess) return fail(FS_EDEVICE, "hipSetDevice failed");
-    const size_t n = (size_t)ps->n[0] * ps->n[1] * ps->n[2];
+    const int nx = ps->n[0], ny = ps->n[1], nz = ps->n[2];
+    const size_t nd = mg_level_floats(ps->mg, 0);
     FS_HIP_CHECK(hipStreamSynchronize(ps->stream));
-    if (dtype == FS_DT_F64) FS_HIP_CHECK(hipMemcpy(x, ps->mg.x64, n * 8, hipMemcpyDeviceToHost));
-    else if (dtype == FS_DT_F32) FS_HIP_CHECK(hipMemcpy(x, ps->mg.p[0], n * 4, hipMemcpyDeviceToHost));
-    else return fail(FS_EINVAL, "dtype must be FS_DT_F32 (the FP32 iterate p) or FS_DT_F64 (the FP64 iterate x)");
+    if (dtype == FS_DT_F64) {
+        std::vector<double> raw(nd);
+        FS_HIP_CHECK(hipMemcpy(raw.data(), ps->mg.x64, nd * 8, hipMemcpyDeviceToHost));
+        rb_to_natural(raw.data(), (double*)x, nx, ny, nz);
+    } else if (dtype == FS_DT_F32) {
+        std::vector<float> raw(nd);
+        FS_HIP_CHECK(hipMemcpy(raw.data(), ps->mg.p[0], nd * 4, hipMemcpyDeviceToHost));
+        rb_to_natural(raw.data(), (float*)x, nx, ny, nz);
+    } else return fail(FS_EINVAL, "dtype must be FS_DT_F32 (the FP32 iterate p) or FS_DT_F64 (the FP64 iterate x)");
     return FS_OK;
 }
 

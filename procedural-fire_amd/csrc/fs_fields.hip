@@ -522,7 +522,7 @@ __global__ __launch_bounds__(256) void k_div(DevParams* __restrict__ P, Dims g, 
     const float T = ldc<S>(v, g, c.x, c.y + 1, c.z).y;
     const float K = ldc<S>(v, g, c.x, c.y, c.z - 1).z, F = ldc<S>(v, g, c.x, c.y, c.z + 1).z;
     if (c.y == 0) B = -me.y;
-    bOut[c.i] = ((R - L) + (T - B) + (F - K)) / (2.f * dx) - me.w;
+    bOut[rb_index(c.x, c.y, c.z, g.nx, g.ny)] = ((R - L) + (T - B) + (F - K)) / (2.f * dx) - me.w;   // parity-split layout
 }
 
 // ---- projection (src/sim.js:532-550) ------------------------------------------
@@ -533,15 +533,15 @@ __global__ __launch_bounds__(256) void k_project(const DevParams* __restrict__ P
     Cell c;
     if (!cell_of(g, c)) return;
     const float dx = P->f[P_dx];
-    const float me = (float)p[c.i];
-    const int sxy = g.nx, sxyz = g.nx * g.ny;
+    // p is in the multigrid's parity-split layout
+    const float me = (float)p[rb_index(c.x, c.y, c.z, g.nx, g.ny)];
     // ghost rules: -p outside x, z and above the top (Dirichlet), +p below the floor (Neumann)
-    const float pxp = c.x + 1 < g.nx ? (float)p[c.i + 1] : -me;
-    const float pxm = c.x > 0 ? (float)p[c.i - 1] : -me;
-    const float pyp = c.y + 1 < g.ny ? (float)p[c.i + sxy] : -me;
-    const float pym = c.y > 0 ? (float)p[c.i - sxy] : me;
-    const float pzp = c.z + 1 < g.nz ? (float)p[c.i + sxyz] : -me;
-    const float pzm = c.z > 0 ? (float)p[c.i - sxyz] : -me;
+    const float pxp = c.x + 1 < g.nx ? (float)p[rb_index(c.x + 1, c.y, c.z, g.nx, g.ny)] : -me;
+    const float pxm = c.x > 0 ? (float)p[rb_index(c.x - 1, c.y, c.z, g.nx, g.ny)] : -me;
+    const float pyp = c.y + 1 < g.ny ? (float)p[rb_index(c.x, c.y + 1, c.z, g.nx, g.ny)] : -me;
+    const float pym = c.y > 0 ? (float)p[rb_index(c.x, c.y - 1, c.z, g.nx, g.ny)] : me;
+    const float pzp = c.z + 1 < g.nz ? (float)p[rb_index(c.x, c.y, c.z + 1, g.nx, g.ny)] : -me;
+    const float pzm = c.z > 0 ? (float)p[rb_index(c.x, c.y, c.z - 1, g.nx, g.ny)] : -me;
     const float gx = (pxp - pxm) / (2.f * dx), gy = (pyp - pym) / (2.f * dx), gz = (pzp - pzm) / (2.f * dx);
     const float4 vi = Codec<S>::ld(vIn, c.i);
     float vx = vi.x - gx, vy = vi.y - gy, vz = vi.z - gz;

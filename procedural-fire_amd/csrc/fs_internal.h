@@ -58,7 +58,8 @@ struct RefineState {            // device-resident FP64 refinement bookkeeping
     double bnorm2;
     int used;                   // V-cycles applied in the current solve
     int done;                   // 1 once rel <= tol or used == max
-    int pad[2];
+    unsigned counter;           // blocks of the running residual kernel that have finished
+    int pad;
 };
 
 struct GpuMG {
@@ -75,19 +76,38 @@ struct GpuMG {
     int npart = 0;
     RefineState* st = nullptr;
     int tail_from = -1;          // first level solved by the single-workgroup tail kernel
+    bool use_tail = true;        // false: one launch per colour pass on every level
+    bool fuse_prolong = true;    // prolongation folded into the first post-smoothing pass
 };
 
+// Multigrid arrays are parity-split: each x-row stores its even-x cells, then its odd-x
+// cells; rb_index is the offset of cell (x, y, z) in a level of width nx.
+__host__ __device__ inline int rb_index(int x, int y, int z, int nx, int ny) {
+    const int nh = (nx + 1) >> 1;
+    return (z * ny + y) * 2 * nh + (x & 1) * nh + (x >> 1);
+}
 int  mg_build(GpuMG& mg, int nx, int ny, int nz, float h, int max_levels, std::string& err);
+size_t mg_level_floats(const GpuMG& mg, int l);   // array length of level l (with row padding)
 void mg_free(GpuMG& mg);
-// FP32 V(2,2) cycle on level-0 p/b; `skip` (device int, may be null) = early exit flag
-void mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip);
+// FP32 V(2,2) cycle on level-0 p/b; `skip` (device int, may be null) = early exit flag.
+// With `xupd` set, x += p (FP64) is fused into the cycle's last colour pass where
+// possible; returns true if it was.
+bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd = nullptr);
 // FP64 refinement: r = bkeep - A x; sets level-0 b = r, p = 0; reduces norms;
 // then up to max_cycles of {V-cycle, x += p, residual}; all device-side.
 void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
+// FP32 V-cycles on level-0 p until the FP64 residual <= tol or max_cycles (device-side test)
+void mg_solve_tol32(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
+// the same two solves split into their starting residual and one (skippable) iteration,
+// for callers that decide on the host how many iterations to enqueue
+void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
+void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
+void mg_tol32_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
+void mg_tol32_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 // relative FP64 residual of the FP32 level-0 p against level-0 b -> st->rel[0]
 void mg_residual_of_p(GpuMG& mg, hipStream_t s);
 // single-operator launches (tests)
-void mg_smooth(GpuMG& mg, hipStream_t s, int level, int iters, const int* skip);
+void mg_smooth(GpuMG& mg, hipStream_t s, int level, int iters, const int* skip, double* xupd = nullptr);
 
 // ---- error handling ----------------------------------------------------------
 void fs_set_error(const std::string& msg);

@@ -3,91 +3,110 @@
 // Operator (src/sim.js:416-495): cell-centred 7-point Laplacian, p = 0 on the
 // x/z faces and the top (ghost = -p), Neumann floor (ghost = +p).  V(2,2) with
 // red-black Gauss-Seidel (omega = 1), residual fused into the 8-child restriction,
-// mirrored trilinear prolongation, `coarse_iters` sweeps on the coarsest level.
+// mirrored trilinear prolongation, `coarse_iters` sweeps on the coarsest level —
+// the same arithmetic, in the same order, as the reference's dispatches.
+//
+// Layout: every level's p and b (and the FP64 iterate) are stored parity-split:
+// each x-row holds its even-x cells, then its odd-x cells (row stride 2*ceil(nx/2)).
+// A colour pass then reads and writes one contiguous half-row per cell class —
+// 12 bytes per updated cell instead of the 24 a stride-2 pass over x-fastest rows
+// moves — and the y/z neighbours sit in the same half of the adjacent rows.
 //
 // Mixed precision: FP32 V-cycles on the correction inside FP64 defect correction
 // (x <- x + V(b - A x)).  The residual and its norms are FP64; the stopping test
 // runs on the device and later kernels of the same step early-exit on a device
-// flag, so a whole step stays free of host round trips (and graph-capturable).
-// Norm reductions are two-pass with a fixed order: bitwise reproducible.
+// flag, so a whole step needs no host round trip (and is graph-capturable).
+// Norm reductions use per-block partials summed in a fixed order: reproducible.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "fs_internal.h"
 
 namespace {
 
-constexpr int RB = 256;   // threads per block of the reduction kernels
+constexpr int RB = 256;   // threads per block of every multigrid kernel (64 x 4)
 
 __device__ __forceinline__ bool skipped(const int* skip) { return skip != nullptr && *skip != 0; }
 
-// one colour pass of red-black GS (sim.js:422-437); threads cover the half grid
-__global__ __launch_bounds__(256) void k_smooth(float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
-                                                float h, int color, const int* __restrict__ skip) {
+// grid over (half-row index or x, y, z) with 64 x 4 threads per block
+inline dim3 grid3(int w, int ny, int nz) { return dim3((unsigned)((w + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz); }
+
+// one colour pass of red-black GS (sim.js:422-437).  Thread (hx, y, z) owns the cell
+// x = 2hx + s of this colour, s = (y + z + color) & 1.  With `xupd` set (the last pass
+// of a correction V-cycle) it also adds the final p of both cells x = 2hx, 2hx+1 to
+// the FP64 iterate.
+__global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
+                                               float h, int color, const int* __restrict__ skip, double* __restrict__ xupd) {
     if (skipped(skip)) return;
-    const int nhx = (nx + 1) >> 1;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nhx * ny * nz) return;
-    const int hx = i % nhx;
-    const int t = i / nhx;
-    const int y = t % ny, z = t / ny;
-    const int x = hx * 2 + ((y + z + color) & 1);
-    if (x >= nx) return;
-    const int sy = nx, sz = nx * ny;
-    const int c = (z * ny + y) * nx + x;
-    float sum = 0.f, diag = 6.f;
-    if (x > 0) sum += p[c - 1]; else diag += 1.f;
-    if (x < nx - 1) sum += p[c + 1]; else diag += 1.f;
-    if (z > 0) sum += p[c - sz]; else diag += 1.f;
-    if (z < nz - 1) sum += p[c + sz]; else diag += 1.f;
-    if (y > 0) sum += p[c - sy]; else diag -= 1.f;
-    if (y < ny - 1) sum += p[c + sy]; else diag += 1.f;
-    p[c] = (sum - h * h * b[c]) / diag;
+    const int nh = (nx + 1) >> 1, rs = 2 * nh;
+    const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (hx >= nh || y >= ny) return;
+    const int sgn = (y + z + color) & 1;
+    const int x = 2 * hx + sgn;
+    const int row = (z * ny + y) * rs;
+    const int own = row + sgn * nh + hx;
+    const int oth = row + (1 - sgn) * nh;             // the other parity's half of this row
+    if (x < nx) {
+        float sum = 0.f, diag = 6.f;
+        if (x > 0) sum += p[oth + hx - 1 + sgn]; else diag += 1.f;
+        if (x < nx - 1) sum += p[oth + hx + sgn]; else diag += 1.f;
+        if (z > 0) sum += p[own - ny * rs]; else diag += 1.f;
+        if (z < nz - 1) sum += p[own + ny * rs]; else diag += 1.f;
+        if (y > 0) sum += p[own - rs]; else diag -= 1.f;
+        if (y < ny - 1) sum += p[own + rs]; else diag += 1.f;
+        const float pn = (sum - h * h * b[own]) / diag;
+        p[own] = pn;
+        if (xupd) xupd[own] = xupd[own] + (double)pn;
+    }
+    if (xupd && 2 * hx + 1 - sgn < nx) {                // the pair's other cell, final already
+        xupd[oth + hx] = xupd[oth + hx] + (double)p[oth + hx];
+    }
 }
 
 // residual of the 8 children averaged onto the coarse cell; coarse p = 0 (sim.js:446-466)
-__global__ __launch_bounds__(256) void k_restrict(const float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
-                                                  float h, float* __restrict__ bc, float* __restrict__ pc,
-                                                  const int* __restrict__ skip) {
+__global__ __launch_bounds__(RB) void k_restrict(const float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
+                                                 float h, float* __restrict__ bc, float* __restrict__ pc,
+                                                 const int* __restrict__ skip) {
     if (skipped(skip)) return;
-    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= cx * cy * cz) return;
-    const int xc = i % cx;
-    const int t = i / cx;
-    const int yc = t % cy, zc = t / cy;
+    const int cx = nx >> 1, cy = ny >> 1;
+    const int xc = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int yc = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int zc = blockIdx.z;
+    if (xc >= cx || yc >= cy) return;
+    const int nh = nx >> 1, rs = nx;                   // levels that restrict have even nx
     const float ih2 = 1.f / (h * h);
-    const int sy = nx, sz = nx * ny;
     float acc = 0.f;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        const int x = xc * 2 + (k & 1), y = yc * 2 + ((k >> 1) & 1), z = zc * 2 + ((k >> 2) & 1);
-        const int c = (z * ny + y) * nx + x;
+        const int dx = k & 1, y = yc * 2 + ((k >> 1) & 1), z = zc * 2 + ((k >> 2) & 1);
+        const int x = xc * 2 + dx;
+        const int row = (z * ny + y) * rs;
+        const int c = row + dx * nh + xc;
         const float me = p[c];
-        const float pxm = x > 0 ? p[c - 1] : -me;
-        const float pxp = x < nx - 1 ? p[c + 1] : -me;
-        const float pym = y > 0 ? p[c - sy] : me;
-        const float pyp = y < ny - 1 ? p[c + sy] : -me;
-        const float pzm = z > 0 ? p[c - sz] : -me;
-        const float pzp = z < nz - 1 ? p[c + sz] : -me;
+        // x-1 / x+1 live in the other parity's half: (2xc)-1 = odd xc-1, (2xc+1)-1 = even xc, etc.
+        const float pxm = x > 0 ? p[row + (1 - dx) * nh + xc - 1 + dx] : -me;
+        const float pxp = x < nx - 1 ? p[row + (1 - dx) * nh + xc + dx] : -me;
+        const float pym = y > 0 ? p[c - rs] : me;
+        const float pyp = y < ny - 1 ? p[c + rs] : -me;
+        const float pzm = z > 0 ? p[c - ny * rs] : -me;
+        const float pzp = z < nz - 1 ? p[c + ny * rs] : -me;
         const float lap = (pxm + pxp + pym + pyp + pzm + pzp - 6.f * me) * ih2;
         acc += b[c] - lap;
     }
-    bc[i] = acc * 0.125f;
-    pc[i] = 0.f;
+    const int nhc = (cx + 1) >> 1;
+    const int ci = (zc * cy + yc) * 2 * nhc + (xc & 1) * nhc + (xc >> 1);
+    bc[ci] = acc * 0.125f;
+    pc[ci] = 0.f;
 }
 
-// p_fine += mirrored trilinear interpolation of the coarse p (sim.js:473-495)
-__global__ __launch_bounds__(256) void k_prolong(float* __restrict__ p, const float* __restrict__ pcoarse, int nx, int ny, int nz,
-                                                 const int* __restrict__ skip) {
-    if (skipped(skip)) return;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nx * ny * nz) return;
-    const int x = i % nx;
-    const int t = i / nx;
-    const int y = t % ny, z = t / ny;
-    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+// p_fine += mirrored trilinear interpolation of the coarse p (sim.js:473-495);
+// thread (hx, y, z) updates the fine cells x = 2hx and 2hx+1
+__device__ __forceinline__ float prolong_at(const float* __restrict__ pc, int x, int y, int z, int cx, int cy, int cz) {
+    const int nhc = (cx + 1) >> 1, rsc = 2 * nhc;
     const float gx = ((float)x + 0.5f) * 0.5f - 0.5f, gy = ((float)y + 0.5f) * 0.5f - 0.5f, gz = ((float)z + 0.5f) * 0.5f - 0.5f;
     const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
     const int g0x = (int)flx, g0y = (int)fly, g0z = (int)flz;
@@ -98,60 +117,67 @@ __global__ __launch_bounds__(256) void k_prolong(float* __restrict__ p, const fl
         const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
         const int qx = g0x + ox, qy = g0y + oy, qz = g0z + oz;
         const int sx = min(max(qx, 0), cx - 1), sy = min(max(qy, 0), cy - 1), sz = min(max(qz, 0), cz - 1);
-        float val = pcoarse[(sz * cy + sy) * cx + sx];
+        float val = pc[(sz * cy + sy) * rsc + (sx & 1) * nhc + (sx >> 1)];
         if (qx < 0 || qx >= cx) val = -val;
         if (qz < 0 || qz >= cz) val = -val;
         if (qy >= cy) val = -val;
         const float wx = ox ? fx : 1.f - fx, wy = oy ? fy : 1.f - fy, wz = oz ? fz : 1.f - fz;
         acc += val * wx * wy * wz;
     }
-    p[i] = p[i] + acc;
+    return acc;
 }
-
-// FP64 residual r = b - A x of the level-0 operator.  Writes the per-block
-// (r.r, b.b) partial sums; with `rhs_out` set it also stores r (f32) as the
-// V-cycle right-hand side and zeroes the V-cycle iterate.
-template <class X>
-__global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const float* __restrict__ b, int nx, int ny, int nz,
-                                                double h, float* __restrict__ rhs_out, float* __restrict__ p_out,
-                                                double* __restrict__ partials, const int* __restrict__ skip) {
+__global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const float* __restrict__ pcoarse, int nx, int ny, int nz,
+                                                const int* __restrict__ skip) {
     if (skipped(skip)) return;
-    __shared__ double sr[RB], sb[RB];
-    const int n = nx * ny * nz;
-    const int i = blockIdx.x * RB + threadIdx.x;
-    double rr = 0.0, bb = 0.0;
-    if (i < n) {
-        const int xx = i % nx;
-        const int t = i / nx;
-        const int y = t % ny, z = t / ny;
-        const int sy = nx, sz = nx * ny;
-        const double me = (double)x[i];
-        double sum = 0.0, diag = 6.0;
-        if (xx > 0) sum += (double)x[i - 1]; else diag += 1.0;
-        if (xx < nx - 1) sum += (double)x[i + 1]; else diag += 1.0;
-        if (z > 0) sum += (double)x[i - sz]; else diag += 1.0;
-        if (z < nz - 1) sum += (double)x[i + sz]; else diag += 1.0;
-        if (y > 0) sum += (double)x[i - sy]; else diag -= 1.0;
-        if (y < ny - 1) sum += (double)x[i + sy]; else diag += 1.0;
-        const double bv = (double)b[i];
-        const double r = bv - (sum - diag * me) * (1.0 / (h * h));
-        rr = r * r; bb = bv * bv;
-        if (rhs_out) { rhs_out[i] = (float)r; p_out[i] = 0.f; }
-    }
-    sr[threadIdx.x] = rr; sb[threadIdx.x] = bb;
-    __syncthreads();
-    for (int s = RB / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) { sr[threadIdx.x] += sr[threadIdx.x + s]; sb[threadIdx.x] += sb[threadIdx.x + s]; }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { partials[2 * blockIdx.x] = sr[0]; partials[2 * blockIdx.x + 1] = sb[0]; }
+    const int nh = nx >> 1, rs = nx;                   // levels that prolong have even nx
+    const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (hx >= nh || y >= ny) return;
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+    const int row = (z * ny + y) * rs;
+    const float a0 = prolong_at(pcoarse, 2 * hx, y, z, cx, cy, cz);
+    const float a1 = prolong_at(pcoarse, 2 * hx + 1, y, z, cx, cy, cz);
+    p[row + hx] = p[row + hx] + a0;
+    p[row + nh + hx] = p[row + nh + hx] + a1;
 }
 
-// Sums the partials in a fixed order.  phase 0: start of a solve; phase k >= 1:
-// after V-cycle k.  Sets st->done once rel <= tol or k == max_cycles.
-__global__ __launch_bounds__(RB) void k_finalize(RefineState* __restrict__ st, const double* __restrict__ partials, int npart,
-                                                 double tol, int max_cycles, int phase) {
-    if (phase > 0 && st->done) return;
+// Prolongation fused into the first post-smoothing pass (colour 0).  A Gauss-Seidel
+// update overwrites its own cell, so the colour-0 pass only needs the prolonged
+// values p + P(pc) of its colour-1 neighbours, which each thread forms exactly as
+// k_prolong would (same f32 operations); the colour-1 pass that follows overwrites
+// every colour-1 cell without reading it.  Saves the prolongation's full read and
+// write of p and one launch per level, with identical results.
+__global__ __launch_bounds__(RB) void k_smooth_prolong(float* __restrict__ p, const float* __restrict__ b,
+                                                       const float* __restrict__ pcoarse, int nx, int ny, int nz, float h,
+                                                       const int* __restrict__ skip) {
+    if (skipped(skip)) return;
+    const int nh = nx >> 1, rs = nx;                   // levels that prolong have even nx
+    const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (hx >= nh || y >= ny) return;
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+    const int sgn = (y + z) & 1;                       // colour 0
+    const int x = 2 * hx + sgn;
+    const int row = (z * ny + y) * rs;
+    const int own = row + sgn * nh + hx;
+    const int oth = row + (1 - sgn) * nh;
+    float sum = 0.f, diag = 6.f;
+    if (x > 0) sum += p[oth + hx - 1 + sgn] + prolong_at(pcoarse, x - 1, y, z, cx, cy, cz); else diag += 1.f;
+    if (x < nx - 1) sum += p[oth + hx + sgn] + prolong_at(pcoarse, x + 1, y, z, cx, cy, cz); else diag += 1.f;
+    if (z > 0) sum += p[own - ny * rs] + prolong_at(pcoarse, x, y, z - 1, cx, cy, cz); else diag += 1.f;
+    if (z < nz - 1) sum += p[own + ny * rs] + prolong_at(pcoarse, x, y, z + 1, cx, cy, cz); else diag += 1.f;
+    if (y > 0) sum += p[own - rs] + prolong_at(pcoarse, x, y - 1, z, cx, cy, cz); else diag -= 1.f;
+    if (y < ny - 1) sum += p[own + rs] + prolong_at(pcoarse, x, y + 1, z, cx, cy, cz); else diag += 1.f;
+    p[own] = (sum - h * h * b[own]) / diag;
+}
+
+// Sums the per-block partials in a fixed order.  `init`: the starting residual of a
+// solve (phase 0); otherwise the residual after one more V-cycle (the phase count
+// lives in the device state, so one captured iteration can be replayed any number
+// of times).  Sets st->done once rel <= tol or the phase reaches max_cycles.
+__device__ void finalize_sums(RefineState* st, const double* partials, int npart, double tol, int max_cycles, int init) {
     __shared__ double sr[RB], sb[RB];
     double rr = 0.0, bb = 0.0;
     for (int j = threadIdx.x; j < npart; j += RB) { rr += partials[2 * j]; bb += partials[2 * j + 1]; }
@@ -163,6 +189,7 @@ __global__ __launch_bounds__(RB) void k_finalize(RefineState* __restrict__ st, c
     }
     if (threadIdx.x == 0) {
         const double rel = sb[0] > 0.0 ? sqrt(sr[0] / sb[0]) : 0.0;
+        const int phase = init ? 0 : st->used + 1;   // V-cycles applied so far in this solve
         st->rel[phase] = rel;
         st->used = phase;
         st->bnorm2 = sb[0];
@@ -170,17 +197,241 @@ __global__ __launch_bounds__(RB) void k_finalize(RefineState* __restrict__ st, c
     }
 }
 
-// x += e (the V-cycle's correction), FP64
-__global__ __launch_bounds__(256) void k_update(double* __restrict__ x, const float* __restrict__ e, int n,
-                                                const int* __restrict__ skip) {
+// FP64 residual r = b - A x of the level-0 operator at one cell of parity s
+template <class X>
+__device__ __forceinline__ double resid_at(const X* __restrict__ x, const float* __restrict__ b, int xx, int y, int z, int sgn,
+                                           int hx, int nx, int ny, int nz, int nh, int rs, double ih2, double* bv) {
+    const int row = (z * ny + y) * rs;
+    const int own = row + sgn * nh + hx, oth = row + (1 - sgn) * nh;
+    const double me = (double)x[own];
+    double sum = 0.0, diag = 6.0;
+    if (xx > 0) sum += (double)x[oth + hx - 1 + sgn]; else diag += 1.0;
+    if (xx < nx - 1) sum += (double)x[oth + hx + sgn]; else diag += 1.0;
+    if (z > 0) sum += (double)x[own - ny * rs]; else diag += 1.0;
+    if (z < nz - 1) sum += (double)x[own + ny * rs]; else diag += 1.0;
+    if (y > 0) sum += (double)x[own - rs]; else diag -= 1.0;
+    if (y < ny - 1) sum += (double)x[own + rs]; else diag += 1.0;
+    *bv = (double)b[own];
+    return *bv - (sum - diag * me) * ih2;
+}
+
+// FP64 residual over the level-0 grid.  Each block writes its (r.r, b.b) partial sums
+// (k_finalize reduces them in index order).  With `rhs_out` set it also stores r (f32)
+// as the V-cycle right-hand side and zeroes the V-cycle iterate.
+template <class X>
+__global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const float* __restrict__ b, int nx, int ny, int nz,
+                                                double h, float* __restrict__ rhs_out, float* __restrict__ p_out,
+                                                double* __restrict__ partials, const int* __restrict__ skip) {
     if (skipped(skip)) return;
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    __shared__ double sr[RB], sb[RB];
+    const int nh = (nx + 1) >> 1, rs = 2 * nh;
+    const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    const double ih2 = 1.0 / (h * h);
+    double rr = 0.0, bb = 0.0;
+    if (hx < nh && y < ny) {
+        for (int sgn = 0; sgn < 2; sgn++) {
+            const int xx = 2 * hx + sgn;
+            if (xx >= nx) continue;
+            double bv;
+            const double r = resid_at(x, b, xx, y, z, sgn, hx, nx, ny, nz, nh, rs, ih2, &bv);
+            rr += r * r;
+            bb += bv * bv;
+            if (rhs_out) {
+                const int own = (z * ny + y) * rs + sgn * nh + hx;
+                rhs_out[own] = (float)r;
+                p_out[own] = 0.f;
+            }
+        }
+    }
+    sr[threadIdx.x] = rr; sb[threadIdx.x] = bb;
+    __syncthreads();
+    for (int s = RB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { sr[threadIdx.x] += sr[threadIdx.x + s]; sb[threadIdx.x] += sb[threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[2 * blk] = sr[0];
+        partials[2 * blk + 1] = sb[0];
+    }
+}
+
+// one block: the fixed-order reduction of the partials and the stopping test
+__global__ __launch_bounds__(RB) void k_finalize(RefineState* __restrict__ st, const double* __restrict__ partials, int npart,
+                                                 double tol, int max_cycles, int init) {
+    if (!init && st->done) return;
+    finalize_sums(st, partials, npart, tol, max_cycles, init);
+}
+
+// x += e (the V-cycle's correction), FP64, over the whole parity-split array
+__global__ __launch_bounds__(RB) void k_update(double* __restrict__ x, const float* __restrict__ e, int n,
+                                               const int* __restrict__ skip) {
+    if (skipped(skip)) return;
+    const int i = blockIdx.x * RB + threadIdx.x;
     if (i < n) x[i] = x[i] + (double)e[i];
 }
 
-inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+// ---- coarse tail: every level from `tail_from` down, one workgroup, all in LDS ----
+// The coarse levels are tiny (<= 8192 cells at the tail's top level) and a launch
+// per colour pass would cost far more than the work; one 1024-thread workgroup runs
+// their whole part of the V-cycle (pre-smooth, restrict, coarse sweeps, prolong,
+// post-smooth) with the same arithmetic as the per-level kernels above.
+constexpr int TAIL_THREADS = 1024;
+constexpr int TAIL_LDS_BYTES = 64 * 1024;
+
+// exact n / d for n, d < 2^16 with one multiply-high: m = ceil(2^32 / d)
+struct FastDiv {
+    unsigned m;
+    int d;
+};
+static FastDiv make_fastdiv(int d) {
+    FastDiv f;
+    f.d = d;
+    f.m = (unsigned)((((unsigned long long)1 << 32) + (unsigned)d - 1) / (unsigned)d);
+    return f;
+}
+__device__ __forceinline__ int fdiv(int n, FastDiv f) { return (int)__umulhi((unsigned)n, f.m); }
+
+struct TailArgs {
+    int nt;                          // number of tail levels
+    FastDiv dhx[MG_MAX_LEVELS], dx[MG_MAX_LEVELS], dy[MG_MAX_LEVELS], dcx[MG_MAX_LEVELS], dcy[MG_MAX_LEVELS];
+    int dims[MG_MAX_LEVELS][3];
+    float h[MG_MAX_LEVELS];
+    int off[MG_MAX_LEVELS];          // float offset of level j's p in LDS; b follows p
+    int coarse_iters;
+    float* p_top;                    // global p / b of the tail's top level (parity-split layout)
+    const float* b_top;
+};
+
+// linear thread-to-cell mapping; the divisions are multiply-highs (FastDiv)
+__device__ __forceinline__ void tail_smooth(float* p, const float* b, int nx, int ny, int nz, float h, int iters,
+                                            FastDiv dhx, FastDiv dy) {
+    const int nhx = (nx + 1) >> 1, nh = nhx * ny * nz, sy = nx, sz = nx * ny;
+    for (int it = 0; it < iters; it++)
+        for (int color = 0; color < 2; color++) {
+            for (int i = threadIdx.x; i < nh; i += TAIL_THREADS) {
+                const int t = fdiv(i, dhx), hx = i - t * nhx;
+                const int z = fdiv(t, dy), y = t - z * ny;
+                const int x = hx * 2 + ((y + z + color) & 1);
+                if (x < nx) {
+                    const int c = (z * ny + y) * nx + x;
+                    float sum = 0.f, diag = 6.f;
+                    if (x > 0) sum += p[c - 1]; else diag += 1.f;
+                    if (x < nx - 1) sum += p[c + 1]; else diag += 1.f;
+                    if (z > 0) sum += p[c - sz]; else diag += 1.f;
+                    if (z < nz - 1) sum += p[c + sz]; else diag += 1.f;
+                    if (y > 0) sum += p[c - sy]; else diag -= 1.f;
+                    if (y < ny - 1) sum += p[c + sy]; else diag += 1.f;
+                    p[c] = (sum - h * h * b[c]) / diag;
+                }
+            }
+            __syncthreads();
+        }
+}
+__device__ __forceinline__ void tail_restrict(const float* p, const float* b, int nx, int ny, int nz, float h, float* bc,
+                                              float* pc, FastDiv dcx, FastDiv dcy) {
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1, n = cx * cy * cz;
+    const float ih2 = 1.f / (h * h);
+    const int sy = nx, sz = nx * ny;
+    for (int i = threadIdx.x; i < n; i += TAIL_THREADS) {
+        const int t = fdiv(i, dcx), xc = i - t * cx;
+        const int zc = fdiv(t, dcy), yc = t - zc * cy;
+        float acc = 0.f;
+        for (int k = 0; k < 8; k++) {
+            const int x = xc * 2 + (k & 1), y = yc * 2 + ((k >> 1) & 1), z = zc * 2 + ((k >> 2) & 1);
+            const int c = (z * ny + y) * nx + x;
+            const float me = p[c];
+            const float pxm = x > 0 ? p[c - 1] : -me;
+            const float pxp = x < nx - 1 ? p[c + 1] : -me;
+            const float pym = y > 0 ? p[c - sy] : me;
+            const float pyp = y < ny - 1 ? p[c + sy] : -me;
+            const float pzm = z > 0 ? p[c - sz] : -me;
+            const float pzp = z < nz - 1 ? p[c + sz] : -me;
+            const float lap = (pxm + pxp + pym + pyp + pzm + pzp - 6.f * me) * ih2;
+            acc += b[c] - lap;
+        }
+        bc[i] = acc * 0.125f;
+        pc[i] = 0.f;
+    }
+    __syncthreads();
+}
+__device__ __forceinline__ void tail_prolong(float* p, const float* pcoarse, int nx, int ny, int nz, FastDiv dx, FastDiv dy) {
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1, n = nx * ny * nz;
+    for (int i = threadIdx.x; i < n; i += TAIL_THREADS) {
+        const int t = fdiv(i, dx), x = i - t * nx;
+        const int z = fdiv(t, dy), y = t - z * ny;
+        const float gx = ((float)x + 0.5f) * 0.5f - 0.5f, gy = ((float)y + 0.5f) * 0.5f - 0.5f, gz = ((float)z + 0.5f) * 0.5f - 0.5f;
+        const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
+        const int g0x = (int)flx, g0y = (int)fly, g0z = (int)flz;
+        const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
+        float acc = 0.f;
+        for (int k = 0; k < 8; k++) {
+            const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+            const int qx = g0x + ox, qy = g0y + oy, qz = g0z + oz;
+            const int sx = min(max(qx, 0), cx - 1), sy = min(max(qy, 0), cy - 1), sz = min(max(qz, 0), cz - 1);
+            float val = pcoarse[(sz * cy + sy) * cx + sx];
+            if (qx < 0 || qx >= cx) val = -val;
+            if (qz < 0 || qz >= cz) val = -val;
+            if (qy >= cy) val = -val;
+            const float wx = ox ? fx : 1.f - fx, wy = oy ? fy : 1.f - fy, wz = oz ? fz : 1.f - fz;
+            acc += val * wx * wy * wz;
+        }
+        p[i] = p[i] + acc;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailArgs a, const int* __restrict__ skip) {
+    if (skipped(skip)) return;
+    extern __shared__ float lds[];
+    const int n0 = a.dims[0][0] * a.dims[0][1] * a.dims[0][2];
+    float* p0 = lds + a.off[0];
+    float* b0 = p0 + n0;
+    // LDS holds the tail levels in natural x-fastest order; the top level's global
+    // arrays are in the parity-split layout
+    const int nx0 = a.dims[0][0], ny0 = a.dims[0][1], nh0 = (nx0 + 1) >> 1;
+    for (int i = threadIdx.x; i < n0; i += TAIL_THREADS) {
+        const int t = fdiv(i, a.dx[0]), x = i - t * nx0;
+        const int g = t * 2 * nh0 + (x & 1) * nh0 + (x >> 1);
+        p0[i] = a.p_top[g];
+        b0[i] = a.b_top[g];
+    }
+    (void)ny0;
+    __syncthreads();
+    for (int j = 0; j < a.nt - 1; j++) {
+        const int nx = a.dims[j][0], ny = a.dims[j][1], nz = a.dims[j][2], n = nx * ny * nz;
+        float* p = lds + a.off[j];
+        float* b = p + n;
+        const int nc = a.dims[j + 1][0] * a.dims[j + 1][1] * a.dims[j + 1][2];
+        float* pc = lds + a.off[j + 1];
+        tail_smooth(p, b, nx, ny, nz, a.h[j], 2, a.dhx[j], a.dy[j]);
+        tail_restrict(p, b, nx, ny, nz, a.h[j], pc + nc, pc, a.dcx[j], a.dcy[j]);
+    }
+    {
+        const int j = a.nt - 1;
+        const int nx = a.dims[j][0], ny = a.dims[j][1], nz = a.dims[j][2], n = nx * ny * nz;
+        float* p = lds + a.off[j];
+        tail_smooth(p, p + n, nx, ny, nz, a.h[j], a.coarse_iters, a.dhx[j], a.dy[j]);
+    }
+    for (int j = a.nt - 2; j >= 0; j--) {
+        const int nx = a.dims[j][0], ny = a.dims[j][1], nz = a.dims[j][2], n = nx * ny * nz;
+        float* p = lds + a.off[j];
+        tail_prolong(p, lds + a.off[j + 1], nx, ny, nz, a.dx[j], a.dy[j]);
+        tail_smooth(p, p + n, nx, ny, nz, a.h[j], 2, a.dhx[j], a.dy[j]);
+    }
+    for (int i = threadIdx.x; i < n0; i += TAIL_THREADS) {
+        const int t = fdiv(i, a.dx[0]), x = i - t * nx0;
+        a.p_top[t * 2 * nh0 + (x & 1) * nh0 + (x >> 1)] = p0[i];
+    }
+}
+
+inline size_t level_floats(const int* d) { return (size_t)2 * ((d[0] + 1) / 2) * d[1] * d[2]; }
 
 }  // namespace
+
+size_t mg_level_floats(const GpuMG& mg, int l) { return level_floats(mg.dims[l]); }
 
 int mg_build(GpuMG& mg, int nx, int ny, int nz, float h, int max_levels, std::string& err) {
     // levels halve while every dimension is even and the smallest exceeds 6 (sim.js:80-87)
@@ -191,7 +442,7 @@ int mg_build(GpuMG& mg, int nx, int ny, int nz, float h, int max_levels, std::st
         if (mg.nlev >= MG_MAX_LEVELS) { err = "too many multigrid levels"; return FS_EINVAL; }
         const int l = mg.nlev++;
         mg.dims[l][0] = x; mg.dims[l][1] = y; mg.dims[l][2] = z; mg.h[l] = hh;
-        const size_t bytes = (size_t)x * y * z * sizeof(float);
+        const size_t bytes = level_floats(mg.dims[l]) * sizeof(float);
         if (hipMalloc(&mg.p[l], bytes) != hipSuccess || hipMalloc(&mg.b[l], bytes) != hipSuccess) {
             err = "hipMalloc failed for multigrid level"; return FS_ENOMEM;
         }
@@ -206,8 +457,19 @@ int mg_build(GpuMG& mg, int nx, int ny, int nz, float h, int max_levels, std::st
     const int cx = mg.dims[mg.nlev - 1][0], cy = mg.dims[mg.nlev - 1][1], cz = mg.dims[mg.nlev - 1][2];
     const double mu = (std::cos(M_PI / cx) + std::cos(M_PI / (2.0 * cy)) + std::cos(M_PI / cz)) / 3.0;
     mg.coarse_iters = std::max(8, std::min(64, (int)std::ceil(std::log(0.05) / std::log(mu * mu))));
-    const long n0 = (long)nx * ny * nz;
-    mg.npart = (int)((n0 + RB - 1) / RB);
+    // the coarse tail: the largest suffix of levels whose p and b fit in one workgroup's LDS
+    mg.tail_from = mg.nlev;
+    {
+        size_t bytes = 0;
+        for (int l = mg.nlev - 1; l >= 0; l--) {
+            bytes += (size_t)mg.dims[l][0] * mg.dims[l][1] * mg.dims[l][2] * 2 * sizeof(float);
+            if (bytes > (size_t)TAIL_LDS_BYTES) break;
+            mg.tail_from = l;
+        }
+    }
+    const size_t n0 = level_floats(mg.dims[0]);
+    const dim3 g = grid3((nx + 1) / 2, ny, nz);
+    mg.npart = (int)(g.x * g.y * g.z);
     if (hipMalloc(&mg.x64, n0 * sizeof(double)) != hipSuccess || hipMalloc(&mg.bkeep, n0 * sizeof(float)) != hipSuccess ||
         hipMalloc(&mg.partials, (size_t)mg.npart * 2 * sizeof(double)) != hipSuccess ||
         hipMalloc(&mg.st, sizeof(RefineState)) != hipSuccess) {
@@ -221,7 +483,10 @@ int mg_build(GpuMG& mg, int nx, int ny, int nz, float h, int max_levels, std::st
 }
 
 void mg_free(GpuMG& mg) {
-    for (int l = 0; l < mg.nlev; l++) { (void)hipFree(mg.p[l]); (void)hipFree(mg.b[l]); mg.p[l] = mg.b[l] = nullptr; }
+    for (int l = 0; l < mg.nlev; l++) {
+        (void)hipFree(mg.p[l]); (void)hipFree(mg.b[l]);
+        mg.p[l] = mg.b[l] = nullptr;
+    }
     if (mg.x64) (void)hipFree(mg.x64);
     if (mg.bkeep) (void)hipFree(mg.bkeep);
     if (mg.partials) (void)hipFree(mg.partials);
@@ -230,48 +495,131 @@ void mg_free(GpuMG& mg) {
     mg.nlev = 0;
 }
 
-void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip) {   // sim.js:576-586
+void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, double* xupd) {   // sim.js:576-586
     const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
-    const long half = (long)((nx + 1) / 2) * ny * nz;
+    const dim3 g = grid3((nx + 1) / 2, ny, nz);
     for (int k = 0; k < iters; k++)
-        for (int color = 0; color < 2; color++)
-            k_smooth<<<nblk(half), 256, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, skip);
+        for (int color = 0; color < 2; color++) {
+            double* xu = (xupd && k == iters - 1 && color == 1) ? xupd : nullptr;
+            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, skip, xu);
+        }
 }
 
-void mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip) {   // sim.js:588-599
+static void launch_restrict(GpuMG& mg, hipStream_t s, int l, const int* skip) {
+    const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
+    k_restrict<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], mg.b[l + 1], mg.p[l + 1], skip);
+}
+static void launch_prolong(GpuMG& mg, hipStream_t s, int l, const int* skip) {
+    const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
+    k_prolong<<<grid3(nx / 2, ny, nz), RB, 0, s>>>(mg.p[l], mg.p[l + 1], nx, ny, nz, skip);
+}
+
+static void launch_tail(GpuMG& mg, hipStream_t s, const int* skip) {
+    TailArgs a;
+    const int T = mg.tail_from;
+    a.nt = mg.nlev - T;
+    int off = 0;
+    for (int j = 0; j < a.nt; j++) {
+        for (int k = 0; k < 3; k++) a.dims[j][k] = mg.dims[T + j][k];
+        a.h[j] = mg.h[T + j];
+        a.dhx[j] = make_fastdiv((mg.dims[T + j][0] + 1) / 2);
+        a.dx[j] = make_fastdiv(mg.dims[T + j][0]);
+        a.dy[j] = make_fastdiv(mg.dims[T + j][1]);
+        a.dcx[j] = make_fastdiv(std::max(mg.dims[T + j][0] / 2, 1));
+        a.dcy[j] = make_fastdiv(std::max(mg.dims[T + j][1] / 2, 1));
+        a.off[j] = off;
+        off += 2 * mg.dims[T + j][0] * mg.dims[T + j][1] * mg.dims[T + j][2];
+    }
+    a.coarse_iters = mg.coarse_iters;
+    a.p_top = mg.p[T];
+    a.b_top = mg.b[T];
+    k_mg_tail<<<1, TAIL_THREADS, off * sizeof(float), s>>>(a, skip);
+}
+
+bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd) {   // sim.js:588-599
     const int n = mg.nlev;
-    for (int l = 0; l < n - 1; l++) {
-        mg_smooth(mg, s, l, 2, skip);
-        const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
-        const long nc = (long)(nx / 2) * (ny / 2) * (nz / 2);
-        k_restrict<<<nblk(nc), 256, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], mg.b[l + 1], mg.p[l + 1], skip);
+    const int T = mg.use_tail ? mg.tail_from : n;      // levels T.. run in the single-workgroup tail
+    if (T == 0) {                                       // the whole hierarchy fits one workgroup's LDS
+        launch_tail(mg, s, skip);
+        return false;
     }
-    mg_smooth(mg, s, n - 1, mg.coarse_iters, skip);
-    for (int l = n - 2; l >= 0; l--) {
-        const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
-        k_prolong<<<nblk((long)nx * ny * nz), 256, 0, s>>>(mg.p[l], mg.p[l + 1], nx, ny, nz, skip);
+    const int last = std::min(T, n - 1);                // levels 0..last-1 smooth and restrict
+    for (int l = 0; l < last; l++) {
         mg_smooth(mg, s, l, 2, skip);
+        launch_restrict(mg, s, l, skip);
     }
+    if (T < n) launch_tail(mg, s, skip);
+    else mg_smooth(mg, s, n - 1, mg.coarse_iters, skip);
+    for (int l = last - 1; l >= 0; l--) {
+        if (mg.fuse_prolong && l >= 2) {   // coarse levels: one launch less; fine levels: cheaper apart
+            // prolongation + colour 0, then colour 1, then one more full sweep (sim.js:468-495, 576-586)
+            const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
+            const dim3 g = grid3(nx / 2, ny, nz);
+            k_smooth_prolong<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], mg.p[l + 1], nx, ny, nz, mg.h[l], skip);
+            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], 1, skip, nullptr);
+            mg_smooth(mg, s, l, 1, skip, l == 0 ? xupd : nullptr);
+        } else {
+            launch_prolong(mg, s, l, skip);
+            mg_smooth(mg, s, l, 2, skip, l == 0 ? xupd : nullptr);
+        }
+    }
+    return xupd != nullptr && last > 0;
+}
+
+void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
+    const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
+    if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
+    k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
+                                                                  mg.partials, nullptr);
+    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
+}
+
+void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
+    const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
+    if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
+    int* done = &mg.st->done;
+    if (!mg_vcycle(mg, s, done, mg.x64)) {
+        const int n = (int)level_floats(mg.dims[0]);
+        k_update<<<(n + RB - 1) / RB, RB, 0, s>>>(mg.x64, mg.p[0], n, done);
+    }
+    k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
+                                                                  mg.partials, done);
+    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
 }
 
 void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
+    if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
+    mg_refine_begin(mg, s, tol, max_cycles);
+    for (int k = 1; k <= max_cycles; k++) mg_refine_iter(mg, s, tol, max_cycles);
+}
+
+void mg_tol32_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
-    const long n = (long)nx * ny * nz;
-    const double h = (double)mg.h[0];
+    if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
+    k_resid64<float><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr,
+                                                                 mg.partials, nullptr);
+    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
+}
+
+void mg_tol32_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
+    const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     int* done = &mg.st->done;
-    k_resid64<double><<<mg.npart, RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, h, mg.b[0], mg.p[0], mg.partials, nullptr);
+    mg_vcycle(mg, s, done);
+    k_resid64<float><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr,
+                                                                 mg.partials, done);
     k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
-    for (int k = 1; k <= max_cycles; k++) {
-        mg_vcycle(mg, s, done);
-        k_update<<<nblk(n), 256, 0, s>>>(mg.x64, mg.p[0], (int)n, done);
-        k_resid64<double><<<mg.npart, RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, h, mg.b[0], mg.p[0], mg.partials, done);
-        k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, k);
-    }
+}
+
+void mg_solve_tol32(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
+    if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
+    mg_tol32_begin(mg, s, tol, max_cycles);
+    for (int k = 1; k <= max_cycles; k++) mg_tol32_iter(mg, s, tol, max_cycles);
 }
 
 void mg_residual_of_p(GpuMG& mg, hipStream_t s) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
-    k_resid64<float><<<mg.npart, RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr, mg.partials, nullptr);
-    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, 0.0, 0, 0);
+    k_resid64<float><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr,
+                                                                 mg.partials, nullptr);
+    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, 0.0, 0, 1);
 }

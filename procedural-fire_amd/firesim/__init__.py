@@ -18,7 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libfiresim.so")
 
 PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
-SOLVE_VCYCLES, SOLVE_REFINE64 = 0, 1
+SOLVE_VCYCLES, SOLVE_REFINE64, SOLVE_TOL32 = 0, 1, 2
 (VEL, S, S2, VORT, GEO, VELX, SX, S2X, VELY, SY, S2Y, P, B, NOISE, CURL) = range(15)
 FIELD_NAMES = {"vel": VEL, "S": S, "S2": S2, "vort": VORT, "geo": GEO, "velX": VELX, "SX": SX, "S2X": S2X,
                "velY": VELY, "SY": SY, "S2Y": S2Y, "p": P, "b": B, "noise": NOISE, "curl": CURL}
@@ -47,7 +47,8 @@ class FsConfig(ctypes.Structure):
         ("device", ctypes.c_int),
         ("stream", ctypes.c_void_p),
         ("use_graph", ctypes.c_int),
-        ("reserved", ctypes.c_int * 7),
+        ("adaptive", ctypes.c_int),
+        ("reserved", ctypes.c_int * 6),
     ]
 
 
@@ -139,7 +140,7 @@ class FireSim:
 
     def __init__(self, res=(80, 120, 80), size=(1.2, 1.8, 1.2), logs=None, vcycles=2, max_levels=0,
                  precision=PREC_FP16, solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128,
-                 noise_seed=1, device=0, stream=None, use_graph=False):
+                 noise_seed=1, device=0, stream=None, use_graph=False, adaptive=True):
         L = lib()
         cfg = FsConfig()
         L.fs_config_default(ctypes.byref(cfg))
@@ -156,6 +157,7 @@ class FireSim:
         cfg.device = int(device)
         cfg.stream = stream
         cfg.use_graph = 1 if use_graph else 0
+        cfg.adaptive = 1 if adaptive else 0
         if logs is None:
             logs, _ = build_campfire(7)   # src/main.js:34-35: buildCampfire(7) packed to 16 slots
         logs = np.ascontiguousarray(logs, np.float32)

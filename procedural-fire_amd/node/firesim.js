@@ -1,0 +1,155 @@
+'use strict';
+// Node.js host API of the gfx950 fire simulation: a drop-in for the reference's WebGPU
+// `FireSim` (src/sim.js:48-649) and the sim-facing part of `window.FIRE` (src/main.js:242-355),
+// over the C ABI (include/firesim.h) through the N-API addon lib/firesim.node.
+//
+// Differences from the browser class, by construction of the boundary:
+//   * no GPUDevice / GPUCommandEncoder arguments: the handle owns its HIP stream;
+//     step(dt) enqueues (like encoding + submit), sync() waits (onSubmittedWorkDone);
+//   * the noise/curl textures are baked by the handle from `noiseSeed` (src/main.js:36-37);
+//   * textures are read back as typed arrays in the same texel order (x fastest, RGBA).
+// Written for Node >= 12 (no optional chaining).
+const path = require('path');
+
+const addon = require(path.join(__dirname, '..', 'lib', 'firesim.node'));
+
+const SIM_FIELDS = addon.paramNames();                       // src/sim.js:14-23
+const GRID_KEYS = new Set(['nx', 'ny', 'nz', 'dx', 'dt', 'time', 'frame', 'ox', 'oy', 'oz', 'nLogs']);
+const SIM_DEFAULTS = (() => {                                 // src/sim.js:25-44
+  const d = addon.paramDefaults();
+  const o = {};
+  SIM_FIELDS.forEach((k, i) => { if (!GRID_KEYS.has(k) && k !== 'seed') o[k] = d[i]; });
+  return o;
+})();
+
+const PRECISION = { fp32: 0, fp16: 1, bf16: 2 };
+const SOLVE = { vcycles: 0, refine64: 1, tol32: 2 };
+const FIELD = { vel: 0, S: 1, S2: 2, vort: 3, geo: 4, velX: 5, SX: 6, S2X: 7, velY: 8, SY: 9, S2Y: 10,
+  p: 11, b: 12, noise: 13, curl: 14 };
+const STAGE_LABELS = ['adv1', 'adv2', 'curl', 'forces', 'div', 'mg', 'project'];
+
+// buildCampfire(seed).logs packed by packLogs (src/scene.js:21-42, 92-98): 16 records of 8 floats.
+function buildCampfireLogs(seed = 7, max = 16) { return addon.buildCampfire(seed, max); }
+
+function fieldId(name) {
+  if (typeof name === 'number') return name;
+  if (!(name in FIELD)) throw new Error(`unknown field ${name}`);
+  return FIELD[name];
+}
+
+class FireSim {
+  constructor({ res = [80, 120, 80], size = [1.2, 1.8, 1.2], logs, vcycles = 2, maxLevels = 0,
+    precision = 'fp16', solve = 'vcycles', divTol = 1e-5, maxVcycles = 8, noiseN = 128, noiseSeed = 1,
+    device = 0, useGraph = true, adaptive = true } = {}) {
+    if (!logs) logs = buildCampfireLogs(7);
+    this.h = addon.create({ res, size, logs, vcycles, maxLevels, precision: PRECISION[precision], solve: SOLVE[solve],
+      divTol, maxVcycles, noiseN, noiseSeed, device, useGraph: useGraph ? 1 : 0, adaptive: adaptive ? 1 : 0 });
+    const info = addon.info(this.h);
+    this.res = info.res;
+    this.size = size;
+    this.dx = info.dx;                                         // src/sim.js:53
+    this.origin = info.origin;                                 // src/sim.js:54
+    this.vcycles = vcycles;
+    this.levels = info.levels.map(dims => ({ dims }));
+    this.coarseIters = info.coarseIters;                       // src/sim.js:88-92
+    this.profileStages = false;
+  }
+
+  get time() { return addon.info(this.h).time; }
+  get frame() { return addon.info(this.h).frame; }
+
+  // the current parameter block as an object (src/sim.js:59)
+  get params() {
+    const v = addon.getParams(this.h);
+    const o = {};
+    SIM_FIELDS.forEach((k, i) => { o[k] = v[i]; });
+    return o;
+  }
+
+  // setParams(p) (src/sim.js:601-604): unknown keys throw and change nothing
+  setParams(p) {
+    const names = Object.keys(p);
+    addon.setParams(this.h, names, names.map(k => Number(p[k])));
+  }
+
+  // the log buffer rewrite + geoReady = false of applyLogs (src/main.js:147-149)
+  setLogs(logs) { addon.setLogs(this.h, logs); }
+
+  reset() { addon.reset(this.h); }                             // src/sim.js:606-618
+
+  // step(dt): enqueue one step (src/sim.js:627-648); n > 1 enqueues n steps
+  step(dt, n = 1) {
+    if (this.profileStages) addon.profile(this.h, true);
+    addon.step(this.h, dt, n);
+  }
+
+  sync() { addon.sync(this.h); }
+
+  // FIRE.step(n, dt) (src/main.js:244): n steps, resolved once the device has finished
+  stepAsync(n = 1, dt = 1 / 90) { return addon.stepAsync(this.h, dt, n); }
+
+  read(name, native = false) { return addon.readField(this.h, fieldId(name), native); }
+  write(name, arr) { addon.writeField(this.h, fieldId(name), arr); }
+
+  residual() { return addon.residual(this.h); }
+
+  // per-kernel device ms of the last profiled step (Profiler labels, src/sim.js:629-644)
+  timings() {
+    const t = addon.timings(this.h);
+    const o = {};
+    STAGE_LABELS.forEach((k, i) => { o[k] = t[i]; });
+    o.adv = o.adv1 + o.adv2;
+    return o;
+  }
+
+  // FIRE.stats(name) (src/main.js:268-294): per-channel min/max/mean/bad/argmax
+  stats(name = 'S') {
+    const [x, y, z] = this.res;
+    const a = this.read(name);
+    const out = [0, 1, 2, 3].map(() => ({ min: Infinity, max: -Infinity, sum: 0, bad: 0, argmax: null }));
+    for (let k = 0; k < z; k++) for (let j = 0; j < y; j++) for (let i = 0; i < x; i++) {
+      const o = ((k * y + j) * x + i) * 4;
+      for (let c = 0; c < 4; c++) {
+        const v = a[o + c];
+        const S = out[c];
+        if (!Number.isFinite(v)) { S.bad++; continue; }
+        if (v < S.min) S.min = v;
+        if (v > S.max) { S.max = v; S.argmax = [i, j, k]; }
+        S.sum += v;
+      }
+    }
+    const n = x * y * z;
+    return out.map(S => ({ min: +S.min.toPrecision(4), max: +S.max.toPrecision(4), mean: +(S.sum / n).toPrecision(4),
+      bad: S.bad, argmax: S.argmax }));
+  }
+
+  // FIRE.profile() (src/main.js:322-355): heat release (kW) and centreline/height rows
+  profile() {
+    const [x, y, z] = this.res;
+    const a = this.read('S');
+    const dx = this.dx, V = dx * dx * dx;
+    const rows = [];
+    for (let j = 0; j < y; j += 4) {
+      let Tmax = 0, Tc = 0, sootMax = 0, fuelMax = 0, hot = 0, nC = 0;
+      for (let k = 0; k < z; k++) for (let i = 0; i < x; i++) {
+        const o = ((k * y + j) * x + i) * 4;
+        const T = 293 + 1000 * Math.abs(a[o]);
+        Tmax = Math.max(Tmax, T); sootMax = Math.max(sootMax, Math.abs(a[o + 2])); fuelMax = Math.max(fuelMax, Math.abs(a[o + 1]));
+        if (T > 1000) hot++;
+        if (Math.abs(i - x / 2) < 3 && Math.abs(k - z / 2) < 3) { Tc += T; nC++; }
+      }
+      rows.push({ h: +((j + 0.5) * dx).toFixed(2), Tc: Math.round(Tc / nC), Tmax: Math.round(Tmax), soot: +sootMax.toFixed(2),
+        fuel: +fuelMax.toFixed(2), hotCells: hot });
+    }
+    let Q = 0;
+    for (let c = 0; c < x * y * z; c++) {
+      const q = Math.abs(a[c * 4 + 3]) * 1e4;
+      if (q <= 0) continue;
+      const T = 293 + 1000 * Math.abs(a[c * 4]);
+      Q += 354800 * q / T * V;
+    }
+    return { Q_kW: +(Q / 1000).toFixed(1), rows };
+  }
+}
+
+module.exports = { FireSim, SIM_FIELDS, SIM_DEFAULTS, PRECISION, SOLVE, FIELD, STAGE_LABELS, buildCampfireLogs, addon };

@@ -149,3 +149,35 @@ def test_deterministic_rerun():
     sims[0].step(DT, 20)
     for f in (F.VEL, F.S, F.S2):
         assert np.array_equal(sims[0].read(f, "native"), sims[1].read(f, "native"))
+
+
+def test_tol32_solve_parity():
+    """FP32 V-cycles until the FP64 residual reaches the tolerance: same cycle counts and bits as the oracle."""
+    print("tol32", flush=True)
+    gpu, orc = make_pair(F.PREC_FP32, warm_steps=20, solve=F.SOLVE_TOL32, div_tol=1e-4, max_vcycles=8)
+    for _ in range(3):
+        gpu.step(DT, 1)
+        orc.step(DT, 1)
+        rel, used = gpu.residual()
+        orel, oused = orc.residual()
+        print(f"  gpu rel {rel:.3e} used {used}; oracle rel {orel:.3e} used {oused}", flush=True)
+        assert used == oused and abs(rel - orel) <= 1e-9 * max(orel, 1e-30) + 1e-15
+    for f in (F.VEL, F.S, F.S2, F.P):
+        assert rel_l2(gpu.read(f, np.float64), orc.read(f)) == 0.0, f
+
+
+@pytest.mark.parametrize("solve", [F.SOLVE_REFINE64, F.SOLVE_TOL32], ids=["refine64", "tol32"])
+def test_adaptive_and_async_solves_agree(solve):
+    """Host-adaptive iteration counts (one sync per step) and the fully asynchronous form
+    (max_vcycles iterations, early exit on the device flag) give identical bits, graph or not."""
+    print(f"adaptive vs async solve {solve}", flush=True)
+    sims = [F.FireSim(res=TEST_RES, size=TEST_SIZE, noise_n=32, precision=F.PREC_FP16, solve=solve, div_tol=1e-5,
+                      max_vcycles=10, use_graph=g, adaptive=a) for a, g in ((True, True), (False, True), (True, False))]
+    for s in sims:
+        s.step(DT, 12)
+    r = [s.residual() for s in sims]
+    assert r[0] == r[1] == r[2], r
+    for f in (F.VEL, F.S, F.S2, F.P):
+        a = sims[0].read(f, "native")
+        for s in sims[1:]:
+            assert np.array_equal(a, s.read(f, "native")), f

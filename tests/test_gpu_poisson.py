@@ -1,0 +1,58 @@
+"""Isolated pressure Poisson solve on the GPU (fs_poisson_*): the sim.js multigrid against the
+oracle (bit-exact V-cycles), the FP64-refined solve against the closed-form DST/DCT solution,
+and odd grid sizes that exercise the parity-split layout's row padding."""
+import numpy as np
+import pytest
+
+import firesim as F
+import fsoracle as O
+from test_oracle_pins import _spectral_solve
+
+pytestmark = pytest.mark.gpu
+
+
+def _rhs(dims, seed=0):
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    b = np.sin(0.3 * x + 0.2) * np.cos(0.21 * y) * np.sin(0.17 * z + 1.0) + 0.05 * rng.standard_normal((nz, ny, nx))
+    return b
+
+
+@pytest.mark.parametrize("dims", [(32, 48, 32), (64, 96, 64), (20, 30, 22), (21, 30, 22), (12, 18, 10)])
+def test_vcycles_bit_exact_vs_oracle(dims):
+    print(f"poisson vcycles {dims}", flush=True)
+    h = 1.0 / 128          # exactly representable: the device keeps the spacing as f32
+    b = _rhs(dims)
+    ps = F.Poisson(dims, h)
+    ps.set_rhs(b)
+    ps.set_x(np.zeros_like(b))
+    hist = ps.vcycles(4)
+    orc = O.OracleMG(dims, h, fp64=False)
+    orc.set(b=b, p=np.zeros_like(b))
+    ohist = orc.vcycles(4)
+    assert ps.info()[0] == orc.levels()[0]
+    assert np.array_equal(ps.read_x(fp64=False), orc.get().astype(np.float32))
+    assert np.allclose(hist, ohist, rtol=1e-9)
+
+
+@pytest.mark.parametrize("dims", [(32, 48, 32), (64, 96, 64), (24, 36, 20)])
+def test_refined_solve_reaches_closed_form(dims):
+    print(f"poisson refine {dims}", flush=True)
+    h = 1.0 / 256
+    b = _rhs(dims, 1)
+    b -= b.mean()
+    exact = _spectral_solve(b, h)
+    ps = F.Poisson(dims, h)
+    ps.set_rhs(b)
+    ps.set_x(np.zeros_like(b))
+    hist = ps.solve(1e-10, 40)
+    assert hist[-1] <= 1e-10 and len(hist) < 40
+    x = ps.read_x()
+    assert np.max(np.abs(x - exact)) <= 1e-8 * np.max(np.abs(exact))
+    # same cycle count and iterate as the oracle's FP32-in-FP64 refinement
+    orc = O.OracleMG(dims, h, fp64=False)
+    orc.set(b=b, p=np.zeros_like(b))
+    ohist = orc.refine(1e-10, 40)
+    assert len(ohist) == len(hist) - 1
+    assert np.max(np.abs(x - orc.get_x())) <= 1e-12 * np.max(np.abs(exact))

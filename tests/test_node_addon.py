@@ -1,0 +1,70 @@
+"""The N-API addon (procedural-fire_amd/lib/firesim.node) and the Node FireSim class over it."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import firesim as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "procedural-fire_amd", "lib", "firesim.node")
+JS = os.path.join(ROOT, "procedural-fire_amd", "node", "firesim.js")
+NODE = shutil.which("node")
+
+needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON),
+                                reason="node or the built addon (make -C procedural-fire_amd/node) is missing")
+
+
+def _node(code):
+    return subprocess.run([NODE, "-e", code], capture_output=True, text=True, timeout=60, cwd=ROOT)
+
+
+@needs_node
+def test_addon_loads_and_mirrors_sim_fields():
+    r = _node(f"const m = require({JS!r}); console.log(JSON.stringify({{fields: m.SIM_FIELDS, "
+              f"defaults: m.SIM_DEFAULTS, logs: Array.from(m.buildCampfireLogs(7))}}))")
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    assert d["fields"] == F.param_names()
+    assert abs(d["defaults"]["buoyancy"] - 26.8) < 1e-5 and d["defaults"]["vortEps"] == 8
+    logs, _ = F.build_campfire(7)
+    assert np.array_equal(np.array(d["logs"], np.float32), logs)
+
+
+@needs_node
+def test_addon_reports_errors_as_js_exceptions():
+    r = _node(f"const m = require({JS!r}); try {{ new m.FireSim({{noiseN: 100}}); console.log('no'); }} "
+              f"catch (e) {{ console.log('threw:' + e.message); }}")
+    assert r.returncode == 0, r.stderr
+    assert "threw:" in r.stdout and "power of two" in r.stdout
+
+
+@pytest.mark.gpu
+@needs_node
+def test_node_steps_match_the_c_abi_bit_for_bit(tmp_path):
+    print("node run_steps 32x48x32 x10", flush=True)
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "run_steps.js"), str(tmp_path), "10"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    meta = json.loads((tmp_path / "meta.json").read_text())
+    assert meta["frame"] == 10 and meta["threwOnUnknown"]
+    sim = F.FireSim(res=(32, 48, 32), size=(1.2, 1.8, 1.2), noise_n=32, use_graph=True)
+    sim.set_params(vortEps=4, intensity=1.2)
+    sim.step(1 / 90, 10)
+    for f, fid in (("vel", F.VEL), ("S", F.S), ("S2", F.S2)):
+        a = np.fromfile(tmp_path / f"{f}.bin", np.uint16)
+        assert np.array_equal(a, sim.read(fid, "native").ravel()), f
+    # FIRE.stats semantics (src/main.js:268-294) against numpy on the same field
+    S = sim.read(F.S)
+    for c in range(4):
+        st = meta["stats"][c]
+        assert abs(st["max"] - float(f"{S[..., c].max():.4g}")) <= 1e-12
+        assert abs(st["mean"] - float(f"{S[..., c].mean():.4g}")) <= 1e-9 + 1e-3 * abs(st["mean"])
+    # FIRE.profile heat release (src/main.js:347-352)
+    q = S[..., 3].astype(np.float64) * 1e4
+    T = 293 + 1000 * S[..., 0].astype(np.float64)
+    Q = np.sum(np.where(q > 0, 354800 * q / T, 0.0)) * sim.dx ** 3 / 1000
+    assert abs(meta["profile"]["Q_kW"] - round(Q, 1)) <= 0.11

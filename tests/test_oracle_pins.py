@@ -107,7 +107,7 @@ def test_oracle_multigrid_converges_to_closed_form(dims):
     """FP64 sim.js multigrid (smooth/restrict/prolong/coarse) converges to the exact discrete solution."""
     rng = np.random.default_rng(1)
     nx, ny, nz = dims
-    h = 0.01
+    h = 1.0 / 128          # a binary fraction: the multigrid keeps its spacing as f32 (src/sim.js:96)
     b = rng.standard_normal((nz, ny, nx))
     exact = _spectral_solve(b, h)
     mg = O.OracleMG(dims, h, fp64=True)
@@ -142,7 +142,7 @@ def test_oracle_multigrid_fp32_rates_match_reference_range():
 def test_oracle_multigrid_matches_dense_solve(dims):
     nx, ny, nz = dims
     n = nx * ny * nz
-    h = 0.02
+    h = 1.0 / 64
     A = np.zeros((n, n))
     for i in range(n):
         e = np.zeros(n)
