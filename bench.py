@@ -55,6 +55,20 @@ def mg_bytes_per_cycle(levels, coarse_iters):
 REFINE_BYTES = 40.0   # per fine cell per refinement iteration: residual (x8 b4 r4 p4) + update (x8 e4 x8)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by the
+    counters' reading for a plain copy of known size), recorded in profiles/pmc_traffic.json."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return {"bytes": k["fetch_bytes"] + k["write_bytes"], "source": "profiles/pmc_traffic.json"}
+
+
 def make_sim(precision, solve, use_graph=True):
     logs, _ = F.build_campfire(7)
     s = F.FireSim(res=RES, size=SIZE, logs=logs, vcycles=2, precision=precision, solve=solve, div_tol=1e-5,
@@ -112,6 +126,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for hardware-counter passes: a 256 MiB copy, then 3 steps of each step form")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the step runs on a single MI355X (no multi-GPU path is built); "
@@ -120,6 +136,17 @@ def main():
 
     ncell = float(np.prod(RES))
     graph = not args.no_graph
+    if args.probe:
+        ms = F.bench_copy(256 << 20, 3)
+        print(f"copy probe: 256 MiB in {ms:.4f} ms -> {2 * (256 << 20) / ms / 1e6:.0f} GB/s (read+write)", flush=True)
+        for name, prec, solve in (("fp32", F.PREC_FP32, F.SOLVE_VCYCLES), ("mixed", F.PREC_FP16, F.SOLVE_REFINE64)):
+            sim = make_sim(prec, solve, False)
+            sim.step(DT, 3)
+            sim.sync()
+            print(f"{name}: 3 steps done", flush=True)
+            sim.close()
+        return
+    stream_ms = F.bench_copy(1 << 30, 10)
     runs = {}
     # fp32: the reference's solve (2 x V(2,2)) on float4 fields; fp32_tol: plain FP32 V-cycles until the
     # residual reaches the same 1e-5 as the mixed step (or 8 cycles); mixed: the headline
@@ -187,7 +214,12 @@ def main():
         "kernel_ms_mixed": profm,
         "kernel_ms_fp32": prof32,
         "kernel_ms_fp32_tol": runs["fp32_tol"]["prof"],
+        "hbm_stream_GBs": 2.0 * (1 << 30) / (stream_ms * 1e-3) / 1e9,
     }
+    traffic = measured_traffic(dom)
+    if traffic:
+        out["roofline"]["traffic"] = traffic["bytes"]
+        out["roofline"]["traffic_source"] = traffic["source"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mix)
         out["vs_cpu"] = vmix / out["cpu_baseline"]["value"]

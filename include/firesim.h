@@ -178,6 +178,11 @@ int  fs_timings(fs_sim* h, float ms[FS_NUM_TIMINGS]);
  * writes max_logs*8 floats (zero-padded) and returns the number of real logs. */
 int  fs_build_campfire(int seed, float* out, int max_logs);
 
+/* Measurement utility with no counterpart in the reference: times `iters`
+ * device-to-device copies of `bytes` with a plain streaming kernel (the box's
+ * achievable HBM bandwidth, and the calibration of bandwidth counters). */
+int  fs_bench_copy(int device, size_t bytes, int iters, float* ms_per_copy);
+
 /* ---- isolated pressure Poisson solve (cell-centred 7-point operator,
  *      Dirichlet p=0 on the x/z faces and the top, Neumann floor) ------------- */
 typedef struct fs_poisson fs_poisson;

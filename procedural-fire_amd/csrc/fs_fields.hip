@@ -662,6 +662,15 @@ __global__ __launch_bounds__(256) void k_convert(int kind, const void* __restric
     else ((double*)dst)[i] = (double)((const float*)src)[i];
 }
 
+// plain streaming copy (bandwidth probe and counter calibration)
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+void launch_copy_probe(hipStream_t s, const void* src, void* dst, size_t bytes) {
+    const size_t n = bytes / 16;
+    k_copy_probe<<<2048, 256, 0, s>>>((const float4*)src, (float4*)dst, n);
+}
+
 // ---- host launchers ---------------------------------------------------------------
 #define DISPATCH_PREC(prec, BODY)                                  \
     do {                                                           \

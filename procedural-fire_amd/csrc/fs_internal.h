@@ -142,3 +142,4 @@ void launch_noise_bake(hipStream_t s, half4_t* noise, half4_t* curl, int n, int 
 void launch_set_params(hipStream_t s, DevParams* dst, const DevParams& src);
 void launch_fill_zero(hipStream_t s, void* ptr, size_t bytes);
 void launch_convert(hipStream_t s, int kind, const void* src, void* dst, size_t n);
+void launch_copy_probe(hipStream_t s, const void* src, void* dst, size_t bytes);

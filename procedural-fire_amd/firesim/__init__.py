@@ -94,6 +94,7 @@ def lib():
         "fs_profile_enable": (ip, [vp, ip]),
         "fs_timings": (ip, [vp, fp]),
         "fs_build_campfire": (ip, [ip, fp, ip]),
+        "fs_bench_copy": (ip, [ip, ctypes.c_size_t, ip, fp]),
         "fs_poisson_create": (ip, [ctypes.POINTER(ctypes.c_int), ctypes.c_double, ip, ip, vp, ctypes.POINTER(vp)]),
         "fs_poisson_destroy": (ip, [vp]),
         "fs_poisson_set_rhs": (ip, [vp, ip, vp]),
@@ -133,6 +134,13 @@ def build_campfire(seed=7, max_logs=16):
     if n < 0:
         _check(n)
     return out, n
+
+
+def bench_copy(nbytes=256 << 20, iters=10, device=0):
+    """ms per device-to-device copy of `nbytes` with a plain streaming kernel."""
+    ms = ctypes.c_float()
+    _check(lib().fs_bench_copy(int(device), int(nbytes), int(iters), ctypes.byref(ms)))
+    return ms.value
 
 
 class FireSim:
