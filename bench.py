@@ -2,7 +2,9 @@
 """Throughput of the fire-simulation step on one MI355X.
 
 Workload: 128 x 192 x 128 cells at dx = 5 mm (0.64 x 0.96 x 0.64 m), dt = 1 ms,
-vorticity confinement eps 0.3, the seed-7 campfire emitter, noise 128^3 seed 1.
+vorticity confinement eps 0.3, fed by 8 pyrolysis spots of U(1.5, 3) g/s drawn with seed 7
+on a 0.3 m disc at 0.1 m height (scene "spots8_128"), noise 128^3 seed 1.  `--scene campfire`
+feeds the same grid from the reference's seed-7 campfire logs instead.
 Two forms of the same step run back to back from the same warmed state:
   * plain FP32:  float4 fields, 2 x V(2,2) FP32 V-cycles per step (the reference's solve);
   * mixed:       rgba16float fields (the reference's storage), FP32 math, FP32 V-cycles
@@ -27,6 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "procedural-fire_amd"))
 import numpy as np  # noqa: E402
 
 import firesim as F  # noqa: E402
+from firesim import scenes  # noqa: E402
 
 RES = (128, 192, 128)
 DX = 0.005
@@ -69,12 +72,18 @@ def measured_traffic(kernel):
     return {"bytes": k["fetch_bytes"] + k["write_bytes"], "source": "profiles/pmc_traffic.json"}
 
 
+SCENE = "spots8_128"
+
+
 def make_sim(precision, solve, use_graph=True):
-    logs, _ = F.build_campfire(7)
-    s = F.FireSim(res=RES, size=SIZE, logs=logs, vcycles=2, precision=precision, solve=solve, div_tol=1e-5,
-                  max_vcycles=8, noise_n=128, use_graph=use_graph)
-    s.set_params(PARAMS)
-    return s
+    if SCENE == "campfire":
+        logs, _ = F.build_campfire(7)
+        s = F.FireSim(res=RES, size=SIZE, logs=logs, vcycles=2, precision=precision, solve=solve, div_tol=1e-5,
+                      max_vcycles=8, noise_n=128, use_graph=use_graph)
+        s.set_params(PARAMS)
+        return s
+    return scenes.make_scene(SCENE, seed=7, precision=precision, solve=solve, div_tol=1e-5, max_vcycles=8,
+                             noise_n=128, use_graph=use_graph)
 
 
 def time_steps(sim, steps):
@@ -100,9 +109,15 @@ def profile_kernels(sim, steps=5):
 def cpu_baseline(gpu_sim, steps=3):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import fsoracle as O
-    logs, _ = F.build_campfire(7)
-    orc = O.OracleSim(res=RES, size=SIZE, logs=logs, n_logs=16, precision=O.FP16, vcycles=2, noise_n=128)
-    orc.set_params(**PARAMS)
+    if SCENE == "campfire":
+        logs, _ = F.build_campfire(7)
+        orc = O.OracleSim(res=RES, size=SIZE, logs=logs, n_logs=16, precision=O.FP16, vcycles=2, noise_n=128)
+        orc.set_params(**PARAMS)
+    else:
+        orc = O.OracleSim(res=RES, size=scenes.scene_size(SCENE), logs=np.zeros(8, np.float32), n_logs=0,
+                          precision=O.FP16, vcycles=2, noise_n=128)
+        orc.set_params(**scenes.scene_params(SCENE))
+        orc.set_spots(scenes.scene_spots(SCENE, 7))
     for f in (F.VEL, F.S, F.S2, F.VORT, F.GEO, F.VELX, F.SX, F.S2X, F.VELY, F.SY, F.S2Y, F.NOISE, F.CURL):
         orc.write(f, gpu_sim.read(f, np.float64))
     orc.write(O.P, gpu_sim.read(F.P, np.float64))
@@ -126,9 +141,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--scene", default="spots8_128", choices=["spots8_128", "campfire"])
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB copy, then 3 steps of each step form")
     args = ap.parse_args()
+    global SCENE
+    SCENE = args.scene
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the step runs on a single MI355X (no multi-GPU path is built); "
               "use --gpus 1", file=sys.stderr)
@@ -194,8 +212,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f16 storage / f32 math / f64 refinement",
-        "data": "synthetic (seed-7 campfire emitter from rest, noise seed 1)",
-        "config": {"workload": "fire step 128x192x128 dx=5mm dt=1ms vortEps=0.3", "grid": list(RES),
+        "data": ("synthetic (8 seeded pyrolysis spots, 1.5-3 g/s, from rest; noise seed 1)" if SCENE != "campfire"
+                 else "synthetic (seed-7 campfire emitter from rest, noise seed 1)"),
+        "config": {"workload": "fire step 128x192x128 dx=5mm dt=1ms vortEps=0.3 "
+                               + ("8 spots (seed 7)" if SCENE != "campfire" else "campfire logs (seed 7)"),
+                   "grid": list(RES),
                    "solve_mixed": "FP32 V(2,2) in FP64 defect correction, tol 1e-5",
                    "solve_fp32": "2 x V(2,2) FP32", "solve_fp32_tol": "FP32 V(2,2) until 1e-5 (cap 8)",
                    "graph": graph},

@@ -135,6 +135,32 @@ int  fs_get_params(fs_sim* h, float out[FS_NUM_PARAMS]);
 /* Rewrites the log buffer and invalidates the geometry (src/main.js:147-149). */
 int  fs_set_logs(fs_sim* h, const float* logs, int n_logs);
 
+/* ---- pyrolysis spots and floor fuel flux (build extension; the reference's
+ *      sources are the log bands of src/sim.js:246-297 and have no mass flux) ----
+ * A spot is a sphere of `radius` m at `pos` (m, sim coordinates) that adds fuel
+ * mass at `flux` kg/s: each step every cell within the radius gains
+ *   dY = flux * dt * w / (rho0 * sum_cells(w) * dx^3),  w = 1 - smoothstep(r/2, r, |p - pos|),
+ * rho0 = 1.204 kg/m^3, so sum(dY) * rho0 * dx^3 = flux * dt exactly per step.  Inside the
+ * spot the gas temperature is held at least at `temp` (theta, like srcTemp of a hot
+ * patch, src/sim.js:268), the fuel is tagged as wood gas and its age restarts, and the
+ * spot counts as a source for the turbulence/confinement gates (srcAll).  The floor
+ * flux field adds face_flux[z][x] * dt / (rho0 * dx) (kg/(m^2 s)) to the fuel of the
+ * y = 0 cells.  Both act inside the forces kernel, after the reference's sources. */
+#define FS_MAX_SPOTS 64
+#define FS_RHO0 1.204f
+typedef struct fs_spot {
+    float pos[3];
+    float radius;
+    float flux;     /* kg/s */
+    float temp;     /* pilot theta = (T - 293 K) / 1000; 0 = no pilot */
+    float norm;     /* filled by fs_set_spots: 1 / (rho0 * sum_cells(w) * dx^3) */
+    float pad;
+} fs_spot;
+int  fs_set_spots(fs_sim* h, const fs_spot* spots, int n);                 /* n <= FS_MAX_SPOTS */
+int  fs_get_spots(fs_sim* h, fs_spot* out, int max, int* n);              /* with `norm` filled */
+/* floor fuel flux, nx*nz values (x fastest); NULL clears it */
+int  fs_set_face_flux(fs_sim* h, const float* flux_xz);
+
 /* Replaces reset() (src/sim.js:606-618): zero the 10 dynamic fields and every
  * level's p; b and geo keep their contents; time = 0, frame = 0. */
 int  fs_reset(fs_sim* h);

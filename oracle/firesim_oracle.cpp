@@ -382,6 +382,8 @@ struct OracleBase {
     virtual int mg_info(int* dims) const = 0;
     virtual int coarse_iters() const = 0;
     virtual double residual(int* used) const = 0;
+    virtual int set_spots(const fs_spot* sp, int n, fs_spot* out) = 0;
+    virtual void set_face(const float* f) = 0;
 };
 
 template <class R, class S, class M>
@@ -401,6 +403,8 @@ struct Oracle : OracleBase {
     NoiseTex noise, curl;
     Multigrid<M> mg;
     std::vector<double> x64;   // FP64 pressure iterate (FS_SOLVE_REFINE64)
+    std::vector<fs_spot> spots;          // pyrolysis spots (build extension)
+    std::vector<float> face;             // floor fuel flux, nx*nz (empty = none)
     double last_rel = -1; int last_used = 0;
 
     Oracle(const fs_config* c, const float* lg, int nl) {
@@ -638,7 +642,7 @@ struct Oracle : OracleBase {
                     s.y = wmax(s.y, bed * u(P_bedFuel));
                     s.x = wmax(s.x, src * u(P_srcTemp));
                     s.x = wmax(s.x, bedHot * u(P_bedTemp));
-                    R srcAll = wmax(srcW, bed);
+                    R srcAll = wmax(srcW, bed);   // raised by the spots below
                     s2.w = mix(s2.w, R(1), bed * (R(1) - srcW));
                     s2.w = mix(s2.w, R(0), srcW);
                     s2.y = s2.y * (R(1) - srcAll);
@@ -657,6 +661,20 @@ struct Oracle : OracleBase {
                         s2.w *= R(1) - bw;
                         s2.y = wmax(s2.y, R(0.05) * bw * wclamp<R>(s.y + s2.x, R(0), R(1)));
                     }
+                    // pyrolysis spots and floor fuel flux (build extension, mass-flux sources)
+                    for (const fs_spot& sp : spots) {
+                        V3<R> dp = p - V3<R>((R)sp.pos[0], (R)sp.pos[1], (R)sp.pos[2]);
+                        R d = length(dp);
+                        if (d < (R)sp.radius) {
+                            R w = R(1) - smoothstep<R>(R(0.5) * (R)sp.radius, (R)sp.radius, d);
+                            s.y = wmin(s.y + (R)sp.flux * dt * (R)sp.norm * w, R(1));
+                            s.x = wmax(s.x, w * (R)sp.temp);
+                            s2.w = mix(s2.w, R(0), w);
+                            s2.y = s2.y * (R(1) - w);
+                            srcAll = wmax(srcAll, w);
+                        }
+                    }
+                    if (!face.empty() && y == 0) s.y = wmin(s.y + (R)face[(size_t)z * nx + x] * dt / ((R)FS_RHO0 * DX), R(1));
                     R ageF = s2.y / wmax(s.y + s2.x, R(1e-4));
                     // mixing-limited combustion (sim.js:300-309)
                     R T0 = R(293) + R(1000) * s.x;
@@ -884,6 +902,42 @@ struct Oracle : OracleBase {
         return (int)mg.L.size();
     }
     int coarse_iters() const override { return mg.coarse_iters; }
+    // normalisation of a spot's mass injection (double; cell centres origin + (i + 1/2) dx)
+    double spot_weight_sum(const fs_spot& sp) const {
+        const double d_x = (double)(float)dx;
+        const double o_x = (double)(float)ox, o_y = (double)(float)oy, o_z = (double)(float)oz;
+        const double r = sp.radius;
+        auto cell = [&](double o, double pc) { return (int)std::floor((pc - o) / d_x - 0.5); };
+        const int x0 = std::max(0, cell(o_x, sp.pos[0] - r) - 1), x1 = std::min(nx - 1, cell(o_x, sp.pos[0] + r) + 2);
+        const int y0 = std::max(0, cell(o_y, sp.pos[1] - r) - 1), y1 = std::min(ny - 1, cell(o_y, sp.pos[1] + r) + 2);
+        const int z0 = std::max(0, cell(o_z, sp.pos[2] - r) - 1), z1 = std::min(nz - 1, cell(o_z, sp.pos[2] + r) + 2);
+        double sum = 0.0;
+        for (int z = z0; z <= z1; z++)
+            for (int y = y0; y <= y1; y++)
+                for (int x = x0; x <= x1; x++) {
+                    const double px = o_x + (x + 0.5) * d_x, py = o_y + (y + 0.5) * d_x, pz = o_z + (z + 0.5) * d_x;
+                    const double d = std::sqrt((px - sp.pos[0]) * (px - sp.pos[0]) + (py - sp.pos[1]) * (py - sp.pos[1]) +
+                                               (pz - sp.pos[2]) * (pz - sp.pos[2]));
+                    if (d >= r) continue;
+                    double t = (d - 0.5 * r) / (0.5 * r);
+                    t = std::min(std::max(t, 0.0), 1.0);
+                    sum += 1.0 - t * t * (3.0 - 2.0 * t);
+                }
+        return sum;
+    }
+    int set_spots(const fs_spot* sp, int n, fs_spot* out) override {
+        spots.assign(sp, sp + n);
+        const double dx3 = std::pow((double)(float)dx, 3.0);
+        for (auto& q : spots) {
+            const double ws = spot_weight_sum(q);
+            q.norm = ws > 0.0 ? (float)(1.0 / ((double)FS_RHO0 * ws * dx3)) : 0.f;
+        }
+        if (out) for (int i = 0; i < n; i++) out[i] = spots[i];
+        return 0;
+    }
+    void set_face(const float* f) override {
+        if (f) face.assign(f, f + (size_t)nx * nz); else face.clear();
+    }
     double residual(int* used) const override { if (used) *used = last_used; return last_rel; }
 };
 
@@ -1044,6 +1098,8 @@ int orc_mg_info(orc_sim* s, int* dims) { return s->o->mg_info(dims); }
 int orc_coarse_iters(orc_sim* s) { return s->o->coarse_iters(); }
 double orc_residual(orc_sim* s, int* used) { return s->o->residual(used); }
 int orc_build_campfire(int seed, float* out, int max_logs) { return build_campfire(seed, out, max_logs); }
+int orc_set_spots(orc_sim* s, const fs_spot* sp, int n, fs_spot* out) { return s->o->set_spots(sp, n, out); }
+void orc_set_face_flux(orc_sim* s, const float* f) { s->o->set_face(f); }
 
 // Noise bake alone (for the noise-texture golden fixtures).
 void orc_bake_noise(int n, int seed, float* noise_out, float* curl_out) {

@@ -58,6 +58,11 @@ def make_config(res=(80, 120, 80), size=(1.2, 1.8, 1.2), vcycles=2, max_levels=0
     return c
 
 
+class FsSpot(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_float * 3), ("radius", ctypes.c_float), ("flux", ctypes.c_float),
+                ("temp", ctypes.c_float), ("norm", ctypes.c_float), ("pad", ctypes.c_float)]
+
+
 _lib = None
 
 
@@ -118,6 +123,8 @@ def lib():
         L.orc_mjs_study.restype = ctypes.c_double
         L.orc_mjs_study.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_spots.argtypes = [vp, ctypes.POINTER(FsSpot), ctypes.c_int, ctypes.POINTER(FsSpot)]
+        L.orc_set_face_flux.argtypes = [vp, fp]
         _lib = L
     return _lib
 
@@ -222,6 +229,26 @@ class OracleSim:
         out = np.zeros(NUM_PARAMS, np.float32)
         lib().orc_get_params(self.h, _fp(out))
         return out
+
+    def set_spots(self, spots):
+        """rows of (x, y, z, radius, flux_kg_s, temp); returns the per-spot mass normalisation"""
+        a = np.asarray(spots, np.float32).reshape(-1, 6)
+        arr = (FsSpot * max(len(a), 1))()
+        out = (FsSpot * max(len(a), 1))()
+        for i, r in enumerate(a):
+            arr[i].pos[:] = [float(r[0]), float(r[1]), float(r[2])]
+            arr[i].radius, arr[i].flux, arr[i].temp = float(r[3]), float(r[4]), float(r[5])
+        lib().orc_set_spots(self.h, arr, len(a), out)
+        return np.array([out[i].norm for i in range(len(a))], np.float64)
+
+    def set_face_flux(self, flux):
+        if flux is None:
+            lib().orc_set_face_flux(self.h, None)
+            return
+        nx, _, nz = self.res
+        a = np.ascontiguousarray(flux, np.float32)
+        assert a.shape == (nz, nx)
+        lib().orc_set_face_flux(self.h, _fp(a))
 
     def reset(self):
         lib().orc_reset(self.h)

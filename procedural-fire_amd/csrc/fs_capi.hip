@@ -73,6 +73,10 @@ struct fs_sim {
     half4_t* curl = nullptr;
     float* logs_d = nullptr;
     int logs_cap = 0;
+    fs_spot* spots_d = nullptr;
+    std::vector<fs_spot> spots_h;
+    float* face_d = nullptr;
+    bool has_face = false;
     std::vector<float> logs_h;
     DevParams* P_d = nullptr;
     float prm[FS_NUM_PARAMS];
@@ -102,6 +106,8 @@ struct fs_sim {
         F.vort = f[F_VORT]; F.geo = f[F_GEO];
         F.noise = noise; F.curl = curl; F.noise_n = cfg.noise_n;
         F.logs = logs_d;
+        F.spots = spots_d;
+        F.face_flux = face_d;
         return F;
     }
     bool refine() const { return cfg.solve == FS_SOLVE_REFINE64; }
@@ -143,6 +149,8 @@ void push_params(fs_sim* h, double dt) {
     d.dt = dt;
     d.frame = h->frame;
     d.n_logs = (int)(h->logs_h.size() / 8);
+    d.n_spots = (int)h->spots_h.size();
+    d.has_face_flux = h->has_face ? 1 : 0;
     launch_set_params(h->stream, h->P_d, d);
     h->params_dirty = false;
     h->dev_dt = dt;
@@ -248,6 +256,8 @@ void free_sim(fs_sim* h) {
     if (h->noise) (void)hipFree(h->noise);
     if (h->curl) (void)hipFree(h->curl);
     if (h->logs_d) (void)hipFree(h->logs_d);
+    if (h->spots_d) (void)hipFree(h->spots_d);
+    if (h->face_d) (void)hipFree(h->face_d);
     if (h->P_d) (void)hipFree(h->P_d);
     mg_free(h->mg);
     for (auto& e : h->ev) if (e) (void)hipEventDestroy(e);
@@ -410,7 +420,11 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     const size_t nb = (size_t)nn * nn * nn * sizeof(half4_t);
     if (hipMalloc(&h->noise, nb) != hipSuccess || hipMalloc(&h->curl, nb) != hipSuccess)
         return bail(fail(FS_ENOMEM, "hipMalloc failed for noise textures"));
-    if (hipMalloc(&h->P_d, sizeof(DevParams)) != hipSuccess) return bail(fail(FS_ENOMEM, "hipMalloc failed"));
+    if (hipMalloc(&h->P_d, sizeof(DevParams)) != hipSuccess || hipMalloc(&h->spots_d, FS_MAX_SPOTS * sizeof(fs_spot)) != hipSuccess ||
+        hipMalloc(&h->face_d, (size_t)nx * nz * sizeof(float)) != hipSuccess)
+        return bail(fail(FS_ENOMEM, "hipMalloc failed"));
+    launch_fill_zero(h->stream, h->spots_d, FS_MAX_SPOTS * sizeof(fs_spot));
+    launch_fill_zero(h->stream, h->face_d, (size_t)nx * nz * sizeof(float));
     std::string err;
     int rc = mg_build(h->mg, nx, ny, nz, (float)h->dx, cfg->max_levels, err);
     if (rc) return bail(fail(rc, err));
@@ -469,6 +483,67 @@ int fs_set_logs(fs_sim* h, const float* logs, int n_logs) {
     if (!h || n_logs < 0 || (n_logs > 0 && !logs)) return fail(FS_EINVAL, "bad logs");
     if (set_device(h)) return FS_EDEVICE;
     return upload_logs(h, logs, n_logs);
+}
+
+// normalisation of a spot's mass injection over the grid (double, cell centres
+// origin + (i + 1/2) dx, the kernel's weight w = 1 - smoothstep(r/2, r, d))
+static double spot_weight_sum(const fs_sim* h, const fs_spot& sp) {
+    const double dx = (double)(float)h->dx;
+    const double ox = (double)(float)h->origin[0], oy = (double)(float)h->origin[1], oz = (double)(float)h->origin[2];
+    const double r = sp.radius;
+    auto cell = [&](double o, double pc) { return (int)std::floor((pc - o) / dx - 0.5); };
+    const int x0 = std::max(0, cell(ox, sp.pos[0] - r) - 1), x1 = std::min(h->nx - 1, cell(ox, sp.pos[0] + r) + 2);
+    const int y0 = std::max(0, cell(oy, sp.pos[1] - r) - 1), y1 = std::min(h->ny - 1, cell(oy, sp.pos[1] + r) + 2);
+    const int z0 = std::max(0, cell(oz, sp.pos[2] - r) - 1), z1 = std::min(h->nz - 1, cell(oz, sp.pos[2] + r) + 2);
+    double sum = 0.0;
+    for (int z = z0; z <= z1; z++)
+        for (int y = y0; y <= y1; y++)
+            for (int x = x0; x <= x1; x++) {
+                const double px = ox + (x + 0.5) * dx, py = oy + (y + 0.5) * dx, pz = oz + (z + 0.5) * dx;
+                const double d = std::sqrt((px - sp.pos[0]) * (px - sp.pos[0]) + (py - sp.pos[1]) * (py - sp.pos[1]) +
+                                           (pz - sp.pos[2]) * (pz - sp.pos[2]));
+                if (d >= r) continue;
+                double t = (d - 0.5 * r) / (0.5 * r);
+                t = std::min(std::max(t, 0.0), 1.0);
+                sum += 1.0 - t * t * (3.0 - 2.0 * t);
+            }
+    return sum;
+}
+
+int fs_set_spots(fs_sim* h, const fs_spot* spots, int n) {
+    if (!h || n < 0 || n > FS_MAX_SPOTS || (n > 0 && !spots)) return fail(FS_EINVAL, "bad spots (at most FS_MAX_SPOTS)");
+    if (set_device(h)) return FS_EDEVICE;
+    std::vector<fs_spot> v(spots, spots + n);
+    const double dx3 = std::pow((double)(float)h->dx, 3.0);
+    for (auto& sp : v) {
+        if (!(sp.radius > 0.f) || !std::isfinite(sp.flux)) return fail(FS_EINVAL, "spot radius must be positive, flux finite");
+        const double ws = spot_weight_sum(h, sp);
+        sp.norm = ws > 0.0 ? (float)(1.0 / ((double)FS_RHO0 * ws * dx3)) : 0.f;
+        sp.pad = 0.f;
+    }
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (n > 0) FS_HIP_CHECK(hipMemcpy(h->spots_d, v.data(), n * sizeof(fs_spot), hipMemcpyHostToDevice));
+    h->spots_h = v;
+    h->params_dirty = true;
+    return FS_OK;
+}
+
+int fs_get_spots(fs_sim* h, fs_spot* out, int max, int* n) {
+    if (!h || (max > 0 && !out)) return fail(FS_EINVAL, "null argument");
+    const int k = std::min(max, (int)h->spots_h.size());
+    for (int i = 0; i < k; i++) out[i] = h->spots_h[i];
+    if (n) *n = (int)h->spots_h.size();
+    return FS_OK;
+}
+
+int fs_set_face_flux(fs_sim* h, const float* flux) {
+    if (!h) return fail(FS_EINVAL, "null handle");
+    if (set_device(h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (flux) FS_HIP_CHECK(hipMemcpy(h->face_d, flux, (size_t)h->nx * h->nz * sizeof(float), hipMemcpyHostToDevice));
+    h->has_face = flux != nullptr;
+    h->params_dirty = true;
+    return FS_OK;
 }
 
 int fs_reset(fs_sim* h) {   // sim.js:606-618

@@ -334,7 +334,8 @@ __global__ __launch_bounds__(256) void k_forces(const DevParams* __restrict__ P,
                                                 const typename S::raw* __restrict__ velY, const typename S::raw* __restrict__ sY,
                                                 const typename S::raw* __restrict__ s2Y, const typename S::raw* __restrict__ vort,
                                                 const typename S::raw* __restrict__ geo, const half4_t* __restrict__ noise,
-                                                const half4_t* __restrict__ curlT, int nn, typename S::raw* __restrict__ velOut,
+                                                const half4_t* __restrict__ curlT, int nn, const fs_spot* __restrict__ spots,
+                                                const float* __restrict__ faceFlux, typename S::raw* __restrict__ velOut,
                                                 typename S::raw* __restrict__ sOut, typename S::raw* __restrict__ s2Out) {
     Cell c;
     if (!cell_of(g, c)) return;
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(256) void k_forces(const DevParams* __restrict__ P,
     s.y = fmaxf(s.y, bed * U[P_bedFuel]);
     s.x = fmaxf(s.x, src * U[P_srcTemp]);
     s.x = fmaxf(s.x, bedHot * U[P_bedTemp]);
-    const float srcAll = fmaxf(srcW, bed);
+    float srcAll = fmaxf(srcW, bed);
     s2.w = mixf(s2.w, 1.f, bed * (1.f - srcW));
     s2.w = mixf(s2.w, 0.f, srcW);
     s2.y = s2.y * (1.f - srcAll);
@@ -391,6 +392,21 @@ __global__ __launch_bounds__(256) void k_forces(const DevParams* __restrict__ P,
         s2.w *= 1.f - bw;
         s2.y = fmaxf(s2.y, 0.05f * bw * wclamp(s.y + s2.x, 0.f, 1.f));
     }
+    // ---- pyrolysis spots and floor fuel flux (build extension: mass-flux sources)
+    for (int k = 0; k < P->n_spots; k++) {
+        const fs_spot sp = spots[k];
+        const float ddx = px - sp.pos[0], ddy = py - sp.pos[1], ddz = pz - sp.pos[2];
+        const float d = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
+        if (d < sp.radius) {
+            const float w = 1.f - sstep(0.5f * sp.radius, sp.radius, d);
+            s.y = fminf(s.y + sp.flux * dt * sp.norm * w, 1.f);
+            s.x = fmaxf(s.x, w * sp.temp);
+            s2.w = mixf(s2.w, 0.f, w);
+            s2.y = s2.y * (1.f - w);
+            srcAll = fmaxf(srcAll, w);
+        }
+    }
+    if (P->has_face_flux && c.y == 0) s.y = fminf(s.y + faceFlux[c.z * g.nx + c.x] * dt / (FS_RHO0 * dx), 1.f);
     float ageF = s2.y / fmaxf(s.y + s2.x, 1e-4f);
 
     // ---- mixing-limited combustion (sim.js:300-309)
@@ -706,7 +722,7 @@ template <class S> static void curl_t(const Dims& g, hipStream_t s, const FieldS
 void launch_curl(int prec, const Dims& g, hipStream_t s, const FieldSet& F) { DISPATCH_PREC(prec, curl_t<S>(g, s, F)); }
 template <class S> static void forces_t(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F) {
     k_forces<S><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.velY), CRAW(F.SY), CRAW(F.S2Y), CRAW(F.vort), CRAW(F.geo), F.noise,
-                                          F.curl, F.noise_n, RAW(F.velX), RAW(F.S), RAW(F.S2));
+                                          F.curl, F.noise_n, F.spots, F.face_flux, RAW(F.velX), RAW(F.S), RAW(F.S2));
 }
 void launch_forces(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F) {
     DISPATCH_PREC(prec, forces_t<S>(g, s, P, F));

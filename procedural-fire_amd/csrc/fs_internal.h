@@ -29,7 +29,9 @@ struct DevParams {
     double dt;
     long long frame;
     int n_logs;
-    int pad;
+    int n_spots;                 // pyrolysis spots in FieldSet::spots
+    int has_face_flux;           // FieldSet::face_flux holds nx*nz floor fluxes
+    int pad[3];
 };
 
 // ---- storage of the 4-channel fields ------------------------------------------
@@ -129,6 +131,8 @@ struct FieldSet {
     void* vort; void* geo;
     const half4_t* noise; const half4_t* curl; int noise_n;
     const float* logs;
+    const fs_spot* spots;        // FS_MAX_SPOTS entries
+    const float* face_flux;      // nx*nz
 };
 void launch_geo(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F);
 void launch_adv1(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F);

@@ -52,6 +52,26 @@ class FsConfig(ctypes.Structure):
     ]
 
 
+class FsSpot(ctypes.Structure):
+    """A pyrolysis spot: mass flux `flux` (kg/s) spread over a sphere of `radius` (m) at `pos`."""
+    _fields_ = [("pos", ctypes.c_float * 3), ("radius", ctypes.c_float), ("flux", ctypes.c_float),
+                ("temp", ctypes.c_float), ("norm", ctypes.c_float), ("pad", ctypes.c_float)]
+
+
+MAX_SPOTS = 64
+RHO0 = 1.204
+
+
+def spots_array(spots):
+    """(n, 6) rows of (x, y, z, radius, flux_kg_s, temp) -> ctypes array of FsSpot."""
+    a = np.asarray(spots, np.float32).reshape(-1, 6)
+    arr = (FsSpot * max(len(a), 1))()
+    for i, r in enumerate(a):
+        arr[i].pos[:] = [float(r[0]), float(r[1]), float(r[2])]
+        arr[i].radius, arr[i].flux, arr[i].temp = float(r[3]), float(r[4]), float(r[5])
+    return arr, len(a)
+
+
 _lib = None
 
 
@@ -79,6 +99,9 @@ def lib():
         "fs_set_params": (ip, [vp, ctypes.POINTER(ctypes.c_char_p), fp, ip]),
         "fs_get_params": (ip, [vp, fp]),
         "fs_set_logs": (ip, [vp, fp, ip]),
+        "fs_set_spots": (ip, [vp, ctypes.POINTER(FsSpot), ip]),
+        "fs_get_spots": (ip, [vp, ctypes.POINTER(FsSpot), ip, ctypes.POINTER(ctypes.c_int)]),
+        "fs_set_face_flux": (ip, [vp, fp]),
         "fs_reset": (ip, [vp]),
         "fs_step": (ip, [vp, ctypes.c_double, ip]),
         "fs_sync": (ip, [vp]),
@@ -217,6 +240,30 @@ class FireSim:
     def set_logs(self, logs):
         logs = np.ascontiguousarray(logs, np.float32)
         _check(lib().fs_set_logs(self._h, logs.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), logs.size // 8))
+
+    # ---- build-extension sources
+    def set_spots(self, spots):
+        """Pyrolysis spots, rows of (x, y, z, radius, flux_kg_s, temp); [] removes them."""
+        arr, n = spots_array(spots)
+        _check(lib().fs_set_spots(self._h, arr, n))
+
+    def spots(self):
+        """The spots as set, with their mass normalisation (columns x, y, z, r, flux, temp, norm)."""
+        arr = (FsSpot * MAX_SPOTS)()
+        n = ctypes.c_int()
+        _check(lib().fs_get_spots(self._h, arr, MAX_SPOTS, ctypes.byref(n)))
+        return np.array([[*a.pos, a.radius, a.flux, a.temp, a.norm] for a in arr[:n.value]], np.float64).reshape(-1, 7)
+
+    def set_face_flux(self, flux):
+        """Floor fuel mass flux (kg/m^2/s), shape (nz, nx); None removes it."""
+        if flux is None:
+            _check(lib().fs_set_face_flux(self._h, None))
+            return
+        a = np.ascontiguousarray(flux, np.float32)
+        nx, _, nz = self.res
+        if a.shape != (nz, nx):
+            raise ValueError(f"face flux must have shape {(nz, nx)}")
+        _check(lib().fs_set_face_flux(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
 
     # ---- stepping
     def reset(self):

@@ -75,6 +75,17 @@ class FireSim {
   // the log buffer rewrite + geoReady = false of applyLogs (src/main.js:147-149)
   setLogs(logs) { addon.setLogs(this.h, logs); }
 
+  // pyrolysis spots (build extension): [{pos: [x, y, z], radius, flux (kg/s), temp}], [] removes them;
+  // returns each spot's mass normalisation 1 / (rho0 * sum(w) * dx^3)
+  setSpots(spots) {
+    const f = new Float32Array(spots.length * 6);
+    spots.forEach((s, i) => { f.set([s.pos[0], s.pos[1], s.pos[2], s.radius, s.flux, s.temp], i * 6); });
+    return addon.setSpots(this.h, f);
+  }
+
+  // floor fuel mass flux (kg/m^2/s) as a Float32Array of nz*nx (x fastest); null removes it
+  setFaceFlux(flux) { addon.setFaceFlux(this.h, flux); }
+
   reset() { addon.reset(this.h); }                             // src/sim.js:606-618
 
   // step(dt): enqueue one step (src/sim.js:627-648); n > 1 enqueues n steps

@@ -246,6 +246,73 @@ static napi_value js_set_logs(napi_env env, napi_callback_info info) {
     return NULL;
 }
 
+/* setSpots(h, Float32Array of n*6: x, y, z, radius, flux_kg_s, temp) -> Float32Array of n normalisations */
+static napi_value js_set_spots(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sim* h = get_sim(env, argv[0]);
+    if (!h) return NULL;
+    void* data;
+    size_t bytes;
+    napi_typedarray_type t;
+    if (!typed_data(env, argv[1], &data, &bytes, &t) || t != napi_float32_array || (bytes / 4) % 6) {
+        napi_throw_type_error(env, NULL, "spots must be a Float32Array of 6 floats per spot");
+        return NULL;
+    }
+    const int n = (int)(bytes / 24);
+    if (n > FS_MAX_SPOTS) {
+        napi_throw_range_error(env, NULL, "too many spots");
+        return NULL;
+    }
+    fs_spot sp[FS_MAX_SPOTS];
+    const float* f = (const float*)data;
+    for (int i = 0; i < n; i++) {
+        memset(&sp[i], 0, sizeof(fs_spot));
+        sp[i].pos[0] = f[6 * i]; sp[i].pos[1] = f[6 * i + 1]; sp[i].pos[2] = f[6 * i + 2];
+        sp[i].radius = f[6 * i + 3]; sp[i].flux = f[6 * i + 4]; sp[i].temp = f[6 * i + 5];
+    }
+    int rc = fs_set_spots(h, sp, n);
+    if (rc) return throw_fs(env, rc);
+    int got = 0;
+    rc = fs_get_spots(h, sp, FS_MAX_SPOTS, &got);
+    if (rc) return throw_fs(env, rc);
+    float norm[FS_MAX_SPOTS];
+    for (int i = 0; i < got; i++) norm[i] = sp[i].norm;
+    return float32_array(env, norm, (size_t)got);
+}
+
+/* setFaceFlux(h, Float32Array nx*nz | null) */
+static napi_value js_set_face_flux(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sim* h = get_sim(env, argv[0]);
+    if (!h) return NULL;
+    napi_valuetype vt;
+    CHECK_NAPI(env, napi_typeof(env, argv[1], &vt));
+    if (vt == napi_null || vt == napi_undefined) {
+        int rc = fs_set_face_flux(h, NULL);
+        if (rc) return throw_fs(env, rc);
+        return NULL;
+    }
+    void* data;
+    size_t bytes;
+    napi_typedarray_type t;
+    int res[3];
+    double dx, origin[3];
+    int rc = fs_grid_info(h, res, &dx, origin);
+    if (rc) return throw_fs(env, rc);
+    if (!typed_data(env, argv[1], &data, &bytes, &t) || t != napi_float32_array ||
+        bytes != (size_t)res[0] * res[2] * sizeof(float)) {
+        napi_throw_type_error(env, NULL, "face flux must be a Float32Array of nx*nz floats");
+        return NULL;
+    }
+    rc = fs_set_face_flux(h, (const float*)data);
+    if (rc) return throw_fs(env, rc);
+    return NULL;
+}
+
 static napi_value js_reset(napi_env env, napi_callback_info info) {
     size_t argc = 1;
     napi_value argv[1];
@@ -508,6 +575,8 @@ static napi_value init(napi_env env, napi_value exports) {
     EXPORT("setParams", js_set_params);
     EXPORT("getParams", js_get_params);
     EXPORT("setLogs", js_set_logs);
+    EXPORT("setSpots", js_set_spots);
+    EXPORT("setFaceFlux", js_set_face_flux);
     EXPORT("reset", js_reset);
     EXPORT("step", js_step);
     EXPORT("stepAsync", js_step_async);

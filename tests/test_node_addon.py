@@ -68,3 +68,28 @@ def test_node_steps_match_the_c_abi_bit_for_bit(tmp_path):
     T = 293 + 1000 * S[..., 0].astype(np.float64)
     Q = np.sum(np.where(q > 0, 354800 * q / T, 0.0)) * sim.dx ** 3 / 1000
     assert abs(meta["profile"]["Q_kW"] - round(Q, 1)) <= 0.11
+
+
+@pytest.mark.gpu
+@needs_node
+def test_node_spot_scene_matches_the_c_abi_bit_for_bit(tmp_path):
+    """setSpots / setFaceFlux through the addon give the C ABI's fields and normalisations."""
+    print("node spot scene adjoint8_64 x8", flush=True)
+    from firesim import scenes
+    name = "adjoint8_64"
+    sc = scenes.SCENES[name]
+    spots = scenes.scene_spots(name)
+    face = scenes.scene_face_flux(name)
+    scene = {"res": list(sc["res"]), "size": list(scenes.scene_size(name)), "params": scenes.scene_params(name),
+             "spots": spots.tolist(), "face": face.ravel().tolist(), "dt": sc["dt"]}
+    (tmp_path / "scene.json").write_text(json.dumps(scene))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "run_steps.js"), str(tmp_path), "8",
+                        str(tmp_path / "scene.json")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    meta = json.loads((tmp_path / "meta.json").read_text())
+    sim = scenes.make_scene(name, noise_n=32, use_graph=True)
+    assert np.array_equal(np.array(meta["norms"], np.float32), sim.spots()[:, 6].astype(np.float32))
+    sim.step(sc["dt"], 8)
+    for f, fid in (("vel", F.VEL), ("S", F.S), ("S2", F.S2)):
+        a = np.fromfile(tmp_path / f"{f}.bin", np.uint16)
+        assert np.array_equal(a, sim.read(fid, "native").ravel()), f
