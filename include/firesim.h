@@ -102,7 +102,11 @@ typedef struct fs_config {
                              once per step (one stream sync per step, no wasted launches);
                              0 = fully asynchronous: max_vcycles iterations are enqueued and the
                              ones after convergence exit at once                              */
-    int    reserved[6];
+    float  omega;         /* FS_SOLVE_REFINE64 / TOL32: over-relaxation factor of the red-black
+                             smoother on every level but the coarsest (0 = default 1.25; 1 =
+                             plain Gauss-Seidel).  FS_SOLVE_VCYCLES, the reference's solve,
+                             always uses 1                                                    */
+    int    reserved[5];
 } fs_config;
 
 /* Fills a config with the reference defaults (res 80x120x80, size 1.2x1.8x1.2,
@@ -226,6 +230,10 @@ int  fs_poisson_solve(fs_poisson* ps, double tol, int max_cycles, double* hist, 
  * hist[k] (may be NULL) = relative residual after cycle k+1. */
 int  fs_poisson_vcycles(fs_poisson* ps, int cycles, double* hist);
 int  fs_poisson_read_x(fs_poisson* ps, int dtype, void* host_x);
+/* Over-relaxation factor of every smoothing pass except the coarsest level's sweeps,
+ * for both fs_poisson_solve and fs_poisson_vcycles (default 1: the reference's
+ * Gauss-Seidel).  No counterpart in the reference. */
+int  fs_poisson_set_omega(fs_poisson* ps, float omega);
 int  fs_poisson_info(fs_poisson* ps, int* levels, int dims[][3], int* coarse_iters);
 
 #ifdef __cplusplus

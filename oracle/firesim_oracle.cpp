@@ -198,6 +198,7 @@ template <class M>
 struct Multigrid {
     std::vector<Level<M>> L;
     int coarse_iters = 8;
+    M omega = 1;   // over-relaxation above the coarsest level (tolerance solves; the reference uses 1)
 
     void build(int x, int y, int z, double h, int max_levels) {   // sim.js:79-92
         L.clear();
@@ -213,9 +214,11 @@ struct Multigrid {
         double mu = (std::cos(M_PI / c.nx) + std::cos(M_PI / (2.0 * c.ny)) + std::cos(M_PI / c.nz)) / 3.0;
         coarse_iters = std::max(8, std::min(64, (int)std::ceil(std::log(0.05) / std::log(mu * mu))));
     }
-    // one colour pass of red-black Gauss-Seidel, omega = 1 (sim.js:422-437)
+    // one colour pass of red-black Gauss-Seidel (sim.js:422-437); with omega != 1 on a
+    // level above the coarsest, p <- p + omega (p_GS - p)
     void smooth_color(int li, int color) {
         Level<M>& l = L[li];
+        const M om = li + 1 < (int)L.size() ? omega : M(1);
         const int nx = l.nx, ny = l.ny, nz = l.nz;
         const M h2 = (M)l.h * (M)l.h;   // L.h is an f32 uniform in the reference
         M* p = l.p.data(); const M* b = l.b.data();
@@ -231,7 +234,9 @@ struct Multigrid {
                     if (z < nz - 1) sum += p[c + (size_t)nx * ny]; else diag += 1;
                     if (y > 0) sum += p[c - nx]; else diag -= 1;
                     if (y < ny - 1) sum += p[c + nx]; else diag += 1;
-                    p[c] = (sum - h2 * b[c]) / diag;
+                    M pn = (sum - h2 * b[c]) / diag;
+                    if (om != M(1)) pn = p[c] + om * (pn - p[c]);
+                    p[c] = pn;
                 }
     }
     void smooth(int li, int iters) {   // sim.js:576-586
@@ -417,6 +422,7 @@ struct Oracle : OracleBase {
         for (auto* f : {&vel, &velX, &velY, &Sf, &SX, &SY, &S2, &S2X, &S2Y, &vort, &geo})
             f->assign(ncell * 4, St::enc(0.0));
         mg.build(nx, ny, nz, (double)(float)dx, c->max_levels);   // L.h is an f32 uniform (sim.js:96)
+        if (solve_mode != FS_SOLVE_VCYCLES) mg.omega = (M)(c->omega > 0.f ? c->omega : 1.25f);
         x64.assign(ncell, 0.0);
         set_logs(lg, nl);
         bake_noise(noise, c->noise_n, c->noise_seed);
@@ -1122,6 +1128,7 @@ orc_mg* orc_mg_create(int nx, int ny, int nz, double h, int max_levels, int fp64
     return m;
 }
 void orc_mg_destroy(orc_mg* m) { delete m->f; delete m->d; delete m; }
+void orc_mg_set_omega(orc_mg* m, float omega) { if (m->f) m->f->omega = omega; else m->d->omega = omega; }
 int orc_mg_levels(orc_mg* m, int* dims, int* coarse) {
     int n = m->f ? (int)m->f->L.size() : (int)m->d->L.size();
     for (int i = 0; i < n; i++) {

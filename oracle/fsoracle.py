@@ -38,12 +38,13 @@ class FsConfig(ctypes.Structure):
         ("stream", ctypes.c_void_p),
         ("use_graph", ctypes.c_int),
         ("adaptive", ctypes.c_int),
-        ("reserved", ctypes.c_int * 6),
+        ("omega", ctypes.c_float),
+        ("reserved", ctypes.c_int * 5),
     ]
 
 
 def make_config(res=(80, 120, 80), size=(1.2, 1.8, 1.2), vcycles=2, max_levels=0, precision=FP16,
-                solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1):
+                solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1, omega=0.0):
     c = FsConfig()
     c.res[:] = list(res)
     c.size[:] = list(size)
@@ -55,6 +56,7 @@ def make_config(res=(80, 120, 80), size=(1.2, 1.8, 1.2), vcycles=2, max_levels=0
     c.max_vcycles = max_vcycles
     c.noise_n = noise_n
     c.noise_seed = noise_seed
+    c.omega = omega
     return c
 
 
@@ -109,6 +111,7 @@ def lib():
         L.orc_mg_create.restype = vp
         L.orc_mg_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.orc_mg_destroy.argtypes = [vp]
+        L.orc_mg_set_omega.argtypes = [vp, ctypes.c_float]
         L.orc_mg_levels.argtypes = [vp, ip, ip]
         L.orc_mg_set.argtypes = [vp, dp, dp]
         L.orc_mg_get.argtypes = [vp, dp]
@@ -188,7 +191,7 @@ class OracleSim:
     """CPU FireSim restatement.  precision: FP32/FP16/BF16 (f32 math) or FP64."""
 
     def __init__(self, res=(64, 96, 64), size=None, logs=None, n_logs=None, precision=FP16, vcycles=2,
-                 max_levels=0, solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1):
+                 max_levels=0, solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1, omega=0.0):
         if size is None:
             size = (res[0] * 0.015, res[1] * 0.015, res[2] * 0.015)
         if logs is None:
@@ -198,7 +201,8 @@ class OracleSim:
         if n_logs is None:
             n_logs = logs.size // 8
         self.res = tuple(res)
-        self.cfg = make_config(res, size, vcycles, max_levels, precision, solve, div_tol, max_vcycles, noise_n, noise_seed)
+        self.cfg = make_config(res, size, vcycles, max_levels, precision, solve, div_tol, max_vcycles, noise_n, noise_seed,
+                               omega)
         self.h = lib().orc_create(ctypes.byref(self.cfg), _fp(logs), n_logs)
         if not self.h:
             raise RuntimeError("oracle create failed")
@@ -306,9 +310,11 @@ class OracleSim:
 class OracleMG:
     """Stand-alone sim.js multigrid (src/sim.js:416-495, 588-599) on an nx*ny*nz grid."""
 
-    def __init__(self, dims, h, max_levels=0, fp64=False):
+    def __init__(self, dims, h, max_levels=0, fp64=False, omega=1.0):
         self.dims = tuple(dims)
         self.h = lib().orc_mg_create(dims[0], dims[1], dims[2], float(h), int(max_levels), 1 if fp64 else 0)
+        if omega != 1.0:
+            lib().orc_mg_set_omega(self.h, float(omega))
 
     def __del__(self):
         try:

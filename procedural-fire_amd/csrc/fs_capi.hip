@@ -366,6 +366,7 @@ void fs_config_default(fs_config* c) {
     c->stream = nullptr;
     c->use_graph = 0;
     c->adaptive = 1;
+    c->omega = 0.f;                                           // tolerance solves: 1.25
 }
 
 int fs_param_count(void) { return FS_NUM_PARAMS; }
@@ -390,6 +391,7 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     if ((long long)nx * ny * nz >= (1LL << 31) / 4) return fail(FS_EINVAL, "grid exceeds 32-bit texel indexing");
     if (cfg->precision < FS_PREC_FP32 || cfg->precision > FS_PREC_BF16) return fail(FS_EINVAL, "bad precision");
     if (cfg->solve < FS_SOLVE_VCYCLES || cfg->solve > FS_SOLVE_TOL32) return fail(FS_EINVAL, "bad solve mode");
+    if (!(cfg->omega == 0.f || (cfg->omega > 0.f && cfg->omega < 2.f))) return fail(FS_EINVAL, "omega must be 0 or in (0, 2)");
     if (cfg->vcycles < 0 || cfg->max_vcycles < 0 || cfg->max_vcycles > MG_MAX_REFINE) return fail(FS_EINVAL, "bad V-cycle count");
     const int nn = cfg->noise_n;
     if (nn < 4 || (nn & (nn - 1)) != 0 || nn > 512) return fail(FS_EINVAL, "noise_n must be a power of two in [4, 512]");
@@ -428,6 +430,8 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     std::string err;
     int rc = mg_build(h->mg, nx, ny, nz, (float)h->dx, cfg->max_levels, err);
     if (rc) return bail(fail(rc, err));
+    // over-relaxed smoothing in the tolerance solves only; the reference's solve is plain GS
+    if (cfg->solve != FS_SOLVE_VCYCLES) h->mg.omega = cfg->omega > 0.f ? cfg->omega : 1.25f;
     for (auto& e : h->ev) if (hipEventCreate(&e) != hipSuccess) return bail(fail(FS_EDEVICE, "event create failed"));
     if (hipHostMalloc((void**)&h->host_state, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return bail(fail(FS_ENOMEM, "pinned host allocation failed"));
@@ -878,6 +882,12 @@ int fs_poisson_create(const int dims[3], double h, int max_levels, int device, v
     int rc = mg_build(ps->mg, dims[0], dims[1], dims[2], (float)h, max_levels, err);
     if (rc) { mg_free(ps->mg); if (ps->own_stream) (void)hipStreamDestroy(ps->stream); delete ps; return fail(rc, err); }
     *out = ps;
+    return FS_OK;
+}
+
+int fs_poisson_set_omega(fs_poisson* ps, float omega) {
+    if (!ps || !(omega > 0.f && omega < 2.f)) return fail(FS_EINVAL, "omega must be in (0, 2)");
+    ps->mg.omega = omega;
     return FS_OK;
 }
 

@@ -80,6 +80,7 @@ struct GpuMG {
     int tail_from = -1;          // first level solved by the single-workgroup tail kernel
     bool use_tail = true;        // false: one launch per colour pass on every level
     bool fuse_prolong = true;    // prolongation folded into the first post-smoothing pass
+    float omega = 1.f;           // over-relaxation of the smoother (all levels but the coarsest)
 };
 
 // Multigrid arrays are parity-split: each x-row stores its even-x cells, then its odd-x

@@ -36,9 +36,11 @@ inline dim3 grid3(int w, int ny, int nz) { return dim3((unsigned)((w + 63) / 64)
 // one colour pass of red-black GS (sim.js:422-437).  Thread (hx, y, z) owns the cell
 // x = 2hx + s of this colour, s = (y + z + color) & 1.  With `xupd` set (the last pass
 // of a correction V-cycle) it also adds the final p of both cells x = 2hx, 2hx+1 to
-// the FP64 iterate.
+// the FP64 iterate.  omega != 1 over-relaxes: p <- p + omega (p_GS - p) (the
+// tolerance-driven solves; the reference's solve is omega = 1 exactly).
 __global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
-                                               float h, int color, const int* __restrict__ skip, double* __restrict__ xupd) {
+                                               float h, int color, const int* __restrict__ skip, double* __restrict__ xupd,
+                                               float omega) {
     if (skipped(skip)) return;
     const int nh = (nx + 1) >> 1, rs = 2 * nh;
     const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -58,7 +60,11 @@ __global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const floa
         if (z < nz - 1) sum += p[own + ny * rs]; else diag += 1.f;
         if (y > 0) sum += p[own - rs]; else diag -= 1.f;
         if (y < ny - 1) sum += p[own + rs]; else diag += 1.f;
-        const float pn = (sum - h * h * b[own]) / diag;
+        float pn = (sum - h * h * b[own]) / diag;
+        if (omega != 1.f) {
+            const float po = p[own];
+            pn = po + omega * (pn - po);
+        }
         p[own] = pn;
         if (xupd) xupd[own] = xupd[own] + (double)pn;
     }
@@ -126,20 +132,53 @@ __device__ __forceinline__ float prolong_at(const float* __restrict__ pc, int x,
     }
     return acc;
 }
+// One thread per coarse cell: its 3x3x3 coarse neighbourhood (mirror rule applied)
+// gives the corrections of its 8 children, each summed in prolong_at's order (the
+// child's two coarse cells per axis sit at offsets {-1, 0} or {0, +1}, weights 1/4
+// and 3/4), so the result is bit-identical to the per-fine-cell form.  The 27 coarse
+// loads are shared by 8 fine cells and every child class is a coalesced half-row.
 __global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const float* __restrict__ pcoarse, int nx, int ny, int nz,
                                                 const int* __restrict__ skip) {
     if (skipped(skip)) return;
-    const int nh = nx >> 1, rs = nx;                   // levels that prolong have even nx
-    const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z;
-    if (hx >= nh || y >= ny) return;
-    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
-    const int row = (z * ny + y) * rs;
-    const float a0 = prolong_at(pcoarse, 2 * hx, y, z, cx, cy, cz);
-    const float a1 = prolong_at(pcoarse, 2 * hx + 1, y, z, cx, cy, cz);
-    p[row + hx] = p[row + hx] + a0;
-    p[row + nh + hx] = p[row + nh + hx] + a1;
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;    // levels that prolong have even dims
+    const int xc = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int yc = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int zc = blockIdx.z;
+    if (xc >= cx || yc >= cy) return;
+    const int nhc = (cx + 1) >> 1, rsc = 2 * nhc;
+    float c[3][3][3];
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++) {
+        const int qz = zc + dz - 1, sz = min(max(qz, 0), cz - 1);
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++) {
+            const int qy = yc + dy - 1, sy = min(max(qy, 0), cy - 1);
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+                const int qx = xc + dx - 1, sx = min(max(qx, 0), cx - 1);
+                float v = pcoarse[(sz * cy + sy) * rsc + (sx & 1) * nhc + (sx >> 1)];
+                if (qx < 0 || qx >= cx) v = -v;
+                if (qz < 0 || qz >= cz) v = -v;
+                if (qy >= cy) v = -v;
+                c[dz][dy][dx] = v;
+            }
+        }
+    }
+    const int nh = nx >> 1, rs = nx;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int a = k & 1, bb = (k >> 1) & 1, cc = (k >> 2) & 1;   // child offsets in x, y, z
+        const float fx = a ? 0.25f : 0.75f, fy = bb ? 0.25f : 0.75f, fz = cc ? 0.25f : 0.75f;
+        float acc = 0.f;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int ox = j & 1, oy = (j >> 1) & 1, oz = (j >> 2) & 1;
+            const float wx = ox ? fx : 1.f - fx, wy = oy ? fy : 1.f - fy, wz = oz ? fz : 1.f - fz;
+            acc += c[cc + oz][bb + oy][a + ox] * wx * wy * wz;
+        }
+        const int i = ((2 * zc + cc) * ny + 2 * yc + bb) * rs + a * nh + xc;
+        p[i] = p[i] + acc;
+    }
 }
 
 // Prolongation fused into the first post-smoothing pass (colour 0).  A Gauss-Seidel
@@ -301,13 +340,14 @@ struct TailArgs {
     float h[MG_MAX_LEVELS];
     int off[MG_MAX_LEVELS];          // float offset of level j's p in LDS; b follows p
     int coarse_iters;
+    float omega;                     // smoother over-relaxation above the coarsest level
     float* p_top;                    // global p / b of the tail's top level (parity-split layout)
     const float* b_top;
 };
 
 // linear thread-to-cell mapping; the divisions are multiply-highs (FastDiv)
 __device__ __forceinline__ void tail_smooth(float* p, const float* b, int nx, int ny, int nz, float h, int iters,
-                                            FastDiv dhx, FastDiv dy) {
+                                            FastDiv dhx, FastDiv dy, float omega) {
     const int nhx = (nx + 1) >> 1, nh = nhx * ny * nz, sy = nx, sz = nx * ny;
     for (int it = 0; it < iters; it++)
         for (int color = 0; color < 2; color++) {
@@ -324,7 +364,9 @@ __device__ __forceinline__ void tail_smooth(float* p, const float* b, int nx, in
                     if (z < nz - 1) sum += p[c + sz]; else diag += 1.f;
                     if (y > 0) sum += p[c - sy]; else diag -= 1.f;
                     if (y < ny - 1) sum += p[c + sy]; else diag += 1.f;
-                    p[c] = (sum - h * h * b[c]) / diag;
+                    float pn = (sum - h * h * b[c]) / diag;
+                    if (omega != 1.f) pn = p[c] + omega * (pn - p[c]);
+                    p[c] = pn;
                 }
             }
             __syncthreads();
@@ -406,20 +448,20 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailArgs a, const int*
         float* b = p + n;
         const int nc = a.dims[j + 1][0] * a.dims[j + 1][1] * a.dims[j + 1][2];
         float* pc = lds + a.off[j + 1];
-        tail_smooth(p, b, nx, ny, nz, a.h[j], 2, a.dhx[j], a.dy[j]);
+        tail_smooth(p, b, nx, ny, nz, a.h[j], 2, a.dhx[j], a.dy[j], a.omega);
         tail_restrict(p, b, nx, ny, nz, a.h[j], pc + nc, pc, a.dcx[j], a.dcy[j]);
     }
     {
         const int j = a.nt - 1;
         const int nx = a.dims[j][0], ny = a.dims[j][1], nz = a.dims[j][2], n = nx * ny * nz;
         float* p = lds + a.off[j];
-        tail_smooth(p, p + n, nx, ny, nz, a.h[j], a.coarse_iters, a.dhx[j], a.dy[j]);
+        tail_smooth(p, p + n, nx, ny, nz, a.h[j], a.coarse_iters, a.dhx[j], a.dy[j], 1.f);
     }
     for (int j = a.nt - 2; j >= 0; j--) {
         const int nx = a.dims[j][0], ny = a.dims[j][1], nz = a.dims[j][2], n = nx * ny * nz;
         float* p = lds + a.off[j];
         tail_prolong(p, lds + a.off[j + 1], nx, ny, nz, a.dx[j], a.dy[j]);
-        tail_smooth(p, p + n, nx, ny, nz, a.h[j], 2, a.dhx[j], a.dy[j]);
+        tail_smooth(p, p + n, nx, ny, nz, a.h[j], 2, a.dhx[j], a.dy[j], a.omega);
     }
     for (int i = threadIdx.x; i < n0; i += TAIL_THREADS) {
         const int t = fdiv(i, a.dx[0]), x = i - t * nx0;
@@ -498,10 +540,11 @@ void mg_free(GpuMG& mg) {
 void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, double* xupd) {   // sim.js:576-586
     const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
     const dim3 g = grid3((nx + 1) / 2, ny, nz);
+    const float om = l == mg.nlev - 1 ? 1.f : mg.omega;   // the coarsest level's sweeps stay plain GS
     for (int k = 0; k < iters; k++)
         for (int color = 0; color < 2; color++) {
             double* xu = (xupd && k == iters - 1 && color == 1) ? xupd : nullptr;
-            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, skip, xu);
+            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, skip, xu, om);
         }
 }
 
@@ -511,7 +554,7 @@ static void launch_restrict(GpuMG& mg, hipStream_t s, int l, const int* skip) {
 }
 static void launch_prolong(GpuMG& mg, hipStream_t s, int l, const int* skip) {
     const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
-    k_prolong<<<grid3(nx / 2, ny, nz), RB, 0, s>>>(mg.p[l], mg.p[l + 1], nx, ny, nz, skip);
+    k_prolong<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(mg.p[l], mg.p[l + 1], nx, ny, nz, skip);
 }
 
 static void launch_tail(GpuMG& mg, hipStream_t s, const int* skip) {
@@ -531,6 +574,7 @@ static void launch_tail(GpuMG& mg, hipStream_t s, const int* skip) {
         off += 2 * mg.dims[T + j][0] * mg.dims[T + j][1] * mg.dims[T + j][2];
     }
     a.coarse_iters = mg.coarse_iters;
+    a.omega = mg.omega;
     a.p_top = mg.p[T];
     a.b_top = mg.b[T];
     k_mg_tail<<<1, TAIL_THREADS, off * sizeof(float), s>>>(a, skip);
@@ -551,12 +595,14 @@ bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd) {   // s
     if (T < n) launch_tail(mg, s, skip);
     else mg_smooth(mg, s, n - 1, mg.coarse_iters, skip);
     for (int l = last - 1; l >= 0; l--) {
-        if (mg.fuse_prolong && l >= 2) {   // coarse levels: one launch less; fine levels: cheaper apart
+        // coarse levels: one launch less; fine levels: cheaper apart.  Over-relaxation reads the
+        // colour-1 cells' own prolonged values, which the fused pass leaves unformed: not fused then.
+        if (mg.fuse_prolong && l >= 2 && mg.omega == 1.f) {
             // prolongation + colour 0, then colour 1, then one more full sweep (sim.js:468-495, 576-586)
             const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
             const dim3 g = grid3(nx / 2, ny, nz);
             k_smooth_prolong<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], mg.p[l + 1], nx, ny, nz, mg.h[l], skip);
-            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], 1, skip, nullptr);
+            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], 1, skip, nullptr, 1.f);
             mg_smooth(mg, s, l, 1, skip, l == 0 ? xupd : nullptr);
         } else {
             launch_prolong(mg, s, l, skip);

@@ -48,7 +48,8 @@ class FsConfig(ctypes.Structure):
         ("stream", ctypes.c_void_p),
         ("use_graph", ctypes.c_int),
         ("adaptive", ctypes.c_int),
-        ("reserved", ctypes.c_int * 6),
+        ("omega", ctypes.c_float),
+        ("reserved", ctypes.c_int * 5),
     ]
 
 
@@ -125,6 +126,7 @@ def lib():
         "fs_poisson_solve": (ip, [vp, ctypes.c_double, ip, dp, ctypes.POINTER(ctypes.c_int)]),
         "fs_poisson_vcycles": (ip, [vp, ip, dp]),
         "fs_poisson_read_x": (ip, [vp, ip, vp]),
+        "fs_poisson_set_omega": (ip, [vp, ctypes.c_float]),
         "fs_poisson_info": (ip, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():
@@ -171,7 +173,7 @@ class FireSim:
 
     def __init__(self, res=(80, 120, 80), size=(1.2, 1.8, 1.2), logs=None, vcycles=2, max_levels=0,
                  precision=PREC_FP16, solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128,
-                 noise_seed=1, device=0, stream=None, use_graph=False, adaptive=True):
+                 noise_seed=1, device=0, stream=None, use_graph=False, adaptive=True, omega=0.0):
         L = lib()
         cfg = FsConfig()
         L.fs_config_default(ctypes.byref(cfg))
@@ -189,6 +191,7 @@ class FireSim:
         cfg.stream = stream
         cfg.use_graph = 1 if use_graph else 0
         cfg.adaptive = 1 if adaptive else 0
+        cfg.omega = float(omega)
         if logs is None:
             logs, _ = build_campfire(7)   # src/main.js:34-35: buildCampfire(7) packed to 16 slots
         logs = np.ascontiguousarray(logs, np.float32)
@@ -365,12 +368,14 @@ class FireSim:
 class Poisson:
     """Isolated pressure solve with the sim's operator and multigrid."""
 
-    def __init__(self, dims, h, max_levels=0, device=0, stream=None):
+    def __init__(self, dims, h, max_levels=0, device=0, stream=None, omega=1.0):
         L = lib()
         d = (ctypes.c_int * 3)(*[int(v) for v in dims])
         self._h = ctypes.c_void_p()
         _check(L.fs_poisson_create(d, float(h), int(max_levels), int(device), stream, ctypes.byref(self._h)))
         self.dims = tuple(int(v) for v in dims)
+        if omega != 1.0:
+            _check(L.fs_poisson_set_omega(self._h, float(omega)))
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -40,10 +40,10 @@ function fieldId(name) {
 class FireSim {
   constructor({ res = [80, 120, 80], size = [1.2, 1.8, 1.2], logs, vcycles = 2, maxLevels = 0,
     precision = 'fp16', solve = 'vcycles', divTol = 1e-5, maxVcycles = 8, noiseN = 128, noiseSeed = 1,
-    device = 0, useGraph = true, adaptive = true } = {}) {
+    device = 0, useGraph = true, adaptive = true, omega = 0 } = {}) {
     if (!logs) logs = buildCampfireLogs(7);
     this.h = addon.create({ res, size, logs, vcycles, maxLevels, precision: PRECISION[precision], solve: SOLVE[solve],
-      divTol, maxVcycles, noiseN, noiseSeed, device, useGraph: useGraph ? 1 : 0, adaptive: adaptive ? 1 : 0 });
+      divTol, maxVcycles, noiseN, noiseSeed, device, useGraph: useGraph ? 1 : 0, adaptive: adaptive ? 1 : 0, omega });
     const info = addon.info(this.h);
     this.res = info.res;
     this.size = size;

@@ -93,7 +93,7 @@ static int typed_data(napi_env env, napi_value v, void** data, size_t* bytes, na
 }
 
 /* create(opts): opts = {res, size, logs(Float32Array), vcycles, maxLevels, precision, solve,
- *                       divTol, maxVcycles, noiseN, noiseSeed, device, useGraph, adaptive} */
+ *                       divTol, maxVcycles, noiseN, noiseSeed, device, useGraph, adaptive, omega} */
 static napi_value js_create(napi_env env, napi_callback_info info) {
     size_t argc = 1;
     napi_value argv[1];
@@ -120,6 +120,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
             if (get_prop_num(env, argv[0], "device", &d)) cfg.device = (int)d;
             if (get_prop_num(env, argv[0], "useGraph", &d)) cfg.use_graph = (int)d;
             if (get_prop_num(env, argv[0], "adaptive", &d)) cfg.adaptive = (int)d;
+            if (get_prop_num(env, argv[0], "omega", &d)) cfg.omega = (float)d;
             bool has = false;
             napi_has_named_property(env, argv[0], "logs", &has);
             if (has) {

@@ -19,16 +19,17 @@ def _rhs(dims, seed=0):
     return b
 
 
+@pytest.mark.parametrize("omega", [1.0, 1.25])
 @pytest.mark.parametrize("dims", [(32, 48, 32), (64, 96, 64), (20, 30, 22), (21, 30, 22), (12, 18, 10)])
-def test_vcycles_bit_exact_vs_oracle(dims):
-    print(f"poisson vcycles {dims}", flush=True)
+def test_vcycles_bit_exact_vs_oracle(dims, omega):
+    print(f"poisson vcycles {dims} omega {omega}", flush=True)
     h = 1.0 / 128          # exactly representable: the device keeps the spacing as f32
     b = _rhs(dims)
-    ps = F.Poisson(dims, h)
+    ps = F.Poisson(dims, h, omega=omega)
     ps.set_rhs(b)
     ps.set_x(np.zeros_like(b))
     hist = ps.vcycles(4)
-    orc = O.OracleMG(dims, h, fp64=False)
+    orc = O.OracleMG(dims, h, fp64=False, omega=omega)
     orc.set(b=b, p=np.zeros_like(b))
     ohist = orc.vcycles(4)
     assert ps.info()[0] == orc.levels()[0]
@@ -36,14 +37,15 @@ def test_vcycles_bit_exact_vs_oracle(dims):
     assert np.allclose(hist, ohist, rtol=1e-9)
 
 
+@pytest.mark.parametrize("omega", [1.0, 1.25])
 @pytest.mark.parametrize("dims", [(32, 48, 32), (64, 96, 64), (24, 36, 20)])
-def test_refined_solve_reaches_closed_form(dims):
-    print(f"poisson refine {dims}", flush=True)
+def test_refined_solve_reaches_closed_form(dims, omega):
+    print(f"poisson refine {dims} omega {omega}", flush=True)
     h = 1.0 / 256
     b = _rhs(dims, 1)
     b -= b.mean()
     exact = _spectral_solve(b, h)
-    ps = F.Poisson(dims, h)
+    ps = F.Poisson(dims, h, omega=omega)
     ps.set_rhs(b)
     ps.set_x(np.zeros_like(b))
     hist = ps.solve(1e-10, 40)
@@ -51,7 +53,7 @@ def test_refined_solve_reaches_closed_form(dims):
     x = ps.read_x()
     assert np.max(np.abs(x - exact)) <= 1e-8 * np.max(np.abs(exact))
     # same cycle count and iterate as the oracle's FP32-in-FP64 refinement
-    orc = O.OracleMG(dims, h, fp64=False)
+    orc = O.OracleMG(dims, h, fp64=False, omega=omega)
     orc.set(b=b, p=np.zeros_like(b))
     ohist = orc.refine(1e-10, 40)
     assert len(ohist) == len(hist) - 1
