@@ -61,7 +61,7 @@ struct RefineState {            // device-resident FP64 refinement bookkeeping
     int used;                   // V-cycles applied in the current solve
     int done;                   // 1 once rel <= tol or used == max
     unsigned counter;           // blocks of the running residual kernel that have finished
-    int pad;
+    int never;                  // always 0: the skip flag of launches that run unconditionally
 };
 
 struct GpuMG {

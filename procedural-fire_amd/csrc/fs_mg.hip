@@ -28,7 +28,21 @@ namespace {
 
 constexpr int RB = 256;   // threads per block of every multigrid kernel (64 x 4)
 
-__device__ __forceinline__ bool skipped(const int* skip) { return skip != nullptr && *skip != 0; }
+
+// An index the compiler cannot see through.  Stencil loads pick an in-bounds neighbour
+// index with a select (the cell itself where the neighbour is outside); without this the
+// compiler folds "load(select(i, own))" back into a conditional load of i and waits on
+// each neighbour in turn instead of issuing all loads at once.
+__device__ __forceinline__ int opaque(int i) {
+    asm("" : "+v"(i));
+    return i;
+}
+
+// The skip test of a V-cycle kernel comes after its data loads: an empty volatile asm
+// that consumes the loaded values (and the flag) pins them before the branch, so the
+// flag's load and the data loads are in flight together (one memory latency, not two).
+// A plain memory clobber does not: loads through const __restrict__ pointers may move
+// across it.
 
 // grid over (half-row index or x, y, z) with 64 x 4 threads per block
 inline dim3 grid3(int w, int ny, int nz) { return dim3((unsigned)((w + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz); }
@@ -41,7 +55,7 @@ inline dim3 grid3(int w, int ny, int nz) { return dim3((unsigned)((w + 63) / 64)
 __global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
                                                float h, int color, const int* __restrict__ skip, double* __restrict__ xupd,
                                                float omega) {
-    if (skipped(skip)) return;
+    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments                             // in flight with the data loads; tested before any store
     const int nh = (nx + 1) >> 1, rs = 2 * nh;
     const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -52,32 +66,43 @@ __global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const floa
     const int row = (z * ny + y) * rs;
     const int own = row + sgn * nh + hx;
     const int oth = row + (1 - sgn) * nh;             // the other parity's half of this row
+    // every neighbour is loaded (an outside one from the cell itself, unused), so the
+    // loads issue together; the boundary rule then picks the same f32 operations
+    const float vw = p[opaque(x > 0 ? oth + hx - 1 + sgn : own)];
+    const float ve = p[opaque(x < nx - 1 ? oth + hx + sgn : own)];
+    const float vk = p[opaque(z > 0 ? own - ny * rs : own)];
+    const float vf = p[opaque(z < nz - 1 ? own + ny * rs : own)];
+    const float vs = p[opaque(y > 0 ? own - rs : own)];
+    const float vn = p[opaque(y < ny - 1 ? own + rs : own)];
+    const float bv = b[own];
+    const float po = omega != 1.f ? p[own] : 0.f;
+    const bool pair = xupd && 2 * hx + 1 - sgn < nx;  // the pair's other cell, final already
+    const float pother = pair ? p[oth + hx] : 0.f;
+    const double xo = xupd ? xupd[own] : 0.0;
+    const double xp = pair ? xupd[oth + hx] : 0.0;
+    float sum = 0.f, diag = 6.f;
+    if (x > 0) sum += vw; else diag += 1.f;
+    if (x < nx - 1) sum += ve; else diag += 1.f;
+    if (z > 0) sum += vk; else diag += 1.f;
+    if (z < nz - 1) sum += vf; else diag += 1.f;
+    if (y > 0) sum += vs; else diag -= 1.f;
+    if (y < ny - 1) sum += vn; else diag += 1.f;
+    float pn = (sum - h * h * bv) / diag;
+    if (omega != 1.f) pn = po + omega * (pn - po);
+    asm volatile("" ::"v"(pn), "v"(pother), "v"(xo), "v"(xp), "v"(sk));
+    if (sk) return;
     if (x < nx) {
-        float sum = 0.f, diag = 6.f;
-        if (x > 0) sum += p[oth + hx - 1 + sgn]; else diag += 1.f;
-        if (x < nx - 1) sum += p[oth + hx + sgn]; else diag += 1.f;
-        if (z > 0) sum += p[own - ny * rs]; else diag += 1.f;
-        if (z < nz - 1) sum += p[own + ny * rs]; else diag += 1.f;
-        if (y > 0) sum += p[own - rs]; else diag -= 1.f;
-        if (y < ny - 1) sum += p[own + rs]; else diag += 1.f;
-        float pn = (sum - h * h * b[own]) / diag;
-        if (omega != 1.f) {
-            const float po = p[own];
-            pn = po + omega * (pn - po);
-        }
         p[own] = pn;
-        if (xupd) xupd[own] = xupd[own] + (double)pn;
+        if (xupd) xupd[own] = xo + (double)pn;
     }
-    if (xupd && 2 * hx + 1 - sgn < nx) {                // the pair's other cell, final already
-        xupd[oth + hx] = xupd[oth + hx] + (double)p[oth + hx];
-    }
+    if (pair) xupd[oth + hx] = xp + (double)pother;
 }
 
 // residual of the 8 children averaged onto the coarse cell; coarse p = 0 (sim.js:446-466)
 __global__ __launch_bounds__(RB) void k_restrict(const float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
                                                  float h, float* __restrict__ bc, float* __restrict__ pc,
                                                  const int* __restrict__ skip) {
-    if (skipped(skip)) return;
+    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     const int cx = nx >> 1, cy = ny >> 1;
     const int xc = blockIdx.x * 64 + (threadIdx.x & 63);
     const int yc = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -85,24 +110,42 @@ __global__ __launch_bounds__(RB) void k_restrict(const float* __restrict__ p, co
     if (xc >= cx || yc >= cy) return;
     const int nh = nx >> 1, rs = nx;                   // levels that restrict have even nx
     const float ih2 = 1.f / (h * h);
-    float acc = 0.f;
+    // the 2x2x2 children and their 24 outer neighbours, loaded together (outside ones from
+    // the child itself, unused); then the ghost rule of sim.js:446-466 per child
+    float me[8], nb[8][6], bv[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int dx = k & 1, y = yc * 2 + ((k >> 1) & 1), z = zc * 2 + ((k >> 2) & 1);
         const int x = xc * 2 + dx;
         const int row = (z * ny + y) * rs;
         const int c = row + dx * nh + xc;
-        const float me = p[c];
         // x-1 / x+1 live in the other parity's half: (2xc)-1 = odd xc-1, (2xc+1)-1 = even xc, etc.
-        const float pxm = x > 0 ? p[row + (1 - dx) * nh + xc - 1 + dx] : -me;
-        const float pxp = x < nx - 1 ? p[row + (1 - dx) * nh + xc + dx] : -me;
-        const float pym = y > 0 ? p[c - rs] : me;
-        const float pyp = y < ny - 1 ? p[c + rs] : -me;
-        const float pzm = z > 0 ? p[c - ny * rs] : -me;
-        const float pzp = z < nz - 1 ? p[c + ny * rs] : -me;
-        const float lap = (pxm + pxp + pym + pyp + pzm + pzp - 6.f * me) * ih2;
-        acc += b[c] - lap;
+        me[k] = p[c];
+        nb[k][0] = p[opaque(x > 0 ? row + (1 - dx) * nh + xc - 1 + dx : c)];
+        nb[k][1] = p[opaque(x < nx - 1 ? row + (1 - dx) * nh + xc + dx : c)];
+        nb[k][2] = p[opaque(y > 0 ? c - rs : c)];
+        nb[k][3] = p[opaque(y < ny - 1 ? c + rs : c)];
+        nb[k][4] = p[opaque(z > 0 ? c - ny * rs : c)];
+        nb[k][5] = p[opaque(z < nz - 1 ? c + ny * rs : c)];
+        bv[k] = b[c];
     }
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int dx = k & 1, y = yc * 2 + ((k >> 1) & 1), z = zc * 2 + ((k >> 2) & 1);
+        const int x = xc * 2 + dx;
+        const float m = me[k];
+        const float pxm = x > 0 ? nb[k][0] : -m;
+        const float pxp = x < nx - 1 ? nb[k][1] : -m;
+        const float pym = y > 0 ? nb[k][2] : m;
+        const float pyp = y < ny - 1 ? nb[k][3] : -m;
+        const float pzm = z > 0 ? nb[k][4] : -m;
+        const float pzp = z < nz - 1 ? nb[k][5] : -m;
+        const float lap = (pxm + pxp + pym + pyp + pzm + pzp - 6.f * m) * ih2;
+        acc += bv[k] - lap;
+    }
+    asm volatile("" ::"v"(acc), "v"(sk));
+    if (sk) return;
     const int nhc = (cx + 1) >> 1;
     const int ci = (zc * cy + yc) * 2 * nhc + (xc & 1) * nhc + (xc >> 1);
     bc[ci] = acc * 0.125f;
@@ -139,7 +182,7 @@ __device__ __forceinline__ float prolong_at(const float* __restrict__ pc, int x,
 // loads are shared by 8 fine cells and every child class is a coalesced half-row.
 __global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const float* __restrict__ pcoarse, int nx, int ny, int nz,
                                                 const int* __restrict__ skip) {
-    if (skipped(skip)) return;
+    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;    // levels that prolong have even dims
     const int xc = blockIdx.x * 64 + (threadIdx.x & 63);
     const int yc = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -165,6 +208,7 @@ __global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const flo
         }
     }
     const int nh = nx >> 1, rs = nx;
+    float fine[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int a = k & 1, bb = (k >> 1) & 1, cc = (k >> 2) & 1;   // child offsets in x, y, z
@@ -176,8 +220,15 @@ __global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const flo
             const float wx = ox ? fx : 1.f - fx, wy = oy ? fy : 1.f - fy, wz = oz ? fz : 1.f - fz;
             acc += c[cc + oz][bb + oy][a + ox] * wx * wy * wz;
         }
-        const int i = ((2 * zc + cc) * ny + 2 * yc + bb) * rs + a * nh + xc;
-        p[i] = p[i] + acc;
+        fine[k] = p[((2 * zc + cc) * ny + 2 * yc + bb) * rs + a * nh + xc] + acc;
+    }
+    asm volatile("" ::"v"(fine[0]), "v"(fine[1]), "v"(fine[2]), "v"(fine[3]), "v"(fine[4]), "v"(fine[5]), "v"(fine[6]),
+                 "v"(fine[7]), "v"(sk));
+    if (sk) return;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int a = k & 1, bb = (k >> 1) & 1, cc = (k >> 2) & 1;
+        p[((2 * zc + cc) * ny + 2 * yc + bb) * rs + a * nh + xc] = fine[k];
     }
 }
 
@@ -190,7 +241,7 @@ __global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const flo
 __global__ __launch_bounds__(RB) void k_smooth_prolong(float* __restrict__ p, const float* __restrict__ b,
                                                        const float* __restrict__ pcoarse, int nx, int ny, int nz, float h,
                                                        const int* __restrict__ skip) {
-    if (skipped(skip)) return;
+    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     const int nh = nx >> 1, rs = nx;                   // levels that prolong have even nx
     const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -202,14 +253,26 @@ __global__ __launch_bounds__(RB) void k_smooth_prolong(float* __restrict__ p, co
     const int row = (z * ny + y) * rs;
     const int own = row + sgn * nh + hx;
     const int oth = row + (1 - sgn) * nh;
+    // all six neighbours loaded (outside ones from the cell itself, unused; prolong_at
+    // clamps its coarse reads for any coordinate), then the boundary rule
+    const float vw = p[opaque(x > 0 ? oth + hx - 1 + sgn : own)] + prolong_at(pcoarse, x - 1, y, z, cx, cy, cz);
+    const float ve = p[opaque(x < nx - 1 ? oth + hx + sgn : own)] + prolong_at(pcoarse, x + 1, y, z, cx, cy, cz);
+    const float vk = p[opaque(z > 0 ? own - ny * rs : own)] + prolong_at(pcoarse, x, y, z - 1, cx, cy, cz);
+    const float vf = p[opaque(z < nz - 1 ? own + ny * rs : own)] + prolong_at(pcoarse, x, y, z + 1, cx, cy, cz);
+    const float vs = p[opaque(y > 0 ? own - rs : own)] + prolong_at(pcoarse, x, y - 1, z, cx, cy, cz);
+    const float vn = p[opaque(y < ny - 1 ? own + rs : own)] + prolong_at(pcoarse, x, y + 1, z, cx, cy, cz);
+    const float bv = b[own];
     float sum = 0.f, diag = 6.f;
-    if (x > 0) sum += p[oth + hx - 1 + sgn] + prolong_at(pcoarse, x - 1, y, z, cx, cy, cz); else diag += 1.f;
-    if (x < nx - 1) sum += p[oth + hx + sgn] + prolong_at(pcoarse, x + 1, y, z, cx, cy, cz); else diag += 1.f;
-    if (z > 0) sum += p[own - ny * rs] + prolong_at(pcoarse, x, y, z - 1, cx, cy, cz); else diag += 1.f;
-    if (z < nz - 1) sum += p[own + ny * rs] + prolong_at(pcoarse, x, y, z + 1, cx, cy, cz); else diag += 1.f;
-    if (y > 0) sum += p[own - rs] + prolong_at(pcoarse, x, y - 1, z, cx, cy, cz); else diag -= 1.f;
-    if (y < ny - 1) sum += p[own + rs] + prolong_at(pcoarse, x, y + 1, z, cx, cy, cz); else diag += 1.f;
-    p[own] = (sum - h * h * b[own]) / diag;
+    if (x > 0) sum += vw; else diag += 1.f;
+    if (x < nx - 1) sum += ve; else diag += 1.f;
+    if (z > 0) sum += vk; else diag += 1.f;
+    if (z < nz - 1) sum += vf; else diag += 1.f;
+    if (y > 0) sum += vs; else diag -= 1.f;
+    if (y < ny - 1) sum += vn; else diag += 1.f;
+    const float pn = (sum - h * h * bv) / diag;
+    asm volatile("" ::"v"(pn), "v"(sk));
+    if (sk) return;
+    p[own] = pn;
 }
 
 // Sums the per-block partials in a fixed order.  `init`: the starting residual of a
@@ -219,7 +282,10 @@ __global__ __launch_bounds__(RB) void k_smooth_prolong(float* __restrict__ p, co
 __device__ void finalize_sums(RefineState* st, const double* partials, int npart, double tol, int max_cycles, int init) {
     __shared__ double sr[RB], sb[RB];
     double rr = 0.0, bb = 0.0;
+    const int done = init ? 0 : st->done;              // in flight with the partials' loads
     for (int j = threadIdx.x; j < npart; j += RB) { rr += partials[2 * j]; bb += partials[2 * j + 1]; }
+    asm volatile("" ::"v"(rr), "v"(bb), "s"(done));
+    if (done) return;                                  // uniform over the block
     sr[threadIdx.x] = rr; sb[threadIdx.x] = bb;
     __syncthreads();
     for (int s = RB / 2; s > 0; s >>= 1) {
@@ -243,14 +309,20 @@ __device__ __forceinline__ double resid_at(const X* __restrict__ x, const float*
     const int row = (z * ny + y) * rs;
     const int own = row + sgn * nh + hx, oth = row + (1 - sgn) * nh;
     const double me = (double)x[own];
-    double sum = 0.0, diag = 6.0;
-    if (xx > 0) sum += (double)x[oth + hx - 1 + sgn]; else diag += 1.0;
-    if (xx < nx - 1) sum += (double)x[oth + hx + sgn]; else diag += 1.0;
-    if (z > 0) sum += (double)x[own - ny * rs]; else diag += 1.0;
-    if (z < nz - 1) sum += (double)x[own + ny * rs]; else diag += 1.0;
-    if (y > 0) sum += (double)x[own - rs]; else diag -= 1.0;
-    if (y < ny - 1) sum += (double)x[own + rs]; else diag += 1.0;
+    const double vw = (double)x[opaque(xx > 0 ? oth + hx - 1 + sgn : own)];
+    const double ve = (double)x[opaque(xx < nx - 1 ? oth + hx + sgn : own)];
+    const double vk = (double)x[opaque(z > 0 ? own - ny * rs : own)];
+    const double vf = (double)x[opaque(z < nz - 1 ? own + ny * rs : own)];
+    const double vs = (double)x[opaque(y > 0 ? own - rs : own)];
+    const double vn = (double)x[opaque(y < ny - 1 ? own + rs : own)];
     *bv = (double)b[own];
+    double sum = 0.0, diag = 6.0;
+    if (xx > 0) sum += vw; else diag += 1.0;
+    if (xx < nx - 1) sum += ve; else diag += 1.0;
+    if (z > 0) sum += vk; else diag += 1.0;
+    if (z < nz - 1) sum += vf; else diag += 1.0;
+    if (y > 0) sum += vs; else diag -= 1.0;
+    if (y < ny - 1) sum += vn; else diag += 1.0;
     return *bv - (sum - diag * me) * ih2;
 }
 
@@ -261,7 +333,6 @@ template <class X>
 __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const float* __restrict__ b, int nx, int ny, int nz,
                                                 double h, float* __restrict__ rhs_out, float* __restrict__ p_out,
                                                 double* __restrict__ partials, const int* __restrict__ skip) {
-    if (skipped(skip)) return;
     __shared__ double sr[RB], sb[RB];
     const int nh = (nx + 1) >> 1, rs = 2 * nh;
     const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -269,17 +340,25 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
     const int z = blockIdx.z;
     const double ih2 = 1.0 / (h * h);
     double rr = 0.0, bb = 0.0;
-    if (hx < nh && y < ny) {
+    const bool inside = hx < nh && y < ny;
+    double r[2] = {0.0, 0.0}, bv[2] = {0.0, 0.0};
+    if (inside) {
+        // both cells of the pair (the odd one of a padded row slot is computed and dropped)
+#pragma unroll
+        for (int sgn = 0; sgn < 2; sgn++) r[sgn] = resid_at(x, b, 2 * hx + sgn, y, z, sgn, hx, nx, ny, nz, nh, rs, ih2, &bv[sgn]);
+    }
+    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
+    asm volatile("" ::"v"(r[0]), "v"(r[1]), "v"(bv[0]), "v"(bv[1]), "v"(sk));
+    if (sk) return;                                    // uniform over the grid
+    if (inside) {
+#pragma unroll
         for (int sgn = 0; sgn < 2; sgn++) {
-            const int xx = 2 * hx + sgn;
-            if (xx >= nx) continue;
-            double bv;
-            const double r = resid_at(x, b, xx, y, z, sgn, hx, nx, ny, nz, nh, rs, ih2, &bv);
-            rr += r * r;
-            bb += bv * bv;
+            if (2 * hx + sgn >= nx) continue;
+            rr += r[sgn] * r[sgn];
+            bb += bv[sgn] * bv[sgn];
             if (rhs_out) {
                 const int own = (z * ny + y) * rs + sgn * nh + hx;
-                rhs_out[own] = (float)r;
+                rhs_out[own] = (float)r[sgn];
                 p_out[own] = 0.f;
             }
         }
@@ -300,16 +379,20 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
 // one block: the fixed-order reduction of the partials and the stopping test
 __global__ __launch_bounds__(RB) void k_finalize(RefineState* __restrict__ st, const double* __restrict__ partials, int npart,
                                                  double tol, int max_cycles, int init) {
-    if (!init && st->done) return;
     finalize_sums(st, partials, npart, tol, max_cycles, init);
 }
 
 // x += e (the V-cycle's correction), FP64, over the whole parity-split array
 __global__ __launch_bounds__(RB) void k_update(double* __restrict__ x, const float* __restrict__ e, int n,
                                                const int* __restrict__ skip) {
-    if (skipped(skip)) return;
+    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     const int i = blockIdx.x * RB + threadIdx.x;
-    if (i < n) x[i] = x[i] + (double)e[i];
+    if (i >= n) return;
+    const double xv = x[i];
+    const float ev = e[i];
+    asm volatile("" ::"v"(xv), "v"(ev), "v"(sk));
+    if (sk) return;
+    x[i] = xv + (double)ev;
 }
 
 // ---- coarse tail: every level from `tail_from` down, one workgroup, all in LDS ----
@@ -357,14 +440,18 @@ __device__ __forceinline__ void tail_smooth(float* p, const float* b, int nx, in
                 const int x = hx * 2 + ((y + z + color) & 1);
                 if (x < nx) {
                     const int c = (z * ny + y) * nx + x;
+                    const float vw = p[opaque(x > 0 ? c - 1 : c)], ve = p[opaque(x < nx - 1 ? c + 1 : c)];
+                    const float vk = p[opaque(z > 0 ? c - sz : c)], vf = p[opaque(z < nz - 1 ? c + sz : c)];
+                    const float vs = p[opaque(y > 0 ? c - sy : c)], vn = p[opaque(y < ny - 1 ? c + sy : c)];
+                    const float bv = b[c];
                     float sum = 0.f, diag = 6.f;
-                    if (x > 0) sum += p[c - 1]; else diag += 1.f;
-                    if (x < nx - 1) sum += p[c + 1]; else diag += 1.f;
-                    if (z > 0) sum += p[c - sz]; else diag += 1.f;
-                    if (z < nz - 1) sum += p[c + sz]; else diag += 1.f;
-                    if (y > 0) sum += p[c - sy]; else diag -= 1.f;
-                    if (y < ny - 1) sum += p[c + sy]; else diag += 1.f;
-                    float pn = (sum - h * h * b[c]) / diag;
+                    if (x > 0) sum += vw; else diag += 1.f;
+                    if (x < nx - 1) sum += ve; else diag += 1.f;
+                    if (z > 0) sum += vk; else diag += 1.f;
+                    if (z < nz - 1) sum += vf; else diag += 1.f;
+                    if (y > 0) sum += vs; else diag -= 1.f;
+                    if (y < ny - 1) sum += vn; else diag += 1.f;
+                    float pn = (sum - h * h * bv) / diag;
                     if (omega != 1.f) pn = p[c] + omega * (pn - p[c]);
                     p[c] = pn;
                 }
@@ -385,12 +472,15 @@ __device__ __forceinline__ void tail_restrict(const float* p, const float* b, in
             const int x = xc * 2 + (k & 1), y = yc * 2 + ((k >> 1) & 1), z = zc * 2 + ((k >> 2) & 1);
             const int c = (z * ny + y) * nx + x;
             const float me = p[c];
-            const float pxm = x > 0 ? p[c - 1] : -me;
-            const float pxp = x < nx - 1 ? p[c + 1] : -me;
-            const float pym = y > 0 ? p[c - sy] : me;
-            const float pyp = y < ny - 1 ? p[c + sy] : -me;
-            const float pzm = z > 0 ? p[c - sz] : -me;
-            const float pzp = z < nz - 1 ? p[c + sz] : -me;
+            const float vxm = p[opaque(x > 0 ? c - 1 : c)], vxp = p[opaque(x < nx - 1 ? c + 1 : c)];
+            const float vym = p[opaque(y > 0 ? c - sy : c)], vyp = p[opaque(y < ny - 1 ? c + sy : c)];
+            const float vzm = p[opaque(z > 0 ? c - sz : c)], vzp = p[opaque(z < nz - 1 ? c + sz : c)];
+            const float pxm = x > 0 ? vxm : -me;
+            const float pxp = x < nx - 1 ? vxp : -me;
+            const float pym = y > 0 ? vym : me;
+            const float pyp = y < ny - 1 ? vyp : -me;
+            const float pzm = z > 0 ? vzm : -me;
+            const float pzp = z < nz - 1 ? vzp : -me;
             const float lap = (pxm + pxp + pym + pyp + pzm + pzp - 6.f * me) * ih2;
             acc += b[c] - lap;
         }
@@ -426,7 +516,7 @@ __device__ __forceinline__ void tail_prolong(float* p, const float* pcoarse, int
 }
 
 __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailArgs a, const int* __restrict__ skip) {
-    if (skipped(skip)) return;
+    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     extern __shared__ float lds[];
     const int n0 = a.dims[0][0] * a.dims[0][1] * a.dims[0][2];
     float* p0 = lds + a.off[0];
@@ -441,6 +531,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailArgs a, const int*
         b0[i] = a.b_top[g];
     }
     (void)ny0;
+    if (sk) return;                                    // uniform over the workgroup
     __syncthreads();
     for (int j = 0; j < a.nt - 1; j++) {
         const int nx = a.dims[j][0], ny = a.dims[j][1], nz = a.dims[j][2], n = nx * ny * nz;
@@ -538,6 +629,7 @@ void mg_free(GpuMG& mg) {
 }
 
 void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, double* xupd) {   // sim.js:576-586
+    if (!skip) skip = &mg.st->never;
     const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
     const dim3 g = grid3((nx + 1) / 2, ny, nz);
     const float om = l == mg.nlev - 1 ? 1.f : mg.omega;   // the coarsest level's sweeps stay plain GS
@@ -581,6 +673,7 @@ static void launch_tail(GpuMG& mg, hipStream_t s, const int* skip) {
 }
 
 bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd) {   // sim.js:588-599
+    if (!skip) skip = &mg.st->never;
     const int n = mg.nlev;
     const int T = mg.use_tail ? mg.tail_from : n;      // levels T.. run in the single-workgroup tail
     if (T == 0) {                                       // the whole hierarchy fits one workgroup's LDS
@@ -616,7 +709,7 @@ void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
-                                                                  mg.partials, nullptr);
+                                                                  mg.partials, &mg.st->never);
     k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
 }
 
@@ -643,7 +736,7 @@ void mg_tol32_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     k_resid64<float><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr,
-                                                                 mg.partials, nullptr);
+                                                                 mg.partials, &mg.st->never);
     k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
 }
 
@@ -666,6 +759,6 @@ void mg_solve_tol32(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
 void mg_residual_of_p(GpuMG& mg, hipStream_t s) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     k_resid64<float><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr,
-                                                                 mg.partials, nullptr);
+                                                                 mg.partials, &mg.st->never);
     k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, 0.0, 0, 1);
 }
