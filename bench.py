@@ -134,6 +134,65 @@ def cpu_baseline(gpu_sim, steps=3):
                       f"(fp16 storage, f32 math, 2 V(2,2)) from the warmed GPU state; {med:.2f} s/step"}
 
 
+def errors_vs_fp64(warmup):
+    """Per-field relative L2 of the mixed and plain FP32 GPU steps against the FP64 oracle
+    (float64 storage and math, pressure converged to 1e-10), after 1 and 10 steps from the
+    GPU's warmed state; the FP64 oracle is the checker, run on this host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import fsoracle as O
+
+    def temp(a):
+        return 293.0 + 1000.0 * np.asarray(a, np.float64)[..., 0]
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
+        return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+    base = make_sim(F.PREC_FP16, F.SOLVE_REFINE64, False)
+    base.step(DT, warmup)
+    base.sync()
+    state = {f: base.read(f, np.float64) for f in (F.VEL, F.S, F.S2, F.VORT, F.GEO, F.VELX, F.SX, F.S2X, F.VELY,
+                                                     F.SY, F.S2Y, F.P)}
+    t0, fr0 = base.time, base.frame
+    base.close()
+    out = {}
+    for name, prec, solve in (("mixed", F.PREC_FP16, F.SOLVE_REFINE64), ("fp32", F.PREC_FP32, F.SOLVE_VCYCLES)):
+        g = make_sim(prec, solve, False)
+        for f, a in state.items():
+            g.write(f, a)
+        g.step(DT, warmup)            # advances the clock to the warmed time; the fields are rewritten next
+        g.sync()
+        for f, a in state.items():
+            g.write(f, a)
+        orc = O.OracleSim(res=RES, size=scenes.scene_size(SCENE), logs=np.zeros(8, np.float32), n_logs=0,
+                          precision=O.FP64, vcycles=2, solve=O.SOLVE_REFINE64, div_tol=1e-10, max_vcycles=40,
+                          noise_n=128)
+        orc.set_params(**scenes.scene_params(SCENE))
+        orc.set_spots(scenes.scene_spots(SCENE, 7))
+        for f, a in state.items():
+            orc.write(f, a)
+        orc.write(F.NOISE, g.read(F.NOISE, np.float64))
+        orc.write(F.CURL, g.read(F.CURL, np.float64))
+        orc.set_clock(t0, fr0)
+        assert abs(g.time - t0) < 1e-12
+        res = {}
+        for done, n in ((1, 1), (10, 9)):
+            for _ in range(n):
+                g.step(DT, 1)
+                orc.step(DT, 1)
+            g.sync()
+            e = {k: rel(g.read(f, np.float64)[..., :3], orc.read(f)[..., :3])
+                 for k, f in (("vel", F.VEL), ("S", F.S), ("S2", F.S2))}
+            e["T"] = rel(temp(g.read(F.S, np.float64)), temp(orc.read(F.S)))
+            e["theta"] = rel(g.read(F.S, np.float64)[..., 0], orc.read(F.S)[..., 0])   # T - 293 K
+            e["residual"] = g.residual()[0] if solve != F.SOLVE_VCYCLES else None
+            res[str(done)] = e
+            print(f"{name} after {done}: {e}", flush=True)
+        out[name] = res
+        g.close()
+    print(json.dumps({"errors_vs_fp64": out, "workload": SCENE, "grid": list(RES), "warmup": warmup}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +201,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--scene", default="spots8_128", choices=["spots8_128", "campfire"])
+    ap.add_argument("--errors", action="store_true",
+                    help="instead of timing: per-field error of the mixed and FP32 steps vs the FP64 oracle "
+                         "after 1 and 10 steps")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB copy, then 3 steps of each step form")
     args = ap.parse_args()
@@ -152,6 +214,9 @@ def main():
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.errors:
+        errors_vs_fp64(args.warmup)
+        return
     ncell = float(np.prod(RES))
     graph = not args.no_graph
     if args.probe:

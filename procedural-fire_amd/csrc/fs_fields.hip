@@ -549,19 +549,28 @@ __global__ __launch_bounds__(256) void k_project(const DevParams* __restrict__ P
     Cell c;
     if (!cell_of(g, c)) return;
     const float dx = P->f[P_dx];
-    // p is in the multigrid's parity-split layout
-    const float me = (float)p[rb_index(c.x, c.y, c.z, g.nx, g.ny)];
-    // ghost rules: -p outside x, z and above the top (Dirichlet), +p below the floor (Neumann)
-    const float pxp = c.x + 1 < g.nx ? (float)p[rb_index(c.x + 1, c.y, c.z, g.nx, g.ny)] : -me;
-    const float pxm = c.x > 0 ? (float)p[rb_index(c.x - 1, c.y, c.z, g.nx, g.ny)] : -me;
-    const float pyp = c.y + 1 < g.ny ? (float)p[rb_index(c.x, c.y + 1, c.z, g.nx, g.ny)] : -me;
-    const float pym = c.y > 0 ? (float)p[rb_index(c.x, c.y - 1, c.z, g.nx, g.ny)] : me;
-    const float pzp = c.z + 1 < g.nz ? (float)p[rb_index(c.x, c.y, c.z + 1, g.nx, g.ny)] : -me;
-    const float pzm = c.z > 0 ? (float)p[rb_index(c.x, c.y, c.z - 1, g.nx, g.ny)] : -me;
-    const float gx = (pxp - pxm) / (2.f * dx), gy = (pyp - pym) / (2.f * dx), gz = (pzp - pzm) / (2.f * dx);
+    // p is in the multigrid's parity-split layout; all seven values are loaded (an outside
+    // neighbour from the cell itself, unused) so the loads issue together
+    const int io = rb_index(c.x, c.y, c.z, g.nx, g.ny);
+    const float me = (float)p[io];
+    const float vxp = (float)p[fs_opaque(c.x + 1 < g.nx ? rb_index(c.x + 1, c.y, c.z, g.nx, g.ny) : io)];
+    const float vxm = (float)p[fs_opaque(c.x > 0 ? rb_index(c.x - 1, c.y, c.z, g.nx, g.ny) : io)];
+    const float vyp = (float)p[fs_opaque(c.y + 1 < g.ny ? rb_index(c.x, c.y + 1, c.z, g.nx, g.ny) : io)];
+    const float vym = (float)p[fs_opaque(c.y > 0 ? rb_index(c.x, c.y - 1, c.z, g.nx, g.ny) : io)];
+    const float vzp = (float)p[fs_opaque(c.z + 1 < g.nz ? rb_index(c.x, c.y, c.z + 1, g.nx, g.ny) : io)];
+    const float vzm = (float)p[fs_opaque(c.z > 0 ? rb_index(c.x, c.y, c.z - 1, g.nx, g.ny) : io)];
     const float4 vi = Codec<S>::ld(vIn, c.i);
+    const float solid = Codec<S>::ld(geo, c.i).z;
+    // ghost rules: -p outside x, z and above the top (Dirichlet), +p below the floor (Neumann)
+    const float pxp = c.x + 1 < g.nx ? vxp : -me;
+    const float pxm = c.x > 0 ? vxm : -me;
+    const float pyp = c.y + 1 < g.ny ? vyp : -me;
+    const float pym = c.y > 0 ? vym : me;
+    const float pzp = c.z + 1 < g.nz ? vzp : -me;
+    const float pzm = c.z > 0 ? vzm : -me;
+    const float gx = (pxp - pxm) / (2.f * dx), gy = (pyp - pym) / (2.f * dx), gz = (pzp - pzm) / (2.f * dx);
     float vx = vi.x - gx, vy = vi.y - gy, vz = vi.z - gz;
-    if (Codec<S>::ld(geo, c.i).z > 0.5f) { vx = 0.f; vy = 0.f; vz = 0.f; }
+    if (solid > 0.5f) { vx = 0.f; vy = 0.f; vz = 0.f; }
     if (c.y == 0) vy = fmaxf(vy, 0.f);
     const float vl = sqrtf(vx * vx + vy * vy + vz * vz);
     if (vl > 20.f || vl != vl) {

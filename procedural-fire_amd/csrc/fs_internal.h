@@ -89,6 +89,15 @@ __host__ __device__ inline int rb_index(int x, int y, int z, int nx, int ny) {
     const int nh = (nx + 1) >> 1;
     return (z * ny + y) * 2 * nh + (x & 1) * nh + (x >> 1);
 }
+// An index the compiler cannot see through.  Stencil loads pick an in-bounds neighbour
+// index with a select (the cell itself where the neighbour is outside); without this the
+// compiler folds "load(select(i, own))" back into a conditional load of i and waits on
+// each neighbour in turn instead of issuing all loads at once.
+__device__ __forceinline__ int fs_opaque(int i) {
+    asm("" : "+v"(i));
+    return i;
+}
+
 int  mg_build(GpuMG& mg, int nx, int ny, int nz, float h, int max_levels, std::string& err);
 size_t mg_level_floats(const GpuMG& mg, int l);   // array length of level l (with row padding)
 void mg_free(GpuMG& mg);

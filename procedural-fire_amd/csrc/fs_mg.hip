@@ -29,15 +29,6 @@ namespace {
 constexpr int RB = 256;   // threads per block of every multigrid kernel (64 x 4)
 
 
-// An index the compiler cannot see through.  Stencil loads pick an in-bounds neighbour
-// index with a select (the cell itself where the neighbour is outside); without this the
-// compiler folds "load(select(i, own))" back into a conditional load of i and waits on
-// each neighbour in turn instead of issuing all loads at once.
-__device__ __forceinline__ int opaque(int i) {
-    asm("" : "+v"(i));
-    return i;
-}
-
 // The skip test of a V-cycle kernel comes after its data loads: an empty volatile asm
 // that consumes the loaded values (and the flag) pins them before the branch, so the
 // flag's load and the data loads are in flight together (one memory latency, not two).
@@ -55,7 +46,8 @@ inline dim3 grid3(int w, int ny, int nz) { return dim3((unsigned)((w + 63) / 64)
 __global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
                                                float h, int color, const int* __restrict__ skip, double* __restrict__ xupd,
                                                float omega) {
-    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments                             // in flight with the data loads; tested before any store
+    const int sk = skip[fs_opaque(0)];   // in flight with the data loads (a vector load: its wait does not hold up the
+                                         // scalar kernel arguments); tested before any store
     const int nh = (nx + 1) >> 1, rs = 2 * nh;
     const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -68,12 +60,12 @@ __global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const floa
     const int oth = row + (1 - sgn) * nh;             // the other parity's half of this row
     // every neighbour is loaded (an outside one from the cell itself, unused), so the
     // loads issue together; the boundary rule then picks the same f32 operations
-    const float vw = p[opaque(x > 0 ? oth + hx - 1 + sgn : own)];
-    const float ve = p[opaque(x < nx - 1 ? oth + hx + sgn : own)];
-    const float vk = p[opaque(z > 0 ? own - ny * rs : own)];
-    const float vf = p[opaque(z < nz - 1 ? own + ny * rs : own)];
-    const float vs = p[opaque(y > 0 ? own - rs : own)];
-    const float vn = p[opaque(y < ny - 1 ? own + rs : own)];
+    const float vw = p[fs_opaque(x > 0 ? oth + hx - 1 + sgn : own)];
+    const float ve = p[fs_opaque(x < nx - 1 ? oth + hx + sgn : own)];
+    const float vk = p[fs_opaque(z > 0 ? own - ny * rs : own)];
+    const float vf = p[fs_opaque(z < nz - 1 ? own + ny * rs : own)];
+    const float vs = p[fs_opaque(y > 0 ? own - rs : own)];
+    const float vn = p[fs_opaque(y < ny - 1 ? own + rs : own)];
     const float bv = b[own];
     const float po = omega != 1.f ? p[own] : 0.f;
     const bool pair = xupd && 2 * hx + 1 - sgn < nx;  // the pair's other cell, final already
@@ -102,7 +94,7 @@ __global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const floa
 __global__ __launch_bounds__(RB) void k_restrict(const float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
                                                  float h, float* __restrict__ bc, float* __restrict__ pc,
                                                  const int* __restrict__ skip) {
-    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
+    const int sk = skip[fs_opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     const int cx = nx >> 1, cy = ny >> 1;
     const int xc = blockIdx.x * 64 + (threadIdx.x & 63);
     const int yc = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -121,12 +113,12 @@ __global__ __launch_bounds__(RB) void k_restrict(const float* __restrict__ p, co
         const int c = row + dx * nh + xc;
         // x-1 / x+1 live in the other parity's half: (2xc)-1 = odd xc-1, (2xc+1)-1 = even xc, etc.
         me[k] = p[c];
-        nb[k][0] = p[opaque(x > 0 ? row + (1 - dx) * nh + xc - 1 + dx : c)];
-        nb[k][1] = p[opaque(x < nx - 1 ? row + (1 - dx) * nh + xc + dx : c)];
-        nb[k][2] = p[opaque(y > 0 ? c - rs : c)];
-        nb[k][3] = p[opaque(y < ny - 1 ? c + rs : c)];
-        nb[k][4] = p[opaque(z > 0 ? c - ny * rs : c)];
-        nb[k][5] = p[opaque(z < nz - 1 ? c + ny * rs : c)];
+        nb[k][0] = p[fs_opaque(x > 0 ? row + (1 - dx) * nh + xc - 1 + dx : c)];
+        nb[k][1] = p[fs_opaque(x < nx - 1 ? row + (1 - dx) * nh + xc + dx : c)];
+        nb[k][2] = p[fs_opaque(y > 0 ? c - rs : c)];
+        nb[k][3] = p[fs_opaque(y < ny - 1 ? c + rs : c)];
+        nb[k][4] = p[fs_opaque(z > 0 ? c - ny * rs : c)];
+        nb[k][5] = p[fs_opaque(z < nz - 1 ? c + ny * rs : c)];
         bv[k] = b[c];
     }
     float acc = 0.f;
@@ -182,7 +174,7 @@ __device__ __forceinline__ float prolong_at(const float* __restrict__ pc, int x,
 // loads are shared by 8 fine cells and every child class is a coalesced half-row.
 __global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const float* __restrict__ pcoarse, int nx, int ny, int nz,
                                                 const int* __restrict__ skip) {
-    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
+    const int sk = skip[fs_opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;    // levels that prolong have even dims
     const int xc = blockIdx.x * 64 + (threadIdx.x & 63);
     const int yc = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -241,7 +233,7 @@ __global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const flo
 __global__ __launch_bounds__(RB) void k_smooth_prolong(float* __restrict__ p, const float* __restrict__ b,
                                                        const float* __restrict__ pcoarse, int nx, int ny, int nz, float h,
                                                        const int* __restrict__ skip) {
-    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
+    const int sk = skip[fs_opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     const int nh = nx >> 1, rs = nx;                   // levels that prolong have even nx
     const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -255,12 +247,12 @@ __global__ __launch_bounds__(RB) void k_smooth_prolong(float* __restrict__ p, co
     const int oth = row + (1 - sgn) * nh;
     // all six neighbours loaded (outside ones from the cell itself, unused; prolong_at
     // clamps its coarse reads for any coordinate), then the boundary rule
-    const float vw = p[opaque(x > 0 ? oth + hx - 1 + sgn : own)] + prolong_at(pcoarse, x - 1, y, z, cx, cy, cz);
-    const float ve = p[opaque(x < nx - 1 ? oth + hx + sgn : own)] + prolong_at(pcoarse, x + 1, y, z, cx, cy, cz);
-    const float vk = p[opaque(z > 0 ? own - ny * rs : own)] + prolong_at(pcoarse, x, y, z - 1, cx, cy, cz);
-    const float vf = p[opaque(z < nz - 1 ? own + ny * rs : own)] + prolong_at(pcoarse, x, y, z + 1, cx, cy, cz);
-    const float vs = p[opaque(y > 0 ? own - rs : own)] + prolong_at(pcoarse, x, y - 1, z, cx, cy, cz);
-    const float vn = p[opaque(y < ny - 1 ? own + rs : own)] + prolong_at(pcoarse, x, y + 1, z, cx, cy, cz);
+    const float vw = p[fs_opaque(x > 0 ? oth + hx - 1 + sgn : own)] + prolong_at(pcoarse, x - 1, y, z, cx, cy, cz);
+    const float ve = p[fs_opaque(x < nx - 1 ? oth + hx + sgn : own)] + prolong_at(pcoarse, x + 1, y, z, cx, cy, cz);
+    const float vk = p[fs_opaque(z > 0 ? own - ny * rs : own)] + prolong_at(pcoarse, x, y, z - 1, cx, cy, cz);
+    const float vf = p[fs_opaque(z < nz - 1 ? own + ny * rs : own)] + prolong_at(pcoarse, x, y, z + 1, cx, cy, cz);
+    const float vs = p[fs_opaque(y > 0 ? own - rs : own)] + prolong_at(pcoarse, x, y - 1, z, cx, cy, cz);
+    const float vn = p[fs_opaque(y < ny - 1 ? own + rs : own)] + prolong_at(pcoarse, x, y + 1, z, cx, cy, cz);
     const float bv = b[own];
     float sum = 0.f, diag = 6.f;
     if (x > 0) sum += vw; else diag += 1.f;
@@ -283,7 +275,16 @@ __device__ void finalize_sums(RefineState* st, const double* partials, int npart
     __shared__ double sr[RB], sb[RB];
     double rr = 0.0, bb = 0.0;
     const int done = init ? 0 : st->done;              // in flight with the partials' loads
-    for (int j = threadIdx.x; j < npart; j += RB) { rr += partials[2 * j]; bb += partials[2 * j + 1]; }
+    // each thread sums partials j, j + RB, j + 2 RB, ... in that order, loading 8 at a time
+    int j = threadIdx.x;
+    for (; j + 7 * RB < npart; j += 8 * RB) {
+        double a[8], c[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) { a[u] = partials[2 * (j + u * RB)]; c[u] = partials[2 * (j + u * RB) + 1]; }
+#pragma unroll
+        for (int u = 0; u < 8; u++) { rr += a[u]; bb += c[u]; }
+    }
+    for (; j < npart; j += RB) { rr += partials[2 * j]; bb += partials[2 * j + 1]; }
     asm volatile("" ::"v"(rr), "v"(bb), "s"(done));
     if (done) return;                                  // uniform over the block
     sr[threadIdx.x] = rr; sb[threadIdx.x] = bb;
@@ -309,12 +310,12 @@ __device__ __forceinline__ double resid_at(const X* __restrict__ x, const float*
     const int row = (z * ny + y) * rs;
     const int own = row + sgn * nh + hx, oth = row + (1 - sgn) * nh;
     const double me = (double)x[own];
-    const double vw = (double)x[opaque(xx > 0 ? oth + hx - 1 + sgn : own)];
-    const double ve = (double)x[opaque(xx < nx - 1 ? oth + hx + sgn : own)];
-    const double vk = (double)x[opaque(z > 0 ? own - ny * rs : own)];
-    const double vf = (double)x[opaque(z < nz - 1 ? own + ny * rs : own)];
-    const double vs = (double)x[opaque(y > 0 ? own - rs : own)];
-    const double vn = (double)x[opaque(y < ny - 1 ? own + rs : own)];
+    const double vw = (double)x[fs_opaque(xx > 0 ? oth + hx - 1 + sgn : own)];
+    const double ve = (double)x[fs_opaque(xx < nx - 1 ? oth + hx + sgn : own)];
+    const double vk = (double)x[fs_opaque(z > 0 ? own - ny * rs : own)];
+    const double vf = (double)x[fs_opaque(z < nz - 1 ? own + ny * rs : own)];
+    const double vs = (double)x[fs_opaque(y > 0 ? own - rs : own)];
+    const double vn = (double)x[fs_opaque(y < ny - 1 ? own + rs : own)];
     *bv = (double)b[own];
     double sum = 0.0, diag = 6.0;
     if (xx > 0) sum += vw; else diag += 1.0;
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
 #pragma unroll
         for (int sgn = 0; sgn < 2; sgn++) r[sgn] = resid_at(x, b, 2 * hx + sgn, y, z, sgn, hx, nx, ny, nz, nh, rs, ih2, &bv[sgn]);
     }
-    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
+    const int sk = skip[fs_opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     asm volatile("" ::"v"(r[0]), "v"(r[1]), "v"(bv[0]), "v"(bv[1]), "v"(sk));
     if (sk) return;                                    // uniform over the grid
     if (inside) {
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(RB) void k_finalize(RefineState* __restrict__ st, c
 // x += e (the V-cycle's correction), FP64, over the whole parity-split array
 __global__ __launch_bounds__(RB) void k_update(double* __restrict__ x, const float* __restrict__ e, int n,
                                                const int* __restrict__ skip) {
-    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
+    const int sk = skip[fs_opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     const int i = blockIdx.x * RB + threadIdx.x;
     if (i >= n) return;
     const double xv = x[i];
@@ -429,21 +430,26 @@ struct TailArgs {
 };
 
 // linear thread-to-cell mapping; the divisions are multiply-highs (FastDiv)
+// Up to TAIL_ILP cells of one colour per thread and pass: all their LDS reads are issued
+// before any of their writes (cells of one colour do not read each other), so the reads
+// overlap instead of running as one dependent chain per cell.
+constexpr int TAIL_ILP = 4;
 __device__ __forceinline__ void tail_smooth(float* p, const float* b, int nx, int ny, int nz, float h, int iters,
                                             FastDiv dhx, FastDiv dy, float omega) {
     const int nhx = (nx + 1) >> 1, nh = nhx * ny * nz, sy = nx, sz = nx * ny;
     for (int it = 0; it < iters; it++)
         for (int color = 0; color < 2; color++) {
-            for (int i = threadIdx.x; i < nh; i += TAIL_THREADS) {
+            if (nh <= TAIL_THREADS) {          // one cell per thread at most: the plain pass
+                const int i = threadIdx.x;
                 const int t = fdiv(i, dhx), hx = i - t * nhx;
                 const int z = fdiv(t, dy), y = t - z * ny;
                 const int x = hx * 2 + ((y + z + color) & 1);
-                if (x < nx) {
+                if (i < nh && x < nx) {
                     const int c = (z * ny + y) * nx + x;
-                    const float vw = p[opaque(x > 0 ? c - 1 : c)], ve = p[opaque(x < nx - 1 ? c + 1 : c)];
-                    const float vk = p[opaque(z > 0 ? c - sz : c)], vf = p[opaque(z < nz - 1 ? c + sz : c)];
-                    const float vs = p[opaque(y > 0 ? c - sy : c)], vn = p[opaque(y < ny - 1 ? c + sy : c)];
-                    const float bv = b[c];
+                    const float vw = p[fs_opaque(x > 0 ? c - 1 : c)], ve = p[fs_opaque(x < nx - 1 ? c + 1 : c)];
+                    const float vk = p[fs_opaque(z > 0 ? c - sz : c)], vf = p[fs_opaque(z < nz - 1 ? c + sz : c)];
+                    const float vs = p[fs_opaque(y > 0 ? c - sy : c)], vn = p[fs_opaque(y < ny - 1 ? c + sy : c)];
+                    const float bv = b[c], po = p[c];
                     float sum = 0.f, diag = 6.f;
                     if (x > 0) sum += vw; else diag += 1.f;
                     if (x < nx - 1) sum += ve; else diag += 1.f;
@@ -452,8 +458,48 @@ __device__ __forceinline__ void tail_smooth(float* p, const float* b, int nx, in
                     if (y > 0) sum += vs; else diag -= 1.f;
                     if (y < ny - 1) sum += vn; else diag += 1.f;
                     float pn = (sum - h * h * bv) / diag;
-                    if (omega != 1.f) pn = p[c] + omega * (pn - p[c]);
+                    if (omega != 1.f) pn = po + omega * (pn - po);
                     p[c] = pn;
+                }
+                __syncthreads();
+                continue;
+            }
+            for (int i0 = threadIdx.x; i0 < nh; i0 += TAIL_ILP * TAIL_THREADS) {
+                float v[TAIL_ILP][8];
+                int cc[TAIL_ILP], xs[TAIL_ILP], ys[TAIL_ILP], zs[TAIL_ILP];
+                bool ok[TAIL_ILP];
+#pragma unroll
+                for (int u = 0; u < TAIL_ILP; u++) {
+                    const int i = i0 + u * TAIL_THREADS;
+                    const int t = fdiv(i, dhx), hx = i - t * nhx;
+                    const int z = fdiv(t, dy), y = t - z * ny;
+                    const int x = hx * 2 + ((y + z + color) & 1);
+                    ok[u] = i < nh && x < nx;
+                    const int c = ok[u] ? (z * ny + y) * nx + x : 0;
+                    cc[u] = c; xs[u] = x; ys[u] = y; zs[u] = z;
+                    v[u][0] = p[fs_opaque(ok[u] && x > 0 ? c - 1 : c)];
+                    v[u][1] = p[fs_opaque(ok[u] && x < nx - 1 ? c + 1 : c)];
+                    v[u][2] = p[fs_opaque(ok[u] && z > 0 ? c - sz : c)];
+                    v[u][3] = p[fs_opaque(ok[u] && z < nz - 1 ? c + sz : c)];
+                    v[u][4] = p[fs_opaque(ok[u] && y > 0 ? c - sy : c)];
+                    v[u][5] = p[fs_opaque(ok[u] && y < ny - 1 ? c + sy : c)];
+                    v[u][6] = b[c];
+                    v[u][7] = p[c];
+                }
+#pragma unroll
+                for (int u = 0; u < TAIL_ILP; u++) {
+                    if (!ok[u]) continue;
+                    const int x = xs[u], y = ys[u], z = zs[u];
+                    float sum = 0.f, diag = 6.f;
+                    if (x > 0) sum += v[u][0]; else diag += 1.f;
+                    if (x < nx - 1) sum += v[u][1]; else diag += 1.f;
+                    if (z > 0) sum += v[u][2]; else diag += 1.f;
+                    if (z < nz - 1) sum += v[u][3]; else diag += 1.f;
+                    if (y > 0) sum += v[u][4]; else diag -= 1.f;
+                    if (y < ny - 1) sum += v[u][5]; else diag += 1.f;
+                    float pn = (sum - h * h * v[u][6]) / diag;
+                    if (omega != 1.f) pn = v[u][7] + omega * (pn - v[u][7]);
+                    p[cc[u]] = pn;
                 }
             }
             __syncthreads();
@@ -472,9 +518,9 @@ __device__ __forceinline__ void tail_restrict(const float* p, const float* b, in
             const int x = xc * 2 + (k & 1), y = yc * 2 + ((k >> 1) & 1), z = zc * 2 + ((k >> 2) & 1);
             const int c = (z * ny + y) * nx + x;
             const float me = p[c];
-            const float vxm = p[opaque(x > 0 ? c - 1 : c)], vxp = p[opaque(x < nx - 1 ? c + 1 : c)];
-            const float vym = p[opaque(y > 0 ? c - sy : c)], vyp = p[opaque(y < ny - 1 ? c + sy : c)];
-            const float vzm = p[opaque(z > 0 ? c - sz : c)], vzp = p[opaque(z < nz - 1 ? c + sz : c)];
+            const float vxm = p[fs_opaque(x > 0 ? c - 1 : c)], vxp = p[fs_opaque(x < nx - 1 ? c + 1 : c)];
+            const float vym = p[fs_opaque(y > 0 ? c - sy : c)], vyp = p[fs_opaque(y < ny - 1 ? c + sy : c)];
+            const float vzm = p[fs_opaque(z > 0 ? c - sz : c)], vzp = p[fs_opaque(z < nz - 1 ? c + sz : c)];
             const float pxm = x > 0 ? vxm : -me;
             const float pxp = x < nx - 1 ? vxp : -me;
             const float pym = y > 0 ? vym : me;
@@ -516,7 +562,7 @@ __device__ __forceinline__ void tail_prolong(float* p, const float* pcoarse, int
 }
 
 __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailArgs a, const int* __restrict__ skip) {
-    const int sk = skip[opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
+    const int sk = skip[fs_opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     extern __shared__ float lds[];
     const int n0 = a.dims[0][0] * a.dims[0][1] * a.dims[0][2];
     float* p0 = lds + a.off[0];
