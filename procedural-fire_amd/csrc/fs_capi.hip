@@ -935,7 +935,17 @@ int fs_poisson_set_x(fs_poisson* ps, int dtype, const void* x) {
 int fs_poisson_solve(fs_poisson* ps, double tol, int max_cycles, double* hist, int* cycles) {
     if (!ps || max_cycles < 0 || max_cycles > MG_MAX_REFINE) return fail(FS_EINVAL, "bad arguments");
     if (hipSetDevice(ps->device) != hipSuccess) return fail(FS_EDEVICE, "hipSetDevice failed");
-    mg_refine(ps->mg, ps->stream, tol, max_cycles);
+    // one refinement iteration at a time, each followed by a read of the device's stopping
+    // test: no V-cycles are launched past convergence (an iteration at this size costs far
+    // more than the round trip)
+    mg_refine_begin(ps->mg, ps->stream, tol, max_cycles);
+    for (int k = 0; k <= max_cycles; k++) {
+        int used_done[2];
+        FS_HIP_CHECK(hipMemcpyAsync(used_done, &ps->mg.st->used, sizeof(used_done), hipMemcpyDeviceToHost, ps->stream));
+        FS_HIP_CHECK(hipStreamSynchronize(ps->stream));
+        if (used_done[1] || k == max_cycles) break;
+        mg_refine_iter(ps->mg, ps->stream, tol, max_cycles);
+    }
     FS_HIP_CHECK(hipGetLastError());
     FS_HIP_CHECK(hipStreamSynchronize(ps->stream));
     RefineState st;

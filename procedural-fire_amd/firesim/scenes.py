@@ -142,3 +142,39 @@ def make_scene(name, seed=7, res=None, **kw):
         sim.set_face_flux(scene_face_flux(name))
     apply_ambient(sim, name)
     return sim
+
+
+# ---- isolated pressure solve workload ------------------------------------------------
+def normals(R):
+    """Standard normals from the uniform generator by Box-Muller (pairs)."""
+    while True:
+        u1 = max(R(), 1e-300)
+        u2 = R()
+        r = math.sqrt(-2.0 * math.log(u1))
+        yield r * math.cos(2.0 * math.pi * u2)
+        yield r * math.sin(2.0 * math.pi * u2)
+
+
+def blob_rhs(dims, n_blobs=64, sigma=(2.0, 8.0), noise=1e-2, seed=11):
+    """Poisson right-hand side, shape (nz, ny, nx): n_blobs Gaussian blobs (centres uniform in
+    cell space, widths U(sigma) cells, amplitudes N(0, 1)) plus white noise N(0, noise^2),
+    then made zero-mean.  Blob parameters come from the reference LCG, the white noise from
+    numpy's PCG64 seeded with `seed` (25 M draws)."""
+    nx, ny, nz = dims
+    R = lcg(seed)
+    G = normals(R)
+    b = np.zeros((nz, ny, nx), np.float64)
+    for _ in range(n_blobs):
+        c = (R() * nx, R() * ny, R() * nz)
+        s = sigma[0] + (sigma[1] - sigma[0]) * R()
+        a = next(G)
+        lo = [max(0, int(c[k] - 4 * s)) for k in range(3)]
+        hi = [min(d, int(c[k] + 4 * s) + 2) for k, d in enumerate(dims)]
+        x = (np.arange(lo[0], hi[0]) + 0.5 - c[0]) ** 2
+        y = (np.arange(lo[1], hi[1]) + 0.5 - c[1]) ** 2
+        z = (np.arange(lo[2], hi[2]) + 0.5 - c[2]) ** 2
+        g = np.exp(-(z[:, None, None] + y[None, :, None] + x[None, None, :]) / (2 * s * s))
+        b[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] += a * g
+    b += np.random.default_rng(seed).normal(0.0, noise, b.shape)
+    b -= b.mean()
+    return b

@@ -58,3 +58,30 @@ def test_refined_solve_reaches_closed_form(dims, omega):
     ohist = orc.refine(1e-10, 40)
     assert len(ohist) == len(hist) - 1
     assert np.max(np.abs(x - orc.get_x())) <= 1e-12 * np.max(np.abs(exact))
+
+
+def test_blob_workload_refined_to_1e8_matches_fp64_oracle():
+    """The isolated-solve workload (seeded Gaussian blobs + noise, zero mean) at 64x96x64:
+    FP32 V-cycles (omega 1.25) in FP64 refinement to 1e-8, against the FP64 oracle converged
+    to ~1e-14; same cycle count as the oracle's own FP32-in-FP64 refinement."""
+    from firesim import scenes
+    print("poisson blobs 64x96x64", flush=True)
+    dims = (64, 96, 64)
+    h = 1.0 / 256
+    b = scenes.blob_rhs(dims)
+    ps = F.Poisson(dims, h, omega=1.25)
+    ps.set_rhs(b)
+    ps.set_x(np.zeros_like(b))
+    hist = ps.solve(1e-8, 40)
+    assert hist[-1] <= 1e-8 and len(hist) - 1 <= 8, hist
+    x = ps.read_x()
+    ref = O.OracleMG(dims, h, fp64=True, omega=1.25)
+    ref.set(b=b, p=np.zeros_like(b))
+    rh = ref.vcycles(30)
+    assert rh[-1] < 1e-12
+    xr = ref.get()
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-7
+    orc = O.OracleMG(dims, h, fp64=False, omega=1.25)
+    orc.set(b=b, p=np.zeros_like(b))
+    ohist = orc.refine(1e-8, 40)
+    assert len(ohist) == len(hist) - 1
