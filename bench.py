@@ -95,14 +95,14 @@ def time_steps(sim, steps):
 
 
 def profile_kernels(sim, steps=5):
-    sim.profile(True)
+    sim.profile_stages(True)
     acc = {k: 0.0 for k in F.STAGE_LABELS}
     for _ in range(steps):
         sim.step(DT, 1)
         t = sim.timings()
         for k in acc:
             acc[k] += t[k]
-    sim.profile(False)
+    sim.profile_stages(False)
     return {k: v / steps for k, v in acc.items()}
 
 
@@ -255,6 +255,57 @@ def poisson_bench(args):
     print(json.dumps(out))
 
 
+def flame_statistics(args):
+    """Long runs of the plain FP32 and the mixed step from rest with the per-step diagnostics
+    record on: heat release, flame height (50 % axis intermittency) and puffing frequency
+    (spectral peak of the heat release) of each, against f = 1.5 / sqrt(D)."""
+    from firesim import estimators as E
+    name = args.flame_scene
+    if name == "campfire":
+        # the reference app's "ultra" preset (src/main.js:19, 32-38): the seed-7 campfire in a
+        # 1.2 x 1.8 x 1.2 m box at 128 x 192 x 128, 3 V-cycles, dt 1/120 s; its own calibration
+        # targets (research/DESIGN.md:427-431): HRR 60 kW +-10 %, puffing 1.94 Hz +-20 %,
+        # 50 % axis intermittency at 0.6 m +-0.1
+        sc = {"res": (128, 192, 128), "dx": 1.2 / 128, "dt": 1.0 / 120, "vcycles": 3}
+        D = 0.5
+    else:
+        sc = scenes.SCENES[name]
+        D = scenes.DISC_DIAMETER
+    steps = args.steps if args.steps != 200 else (2400 if name == "campfire" else 3500)
+    skip = min(args.warmup if args.warmup != 100 else 500, steps // 2)
+    out = {"workload": name, "grid": list(sc["res"]), "dt": sc["dt"], "steps": steps, "skipped": skip,
+           "fire_diameter_m": D, "puffing_correlation_Hz": E.puffing_correlation(D)}
+    # reference: the reference's own step (fp16 storage, fixed V-cycles); fp32: the same with
+    # float4 storage; mixed: fp16 storage with the pressure refined to 1e-5 in FP64
+    for form, prec, solve in (("reference", F.PREC_FP16, F.SOLVE_VCYCLES), ("fp32", F.PREC_FP32, F.SOLVE_VCYCLES),
+                              ("mixed", F.PREC_FP16, F.SOLVE_REFINE64)):
+        if name == "campfire":
+            logs, _ = F.build_campfire(7)
+            sim = F.FireSim(res=sc["res"], size=(1.2, 1.8, 1.2), logs=logs, vcycles=sc["vcycles"], precision=prec,
+                            solve=solve, div_tol=1e-5, max_vcycles=8, noise_n=128, use_graph=True)
+        else:
+            sim = scenes.make_scene(name, seed=7, precision=prec, solve=solve, div_tol=1e-5, max_vcycles=8,
+                                    noise_n=128, use_graph=True)
+        sim.diag_enable(steps, t_flame=1000.0)
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            n = min(500, steps - done)
+            sim.step(sc["dt"], n)
+            sim.sync()
+            done += n
+            print(f"{form}: {done}/{steps} steps", flush=True)
+        wall = time.perf_counter() - t0
+        s = sim.diag_series()
+        if args.save:
+            os.makedirs(args.save, exist_ok=True)
+            np.savez(os.path.join(args.save, f"flames_{name}_{form}.npz"), **s)
+        out[form] = E.summary(s, float(np.float32(sc["dx"])), sc["dt"], 1000.0, skip)
+        out[form]["wall_s"] = wall
+        sim.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +316,11 @@ def main():
     ap.add_argument("--scene", default="spots8_128", choices=["spots8_128", "campfire"])
     ap.add_argument("--poisson", action="store_true",
                     help="instead of the step: the isolated 256x384x256 pressure solve (FP64 refinement to 1e-8)")
+    ap.add_argument("--flames", action="store_true",
+                    help="instead of timing: long FP32 and mixed runs with flame-height / puffing statistics")
+    ap.add_argument("--save", default=None, help="with --flames: directory for the per-step series (.npz)")
+    ap.add_argument("--flame-scene", default="spots8_128",
+                    choices=["spots8_128", "segments12_192", "spots4_64", "campfire"])
     ap.add_argument("--errors", action="store_true",
                     help="instead of timing: per-field error of the mixed and FP32 steps vs the FP64 oracle "
                          "after 1 and 10 steps")
@@ -283,6 +339,9 @@ def main():
         return
     if args.poisson:
         poisson_bench(args)
+        return
+    if args.flames:
+        flame_statistics(args)
         return
     ncell = float(np.prod(RES))
     graph = not args.no_graph

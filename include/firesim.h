@@ -208,6 +208,31 @@ int  fs_timings(fs_sim* h, float ms[FS_NUM_TIMINGS]);
  * writes max_logs*8 floats (zero-padded) and returns the number of real logs. */
 int  fs_build_campfire(int seed, float* out, int max_logs);
 
+/* ---- diagnostics ------------------------------------------------------------ */
+/* FIRE.stats(name) (src/main.js:268-294), reduced on the device, for a 4-channel
+ * state field (FS_VEL .. FS_S2Y, FS_VORT, FS_GEO): for channel c, out[5c + 0..4] =
+ * min, max, mean over all cells, count of non-finite values, and the linear index
+ * (x fastest) of the first cell holding the maximum (-1 if the channel has no finite
+ * value).  Min, max and mean are over the finite values; the mean divides by the
+ * cell count, as the reference does.  Waits for the handle's stream. */
+int  fs_stats(fs_sim* h, int field, double out[20]);
+/* FIRE.profile() (src/main.js:322-355), reduced on the device from S: *q_watts =
+ * heat release sum(354800 q / T * dx^3) over cells with q = |S.w| 1e4 > 0, T = 293 +
+ * 1000 |S.x|; then for y-planes j = 0, 4, 8, ... one row of 6 doubles: height
+ * (j + 0.5) dx, mean T of the 5x5 centre columns, max T, max |soot|, max |fuel|,
+ * cells above 1000 K.  *n_rows = ceil(ny / 4); at most max_rows rows are written.
+ * Rounding for display is left to the caller.  Waits for the handle's stream. */
+int  fs_profile(fs_sim* h, double* q_watts, double* rows, int max_rows, int* n_rows);
+/* Per-step diagnostics record, no counterpart in the reference: with capacity > 0
+ * every later step appends, after its last kernel, one record of (4 + ny) doubles
+ * to a device ring of `capacity` records: time after the step (s), heat release
+ * (W, as fs_profile), max T (K), flame tip (m: top of the highest y-plane whose max
+ * T exceeds t_flame, 0 if none), then per y-plane the max T of the 5x5 centre
+ * columns (the axis profile behind flame intermittency).  capacity 0 turns it off. */
+int  fs_diag_enable(fs_sim* h, int capacity, double t_flame);
+/* Copies the held records, oldest first (at most max_records, the newest kept). */
+int  fs_diag_read(fs_sim* h, double* out, int max_records, int* n_records);
+
 /* Measurement utility with no counterpart in the reference: times `iters`
  * device-to-device copies of `bytes` with a plain streaming kernel (the box's
  * achievable HBM bandwidth, and the calibration of bandwidth counters). */

@@ -97,6 +97,13 @@ struct fs_sim {
     bool profile = false;
     hipEvent_t ev[8] = {};
     bool timings_valid = false;
+    // diagnostics: per-plane partials, stats partials, the per-step record ring
+    double* planes_d = nullptr;      // ny * 8
+    double* stats_d = nullptr;       // stats blocks * 20
+    double* ring_d = nullptr;        // diag_cap * (4 + ny)
+    int* ring_count_d = nullptr;
+    int diag_cap = 0;
+    double diag_tflame = 1000.0;
 
     FieldSet fields() const {
         FieldSet F;
@@ -184,6 +191,11 @@ void enqueue_post(fs_sim* h, bool prof) {
     if (prof) (void)hipEventRecord(h->ev[6], s);
     launch_project(h->prec, h->g, s, h->P_d, F, h->refine() ? nullptr : h->mg.p[0], h->refine() ? h->mg.x64 : nullptr);
     if (prof) (void)hipEventRecord(h->ev[7], s);
+    if (h->diag_cap > 0) {           // the per-step record, after the step's last kernel
+        launch_planes(h->prec, s, F.S, h->nx, h->ny, h->nz, std::pow((double)(float)h->dx, 3.0), h->planes_d);
+        launch_record(s, h->P_d, h->planes_d, h->ny, (double)(float)h->dx, h->diag_tflame, h->ring_d, h->diag_cap,
+                      h->ring_count_d);
+    }
 }
 void enqueue_step(fs_sim* h, bool prof) {
     enqueue_pre(h, prof, false);
@@ -259,6 +271,10 @@ void free_sim(fs_sim* h) {
     if (h->spots_d) (void)hipFree(h->spots_d);
     if (h->face_d) (void)hipFree(h->face_d);
     if (h->P_d) (void)hipFree(h->P_d);
+    if (h->planes_d) (void)hipFree(h->planes_d);
+    if (h->stats_d) (void)hipFree(h->stats_d);
+    if (h->ring_d) (void)hipFree(h->ring_d);
+    if (h->ring_count_d) (void)hipFree(h->ring_count_d);
     mg_free(h->mg);
     for (auto& e : h->ev) if (e) (void)hipEventDestroy(e);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
@@ -856,6 +872,114 @@ int fs_build_campfire(int seed, float* out, int max_logs) {
 }
 
 // ---- isolated Poisson solve ---------------------------------------------------------
+// ---- diagnostics (src/main.js:268-355) -------------------------------------------
+namespace {
+int ensure_planes(fs_sim* h) {
+    if (!h->planes_d) FS_HIP_CHECK(hipMalloc(&h->planes_d, (size_t)h->ny * 8 * sizeof(double)));
+    return FS_OK;
+}
+}  // namespace
+
+int fs_stats(fs_sim* h, int field, double out[20]) {
+    if (!h || !out) return fail(FS_EINVAL, "null argument");
+    const int slot = field_slot(field);
+    if (slot < 0) return fail(FS_EINVAL, "stats needs a 4-channel state field");
+    if (set_device(h)) return FS_EDEVICE;
+    const int nb = stats_blocks(h->ncell);
+    if (!h->stats_d) FS_HIP_CHECK(hipMalloc(&h->stats_d, (size_t)nb * 20 * sizeof(double)));
+    launch_stats(h->prec, h->stream, h->f[slot], h->ncell, h->stats_d);
+    FS_HIP_CHECK(hipGetLastError());
+    std::vector<double> part((size_t)nb * 20);
+    FS_HIP_CHECK(hipMemcpyAsync(part.data(), h->stats_d, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < 4; c++) {
+        double mn = INFINITY, mx = -INFINITY, sum = 0.0, bad = 0.0, arg = -1.0;
+        for (int b = 0; b < nb; b++) {   // blocks in index order: ties keep the earlier maximum
+            const double* q = &part[((size_t)b * 4 + c) * 5];
+            mn = std::min(mn, q[0]);
+            if (q[4] >= 0 && (arg < 0 || q[1] > mx)) { mx = q[1]; arg = q[4]; }
+            sum += q[2];
+            bad += q[3];
+        }
+        out[c * 5 + 0] = mn;
+        out[c * 5 + 1] = mx;
+        out[c * 5 + 2] = sum / (double)h->ncell;
+        out[c * 5 + 3] = bad;
+        out[c * 5 + 4] = arg;
+    }
+    return FS_OK;
+}
+
+int fs_profile(fs_sim* h, double* q_watts, double* rows, int max_rows, int* n_rows) {
+    if (!h || (max_rows > 0 && !rows)) return fail(FS_EINVAL, "null argument");
+    if (set_device(h)) return FS_EDEVICE;
+    if (ensure_planes(h)) return FS_EDEVICE;
+    const double dx = (double)(float)h->dx;
+    launch_planes(h->prec, h->stream, h->f[F_S], h->nx, h->ny, h->nz, dx * dx * dx, h->planes_d);
+    FS_HIP_CHECK(hipGetLastError());
+    std::vector<double> pl((size_t)h->ny * 8);
+    FS_HIP_CHECK(hipMemcpyAsync(pl.data(), h->planes_d, pl.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    double Q = 0.0;
+    for (int j = 0; j < h->ny; j++) Q += pl[(size_t)j * 8 + 6];
+    if (q_watts) *q_watts = Q;
+    int r = 0;
+    for (int j = 0; j < h->ny; j += 4, r++) {
+        if (r >= max_rows) continue;
+        const double* q = &pl[(size_t)j * 8];
+        double* o = rows + (size_t)r * 6;
+        o[0] = (j + 0.5) * h->dx;
+        o[1] = q[2] > 0 ? q[1] / q[2] : 0.0;
+        o[2] = q[0];
+        o[3] = q[3];
+        o[4] = q[4];
+        o[5] = q[5];
+    }
+    if (n_rows) *n_rows = r;
+    return FS_OK;
+}
+
+int fs_diag_enable(fs_sim* h, int capacity, double t_flame) {
+    if (!h || capacity < 0 || !(t_flame > 0)) return fail(FS_EINVAL, "bad diagnostics arguments");
+    if (set_device(h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    drop_graphs(h);                  // the record kernels join (or leave) the captured step
+    if (h->ring_d) { (void)hipFree(h->ring_d); h->ring_d = nullptr; }
+    h->diag_cap = 0;
+    if (capacity == 0) return FS_OK;
+    if (ensure_planes(h)) return FS_EDEVICE;
+    if (!h->ring_count_d) FS_HIP_CHECK(hipMalloc(&h->ring_count_d, sizeof(int)));
+    FS_HIP_CHECK(hipMalloc(&h->ring_d, (size_t)capacity * (4 + h->ny) * sizeof(double)));
+    launch_fill_zero(h->stream, h->ring_count_d, sizeof(int));
+    FS_HIP_CHECK(hipGetLastError());
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->diag_cap = capacity;
+    h->diag_tflame = t_flame;
+    return FS_OK;
+}
+
+int fs_diag_read(fs_sim* h, double* out, int max_records, int* n_records) {
+    if (!h || (max_records > 0 && !out)) return fail(FS_EINVAL, "null argument");
+    if (h->diag_cap == 0) { if (n_records) *n_records = 0; return FS_OK; }
+    if (set_device(h)) return FS_EDEVICE;
+    int count = 0;
+    FS_HIP_CHECK(hipMemcpyAsync(&count, h->ring_count_d, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    const int have = std::min(count, h->diag_cap);
+    const size_t rec = 4 + (size_t)h->ny;
+    std::vector<double> ring((size_t)h->diag_cap * rec);
+    FS_HIP_CHECK(hipMemcpy(ring.data(), h->ring_d, ring.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const int first = count - have;               // oldest record still held
+    const int k0 = std::max(0, have - max_records);   // keep the newest max_records
+    int w = 0;
+    for (int k = k0; k < have; k++, w++) {
+        const int idx = (first + k) % h->diag_cap;
+        std::copy(&ring[(size_t)idx * rec], &ring[(size_t)idx * rec] + rec, out + (size_t)w * rec);
+    }
+    if (n_records) *n_records = w;
+    return FS_OK;
+}
+
 struct fs_poisson {
     GpuMG mg;
     hipStream_t stream = nullptr;

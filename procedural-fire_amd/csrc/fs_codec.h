@@ -1,0 +1,43 @@
+// Storage codecs of the 4-channel fields (float4, binary16 x4, bfloat16 x4): load as
+// float4, store with round-to-nearest-even.  Shared by the step and diagnostics kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fs_internal.h"
+
+// ---- storage codecs -----------------------------------------------------------
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    unsigned x = __float_as_uint(f);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((x >> 16) | 0x40u);
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return (unsigned short)(x >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short u) { return __uint_as_float((unsigned)u << 16); }
+
+template <class S> struct Codec;
+template <> struct Codec<StF32> {
+    static __device__ __forceinline__ float4 ld(const float4* p, int i) { return p[i]; }
+    static __device__ __forceinline__ void st(float4* p, int i, float4 v) { p[i] = v; }
+};
+template <> struct Codec<StF16> {
+    static __device__ __forceinline__ float4 ld(const half4_t* p, int i) {
+        half4_t h = p[i];
+        return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
+    }
+    static __device__ __forceinline__ void st(half4_t* p, int i, float4 v) {
+        half4_t h;
+        h.x = (_Float16)v.x; h.y = (_Float16)v.y; h.z = (_Float16)v.z; h.w = (_Float16)v.w;
+        p[i] = h;
+    }
+};
+template <> struct Codec<StBF16> {
+    static __device__ __forceinline__ float4 ld(const ushort4_t* p, int i) {
+        ushort4_t u = p[i];
+        return make_float4(bf2f(u.x), bf2f(u.y), bf2f(u.z), bf2f(u.w));
+    }
+    static __device__ __forceinline__ void st(ushort4_t* p, int i, float4 v) {
+        ushort4_t u;
+        u.x = f2bf(v.x); u.y = f2bf(v.y); u.z = f2bf(v.z); u.w = f2bf(v.w);
+        p[i] = u;
+    }
+};

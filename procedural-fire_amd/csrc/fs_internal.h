@@ -157,3 +157,9 @@ void launch_set_params(hipStream_t s, DevParams* dst, const DevParams& src);
 void launch_fill_zero(hipStream_t s, void* ptr, size_t bytes);
 void launch_convert(hipStream_t s, int kind, const void* src, void* dst, size_t n);
 void launch_copy_probe(hipStream_t s, const void* src, void* dst, size_t bytes);
+// ---- diagnostics (fs_diag.hip) -------------------------------------------------
+int  stats_blocks(size_t n);
+void launch_stats(int prec, hipStream_t s, const void* field, size_t n, double* part);
+void launch_planes(int prec, hipStream_t s, const void* S_field, int nx, int ny, int nz, double V, double* out);
+void launch_record(hipStream_t s, const DevParams* P, const double* planes, int ny, double dx, double t_flame,
+                   double* ring, int cap, int* count);

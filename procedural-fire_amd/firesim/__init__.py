@@ -127,6 +127,10 @@ def lib():
         "fs_poisson_vcycles": (ip, [vp, ip, dp]),
         "fs_poisson_read_x": (ip, [vp, ip, vp]),
         "fs_poisson_set_omega": (ip, [vp, ctypes.c_float]),
+        "fs_stats": (ip, [vp, ip, dp]),
+        "fs_profile": (ip, [vp, dp, dp, ip, ctypes.POINTER(ctypes.c_int)]),
+        "fs_diag_enable": (ip, [vp, ip, ctypes.c_double]),
+        "fs_diag_read": (ip, [vp, dp, ip, ctypes.POINTER(ctypes.c_int)]),
         "fs_poisson_info": (ip, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():
@@ -268,6 +272,51 @@ class FireSim:
             raise ValueError(f"face flux must have shape {(nz, nx)}")
         _check(lib().fs_set_face_flux(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
 
+    # ---- diagnostics (src/main.js:268-355)
+    def stats(self, field="S"):
+        """FIRE.stats(name) reduced on the device: per channel min, max, mean, bad (non-finite
+        count) and argmax (i, j, k) of the first maximum, unrounded."""
+        out = np.zeros(20, np.float64)
+        _check(lib().fs_stats(self._h, FIELD_NAMES.get(field, field), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        nx, ny, _ = self.res
+        res = []
+        for c in range(4):
+            mn, mx, mean, bad, arg = out[5 * c:5 * c + 5]
+            a = int(arg)
+            res.append({"min": mn, "max": mx, "mean": mean, "bad": int(bad),
+                        "argmax": None if a < 0 else (a % nx, (a // nx) % ny, a // (nx * ny))})
+        return res
+
+    def profile(self):
+        """FIRE.profile() reduced on the device: heat release (kW) and rows every 4th y-plane
+        (h, Tc, Tmax, soot, fuel, hotCells), unrounded."""
+        nrow = (self.res[1] + 3) // 4
+        rows = np.zeros(nrow * 6, np.float64)
+        q = ctypes.c_double()
+        n = ctypes.c_int()
+        _check(lib().fs_profile(self._h, ctypes.byref(q), rows.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), nrow,
+                                ctypes.byref(n)))
+        keys = ("h", "Tc", "Tmax", "soot", "fuel", "hotCells")
+        return {"Q_kW": q.value / 1000.0, "rows": [dict(zip(keys, rows[6 * r:6 * r + 6])) for r in range(n.value)]}
+
+    def diag_enable(self, capacity=4096, t_flame=1000.0):
+        """Per-step record (time, heat release, max T, flame tip, axis T per plane) into a
+        device ring of `capacity` steps; 0 turns it off."""
+        _check(lib().fs_diag_enable(self._h, int(capacity), float(t_flame)))
+        self._diag_cap = int(capacity)
+
+    def diag_series(self):
+        """The held per-step records, oldest first: dict of arrays time, Q (W), Tmax (K), tip (m),
+        axisT (steps x ny, K)."""
+        cap = getattr(self, "_diag_cap", 0)
+        ny = self.res[1]
+        buf = np.zeros(max(cap, 1) * (4 + ny), np.float64)
+        n = ctypes.c_int()
+        _check(lib().fs_diag_read(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), cap, ctypes.byref(n)))
+        a = buf[:n.value * (4 + ny)].reshape(n.value, 4 + ny)
+        return {"time": a[:, 0].copy(), "Q": a[:, 1].copy(), "Tmax": a[:, 2].copy(), "tip": a[:, 3].copy(),
+                "axisT": a[:, 4:].copy()}
+
     # ---- stepping
     def reset(self):
         _check(lib().fs_reset(self._h))
@@ -356,7 +405,8 @@ class FireSim:
         _check(lib().fs_residual(self._h, ctypes.byref(r), ctypes.byref(u)))
         return r.value, u.value
 
-    def profile(self, on=True):
+    def profile_stages(self, on=True):
+        """Per-stage device timing of the following steps (the Profiler labels, src/sim.js:629-644)."""
         _check(lib().fs_profile_enable(self._h, 1 if on else 0))
 
     def timings(self):

@@ -113,53 +113,45 @@ class FireSim {
     return o;
   }
 
-  // FIRE.stats(name) (src/main.js:268-294): per-channel min/max/mean/bad/argmax
+  // FIRE.stats(name) (src/main.js:268-294), reduced on the device: per-channel
+  // min/max/mean/bad/argmax with the reference's toPrecision(4) display rounding
   stats(name = 'S') {
-    const [x, y, z] = this.res;
-    const a = this.read(name);
-    const out = [0, 1, 2, 3].map(() => ({ min: Infinity, max: -Infinity, sum: 0, bad: 0, argmax: null }));
-    for (let k = 0; k < z; k++) for (let j = 0; j < y; j++) for (let i = 0; i < x; i++) {
-      const o = ((k * y + j) * x + i) * 4;
-      for (let c = 0; c < 4; c++) {
-        const v = a[o + c];
-        const S = out[c];
-        if (!Number.isFinite(v)) { S.bad++; continue; }
-        if (v < S.min) S.min = v;
-        if (v > S.max) { S.max = v; S.argmax = [i, j, k]; }
-        S.sum += v;
-      }
+    const [x, y] = this.res;
+    const a = addon.stats(this.h, fieldId(name));
+    const out = [];
+    for (let c = 0; c < 4; c++) {
+      const [mn, mx, mean, bad, arg] = a.subarray(c * 5, c * 5 + 5);
+      const i = arg < 0 ? null : [arg % x, Math.floor(arg / x) % y, Math.floor(arg / (x * y))];
+      out.push({ min: +mn.toPrecision(4), max: +mx.toPrecision(4), mean: +mean.toPrecision(4), bad, argmax: i });
     }
-    const n = x * y * z;
-    return out.map(S => ({ min: +S.min.toPrecision(4), max: +S.max.toPrecision(4), mean: +(S.sum / n).toPrecision(4),
-      bad: S.bad, argmax: S.argmax }));
+    return out;
   }
 
-  // FIRE.profile() (src/main.js:322-355): heat release (kW) and centreline/height rows
+  // FIRE.profile() (src/main.js:322-355), reduced on the device: heat release (kW) and
+  // centreline/height rows, rounded as the reference rounds them
   profile() {
-    const [x, y, z] = this.res;
-    const a = this.read('S');
-    const dx = this.dx, V = dx * dx * dx;
+    const a = addon.profileRows(this.h);
     const rows = [];
-    for (let j = 0; j < y; j += 4) {
-      let Tmax = 0, Tc = 0, sootMax = 0, fuelMax = 0, hot = 0, nC = 0;
-      for (let k = 0; k < z; k++) for (let i = 0; i < x; i++) {
-        const o = ((k * y + j) * x + i) * 4;
-        const T = 293 + 1000 * Math.abs(a[o]);
-        Tmax = Math.max(Tmax, T); sootMax = Math.max(sootMax, Math.abs(a[o + 2])); fuelMax = Math.max(fuelMax, Math.abs(a[o + 1]));
-        if (T > 1000) hot++;
-        if (Math.abs(i - x / 2) < 3 && Math.abs(k - z / 2) < 3) { Tc += T; nC++; }
-      }
-      rows.push({ h: +((j + 0.5) * dx).toFixed(2), Tc: Math.round(Tc / nC), Tmax: Math.round(Tmax), soot: +sootMax.toFixed(2),
-        fuel: +fuelMax.toFixed(2), hotCells: hot });
+    for (let r = 0; 1 + r * 6 < a.length; r++) {
+      const o = 1 + r * 6;
+      rows.push({ h: +a[o].toFixed(2), Tc: Math.round(a[o + 1]), Tmax: Math.round(a[o + 2]), soot: +a[o + 3].toFixed(2),
+        fuel: +a[o + 4].toFixed(2), hotCells: a[o + 5] });
     }
-    let Q = 0;
-    for (let c = 0; c < x * y * z; c++) {
-      const q = Math.abs(a[c * 4 + 3]) * 1e4;
-      if (q <= 0) continue;
-      const T = 293 + 1000 * Math.abs(a[c * 4]);
-      Q += 354800 * q / T * V;
+    return { Q_kW: +(a[0] / 1000).toFixed(1), rows };
+  }
+
+  // per-step diagnostics (time, heat release W, max T, flame tip m, axis T per y-plane)
+  diagEnable(capacity = 4096, tFlame = 1000) { this._diagCap = capacity; addon.diagEnable(this.h, capacity, tFlame); }
+  diagSeries() {
+    const ny = this.res[1], rec = 4 + ny;
+    const a = addon.diagRead(this.h, this._diagCap || 0);
+    const n = a.length / rec;
+    const o = { time: [], Q: [], Tmax: [], tip: [], axisT: [] };
+    for (let k = 0; k < n; k++) {
+      o.time.push(a[k * rec]); o.Q.push(a[k * rec + 1]); o.Tmax.push(a[k * rec + 2]); o.tip.push(a[k * rec + 3]);
+      o.axisT.push(Array.from(a.subarray(k * rec + 4, (k + 1) * rec)));
     }
-    return { Q_kW: +(Q / 1000).toFixed(1), rows };
+    return o;
   }
 }
 

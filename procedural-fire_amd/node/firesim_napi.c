@@ -496,6 +496,90 @@ static napi_value js_info(napi_env env, napi_callback_info info) {
     return o;
 }
 
+static napi_value float64_array(napi_env env, const double* src, size_t n) {
+    napi_value ab, ta;
+    void* data;
+    napi_create_arraybuffer(env, n * sizeof(double), &data, &ab);
+    if (n) memcpy(data, src, n * sizeof(double));
+    napi_create_typedarray(env, napi_float64_array, n, ab, 0, &ta);
+    return ta;
+}
+
+/* stats(h, field) -> Float64Array(20): per channel min, max, mean, bad, argmax index */
+static napi_value js_stats(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sim* h = get_sim(env, argv[0]);
+    if (!h) return NULL;
+    int32_t field = 0;
+    napi_get_value_int32(env, argv[1], &field);
+    double out[20];
+    int rc = fs_stats(h, field, out);
+    if (rc) return throw_fs(env, rc);
+    return float64_array(env, out, 20);
+}
+
+/* profileRows(h) -> Float64Array: [Q_W, then 6 doubles per row] */
+static napi_value js_profile_rows(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sim* h = get_sim(env, argv[0]);
+    if (!h) return NULL;
+    int res[3];
+    double dx, org[3];
+    if (fs_grid_info(h, res, &dx, org)) return throw_fs(env, FS_EINVAL);
+    const int max_rows = (res[1] + 3) / 4;
+    double* buf = (double*)malloc(sizeof(double) * (1 + 6 * (size_t)max_rows));
+    if (!buf) { napi_throw_error(env, NULL, "out of memory"); return NULL; }
+    int n = 0;
+    int rc = fs_profile(h, buf, buf + 1, max_rows, &n);
+    if (rc) { free(buf); return throw_fs(env, rc); }
+    napi_value ta = float64_array(env, buf, 1 + 6 * (size_t)n);
+    free(buf);
+    return ta;
+}
+
+/* diagEnable(h, capacity, tFlame) */
+static napi_value js_diag_enable(napi_env env, napi_callback_info info) {
+    size_t argc = 3;
+    napi_value argv[3];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sim* h = get_sim(env, argv[0]);
+    if (!h) return NULL;
+    int32_t cap = 0;
+    double tf = 1000.0;
+    napi_get_value_int32(env, argv[1], &cap);
+    if (argc > 2) napi_get_value_double(env, argv[2], &tf);
+    int rc = fs_diag_enable(h, cap, tf);
+    if (rc) return throw_fs(env, rc);
+    return NULL;
+}
+
+/* diagRead(h, maxRecords) -> Float64Array of n * (4 + ny) doubles, oldest first */
+static napi_value js_diag_read(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sim* h = get_sim(env, argv[0]);
+    if (!h) return NULL;
+    int32_t maxr = 0;
+    napi_get_value_int32(env, argv[1], &maxr);
+    int res[3];
+    double dx, org[3];
+    if (fs_grid_info(h, res, &dx, org)) return throw_fs(env, FS_EINVAL);
+    const size_t rec = 4 + (size_t)res[1];
+    double* buf = (double*)malloc(sizeof(double) * rec * (size_t)(maxr > 0 ? maxr : 1));
+    if (!buf) { napi_throw_error(env, NULL, "out of memory"); return NULL; }
+    int n = 0;
+    int rc = fs_diag_read(h, buf, maxr, &n);
+    if (rc) { free(buf); return throw_fs(env, rc); }
+    napi_value ta = float64_array(env, buf, rec * (size_t)n);
+    free(buf);
+    return ta;
+}
+
 static napi_value js_residual(napi_env env, napi_callback_info info) {
     size_t argc = 1;
     napi_value argv[1];
@@ -586,6 +670,10 @@ static napi_value init(napi_env env, napi_value exports) {
     EXPORT("writeField", js_write_field);
     EXPORT("info", js_info);
     EXPORT("residual", js_residual);
+    EXPORT("stats", js_stats);
+    EXPORT("profileRows", js_profile_rows);
+    EXPORT("diagEnable", js_diag_enable);
+    EXPORT("diagRead", js_diag_read);
     EXPORT("profile", js_profile);
     EXPORT("timings", js_timings);
     EXPORT("buildCampfire", js_build_campfire);
