@@ -44,14 +44,15 @@ def kernel_bytes(E):
     return {"adv1": 6 * E, "adv2": 9 * E, "curl": 2 * E, "forces": 8 * E, "div": E + 4, "project": 3 * E + 4}
 
 
-def mg_bytes_per_cycle(levels, coarse_iters):
-    """V(2,2) traffic per fine cell: 4 sweeps x 12 B, restrict 9 B, prolong 8.5 B per cell of each
-    non-coarsest level; 12 B per sweep on the coarsest (src/sim.js:588-599)."""
+def mg_bytes_per_cycle(levels, coarse_iters, sweeps=4):
+    """Traffic of one V-cycle per fine cell: `sweeps` red-black sweeps x 12 B, restriction 9 B and
+    prolongation 8.5 B per cell of each non-coarsest level; 12 B per sweep on the coarsest
+    (src/sim.js:588-599).  V(2,2) has 4 sweeps per level, V(1,2) 3."""
     n0 = float(np.prod(levels[0]))
     tot = 0.0
     for i, d in enumerate(levels):
         w = np.prod(d) / n0
-        tot += w * (12.0 * coarse_iters if i == len(levels) - 1 else 65.5)
+        tot += w * (12.0 * coarse_iters if i == len(levels) - 1 else 12.0 * sweeps + 17.5)
     return tot
 
 
@@ -73,17 +74,19 @@ def measured_traffic(kernel):
 
 
 SCENE = "spots8_128"
+SWEEPS = (0, 0)        # red-black sweeps before / after the coarse correction in the tolerance solves (0: default)
 
 
 def make_sim(precision, solve, use_graph=True):
+    sw = SWEEPS if solve != F.SOLVE_VCYCLES else (0, 0)
     if SCENE == "campfire":
         logs, _ = F.build_campfire(7)
         s = F.FireSim(res=RES, size=SIZE, logs=logs, vcycles=2, precision=precision, solve=solve, div_tol=1e-5,
-                      max_vcycles=8, noise_n=128, use_graph=use_graph)
+                      max_vcycles=8, noise_n=128, use_graph=use_graph, pre_sweeps=sw[0], post_sweeps=sw[1])
         s.set_params(PARAMS)
         return s
     return scenes.make_scene(SCENE, seed=7, precision=precision, solve=solve, div_tol=1e-5, max_vcycles=8,
-                             noise_n=128, use_graph=use_graph)
+                             noise_n=128, use_graph=use_graph, pre_sweeps=sw[0], post_sweeps=sw[1])
 
 
 def time_steps(sim, steps):
@@ -314,6 +317,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--scene", default="spots8_128", choices=["spots8_128", "campfire"])
+    ap.add_argument("--sweeps", default="0,0",
+                    help="pre,post red-black sweeps of the tolerance solves' V-cycles (0,0: the library default V(1,2))")
     ap.add_argument("--poisson", action="store_true",
                     help="instead of the step: the isolated 256x384x256 pressure solve (FP64 refinement to 1e-8)")
     ap.add_argument("--flames", action="store_true",
@@ -327,8 +332,9 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB copy, then 3 steps of each step form")
     args = ap.parse_args()
-    global SCENE
+    global SCENE, SWEEPS
     SCENE = args.scene
+    SWEEPS = tuple(int(v) for v in args.sweeps.split(","))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the step runs on a single MI355X (no multi-GPU path is built); "
               "use --gpus 1", file=sys.stderr)
@@ -387,10 +393,11 @@ def main():
     dom = max(kb, key=lambda k: profm[k])
     dom_ms = profm[dom]
     achieved = kb[dom] * ncell / (dom_ms * 1e-3) / 1e9
-    mgc = mg_bytes_per_cycle(levels, coarse)
+    sw_tol = sum(SWEEPS) if all(SWEEPS) else 3                # the tolerance solves' sweeps per level (default V(1,2))
+    mgc = mg_bytes_per_cycle(levels, coarse, sw_tol)
     mg_alg = (mgc + REFINE_BYTES) * nv * ncell + 20.0 * ncell
     mg_gbs = mg_alg / (profm["mg"] * 1e-3) / 1e9
-    mg32_gbs = mgc * 2 * ncell / (prof32["mg"] * 1e-3) / 1e9
+    mg32_gbs = mg_bytes_per_cycle(levels, coarse, 4) * 2 * ncell / (prof32["mg"] * 1e-3) / 1e9
     out = {
         "metric": "sim voxel-steps/s @128x192x128 (mixed vs FP32) + MG HBM GB/s % of peak",
         "value": vmix,
@@ -408,8 +415,8 @@ def main():
         "config": {"workload": "fire step 128x192x128 dx=5mm dt=1ms vortEps=0.3 "
                                + ("8 spots (seed 7)" if SCENE != "campfire" else "campfire logs (seed 7)"),
                    "grid": list(RES),
-                   "solve_mixed": "FP32 V(2,2) in FP64 defect correction, tol 1e-5",
-                   "solve_fp32": "2 x V(2,2) FP32", "solve_fp32_tol": "FP32 V(2,2) until 1e-5 (cap 8)",
+                   "solve_mixed": "FP32 V(1,2) cycles (red-black SOR, omega 1.25) in FP64 defect correction, tol 1e-5",
+                   "solve_fp32": "2 x V(2,2) FP32", "solve_fp32_tol": "FP32 V(1,2) (omega 1.25) until 1e-5 (cap 8)",
                    "graph": graph},
         "fp32_value": v32,
         "fp32_ms_per_step": t32 / args.steps * 1e3,

@@ -106,7 +106,11 @@ typedef struct fs_config {
                              smoother on every level but the coarsest (0 = default 1.25; 1 =
                              plain Gauss-Seidel).  FS_SOLVE_VCYCLES, the reference's solve,
                              always uses 1                                                    */
-    int    reserved[5];
+    int    pre_sweeps;    /* FS_SOLVE_REFINE64 / TOL32: red-black sweeps before / after the     */
+    int    post_sweeps;   /* coarse correction on every level but the coarsest (0 = default:
+                             V(1,2), which with omega 1.25 converges faster per unit of work
+                             than the reference's V(2,2)); FS_SOLVE_VCYCLES always uses V(2,2) */
+    int    reserved[3];
 } fs_config;
 
 /* Fills a config with the reference defaults (res 80x120x80, size 1.2x1.8x1.2,

@@ -199,6 +199,7 @@ struct Multigrid {
     std::vector<Level<M>> L;
     int coarse_iters = 8;
     M omega = 1;   // over-relaxation above the coarsest level (tolerance solves; the reference uses 1)
+    int pre = 2, post = 2;   // sweeps per level (tolerance solves may set them; the reference uses V(2,2))
 
     void build(int x, int y, int z, double h, int max_levels) {   // sim.js:79-92
         L.clear();
@@ -302,7 +303,7 @@ struct Multigrid {
                     f.p[i] = f.p[i] + acc;
                 }
     }
-    void vcycle(int pre = 2, int post = 2) {   // sim.js:588-599
+    void vcycle() {   // sim.js:588-599
         const int n = (int)L.size();
         for (int i = 0; i < n - 1; i++) { smooth(i, pre); restrict_(i); }
         smooth(n - 1, coarse_iters);
@@ -422,7 +423,11 @@ struct Oracle : OracleBase {
         for (auto* f : {&vel, &velX, &velY, &Sf, &SX, &SY, &S2, &S2X, &S2Y, &vort, &geo})
             f->assign(ncell * 4, St::enc(0.0));
         mg.build(nx, ny, nz, (double)(float)dx, c->max_levels);   // L.h is an f32 uniform (sim.js:96)
-        if (solve_mode != FS_SOLVE_VCYCLES) mg.omega = (M)(c->omega > 0.f ? c->omega : 1.25f);
+        if (solve_mode != FS_SOLVE_VCYCLES) {
+            mg.omega = (M)(c->omega > 0.f ? c->omega : 1.25f);
+            mg.pre = c->pre_sweeps > 0 ? c->pre_sweeps : 1;
+            mg.post = c->post_sweeps > 0 ? c->post_sweeps : 2;
+        }
         x64.assign(ncell, 0.0);
         set_logs(lg, nl);
         bake_noise(noise, c->noise_n, c->noise_seed);

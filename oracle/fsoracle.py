@@ -39,12 +39,15 @@ class FsConfig(ctypes.Structure):
         ("use_graph", ctypes.c_int),
         ("adaptive", ctypes.c_int),
         ("omega", ctypes.c_float),
-        ("reserved", ctypes.c_int * 5),
+        ("pre_sweeps", ctypes.c_int),
+        ("post_sweeps", ctypes.c_int),
+        ("reserved", ctypes.c_int * 3),
     ]
 
 
 def make_config(res=(80, 120, 80), size=(1.2, 1.8, 1.2), vcycles=2, max_levels=0, precision=FP16,
-                solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1, omega=0.0):
+                solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1, omega=0.0, pre_sweeps=0,
+                post_sweeps=0):
     c = FsConfig()
     c.res[:] = list(res)
     c.size[:] = list(size)
@@ -57,6 +60,8 @@ def make_config(res=(80, 120, 80), size=(1.2, 1.8, 1.2), vcycles=2, max_levels=0
     c.noise_n = noise_n
     c.noise_seed = noise_seed
     c.omega = omega
+    c.pre_sweeps = pre_sweeps
+    c.post_sweeps = post_sweeps
     return c
 
 
@@ -191,7 +196,8 @@ class OracleSim:
     """CPU FireSim restatement.  precision: FP32/FP16/BF16 (f32 math) or FP64."""
 
     def __init__(self, res=(64, 96, 64), size=None, logs=None, n_logs=None, precision=FP16, vcycles=2,
-                 max_levels=0, solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1, omega=0.0):
+                 max_levels=0, solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1, omega=0.0,
+                 pre_sweeps=0, post_sweeps=0):
         if size is None:
             size = (res[0] * 0.015, res[1] * 0.015, res[2] * 0.015)
         if logs is None:
@@ -202,7 +208,7 @@ class OracleSim:
             n_logs = logs.size // 8
         self.res = tuple(res)
         self.cfg = make_config(res, size, vcycles, max_levels, precision, solve, div_tol, max_vcycles, noise_n, noise_seed,
-                               omega)
+                               omega, pre_sweeps, post_sweeps)
         self.h = lib().orc_create(ctypes.byref(self.cfg), _fp(logs), n_logs)
         if not self.h:
             raise RuntimeError("oracle create failed")

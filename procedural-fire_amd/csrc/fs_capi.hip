@@ -383,6 +383,8 @@ void fs_config_default(fs_config* c) {
     c->use_graph = 0;
     c->adaptive = 1;
     c->omega = 0.f;                                           // tolerance solves: 1.25
+    c->pre_sweeps = 0;                                        // tolerance solves: V(1,2)
+    c->post_sweeps = 0;
 }
 
 int fs_param_count(void) { return FS_NUM_PARAMS; }
@@ -408,6 +410,8 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     if (cfg->precision < FS_PREC_FP32 || cfg->precision > FS_PREC_BF16) return fail(FS_EINVAL, "bad precision");
     if (cfg->solve < FS_SOLVE_VCYCLES || cfg->solve > FS_SOLVE_TOL32) return fail(FS_EINVAL, "bad solve mode");
     if (!(cfg->omega == 0.f || (cfg->omega > 0.f && cfg->omega < 2.f))) return fail(FS_EINVAL, "omega must be 0 or in (0, 2)");
+    if (cfg->pre_sweeps < 0 || cfg->pre_sweeps > 8 || cfg->post_sweeps < 0 || cfg->post_sweeps > 8)
+        return fail(FS_EINVAL, "pre/post sweeps must be in 0..8");
     if (cfg->vcycles < 0 || cfg->max_vcycles < 0 || cfg->max_vcycles > MG_MAX_REFINE) return fail(FS_EINVAL, "bad V-cycle count");
     const int nn = cfg->noise_n;
     if (nn < 4 || (nn & (nn - 1)) != 0 || nn > 512) return fail(FS_EINVAL, "noise_n must be a power of two in [4, 512]");
@@ -447,7 +451,11 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     int rc = mg_build(h->mg, nx, ny, nz, (float)h->dx, cfg->max_levels, err);
     if (rc) return bail(fail(rc, err));
     // over-relaxed smoothing in the tolerance solves only; the reference's solve is plain GS
-    if (cfg->solve != FS_SOLVE_VCYCLES) h->mg.omega = cfg->omega > 0.f ? cfg->omega : 1.25f;
+    if (cfg->solve != FS_SOLVE_VCYCLES) {
+        h->mg.omega = cfg->omega > 0.f ? cfg->omega : 1.25f;
+        h->mg.pre = cfg->pre_sweeps > 0 ? cfg->pre_sweeps : 1;
+        h->mg.post = cfg->post_sweeps > 0 ? cfg->post_sweeps : 2;
+    }
     for (auto& e : h->ev) if (hipEventCreate(&e) != hipSuccess) return bail(fail(FS_EDEVICE, "event create failed"));
     if (hipHostMalloc((void**)&h->host_state, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return bail(fail(FS_ENOMEM, "pinned host allocation failed"));

@@ -81,6 +81,7 @@ struct GpuMG {
     bool use_tail = true;        // false: one launch per colour pass on every level
     bool fuse_prolong = true;    // prolongation folded into the first post-smoothing pass
     float omega = 1.f;           // over-relaxation of the smoother (all levels but the coarsest)
+    int pre = 2, post = 2;       // sweeps before / after the coarse correction (all levels but the coarsest)
 };
 
 // Multigrid arrays are parity-split: each x-row stores its even-x cells, then its odd-x

@@ -425,6 +425,7 @@ struct TailArgs {
     int off[MG_MAX_LEVELS];          // float offset of level j's p in LDS; b follows p
     int coarse_iters;
     float omega;                     // smoother over-relaxation above the coarsest level
+    int pre, post;                   // sweeps before / after the coarse correction
     float* p_top;                    // global p / b of the tail's top level (parity-split layout)
     const float* b_top;
 };
@@ -585,7 +586,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailArgs a, const int*
         float* b = p + n;
         const int nc = a.dims[j + 1][0] * a.dims[j + 1][1] * a.dims[j + 1][2];
         float* pc = lds + a.off[j + 1];
-        tail_smooth(p, b, nx, ny, nz, a.h[j], 2, a.dhx[j], a.dy[j], a.omega);
+        tail_smooth(p, b, nx, ny, nz, a.h[j], a.pre, a.dhx[j], a.dy[j], a.omega);
         tail_restrict(p, b, nx, ny, nz, a.h[j], pc + nc, pc, a.dcx[j], a.dcy[j]);
     }
     {
@@ -598,7 +599,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailArgs a, const int*
         const int nx = a.dims[j][0], ny = a.dims[j][1], nz = a.dims[j][2], n = nx * ny * nz;
         float* p = lds + a.off[j];
         tail_prolong(p, lds + a.off[j + 1], nx, ny, nz, a.dx[j], a.dy[j]);
-        tail_smooth(p, p + n, nx, ny, nz, a.h[j], 2, a.dhx[j], a.dy[j], a.omega);
+        tail_smooth(p, p + n, nx, ny, nz, a.h[j], a.post, a.dhx[j], a.dy[j], a.omega);
     }
     for (int i = threadIdx.x; i < n0; i += TAIL_THREADS) {
         const int t = fdiv(i, a.dx[0]), x = i - t * nx0;
@@ -713,6 +714,8 @@ static void launch_tail(GpuMG& mg, hipStream_t s, const int* skip) {
     }
     a.coarse_iters = mg.coarse_iters;
     a.omega = mg.omega;
+    a.pre = mg.pre;
+    a.post = mg.post;
     a.p_top = mg.p[T];
     a.b_top = mg.b[T];
     k_mg_tail<<<1, TAIL_THREADS, off * sizeof(float), s>>>(a, skip);
@@ -728,7 +731,7 @@ bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd) {   // s
     }
     const int last = std::min(T, n - 1);                // levels 0..last-1 smooth and restrict
     for (int l = 0; l < last; l++) {
-        mg_smooth(mg, s, l, 2, skip);
+        mg_smooth(mg, s, l, mg.pre, skip);
         launch_restrict(mg, s, l, skip);
     }
     if (T < n) launch_tail(mg, s, skip);
@@ -736,7 +739,7 @@ bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd) {   // s
     for (int l = last - 1; l >= 0; l--) {
         // coarse levels: one launch less; fine levels: cheaper apart.  Over-relaxation reads the
         // colour-1 cells' own prolonged values, which the fused pass leaves unformed: not fused then.
-        if (mg.fuse_prolong && l >= 2 && mg.omega == 1.f) {
+        if (mg.fuse_prolong && l >= 2 && mg.omega == 1.f && mg.post == 2) {
             // prolongation + colour 0, then colour 1, then one more full sweep (sim.js:468-495, 576-586)
             const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
             const dim3 g = grid3(nx / 2, ny, nz);
@@ -745,7 +748,7 @@ bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd) {   // s
             mg_smooth(mg, s, l, 1, skip, l == 0 ? xupd : nullptr);
         } else {
             launch_prolong(mg, s, l, skip);
-            mg_smooth(mg, s, l, 2, skip, l == 0 ? xupd : nullptr);
+            mg_smooth(mg, s, l, mg.post, skip, l == 0 ? xupd : nullptr);
         }
     }
     return xupd != nullptr && last > 0;

@@ -151,10 +151,11 @@ def test_deterministic_rerun():
         assert np.array_equal(sims[0].read(f, "native"), sims[1].read(f, "native"))
 
 
-def test_tol32_solve_parity():
+@pytest.mark.parametrize("sweeps", [(0, 0), (1, 2)], ids=["V22", "V12"])
+def test_tol32_solve_parity(sweeps):
     """FP32 V-cycles until the FP64 residual reaches the tolerance: same cycle counts and bits as the oracle."""
-    print("tol32", flush=True)
-    gpu, orc = make_pair(F.PREC_FP32, warm_steps=20, solve=F.SOLVE_TOL32, div_tol=1e-4, max_vcycles=8)
+    print(f"tol32 sweeps {sweeps}", flush=True)
+    gpu, orc = make_pair(F.PREC_FP32, warm_steps=20, solve=F.SOLVE_TOL32, div_tol=1e-4, max_vcycles=8, sweeps=sweeps)
     for _ in range(3):
         gpu.step(DT, 1)
         orc.step(DT, 1)
