@@ -105,7 +105,9 @@ void mg_free(GpuMG& mg);
 // FP32 V(2,2) cycle on level-0 p/b; `skip` (device int, may be null) = early exit flag.
 // With `xupd` set, x += p (FP64) is fused into the cycle's last colour pass where
 // possible; returns true if it was.
-bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd = nullptr);
+// With `red0_done`, level 0's first colour-0 pass has already been written (level 0 started
+// from p = 0 and the kernel that zeroed it did that pass).
+bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd = nullptr, bool red0_done = false);
 // FP64 refinement: r = bkeep - A x; sets level-0 b = r, p = 0; reduces norms;
 // then up to max_cycles of {V-cycle, x += p, residual}; all device-side.
 void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
@@ -120,7 +122,8 @@ void mg_tol32_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 // relative FP64 residual of the FP32 level-0 p against level-0 b -> st->rel[0]
 void mg_residual_of_p(GpuMG& mg, hipStream_t s);
 // single-operator launches (tests)
-void mg_smooth(GpuMG& mg, hipStream_t s, int level, int iters, const int* skip, double* xupd = nullptr);
+void mg_smooth(GpuMG& mg, hipStream_t s, int level, int iters, const int* skip, double* xupd = nullptr,
+               bool first_red_done = false);
 
 // ---- error handling ----------------------------------------------------------
 void fs_set_error(const std::string& msg);

@@ -38,6 +38,27 @@ constexpr int RB = 256;   // threads per block of every multigrid kernel (64 x 4
 // grid over (half-row index or x, y, z) with 64 x 4 threads per block
 inline dim3 grid3(int w, int ny, int nz) { return dim3((unsigned)((w + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz); }
 
+// The colour-0 Gauss-Seidel update of cell (x, y, z) on a level whose iterate is all
+// zero: every neighbour contributes +0, so the update needs only the cell's own b.  The
+// kernels that zero a level (restriction, the refinement's residual) write it directly;
+// the operations are k_smooth's, in its order, so the values are identical.
+__device__ __forceinline__ float red_from_zero(float bv, int x, int y, int z, int nx, int ny, int nz, float h, float omega) {
+    float diag = 6.f;
+    if (x == 0) diag += 1.f;
+    if (x == nx - 1) diag += 1.f;
+    if (z == 0) diag += 1.f;
+    if (z == nz - 1) diag += 1.f;
+    if (y == 0) diag -= 1.f;
+    if (y == ny - 1) diag += 1.f;
+    const float sum = 0.f;
+    float pn = (sum - h * h * bv) / diag;
+    if (omega != 1.f) {
+        const float po = 0.f;
+        pn = po + omega * (pn - po);
+    }
+    return pn;
+}
+
 // one colour pass of red-black GS (sim.js:422-437).  Thread (hx, y, z) owns the cell
 // x = 2hx + s of this colour, s = (y + z + color) & 1.  With `xupd` set (the last pass
 // of a correction V-cycle) it also adds the final p of both cells x = 2hx, 2hx+1 to
@@ -93,7 +114,7 @@ __global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const floa
 // residual of the 8 children averaged onto the coarse cell; coarse p = 0 (sim.js:446-466)
 __global__ __launch_bounds__(RB) void k_restrict(const float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
                                                  float h, float* __restrict__ bc, float* __restrict__ pc,
-                                                 const int* __restrict__ skip) {
+                                                 const int* __restrict__ skip, float hc, float omega, int red_first) {
     const int sk = skip[fs_opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     const int cx = nx >> 1, cy = ny >> 1;
     const int xc = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -140,8 +161,10 @@ __global__ __launch_bounds__(RB) void k_restrict(const float* __restrict__ p, co
     if (sk) return;
     const int nhc = (cx + 1) >> 1;
     const int ci = (zc * cy + yc) * 2 * nhc + (xc & 1) * nhc + (xc >> 1);
-    bc[ci] = acc * 0.125f;
-    pc[ci] = 0.f;
+    const float bnew = acc * 0.125f;
+    bc[ci] = bnew;
+    // the coarse iterate starts at zero; with red_first its colour-0 cells get their first update
+    pc[ci] = (red_first && ((xc + yc + zc) & 1) == 0) ? red_from_zero(bnew, xc, yc, zc, cx, cy, nz >> 1, hc, omega) : 0.f;
 }
 
 // p_fine += mirrored trilinear interpolation of the coarse p (sim.js:473-495);
@@ -333,7 +356,8 @@ __device__ __forceinline__ double resid_at(const X* __restrict__ x, const float*
 template <class X>
 __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const float* __restrict__ b, int nx, int ny, int nz,
                                                 double h, float* __restrict__ rhs_out, float* __restrict__ p_out,
-                                                double* __restrict__ partials, const int* __restrict__ skip) {
+                                                double* __restrict__ partials, const int* __restrict__ skip,
+                                                float hred = 0.f, float omega = 1.f, int red_first = 0) {
     __shared__ double sr[RB], sb[RB];
     const int nh = (nx + 1) >> 1, rs = 2 * nh;
     const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -359,8 +383,11 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
             bb += bv[sgn] * bv[sgn];
             if (rhs_out) {
                 const int own = (z * ny + y) * rs + sgn * nh + hx;
-                rhs_out[own] = (float)r[sgn];
-                p_out[own] = 0.f;
+                const int xx = 2 * hx + sgn;
+                const float bf = (float)r[sgn];
+                rhs_out[own] = bf;
+                p_out[own] = (red_first && ((xx + y + z) & 1) == 0) ? red_from_zero(bf, xx, y, z, nx, ny, nz, hred, omega)
+                                                                   : 0.f;
             }
         }
     }
@@ -426,6 +453,7 @@ struct TailArgs {
     int coarse_iters;
     float omega;                     // smoother over-relaxation above the coarsest level
     int pre, post;                   // sweeps before / after the coarse correction
+    int red_done;                    // the top level's first colour-0 pass was done by the restriction
     float* p_top;                    // global p / b of the tail's top level (parity-split layout)
     const float* b_top;
 };
@@ -436,10 +464,11 @@ struct TailArgs {
 // overlap instead of running as one dependent chain per cell.
 constexpr int TAIL_ILP = 4;
 __device__ __forceinline__ void tail_smooth(float* p, const float* b, int nx, int ny, int nz, float h, int iters,
-                                            FastDiv dhx, FastDiv dy, float omega) {
+                                            FastDiv dhx, FastDiv dy, float omega, int first_red_done = 0) {
     const int nhx = (nx + 1) >> 1, nh = nhx * ny * nz, sy = nx, sz = nx * ny;
     for (int it = 0; it < iters; it++)
         for (int color = 0; color < 2; color++) {
+            if (it == 0 && color == 0 && first_red_done) continue;   // written by the restriction
             if (nh <= TAIL_THREADS) {          // one cell per thread at most: the plain pass
                 const int i = threadIdx.x;
                 const int t = fdiv(i, dhx), hx = i - t * nhx;
@@ -586,7 +615,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_mg_tail(TailArgs a, const int*
         float* b = p + n;
         const int nc = a.dims[j + 1][0] * a.dims[j + 1][1] * a.dims[j + 1][2];
         float* pc = lds + a.off[j + 1];
-        tail_smooth(p, b, nx, ny, nz, a.h[j], a.pre, a.dhx[j], a.dy[j], a.omega);
+        tail_smooth(p, b, nx, ny, nz, a.h[j], a.pre, a.dhx[j], a.dy[j], a.omega, j == 0 ? a.red_done : 0);
         tail_restrict(p, b, nx, ny, nz, a.h[j], pc + nc, pc, a.dcx[j], a.dcy[j]);
     }
     {
@@ -675,29 +704,32 @@ void mg_free(GpuMG& mg) {
     mg.nlev = 0;
 }
 
-void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, double* xupd) {   // sim.js:576-586
+void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, double* xupd, bool first_red_done) {   // sim.js:576-586
     if (!skip) skip = &mg.st->never;
     const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
     const dim3 g = grid3((nx + 1) / 2, ny, nz);
     const float om = l == mg.nlev - 1 ? 1.f : mg.omega;   // the coarsest level's sweeps stay plain GS
     for (int k = 0; k < iters; k++)
         for (int color = 0; color < 2; color++) {
+            if (k == 0 && color == 0 && first_red_done) continue;
             double* xu = (xupd && k == iters - 1 && color == 1) ? xupd : nullptr;
             k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, skip, xu, om);
         }
 }
 
-static void launch_restrict(GpuMG& mg, hipStream_t s, int l, const int* skip) {
+static void launch_restrict(GpuMG& mg, hipStream_t s, int l, const int* skip, bool red_first) {
     const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
-    k_restrict<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], mg.b[l + 1], mg.p[l + 1], skip);
+    k_restrict<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], mg.b[l + 1], mg.p[l + 1], skip,
+                                                            mg.h[l + 1], mg.omega, red_first ? 1 : 0);
 }
 static void launch_prolong(GpuMG& mg, hipStream_t s, int l, const int* skip) {
     const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
     k_prolong<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(mg.p[l], mg.p[l + 1], nx, ny, nz, skip);
 }
 
-static void launch_tail(GpuMG& mg, hipStream_t s, const int* skip) {
+static void launch_tail(GpuMG& mg, hipStream_t s, const int* skip, bool red_done) {
     TailArgs a;
+    a.red_done = red_done ? 1 : 0;
     const int T = mg.tail_from;
     a.nt = mg.nlev - T;
     int off = 0;
@@ -721,20 +753,29 @@ static void launch_tail(GpuMG& mg, hipStream_t s, const int* skip) {
     k_mg_tail<<<1, TAIL_THREADS, off * sizeof(float), s>>>(a, skip);
 }
 
-bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd) {   // sim.js:588-599
+// Whether the restriction into level l + 1 writes that level's first colour-0 pass: it
+// must be a pre-smoothed level that is not the coarsest (whose sweeps stay plain GS).
+static bool fuse_red_into(const GpuMG& mg, int lc, int T) {
+    if (mg.pre < 1 || lc >= mg.nlev - 1) return false;
+    return lc <= T;                                    // a per-level pass, or the tail's top level
+}
+
+bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd, bool red0_done) {   // sim.js:588-599
     if (!skip) skip = &mg.st->never;
     const int n = mg.nlev;
     const int T = mg.use_tail ? mg.tail_from : n;      // levels T.. run in the single-workgroup tail
     if (T == 0) {                                       // the whole hierarchy fits one workgroup's LDS
-        launch_tail(mg, s, skip);
+        launch_tail(mg, s, skip, false);
         return false;
     }
     const int last = std::min(T, n - 1);                // levels 0..last-1 smooth and restrict
+    bool red_done = red0_done;
     for (int l = 0; l < last; l++) {
-        mg_smooth(mg, s, l, mg.pre, skip);
-        launch_restrict(mg, s, l, skip);
+        mg_smooth(mg, s, l, mg.pre, skip, nullptr, red_done);
+        red_done = fuse_red_into(mg, l + 1, T);
+        launch_restrict(mg, s, l, skip, red_done);
     }
-    if (T < n) launch_tail(mg, s, skip);
+    if (T < n) launch_tail(mg, s, skip, red_done);
     else mg_smooth(mg, s, n - 1, mg.coarse_iters, skip);
     for (int l = last - 1; l >= 0; l--) {
         // coarse levels: one launch less; fine levels: cheaper apart.  Over-relaxation reads the
@@ -754,11 +795,19 @@ bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd) {   // s
     return xupd != nullptr && last > 0;
 }
 
+// In the refinement every V-cycle starts from p = 0 on level 0 (the residual kernel zeroes
+// it), so the residual kernel can write level 0's first colour-0 pass as well.
+static bool fuse_red0(const GpuMG& mg) {
+    const int T = mg.use_tail ? mg.tail_from : mg.nlev;
+    return mg.pre >= 1 && mg.nlev > 1 && T >= 1;
+}
+
 void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
+    const bool red = fuse_red0(mg);
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
-                                                                  mg.partials, &mg.st->never);
+                                                                  mg.partials, &mg.st->never, mg.h[0], mg.omega, red ? 1 : 0);
     k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
 }
 
@@ -766,12 +815,13 @@ void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     int* done = &mg.st->done;
-    if (!mg_vcycle(mg, s, done, mg.x64)) {
+    const bool red = fuse_red0(mg);
+    if (!mg_vcycle(mg, s, done, mg.x64, red)) {
         const int n = (int)level_floats(mg.dims[0]);
         k_update<<<(n + RB - 1) / RB, RB, 0, s>>>(mg.x64, mg.p[0], n, done);
     }
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
-                                                                  mg.partials, done);
+                                                                  mg.partials, done, mg.h[0], mg.omega, red ? 1 : 0);
     k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
 }
 
