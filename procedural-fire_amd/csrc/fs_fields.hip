@@ -11,87 +11,11 @@
 
 #include "fs_codec.h"
 #include "fs_internal.h"
+#include "fs_math.h"
+#include "fs_forces_cell.h"
 
 
 
-// ---- WGSL built-in semantics (clamp = min(max()), smoothstep, mix) -------------
-__device__ __forceinline__ float wclamp(float e, float lo, float hi) { return fminf(fmaxf(e, lo), hi); }
-__device__ __forceinline__ float sstep(float e0, float e1, float x) {
-    float t = wclamp((x - e0) / (e1 - e0), 0.f, 1.f);
-    return t * t * (3.f - 2.f * t);
-}
-__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.f - t) + b * t; }
-__device__ __forceinline__ float4 mix4(float4 a, float4 b, float t) {
-    return make_float4(mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t), mixf(a.w, b.w, t));
-}
-__device__ __forceinline__ float4 scale4(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
-
-// ---- exp / log / pow ----------------------------------------------------------
-// WGSL leaves exp and pow to the implementation; these are a fixed textbook
-// construction (identical to the oracle's, so results agree bit for bit):
-//   exp: Cody-Waite reduction x = k ln2 + r (hi part of ln2 has 9 trailing zero
-//        bits, so k*hi is exact for |k| < 512), degree-7 Taylor polynomial on
-//        |r| <= ln2/2 in Horner form, then ldexp(., k): <= 1.2 ulp.
-//   log: frexp to m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
-//        log m = 2s(1 + s^2/3 + s^4/5 + s^6/7 + s^8/9) + e ln2: <= 1.8 ulp.
-//   pow(a, b) = exp(b log a) for finite a > 0.
-__device__ __forceinline__ float fs_exp(float x) {
-    if (x != x) return x;
-    if (x > 88.72283f) return __int_as_float(0x7f800000);
-    if (x < -103.97208f) return 0.0f;
-    const float k = rintf(x * 1.44269504f);
-    const float r = (x - k * 0.693145751953125f) - k * 1.42860682e-6f;
-    float p = 1.0f / 5040.0f;
-    p = 1.0f / 720.0f + r * p;
-    p = 1.0f / 120.0f + r * p;
-    p = 1.0f / 24.0f + r * p;
-    p = 1.0f / 6.0f + r * p;
-    p = 0.5f + r * p;
-    p = 1.0f + r * p;
-    p = 1.0f + r * p;
-    return ldexpf(p, (int)k);
-}
-__device__ __forceinline__ float fs_log(float a) {
-    if (a != a || a < 0.0f) return __int_as_float(0x7fc00000);
-    if (a == 0.0f) return -__int_as_float(0x7f800000);
-    if (a == __int_as_float(0x7f800000)) return a;
-    int e;
-    float m = frexpf(a, &e);
-    if (m < 0.70710678f) { m = m * 2.0f; e = e - 1; }
-    const float s = (m - 1.0f) / (m + 1.0f);
-    const float s2 = s * s;
-    float q = 1.0f / 9.0f;
-    q = 1.0f / 7.0f + s2 * q;
-    q = 1.0f / 5.0f + s2 * q;
-    q = 1.0f / 3.0f + s2 * q;
-    const float t = 2.0f * s;
-    const float fe = (float)e;
-    return fe * 0.693145751953125f + (fe * 1.42860682e-6f + (t + t * (s2 * q)));
-}
-__device__ __forceinline__ float fs_pow(float a, float b) {
-    if (a == 1.0f || b == 0.0f) return 1.0f;
-    if (a == 0.0f) return b > 0.0f ? 0.0f : __int_as_float(0x7f800000);
-    return fs_exp(b * fs_log(a));
-}
-
-// ---- tiling -----------------------------------------------------------------
-__device__ __forceinline__ int xcd_remap(int hw, int total) {
-    const int xcd = hw & 7, q = total >> 3, r = total & 7;
-    return xcd * q + min(xcd, r) + (hw >> 3);
-}
-struct Cell { int x, y, z, i; };
-__device__ __forceinline__ bool cell_of(const Dims& g, Cell& c) {
-    const int lb = xcd_remap((int)blockIdx.x, g.nblocks);
-    const int bx = lb % g.nbx;
-    const int t = lb / g.nbx;
-    const int by = t % g.nby;
-    c.z = t / g.nby;
-    c.x = bx * TILE_X + ((int)threadIdx.x & 63);
-    c.y = by * TILE_Y + ((int)threadIdx.x >> 6);
-    c.i = (c.z * g.ny + c.y) * g.nx + c.x;
-    return c.x < g.nx && c.y < g.ny;
-}
 Dims make_dims(int nx, int ny, int nz) {
     Dims g;
     g.nx = nx; g.ny = ny; g.nz = nz;
@@ -99,55 +23,6 @@ Dims make_dims(int nx, int ny, int nz) {
     g.nby = (ny + TILE_Y - 1) / TILE_Y;
     g.nblocks = g.nbx * g.nby * nz;
     return g;
-}
-
-// clamped texel load (ld(), src/sim.js:114)
-template <class S>
-__device__ __forceinline__ float4 ldc(const typename S::raw* f, const Dims& g, int x, int y, int z) {
-    x = clampi(x, 0, g.nx - 1); y = clampi(y, 0, g.ny - 1); z = clampi(z, 0, g.nz - 1);
-    return Codec<S>::ld(f, (z * g.ny + y) * g.nx + x);
-}
-// trilinear, clamp-to-edge, at cell-space position b (texel centres at i + 0.5).
-// Clamping the coordinate to [-1, n] first leaves the clamped-texel result unchanged.
-template <class S>
-__device__ __forceinline__ float4 sample_clamp(const typename S::raw* f, const Dims& g, float bx, float by, float bz) {
-    float tx = wclamp(bx - 0.5f, -1.f, (float)g.nx);
-    float ty = wclamp(by - 0.5f, -1.f, (float)g.ny);
-    float tz = wclamp(bz - 0.5f, -1.f, (float)g.nz);
-    const float flx = floorf(tx), fly = floorf(ty), flz = floorf(tz);
-    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-    const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
-    const int x0 = clampi(ix, 0, g.nx - 1), x1 = clampi(ix + 1, 0, g.nx - 1);
-    const int y0 = clampi(iy, 0, g.ny - 1), y1 = clampi(iy + 1, 0, g.ny - 1);
-    const int z0 = clampi(iz, 0, g.nz - 1), z1 = clampi(iz + 1, 0, g.nz - 1);
-    const int r00 = (z0 * g.ny + y0) * g.nx, r10 = (z0 * g.ny + y1) * g.nx;
-    const int r01 = (z1 * g.ny + y0) * g.nx, r11 = (z1 * g.ny + y1) * g.nx;
-    const float4 c000 = Codec<S>::ld(f, r00 + x0), c100 = Codec<S>::ld(f, r00 + x1);
-    const float4 c010 = Codec<S>::ld(f, r10 + x0), c110 = Codec<S>::ld(f, r10 + x1);
-    const float4 c001 = Codec<S>::ld(f, r01 + x0), c101 = Codec<S>::ld(f, r01 + x1);
-    const float4 c011 = Codec<S>::ld(f, r11 + x0), c111 = Codec<S>::ld(f, r11 + x1);
-    const float4 x00 = mix4(c000, c100, fx), x10 = mix4(c010, c110, fx);
-    const float4 x01 = mix4(c001, c101, fx), x11 = mix4(c011, c111, fx);
-    return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
-}
-// trilinear with repeat addressing at normalised coordinate q (noise textures, n = 2^k)
-__device__ __forceinline__ float4 ld_h4(const half4_t* t, int i) {
-    half4_t h = t[i];
-    return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
-}
-__device__ __forceinline__ float4 sample_rep(const half4_t* t, int n, float qx, float qy, float qz) {
-    const float tx = qx * (float)n - 0.5f, ty = qy * (float)n - 0.5f, tz = qz * (float)n - 0.5f;
-    const float flx = floorf(tx), fly = floorf(ty), flz = floorf(tz);
-    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-    const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
-    const int m = n - 1;
-    const int x0 = ix & m, x1 = (ix + 1) & m, y0 = iy & m, y1 = (iy + 1) & m, z0 = iz & m, z1 = (iz + 1) & m;
-    const int r00 = (z0 * n + y0) * n, r10 = (z0 * n + y1) * n, r01 = (z1 * n + y0) * n, r11 = (z1 * n + y1) * n;
-    const float4 x00 = mix4(ld_h4(t, r00 + x0), ld_h4(t, r00 + x1), fx);
-    const float4 x10 = mix4(ld_h4(t, r10 + x0), ld_h4(t, r10 + x1), fx);
-    const float4 x01 = mix4(ld_h4(t, r01 + x0), ld_h4(t, r01 + x1), fx);
-    const float4 x11 = mix4(ld_h4(t, r11 + x0), ld_h4(t, r11 + x1), fx);
-    return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
 }
 
 // ---- geo: emitter geometry (src/sim.js:498-523, sdCapsule src/shaders/lib.js:58-62)
@@ -186,20 +61,6 @@ __global__ __launch_bounds__(256) void k_geo(const DevParams* __restrict__ P, Di
 
 // ---- advection (src/sim.js:123-197) -----------------------------------------
 template <class S>
-__device__ __forceinline__ float3 backtrace(const typename S::raw* vel, const Dims& g, float cx, float cy, float cz,
-                                            float dt, float dx) {   // sim.js:130-135
-    const float4 v0 = sample_clamp<S>(vel, g, cx, cy, cz);
-    const float hx = cx - v0.x * (0.5f * dt) / dx, hy = cy - v0.y * (0.5f * dt) / dx, hz = cz - v0.z * (0.5f * dt) / dx;
-    const float4 v1 = sample_clamp<S>(vel, g, hx, hy, hz);
-    return make_float3(cx - v1.x * dt / dx, cy - v1.y * dt / dx, cz - v1.z * dt / dx);
-}
-__device__ __forceinline__ float outside_fade(const Dims& g, float x, float y, float z) {   // sim.js:137-140
-    const float lo = fminf(x, z);
-    const float hi = fmaxf(x - (float)g.nx, fmaxf(y - (float)g.ny, z - (float)g.nz));
-    return wclamp(1.f - fmaxf(-lo, hi), 0.f, 1.f);
-}
-
-template <class S>
 __global__ __launch_bounds__(256) void k_adv1(const DevParams* __restrict__ P, Dims g,
                                               const typename S::raw* __restrict__ vel, const typename S::raw* __restrict__ sF,
                                               const typename S::raw* __restrict__ s2F, typename S::raw* __restrict__ velX,
@@ -218,7 +79,6 @@ __global__ __launch_bounds__(256) void k_adv1(const DevParams* __restrict__ P, D
     Codec<S>::st(s2X, c.i, scale4(sample_clamp<S>(s2F, g, b.x, b.y, b.z), f));
 }
 
-__device__ __forceinline__ float clamp_lim(float v, float mn, float mx, float h) { return wclamp(v, fminf(mn, h), fmaxf(mx, h)); }
 
 template <class S>
 __global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, Dims g,
@@ -305,184 +165,22 @@ __global__ __launch_bounds__(256) void k_forces(const DevParams* __restrict__ P,
                                                 typename S::raw* __restrict__ sOut, typename S::raw* __restrict__ s2Out) {
     Cell c;
     if (!cell_of(g, c)) return;
-    const float* U = P->f;
-    const float dt = (float)P->dt, dx = U[P_dx], tm = (float)P->time, I = U[P_intensity];
-    const float px = U[P_ox] + ((float)c.x + 0.5f) * dx;
-    const float py = U[P_oy] + ((float)c.y + 0.5f) * dx;
-    const float pz = U[P_oz] + ((float)c.z + 0.5f) * dx;
+    const ForcesCell k = forces_setup<S>(P, g, c, velY, geo, noise, curlT, nn);
     const float4 v4 = Codec<S>::ld(velY, c.i);
-    float vx = v4.x, vy = v4.y, vz = v4.z;
-    float4 s = Codec<S>::ld(sY, c.i);
-    float4 s2 = Codec<S>::ld(s2Y, c.i);
-    const float4 gg = Codec<S>::ld(geo, c.i);
-
-    // ---- sources (sim.js:246-297)
-    float src = 0.f, anchor = 0.f;
-    if (gg.x > 0.001f) {
-        const float af = U[P_anchorFreq];
-        const float4 na = sample_rep(noise, nn, px * af + U[P_seed] * 0.37f, py * af + 0.f, pz * af + 0.11f);
-        anchor = sstep(-0.05f, 0.25f, na.z * 2.f - 1.f);
-        const float sf = U[P_srcFreq] * 0.06f;
-        const float4 nq = sample_rep(noise, nn, px * sf + U[P_seed] * 0.131f, py * sf + 0.f, pz * sf + tm * U[P_srcSpeed]);
-        const float pulse = sstep(U[P_srcThresh] - 0.25f, U[P_srcThresh] + 0.25f, nq.x * 2.f - 1.f);
-        src = wclamp(gg.x * U[P_logSrc] * (U[P_srcFloor] + U[P_anchorGain] * anchor) * mixf(0.55f, 1.f, pulse) * 1.2f, 0.f, 1.f) * I;
-        src *= 1.f - gg.z;
-    }
-    const float srcW = fminf(src, 1.f);
-    const float bedHot = gg.y;
-    float bed = gg.y * wclamp(I * 1.4f, 0.3f, 1.f);
-    if (bed > 0.001f) {
-        const float4 nb = sample_rep(noise, nn, px * 2.4f + 0.17f, py * 2.4f + -tm * 0.12f, pz * 2.4f + tm * 0.05f);
-        bed *= sstep(0.f, 0.3f, nb.y * 2.f - 1.f);
-    }
-    s.y = fmaxf(s.y, src * U[P_fuelSrc]);
-    s.y = fmaxf(s.y, bed * U[P_bedFuel]);
-    s.x = fmaxf(s.x, src * U[P_srcTemp]);
-    s.x = fmaxf(s.x, bedHot * U[P_bedTemp]);
-    float srcAll = fmaxf(srcW, bed);
-    s2.w = mixf(s2.w, 1.f, bed * (1.f - srcW));
-    s2.w = mixf(s2.w, 0.f, srcW);
-    s2.y = s2.y * (1.f - srcAll);
-    s2.x = fmaxf(s2.x, (1.f - s.y) * srcW);
-    s2.x = fmaxf(s2.x, (1.f - s.y) * bed * 0.6f);
-    const float wsrc = 1.f - fs_pow(1.f - wclamp(srcW + bed * 0.4f, 0.f, 1.f) * 0.5f, dt * 90.f);
-    vy = mixf(vy, fmaxf(vy, U[P_srcVel]), wsrc);
-    s2.z += gg.x * (1.f - anchor) * U[P_smolder] * dt * I;
-    if (U[P_burstAmt] > 0.f) {
-        const float ddx = px - U[P_burstX], ddy = py - U[P_burstY], ddz = pz - U[P_burstZ];
-        const float bd = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
-        const float bw = wclamp(fs_exp(-bd * bd / 0.006f) * U[P_burstAmt], 0.f, 1.f);
-        s.y = fmaxf(s.y, 0.17f * bw);
-        s.x = fmaxf(s.x, 0.9f * bw);
-        s2.x *= 1.f - bw;
-        s2.w *= 1.f - bw;
-        s2.y = fmaxf(s2.y, 0.05f * bw * wclamp(s.y + s2.x, 0.f, 1.f));
-    }
-    // ---- pyrolysis spots and floor fuel flux (build extension: mass-flux sources)
-    for (int k = 0; k < P->n_spots; k++) {
-        const fs_spot sp = spots[k];
-        const float ddx = px - sp.pos[0], ddy = py - sp.pos[1], ddz = pz - sp.pos[2];
-        const float d = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
-        if (d < sp.radius) {
-            const float w = 1.f - sstep(0.5f * sp.radius, sp.radius, d);
-            s.y = fminf(s.y + sp.flux * dt * sp.norm * w, 1.f);
-            s.x = fmaxf(s.x, w * sp.temp);
-            s2.w = mixf(s2.w, 0.f, w);
-            s2.y = s2.y * (1.f - w);
-            srcAll = fmaxf(srcAll, w);
-        }
-    }
-    if (P->has_face_flux && c.y == 0) s.y = fminf(s.y + faceFlux[c.z * g.nx + c.x] * dt / (FS_RHO0 * dx), 1.f);
-    float ageF = s2.y / fmaxf(s.y + s2.x, 1e-4f);
-
-    // ---- mixing-limited combustion (sim.js:300-309)
-    const float T0 = 293.f + 1000.f * s.x;
-    const float o = wclamp(1.f - s.y - s2.x, 0.f, 1.f);
-    const float ign = sstep(U[P_ignT] - 50.f, U[P_ignT] + 50.f, T0);
-    const float b = fminf(s.y, o / U[P_rs]) * (1.f - fs_exp(-U[P_kMix] * ign * dt));
-    s.y -= b;
-    s2.x = fminf(s2.x + (1.f + U[P_rs]) * b, 1.f);
-    const float dT = fminf(U[P_heatOut] * b, fmaxf(2100.f - T0, 0.f));
-    float T = T0 + dT;
-    const float q = dT / dt;
-    const float phi = gg.z > 0.5f ? 0.f : fminf(U[P_expansion] * q / T, 60.f);
-
-    // ---- soot (sim.js:312-318)
-    ageF = fminf(ageF + dt, 4.f);
-    s.z += U[P_sootForm] * b * sstep(0.015f, 0.05f, ageF) * sstep(1000.f, 1400.f, T) * (1.f - s2.w);
-    s.z *= fs_exp(-U[P_sootOx] * (0.25f + o) * sstep(1100.f, 1400.f, T) * dt);
-    s.z *= fs_exp(-U[P_sootOx] * 0.5f * o * sstep(850.f, 1100.f, T) * (1.f - sstep(1100.f, 1300.f, T)) * dt);
-    s.z *= fs_exp(-U[P_sootDissip] * dt);
-
-    // ---- cooling (sim.js:321-330)
-    const float th = T * 0.001f;
-    T -= fminf(U[P_radCool] * 970.f * s.z * th * th * (th * th * th * th - 0.00737f), 6000.f) * dt;
-    const float notBurning = 1.f - sstep(500.f, 5000.f, q);
-    T = 293.f + fmaxf(T - 293.f, 0.f) * fs_exp(-U[P_coolRate] * notBurning * dt);
-    s.x = (T - 293.f) * 0.001f;
-    s.w = q * 1e-4f;
-    s.y *= fs_exp(-U[P_fuelDissip] * dt);
-    s2.x *= fs_exp(-U[P_vitDecay] * dt);
-    s2.z *= fs_exp(-U[P_smokeDissip] * dt);
-    s2.y = ageF * wclamp(s.y + s2.x, 0.f, 1.f);
-
-    // ---- buoyancy (sim.js:333)
-    vy += U[P_buoyancy] * s.x * dt;
-
-    // ---- hot-gas viscosity (sim.js:339-346)
-    if (U[P_visc] > 0.f) {
-        const float4 a = ldc<S>(velY, g, c.x + 1, c.y, c.z), bb = ldc<S>(velY, g, c.x - 1, c.y, c.z);
-        const float4 cc = ldc<S>(velY, g, c.x, c.y + 1, c.z), dd = ldc<S>(velY, g, c.x, c.y - 1, c.z);
-        const float4 ee = ldc<S>(velY, g, c.x, c.y, c.z + 1), ff = ldc<S>(velY, g, c.x, c.y, c.z - 1);
-        const float4 me = Codec<S>::ld(velY, c.i);
-        const float lx = a.x + bb.x + cc.x + dd.x + ee.x + ff.x - me.x * 6.f;
-        const float ly = a.y + bb.y + cc.y + dd.y + ee.y + ff.y - me.y * 6.f;
-        const float lz = a.z + bb.z + cc.z + dd.z + ee.z + ff.z - me.z * 6.f;
-        const float nu = U[P_visc] * 1.5e-5f * fs_pow(fmaxf(T, 293.f) / 293.f, 1.7f);
-        const float k = fminf(nu * dt / (dx * dx), 0.15f);
-        vx = vx + lx * k; vy = vy + ly * k; vz = vz + lz * k;
-    }
-    // ---- laminar base gate (sim.js:350)
-    const float lamAge = U[P_lamAge];
-    const float lam = lamAge > 0.f ? sstep(0.5f * lamAge, fmaxf(lamAge, 1e-3f), ageF) : 1.f;
-
-    // ---- vorticity confinement (sim.js:353-361)
+    const float4 s4 = Codec<S>::ld(sY, c.i), t4 = Codec<S>::ld(s2Y, c.i);
     const float4 w = Codec<S>::ld(vort, c.i);
     const float gx = ldc<S>(vort, g, c.x + 1, c.y, c.z).w - ldc<S>(vort, g, c.x - 1, c.y, c.z).w;
     const float gy = ldc<S>(vort, g, c.x, c.y + 1, c.z).w - ldc<S>(vort, g, c.x, c.y - 1, c.z).w;
     const float gz = ldc<S>(vort, g, c.x, c.y, c.z + 1).w - ldc<S>(vort, g, c.x, c.y, c.z - 1).w;
-    const float gl = sqrtf(gx * gx + gy * gy + gz * gz);
-    if (gl > 1e-6f) {
-        const float gate = sstep(450.f, 900.f, T) * (1.f - srcAll) * sstep(0.02f, 0.08f, py) * lam;
-        const float nx_ = gx / gl, ny_ = gy / gl, nz_ = gz / gl;
-        const float k = U[P_vortEps] * gate * dt;
-        vx = vx + (ny_ * w.z - nz_ * w.y) * k;
-        vy = vy + (nz_ * w.x - nx_ * w.z) * k;
-        vz = vz + (nx_ * w.y - ny_ * w.x) * k;
-    }
-    // ---- rising curl noise (sim.js:364-368)
-    const float tf = U[P_turbFreq];
-    const float tqx = (px - 0.f) * tf + 0.f;
-    const float tqy = (py - tm * U[P_turbRise]) * tf + 0.f;
-    const float tqz = (pz - 0.f) * tf + tm * U[P_turbEvolve];
-    const float4 cn = sample_rep(curlT, nn, tqx, tqy, tqz);
-    const float tgate = fmaxf(sstep(500.f, 1100.f, T), sstep(0.f, 3.f, s.w)) * (1.f - srcAll) *
-                        mixf(0.6f, 1.8f, sstep(0.2f, 0.75f, py)) * lam;
-    {
-        const float k = (U[P_turbAmp] / 12.f) * tgate * dt * 10.f;
-        vx = vx + cn.x * k; vy = vy + cn.y * k; vz = vz + cn.z * k;
-    }
-    // ---- wind (sim.js:372-375)
-    const float windK = sstep(0.05f, 0.5f, py) * (1.f + U[P_gust]);
-    vx += (U[P_windX] * windK - vx) * dt * 0.35f * windK;
-    vz += (U[P_windZ] * windK - vz) * dt * 0.35f * windK;
-    {
-        const float e = fs_exp(-U[P_velDissip] * dt);
-        vx = vx * e; vy = vy * e; vz = vz * e;
-    }
-    // ---- solids and sponge layers (sim.js:378-388)
-    if (gg.z > 0.5f) { vx = 0.f; vy = 0.f; vz = 0.f; s.y = 0.f; s2.y = 0.f; }
-    const float fpx = (float)c.x + 0.5f, fpy = (float)c.y + 0.5f, fpz = (float)c.z + 0.5f;
-    const float side = fminf(fminf(fpx, (float)g.nx - fpx), fminf(fpz, (float)g.nz - fpz));
-    const float sp = (1.f - sstep(0.f, 5.f, side)) * U[P_sponge];
-    const float wsg = sstep(0.03f, 0.4f, py);
-    const float wSx = U[P_windX] * wsg, wSz = U[P_windZ] * wsg;
-    const float ks = 1.f - fs_pow(1.f - 0.15f * wclamp(sp, 0.f, 1.f), dt * 90.f);
-    vx = mixf(vx, wSx, ks);
-    vz = mixf(vz, wSz, ks);
-    const float top = sstep((float)g.ny - 10.f, (float)g.ny, fpy);
-    const float kt = fs_pow(1.f - 0.1f * top, dt * 90.f);
-    s.z *= kt;
-    s2.z *= kt;
-    // ---- clamps and NaN reset (sim.js:390-392)
-    const float vl = sqrtf(vx * vx + vy * vy + vz * vz);
-    if (vl > 15.f) { const float k = 15.f / vl; vx = vx * k; vy = vy * k; vz = vz * k; }
-    const bool bad = isnan(s.x) || isnan(s.y) || isnan(s.z) || isnan(s.w) || isnan(s2.x) || isnan(s2.y) ||
-                     isnan(s2.z) || isnan(s2.w) || isnan(vx) || isnan(vy) || isnan(vz);
-    if (bad) { s = make_float4(0.f, 0.f, 0.f, 0.f); s2 = s; vx = 0.f; vy = 0.f; vz = 0.f; }
+    float vx = v4.x, vy = v4.y, vz = v4.z, phi;
+    float s[4] = {s4.x, s4.y, s4.z, s4.w}, s2[4] = {t4.x, t4.y, t4.z, t4.w};
+    const float wv[3] = {w.x, w.y, w.z};
+    const bool face = P->has_face_flux != 0;
+    const float fv = (face && c.y == 0) ? faceFlux[c.z * g.nx + c.x] : 0.f;
+    forces_cell<float>(P->f, k, spots, P->n_spots, -1, face, fv, vx, vy, vz, s, s2, wv, gx, gy, gz, phi);
     Codec<S>::st(velOut, c.i, make_float4(vx, vy, vz, phi));
-    Codec<S>::st(sOut, c.i, s);
-    Codec<S>::st(s2Out, c.i, s2);
+    Codec<S>::st(sOut, c.i, make_float4(s[0], s[1], s[2], s[3]));
+    Codec<S>::st(s2Out, c.i, make_float4(s2[0], s2[1], s2[2], s2[3]));
 }
 
 // ---- divergence minus the expansion target (src/sim.js:403-413) ----------------

@@ -1,0 +1,152 @@
+// Device helpers shared by the step, diagnostics and adjoint kernels: WGSL built-in
+// semantics, the textbook exp/log/pow (bit-identical to the oracle's), the tiling of
+// the per-cell kernels, clamped texel loads and trilinear sampling, and the
+// semi-Lagrangian backtrace (src/sim.js:123-140).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fs_codec.h"
+#include "fs_internal.h"
+
+// ---- WGSL built-in semantics (clamp = min(max()), smoothstep, mix) -------------
+__device__ __forceinline__ float wclamp(float e, float lo, float hi) { return fminf(fmaxf(e, lo), hi); }
+__device__ __forceinline__ float sstep(float e0, float e1, float x) {
+    float t = wclamp((x - e0) / (e1 - e0), 0.f, 1.f);
+    return t * t * (3.f - 2.f * t);
+}
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.f - t) + b * t; }
+__device__ __forceinline__ float4 mix4(float4 a, float4 b, float t) {
+    return make_float4(mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t), mixf(a.w, b.w, t));
+}
+__device__ __forceinline__ float4 scale4(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// ---- exp / log / pow ----------------------------------------------------------
+// WGSL leaves exp and pow to the implementation; these are a fixed textbook
+// construction (identical to the oracle's, so results agree bit for bit):
+//   exp: Cody-Waite reduction x = k ln2 + r (hi part of ln2 has 9 trailing zero
+//        bits, so k*hi is exact for |k| < 512), degree-7 Taylor polynomial on
+//        |r| <= ln2/2 in Horner form, then ldexp(., k): <= 1.2 ulp.
+//   log: frexp to m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
+//        log m = 2s(1 + s^2/3 + s^4/5 + s^6/7 + s^8/9) + e ln2: <= 1.8 ulp.
+//   pow(a, b) = exp(b log a) for finite a > 0.
+__device__ __forceinline__ float fs_exp(float x) {
+    if (x != x) return x;
+    if (x > 88.72283f) return __int_as_float(0x7f800000);
+    if (x < -103.97208f) return 0.0f;
+    const float k = rintf(x * 1.44269504f);
+    const float r = (x - k * 0.693145751953125f) - k * 1.42860682e-6f;
+    float p = 1.0f / 5040.0f;
+    p = 1.0f / 720.0f + r * p;
+    p = 1.0f / 120.0f + r * p;
+    p = 1.0f / 24.0f + r * p;
+    p = 1.0f / 6.0f + r * p;
+    p = 0.5f + r * p;
+    p = 1.0f + r * p;
+    p = 1.0f + r * p;
+    return ldexpf(p, (int)k);
+}
+__device__ __forceinline__ float fs_log(float a) {
+    if (a != a || a < 0.0f) return __int_as_float(0x7fc00000);
+    if (a == 0.0f) return -__int_as_float(0x7f800000);
+    if (a == __int_as_float(0x7f800000)) return a;
+    int e;
+    float m = frexpf(a, &e);
+    if (m < 0.70710678f) { m = m * 2.0f; e = e - 1; }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float q = 1.0f / 9.0f;
+    q = 1.0f / 7.0f + s2 * q;
+    q = 1.0f / 5.0f + s2 * q;
+    q = 1.0f / 3.0f + s2 * q;
+    const float t = 2.0f * s;
+    const float fe = (float)e;
+    return fe * 0.693145751953125f + (fe * 1.42860682e-6f + (t + t * (s2 * q)));
+}
+__device__ __forceinline__ float fs_pow(float a, float b) {
+    if (a == 1.0f || b == 0.0f) return 1.0f;
+    if (a == 0.0f) return b > 0.0f ? 0.0f : __int_as_float(0x7f800000);
+    return fs_exp(b * fs_log(a));
+}
+
+// ---- tiling -----------------------------------------------------------------
+__device__ __forceinline__ int xcd_remap(int hw, int total) {
+    const int xcd = hw & 7, q = total >> 3, r = total & 7;
+    return xcd * q + min(xcd, r) + (hw >> 3);
+}
+struct Cell { int x, y, z, i; };
+__device__ __forceinline__ bool cell_of(const Dims& g, Cell& c) {
+    const int lb = xcd_remap((int)blockIdx.x, g.nblocks);
+    const int bx = lb % g.nbx;
+    const int t = lb / g.nbx;
+    const int by = t % g.nby;
+    c.z = t / g.nby;
+    c.x = bx * TILE_X + ((int)threadIdx.x & 63);
+    c.y = by * TILE_Y + ((int)threadIdx.x >> 6);
+    c.i = (c.z * g.ny + c.y) * g.nx + c.x;
+    return c.x < g.nx && c.y < g.ny;
+}
+// clamped texel load (ld(), src/sim.js:114)
+template <class S>
+__device__ __forceinline__ float4 ldc(const typename S::raw* f, const Dims& g, int x, int y, int z) {
+    x = clampi(x, 0, g.nx - 1); y = clampi(y, 0, g.ny - 1); z = clampi(z, 0, g.nz - 1);
+    return Codec<S>::ld(f, (z * g.ny + y) * g.nx + x);
+}
+// trilinear, clamp-to-edge, at cell-space position b (texel centres at i + 0.5).
+// Clamping the coordinate to [-1, n] first leaves the clamped-texel result unchanged.
+template <class S>
+__device__ __forceinline__ float4 sample_clamp(const typename S::raw* f, const Dims& g, float bx, float by, float bz) {
+    float tx = wclamp(bx - 0.5f, -1.f, (float)g.nx);
+    float ty = wclamp(by - 0.5f, -1.f, (float)g.ny);
+    float tz = wclamp(bz - 0.5f, -1.f, (float)g.nz);
+    const float flx = floorf(tx), fly = floorf(ty), flz = floorf(tz);
+    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+    const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
+    const int x0 = clampi(ix, 0, g.nx - 1), x1 = clampi(ix + 1, 0, g.nx - 1);
+    const int y0 = clampi(iy, 0, g.ny - 1), y1 = clampi(iy + 1, 0, g.ny - 1);
+    const int z0 = clampi(iz, 0, g.nz - 1), z1 = clampi(iz + 1, 0, g.nz - 1);
+    const int r00 = (z0 * g.ny + y0) * g.nx, r10 = (z0 * g.ny + y1) * g.nx;
+    const int r01 = (z1 * g.ny + y0) * g.nx, r11 = (z1 * g.ny + y1) * g.nx;
+    const float4 c000 = Codec<S>::ld(f, r00 + x0), c100 = Codec<S>::ld(f, r00 + x1);
+    const float4 c010 = Codec<S>::ld(f, r10 + x0), c110 = Codec<S>::ld(f, r10 + x1);
+    const float4 c001 = Codec<S>::ld(f, r01 + x0), c101 = Codec<S>::ld(f, r01 + x1);
+    const float4 c011 = Codec<S>::ld(f, r11 + x0), c111 = Codec<S>::ld(f, r11 + x1);
+    const float4 x00 = mix4(c000, c100, fx), x10 = mix4(c010, c110, fx);
+    const float4 x01 = mix4(c001, c101, fx), x11 = mix4(c011, c111, fx);
+    return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+}
+// trilinear with repeat addressing at normalised coordinate q (noise textures, n = 2^k)
+__device__ __forceinline__ float4 ld_h4(const half4_t* t, int i) {
+    half4_t h = t[i];
+    return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
+}
+__device__ __forceinline__ float4 sample_rep(const half4_t* t, int n, float qx, float qy, float qz) {
+    const float tx = qx * (float)n - 0.5f, ty = qy * (float)n - 0.5f, tz = qz * (float)n - 0.5f;
+    const float flx = floorf(tx), fly = floorf(ty), flz = floorf(tz);
+    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+    const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
+    const int m = n - 1;
+    const int x0 = ix & m, x1 = (ix + 1) & m, y0 = iy & m, y1 = (iy + 1) & m, z0 = iz & m, z1 = (iz + 1) & m;
+    const int r00 = (z0 * n + y0) * n, r10 = (z0 * n + y1) * n, r01 = (z1 * n + y0) * n, r11 = (z1 * n + y1) * n;
+    const float4 x00 = mix4(ld_h4(t, r00 + x0), ld_h4(t, r00 + x1), fx);
+    const float4 x10 = mix4(ld_h4(t, r10 + x0), ld_h4(t, r10 + x1), fx);
+    const float4 x01 = mix4(ld_h4(t, r01 + x0), ld_h4(t, r01 + x1), fx);
+    const float4 x11 = mix4(ld_h4(t, r11 + x0), ld_h4(t, r11 + x1), fx);
+    return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+}
+
+template <class S>
+__device__ __forceinline__ float3 backtrace(const typename S::raw* vel, const Dims& g, float cx, float cy, float cz,
+                                            float dt, float dx) {   // sim.js:130-135
+    const float4 v0 = sample_clamp<S>(vel, g, cx, cy, cz);
+    const float hx = cx - v0.x * (0.5f * dt) / dx, hy = cy - v0.y * (0.5f * dt) / dx, hz = cz - v0.z * (0.5f * dt) / dx;
+    const float4 v1 = sample_clamp<S>(vel, g, hx, hy, hz);
+    return make_float3(cx - v1.x * dt / dx, cy - v1.y * dt / dx, cz - v1.z * dt / dx);
+}
+__device__ __forceinline__ float outside_fade(const Dims& g, float x, float y, float z) {   // sim.js:137-140
+    const float lo = fminf(x, z);
+    const float hi = fmaxf(x - (float)g.nx, fmaxf(y - (float)g.ny, z - (float)g.nz));
+    return wclamp(1.f - fmaxf(-lo, hi), 0.f, 1.f);
+}
+
+__device__ __forceinline__ float clamp_lim(float v, float mn, float mx, float h) { return wclamp(v, fminf(mn, h), fmaxf(mx, h)); }
