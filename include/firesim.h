@@ -193,6 +193,9 @@ void* fs_device_ptr(fs_sim* h, int field);
 
 /* time (s), frame, grid, multigrid hierarchy (src/sim.js:80-92). */
 int  fs_get_clock(fs_sim* h, double* time, int64_t* frame);
+/* Sets the clock (time in s, frame count), e.g. to restart from a stored state; no
+ * counterpart in the reference, whose clock only advances (src/sim.js:646-647). */
+int  fs_set_clock(fs_sim* h, double time, int64_t frame);
 int  fs_grid_info(fs_sim* h, int res[3], double* dx, double origin[3]);
 int  fs_mg_info(fs_sim* h, int* levels, int dims[][3], int* coarse_iters);
 
@@ -264,6 +267,45 @@ int  fs_poisson_read_x(fs_poisson* ps, int dtype, void* host_x);
  * Gauss-Seidel).  No counterpart in the reference. */
 int  fs_poisson_set_omega(fs_poisson* ps, float omega);
 int  fs_poisson_info(fs_poisson* ps, int* levels, int dims[][3], int* coarse_iters);
+
+/* ---- reverse-mode derivative of a run of steps -------------------------------
+ * No counterpart in the reference (its gradients are planned, not built); these are
+ * the entry points a torch.autograd.Function binds.  An fs_adjoint belongs to one
+ * fs_sim; it runs the steps itself (fs_adjoint_forward, the same kernels as fs_step),
+ * keeping the state (vel, S, S2 and the pressure) every `checkpoint_every` steps,
+ * and fs_adjoint_backward sweeps the steps in reverse: each checkpoint interval is
+ * recomputed, then every step is replayed once more and its hand-written adjoint
+ * kernels run (projection, the pressure solve, divergence, sources, curl, MacCormack
+ * correction with the limiter's active set frozen, semi-Lagrangian pass).
+ * The derivative is that of the step with a converged pressure solve (the solve is
+ * x = A^-1 b for the symmetric 7-point operator A, so its adjoint is the same solve);
+ * requirements: FS_PREC_FP32 storage, FS_SOLVE_REFINE64, `visc` = 0.  Float atomics
+ * accumulate scattered adjoints, so results vary in the last bits run to run.
+ * Diagnostics records are not written by these steps. */
+typedef struct fs_adjoint fs_adjoint;
+
+int  fs_adjoint_create(fs_sim* h, int max_steps, int checkpoint_every, fs_adjoint** out);
+int  fs_adjoint_destroy(fs_adjoint* a);
+/* nsteps (<= max_steps) steps of dt from the sim's current state; the sim ends in the
+ * final state, its clock advanced, as after fs_step.  Clears any seed. */
+int  fs_adjoint_forward(fs_adjoint* a, double dt, int nsteps);
+/* The adjoint of the final state for field FS_VEL (xyz; w ignored), FS_S or FS_S2:
+ * host array of nx*ny*nz*4 values, x fastest (FS_DT_F32 or FS_DT_F64).  Unseeded
+ * fields are zero. */
+int  fs_adjoint_seed(fs_adjoint* a, int field, int dtype, const void* host, size_t bytes);
+/* Seeds the mismatch L = sum over cells of (theta - T*)^2 + (soot - soot*)^2 of the
+ * final state (theta, soot = channels 0, 2 of FS_S; targets: host arrays of nx*ny*nz
+ * floats, x fastest); returns L (FP64).  Replaces every seed. */
+int  fs_adjoint_seed_mismatch(fs_adjoint* a, const float* target_theta, const float* target_soot, double* loss);
+/* The reverse sweep; each adjoint pressure solve runs FP64 refinement to solve_tol
+ * (<= 0: the sim's div_tol) within max_cycles (<= 0: the sim's max_vcycles). */
+int  fs_adjoint_backward(fs_adjoint* a, double solve_tol, int max_cycles);
+/* The adjoint of the initial state (FS_VEL, FS_S or FS_S2), layout as fs_adjoint_seed. */
+int  fs_adjoint_read(fs_adjoint* a, int field, int dtype, void* host, size_t bytes);
+/* dL/d(flux) of each pyrolysis spot (kg/s; up to max_spots entries) and, when a floor
+ * flux is set (fs_set_face_flux), dL/d(face flux) per floor cell (nz*nx, x fastest;
+ * may be NULL). */
+int  fs_adjoint_grads(fs_adjoint* a, double* spot_flux, int max_spots, double* face_flux);
 
 #ifdef __cplusplus
 }

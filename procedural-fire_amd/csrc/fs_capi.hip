@@ -778,6 +778,15 @@ int fs_get_clock(fs_sim* h, double* time, int64_t* frame) {
     return FS_OK;
 }
 
+int fs_set_clock(fs_sim* h, double time, int64_t frame) {
+    if (!h) return fail(FS_EINVAL, "null handle");
+    if (!std::isfinite(time) || frame < 0) return fail(FS_EINVAL, "time must be finite and frame >= 0");
+    h->time = time;
+    h->frame = frame;
+    h->params_dirty = true;
+    return FS_OK;
+}
+
 int fs_grid_info(fs_sim* h, int res[3], double* dx, double origin[3]) {
     if (!h) return fail(FS_EINVAL, "null handle");
     if (res) { res[0] = h->nx; res[1] = h->ny; res[2] = h->nz; }
@@ -1131,6 +1140,346 @@ int fs_poisson_info(fs_poisson* ps, int* levels, int dims[][3], int* coarse_iter
     if (levels) *levels = ps->mg.nlev;
     if (dims) for (int l = 0; l < ps->mg.nlev; l++) for (int k = 0; k < 3; k++) dims[l][k] = ps->mg.dims[l][k];
     if (coarse_iters) *coarse_iters = ps->mg.coarse_iters;
+    return FS_OK;
+}
+
+}  // extern "C"
+
+// ---- reverse-mode derivative of a run of steps ------------------------------------
+struct AdjSlot {                     // the state at the start of a step
+    float4* vel = nullptr;
+    float4* S = nullptr;
+    float4* S2 = nullptr;
+    double* p = nullptr;             // FP64 pressure of the previous step (the solve's starting guess)
+};
+
+struct fs_adjoint {
+    fs_sim* h = nullptr;
+    int max_steps = 0, every = 1;
+    std::vector<AdjSlot> ckpt;       // states 0, every, 2 every, ...
+    std::vector<AdjSlot> seg;        // states inside one checkpoint interval (index = step % every; 0 unused)
+    AdjSlot fin;                     // the final state (seg is reused by the reverse sweep)
+    std::vector<double> times;       // clock at the start of each step
+    std::vector<int64_t> frames;
+    double dt = 0.0, t_end = 0.0;
+    int64_t frame_end = 0;
+    int nsteps = 0;
+    bool have_forward = false, have_grads = false;
+    bool last_interval_held = false;   // seg holds the last interval's states (true after the forward)
+    // adjoint fields: the state (l_vel, l_S, l_S2 = seed, then the result), the
+    // intermediates of one step, and the forces kernel's velocity output (vF)
+    enum { L_VEL, L_S, L_S2, L_VF, L_VY, L_SY, L_S2Y, L_VORT, L_VH, L_SH, L_S2H, VF, L_COUNT };
+    float4* l[L_COUNT] = {};
+    double* gspot = nullptr;         // FS_MAX_SPOTS
+    double* gface = nullptr;         // nx * nz
+    float* target = nullptr;         // 2 * ncell (mismatch targets)
+    double* partial = nullptr;       // per-block loss partials
+    std::vector<void*> owned;
+
+    const AdjSlot& slot(int k) const { return k % every == 0 ? ckpt[k / every] : seg[k % every]; }
+};
+
+namespace {
+
+int adj_alloc(fs_adjoint* a, void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) return fail(FS_ENOMEM, "hipMalloc failed for adjoint storage");
+    a->owned.push_back(*p);
+    return FS_OK;
+}
+void adj_free(fs_adjoint* a) {
+    for (void* p : a->owned) (void)hipFree(p);
+    a->owned.clear();
+}
+int adj_slot_alloc(fs_adjoint* a, AdjSlot& s) {
+    const size_t fb = a->h->ncell * sizeof(float4), pb = mg_level_floats(a->h->mg, 0) * sizeof(double);
+    if (adj_alloc(a, (void**)&s.vel, fb) || adj_alloc(a, (void**)&s.S, fb) || adj_alloc(a, (void**)&s.S2, fb) ||
+        adj_alloc(a, (void**)&s.p, pb))
+        return FS_ENOMEM;
+    return FS_OK;
+}
+int adj_check(const fs_sim* h) {
+    if (h->prec != FS_PREC_FP32) return fail(FS_ESTATE, "the adjoint needs FS_PREC_FP32 storage");
+    if (!h->refine()) return fail(FS_ESTATE, "the adjoint needs the converged solve FS_SOLVE_REFINE64");
+    if (h->prm[P_visc] != 0.f) return fail(FS_ESTATE, "the adjoint needs visc = 0");
+    return FS_OK;
+}
+// copies between the sim's own state and a slot
+void slot_copy(fs_sim* h, const AdjSlot& s, bool to_slot) {
+    const size_t fb = h->ncell * sizeof(float4), pb = mg_level_floats(h->mg, 0) * sizeof(double);
+    void* sim[4] = {h->f[F_VEL], h->f[F_S], h->f[F_S2], h->mg.x64};
+    void* sl[4] = {s.vel, s.S, s.S2, s.p};
+    const size_t nb[4] = {fb, fb, fb, pb};
+    for (int i = 0; i < 4; i++)
+        (void)hipMemcpyAsync(to_slot ? sl[i] : sim[i], to_slot ? sim[i] : sl[i], nb[i], hipMemcpyDeviceToDevice, h->stream);
+}
+
+// Step k from slot `in`, through the pressure solve (sim.js:633-642): adv1 -> the sim's
+// velX/SX/S2X, adv2 -> velY/SY/S2Y, curl -> vort, forces -> vF and (out's or the sim's)
+// S/S2, div -> bkeep, solve -> x64 (from in.p).  With `out`, the projection writes
+// out.vel and the pressure is kept in out.p.
+void replay_step(fs_adjoint* a, int k, const AdjSlot& in, const AdjSlot* out) {
+    fs_sim* h = a->h;
+    hipStream_t s = h->stream;
+    h->time = a->times[k];
+    h->frame = a->frames[k];
+    push_params(h, a->dt);
+    FieldSet A = h->fields();
+    A.vel = in.vel; A.S = in.S; A.S2 = in.S2;
+    launch_adv1(h->prec, h->g, s, h->P_d, A);
+    launch_adv2(h->prec, h->g, s, h->P_d, A);
+    launch_curl(h->prec, h->g, s, A);
+    FieldSet B = h->fields();
+    B.velX = a->l[fs_adjoint::VF];
+    if (out) { B.S = out->S; B.S2 = out->S2; }
+    launch_forces(h->prec, h->g, s, h->P_d, B);
+    launch_div(h->prec, h->g, s, h->P_d, B, h->mg.bkeep, 0);
+    const size_t pb = mg_level_floats(h->mg, 0) * sizeof(double);
+    (void)hipMemcpyAsync(h->mg.x64, in.p, pb, hipMemcpyDeviceToDevice, s);
+    h->solve(s);
+    if (out) {
+        FieldSet C = h->fields();
+        C.velX = a->l[fs_adjoint::VF];
+        C.vel = out->vel;
+        launch_project(h->prec, h->g, s, h->P_d, C, nullptr, h->mg.x64);
+        (void)hipMemcpyAsync(out->p, h->mg.x64, pb, hipMemcpyDeviceToDevice, s);
+    }
+}
+
+// The adjoint of step k: l_vel/l_S/l_S2 hold the adjoint of the state after the step on
+// entry and of the state before it on return; parameter gradients accumulate.
+void reverse_step(fs_adjoint* a, int k, double tol, int maxc) {
+    fs_sim* h = a->h;
+    hipStream_t s = h->stream;
+    const Dims& g = h->g;
+    const AdjSlot& in = a->slot(k);
+    replay_step(a, k, in, nullptr);
+    float4** l = a->l;
+    const size_t fb = h->ncell * sizeof(float4), nd = mg_level_floats(h->mg, 0);
+    const FieldSet F = h->fields();
+    // projection: vF, p -> vel (sim.js:526-550); the pressure adjoint goes to bkeep
+    launch_fill_zero(s, h->mg.bkeep, nd * sizeof(float));
+    launch_project_adj(g, s, h->P_d, l[fs_adjoint::VF], h->mg.x64, (const float4*)F.geo, l[fs_adjoint::L_VEL],
+                       l[fs_adjoint::L_VF], h->mg.bkeep);
+    // the pressure solve: A lb = lp, from zero
+    launch_fill_zero(s, h->mg.x64, nd * sizeof(double));
+    mg_refine(h->mg, s, tol, maxc);
+    // divergence (sim.js:399-413)
+    launch_div_adj(g, s, h->P_d, h->mg.x64, l[fs_adjoint::L_VF]);
+    // sources (sim.js:214-396)
+    launch_fill_zero(s, l[fs_adjoint::L_VORT], fb);
+    launch_forces_adj(g, s, h->P_d, F, (const float4*)F.velY, (const float4*)F.SY, (const float4*)F.S2Y,
+                      (const float4*)F.vort, l[fs_adjoint::L_VF], l[fs_adjoint::L_S], l[fs_adjoint::L_S2],
+                      l[fs_adjoint::L_VY], l[fs_adjoint::L_SY], l[fs_adjoint::L_S2Y], l[fs_adjoint::L_VORT], a->gspot,
+                      a->gface);
+    // curl (sim.js:200-211)
+    launch_curl_adj(g, s, (const float4*)F.velY, l[fs_adjoint::L_VORT], l[fs_adjoint::L_VY]);
+    // MacCormack correction and semi-Lagrangian pass (sim.js:142-197)
+    for (int i : {fs_adjoint::L_VEL, fs_adjoint::L_S, fs_adjoint::L_S2, fs_adjoint::L_VH, fs_adjoint::L_SH, fs_adjoint::L_S2H})
+        launch_fill_zero(s, l[i], fb);
+    launch_adv2_adj(g, s, h->P_d, in.vel, in.S, in.S2, (const float4*)F.velX, (const float4*)F.SX, (const float4*)F.S2X,
+                    l[fs_adjoint::L_VY], l[fs_adjoint::L_SY], l[fs_adjoint::L_S2Y], l[fs_adjoint::L_VEL],
+                    l[fs_adjoint::L_S], l[fs_adjoint::L_S2], l[fs_adjoint::L_VH], l[fs_adjoint::L_SH], l[fs_adjoint::L_S2H]);
+    launch_adv1_adj(g, s, h->P_d, in.vel, in.S, in.S2, l[fs_adjoint::L_VH], l[fs_adjoint::L_SH], l[fs_adjoint::L_S2H],
+                    l[fs_adjoint::L_VEL], l[fs_adjoint::L_S], l[fs_adjoint::L_S2]);
+}
+
+float4* adj_field(fs_adjoint* a, int field) {
+    switch (field) {
+        case FS_VEL: return a->l[fs_adjoint::L_VEL];
+        case FS_S: return a->l[fs_adjoint::L_S];
+        case FS_S2: return a->l[fs_adjoint::L_S2];
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fs_adjoint_create(fs_sim* h, int max_steps, int checkpoint_every, fs_adjoint** out) {
+    if (!h || !out) return fail(FS_EINVAL, "null argument");
+    *out = nullptr;
+    if (max_steps < 1 || checkpoint_every < 1) return fail(FS_EINVAL, "max_steps and checkpoint_every must be >= 1");
+    if (adj_check(h)) return FS_ESTATE;
+    if (set_device(h)) return FS_EDEVICE;
+    fs_adjoint* a = new fs_adjoint();
+    a->h = h;
+    a->max_steps = max_steps;
+    a->every = std::min(checkpoint_every, max_steps);
+    auto bail = [&](int code) { std::string e = g_last_error; adj_free(a); delete a; g_last_error = e; return code; };
+    a->ckpt.resize((max_steps + a->every - 1) / a->every + 1);
+    a->seg.resize(a->every);
+    for (auto& s : a->ckpt) if (adj_slot_alloc(a, s)) return bail(FS_ENOMEM);
+    for (int i = 1; i < a->every; i++) if (adj_slot_alloc(a, a->seg[i])) return bail(FS_ENOMEM);
+    if (adj_slot_alloc(a, a->fin)) return bail(FS_ENOMEM);
+    const size_t fb = h->ncell * sizeof(float4);
+    for (int i = 0; i < fs_adjoint::L_COUNT; i++) {
+        if (adj_alloc(a, (void**)&a->l[i], fb)) return bail(FS_ENOMEM);
+        launch_fill_zero(h->stream, a->l[i], fb);
+    }
+    if (adj_alloc(a, (void**)&a->gspot, FS_MAX_SPOTS * sizeof(double)) ||
+        adj_alloc(a, (void**)&a->gface, (size_t)h->nx * h->nz * sizeof(double)) ||
+        adj_alloc(a, (void**)&a->target, 2 * h->ncell * sizeof(float)) ||
+        adj_alloc(a, (void**)&a->partial, (size_t)h->g.nblocks * sizeof(double)))
+        return bail(FS_ENOMEM);
+    launch_fill_zero(h->stream, a->gspot, FS_MAX_SPOTS * sizeof(double));
+    launch_fill_zero(h->stream, a->gface, (size_t)h->nx * h->nz * sizeof(double));
+    a->times.resize(max_steps + 1);
+    a->frames.resize(max_steps + 1);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(FS_EDEVICE, "device error in adjoint create"));
+    *out = a;
+    return FS_OK;
+}
+
+int fs_adjoint_destroy(fs_adjoint* a) {
+    if (!a) return FS_OK;
+    if (a->h) { (void)set_device(a->h); (void)hipStreamSynchronize(a->h->stream); }
+    adj_free(a);
+    delete a;
+    return FS_OK;
+}
+
+int fs_adjoint_forward(fs_adjoint* a, double dt, int nsteps) {
+    if (!a) return fail(FS_EINVAL, "null handle");
+    if (nsteps < 1 || nsteps > a->max_steps) return fail(FS_EINVAL, "nsteps must be in 1..max_steps");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(FS_EINVAL, "dt must be positive and finite");
+    fs_sim* h = a->h;
+    if (adj_check(h)) return FS_ESTATE;
+    if (set_device(h)) return FS_EDEVICE;
+    a->dt = dt;
+    a->nsteps = nsteps;
+    a->have_forward = a->have_grads = false;
+    sync_params(h, dt);
+    if (!h->geo_ready) {   // sim.js:632
+        launch_geo(h->prec, h->g, h->stream, h->P_d, h->fields());
+        h->geo_ready = true;
+    }
+    slot_copy(h, a->slot(0), true);
+    for (int k = 0; k < nsteps; k++) {
+        a->times[k] = h->time;
+        a->frames[k] = h->frame;
+        replay_step(a, k, a->slot(k), &a->slot(k + 1));
+        h->time = a->times[k] + dt;   // sim.js:646-647
+        h->frame = a->frames[k] + 1;
+    }
+    a->t_end = h->time;
+    a->frame_end = h->frame;
+    slot_copy(h, a->slot(nsteps), false);
+    slot_copy(h, a->fin, true);
+    h->params_dirty = true;
+    const size_t fb = h->ncell * sizeof(float4);
+    for (int i : {fs_adjoint::L_VEL, fs_adjoint::L_S, fs_adjoint::L_S2}) launch_fill_zero(h->stream, a->l[i], fb);
+    FS_HIP_CHECK(hipGetLastError());
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    a->have_forward = true;
+    a->last_interval_held = true;
+    return FS_OK;
+}
+
+int fs_adjoint_seed(fs_adjoint* a, int field, int dtype, const void* host, size_t bytes) {
+    if (!a || !host) return fail(FS_EINVAL, "null argument");
+    float4* dst = adj_field(a, field);
+    if (!dst) return fail(FS_EINVAL, "the seed field must be FS_VEL, FS_S or FS_S2");
+    if (!a->have_forward) return fail(FS_ESTATE, "run fs_adjoint_forward first");
+    fs_sim* h = a->h;
+    const size_t nval = h->ncell * 4;
+    if (dtype != FS_DT_F32 && dtype != FS_DT_F64) return fail(FS_EINVAL, "dtype must be FS_DT_F32 or FS_DT_F64");
+    if (bytes != nval * (dtype == FS_DT_F32 ? 4 : 8)) return fail(FS_EINVAL, "seed size mismatch");
+    if (set_device(h)) return FS_EDEVICE;
+    std::vector<float> tmp(nval);
+    if (dtype == FS_DT_F32) std::memcpy(tmp.data(), host, nval * 4);
+    else for (size_t i = 0; i < nval; i++) tmp[i] = (float)((const double*)host)[i];
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    FS_HIP_CHECK(hipMemcpy(dst, tmp.data(), nval * 4, hipMemcpyHostToDevice));
+    a->have_grads = false;
+    return FS_OK;
+}
+
+int fs_adjoint_seed_mismatch(fs_adjoint* a, const float* target_theta, const float* target_soot, double* loss) {
+    if (!a || !target_theta || !target_soot) return fail(FS_EINVAL, "null argument");
+    if (!a->have_forward) return fail(FS_ESTATE, "run fs_adjoint_forward first");
+    fs_sim* h = a->h;
+    if (set_device(h)) return FS_EDEVICE;
+    const size_t n = h->ncell, fb = n * sizeof(float4);
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    FS_HIP_CHECK(hipMemcpy(a->target, target_theta, n * sizeof(float), hipMemcpyHostToDevice));
+    FS_HIP_CHECK(hipMemcpy(a->target + n, target_soot, n * sizeof(float), hipMemcpyHostToDevice));
+    launch_fill_zero(h->stream, a->l[fs_adjoint::L_VEL], fb);
+    launch_fill_zero(h->stream, a->l[fs_adjoint::L_S2], fb);
+    launch_loss_seed(h->g, h->stream, a->fin.S, a->target, a->target + n, a->l[fs_adjoint::L_S], a->partial);
+    std::vector<double> part(h->g.nblocks);
+    FS_HIP_CHECK(hipMemcpyAsync(part.data(), a->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    double sum = 0.0;
+    for (double v : part) sum += v;
+    if (loss) *loss = sum;
+    a->have_grads = false;
+    return FS_OK;
+}
+
+int fs_adjoint_backward(fs_adjoint* a, double solve_tol, int max_cycles) {
+    if (!a) return fail(FS_EINVAL, "null handle");
+    if (!a->have_forward) return fail(FS_ESTATE, "run fs_adjoint_forward first");
+    fs_sim* h = a->h;
+    if (adj_check(h)) return FS_ESTATE;
+    if (set_device(h)) return FS_EDEVICE;
+    const double tol = solve_tol > 0.0 ? solve_tol : h->cfg.div_tol;
+    const int maxc = std::min(max_cycles > 0 ? max_cycles : h->cfg.max_vcycles, MG_MAX_REFINE);
+    launch_fill_zero(h->stream, a->gspot, FS_MAX_SPOTS * sizeof(double));
+    launch_fill_zero(h->stream, a->gface, (size_t)h->nx * h->nz * sizeof(double));
+    const int N = a->nsteps, E = a->every;
+    const int nseg = (N + E - 1) / E;
+    for (int j = nseg - 1; j >= 0; j--) {
+        const int k0 = j * E, k1 = std::min(k0 + E, N);
+        // the states inside the interval; the forward left the last interval's in place
+        if (j != nseg - 1 || !a->last_interval_held) {
+            for (int k = k0; k < k1 - 1; k++) replay_step(a, k, a->slot(k), &a->slot(k + 1));
+            a->last_interval_held = j == nseg - 1;
+        }
+        for (int k = k1 - 1; k >= k0; k--) reverse_step(a, k, tol, maxc);
+        FS_HIP_CHECK(hipGetLastError());
+    }
+    // the sim's state, pressure and clock as the forward left them
+    slot_copy(h, a->fin, false);
+    h->time = a->t_end;
+    h->frame = a->frame_end;
+    h->params_dirty = true;
+    FS_HIP_CHECK(hipGetLastError());
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    a->have_grads = true;
+    return FS_OK;
+}
+
+int fs_adjoint_read(fs_adjoint* a, int field, int dtype, void* host, size_t bytes) {
+    if (!a || !host) return fail(FS_EINVAL, "null argument");
+    float4* src = adj_field(a, field);
+    if (!src) return fail(FS_EINVAL, "the field must be FS_VEL, FS_S or FS_S2");
+    if (!a->have_grads) return fail(FS_ESTATE, "run fs_adjoint_backward first");
+    fs_sim* h = a->h;
+    const size_t nval = h->ncell * 4;
+    if (dtype != FS_DT_F32 && dtype != FS_DT_F64) return fail(FS_EINVAL, "dtype must be FS_DT_F32 or FS_DT_F64");
+    if (bytes != nval * (dtype == FS_DT_F32 ? 4 : 8)) return fail(FS_EINVAL, "destination size mismatch");
+    if (set_device(h)) return FS_EDEVICE;
+    std::vector<float> tmp(nval);
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    FS_HIP_CHECK(hipMemcpy(tmp.data(), src, nval * 4, hipMemcpyDeviceToHost));
+    if (dtype == FS_DT_F32) std::memcpy(host, tmp.data(), nval * 4);
+    else for (size_t i = 0; i < nval; i++) ((double*)host)[i] = (double)tmp[i];
+    return FS_OK;
+}
+
+int fs_adjoint_grads(fs_adjoint* a, double* spot_flux, int max_spots, double* face_flux) {
+    if (!a) return fail(FS_EINVAL, "null handle");
+    if (!a->have_grads) return fail(FS_ESTATE, "run fs_adjoint_backward first");
+    fs_sim* h = a->h;
+    if (set_device(h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    const int n = std::min(max_spots, (int)h->spots_h.size());
+    if (spot_flux && n > 0) FS_HIP_CHECK(hipMemcpy(spot_flux, a->gspot, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (face_flux) {
+        if (!h->has_face) return fail(FS_ESTATE, "no floor flux is set");
+        FS_HIP_CHECK(hipMemcpy(face_flux, a->gface, (size_t)h->nx * h->nz * sizeof(double), hipMemcpyDeviceToHost));
+    }
     return FS_OK;
 }
 

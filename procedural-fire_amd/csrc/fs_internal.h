@@ -161,6 +161,27 @@ void launch_set_params(hipStream_t s, DevParams* dst, const DevParams& src);
 void launch_fill_zero(hipStream_t s, void* ptr, size_t bytes);
 void launch_convert(hipStream_t s, int kind, const void* src, void* dst, size_t n);
 void launch_copy_probe(hipStream_t s, const void* src, void* dst, size_t bytes);
+// ---- adjoint of one step (fs_adjoint.hip), float4 storage ------------------------
+// Stage inputs are the forward's recorded fields; l* arguments are adjoint fields.
+// Gathers store (lvIn of project, lvY/lsY/ls2Y and the xyz of lvort in forces); every
+// other output is accumulated with atomics and must be cleared first.
+void launch_adv1_adj(const Dims& g, hipStream_t s, const DevParams* P, const float4* vel, const float4* sF,
+                     const float4* s2F, const float4* lvX, const float4* lsX, const float4* ls2X, float4* lvel,
+                     float4* ls, float4* ls2);
+void launch_adv2_adj(const Dims& g, hipStream_t s, const DevParams* P, const float4* vel, const float4* sF,
+                     const float4* s2F, const float4* vH, const float4* sH, const float4* s2H, const float4* lvY,
+                     const float4* lsY, const float4* ls2Y, float4* lvel, float4* ls, float4* ls2, float4* lvH, float4* lsH,
+                     float4* ls2H);
+void launch_curl_adj(const Dims& g, hipStream_t s, const float4* v, const float4* lvort, float4* lv);
+void launch_forces_adj(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, const float4* velY,
+                       const float4* sY, const float4* s2Y, const float4* vort, const float4* lvOut, const float4* lsOut,
+                       const float4* ls2Out, float4* lvY, float4* lsY, float4* ls2Y, float4* lvort, double* gspot,
+                       double* gface);
+void launch_div_adj(const Dims& g, hipStream_t s, const DevParams* P, const double* lb, float4* lvF);
+void launch_project_adj(const Dims& g, hipStream_t s, const DevParams* P, const float4* vIn, const double* p,
+                        const float4* geo, const float4* lvOut, float4* lvIn, float* lp);
+void launch_loss_seed(const Dims& g, hipStream_t s, const float4* Sf, const float* tT, const float* tSoot, float4* lS,
+                      double* partial);
 // ---- diagnostics (fs_diag.hip) -------------------------------------------------
 int  stats_blocks(size_t n);
 void launch_stats(int prec, hipStream_t s, const void* field, size_t n, double* part);

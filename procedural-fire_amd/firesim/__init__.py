@@ -134,6 +134,15 @@ def lib():
         "fs_diag_enable": (ip, [vp, ip, ctypes.c_double]),
         "fs_diag_read": (ip, [vp, dp, ip, ctypes.POINTER(ctypes.c_int)]),
         "fs_poisson_info": (ip, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "fs_set_clock": (ip, [vp, ctypes.c_double, ctypes.c_int64]),
+        "fs_adjoint_create": (ip, [vp, ip, ip, ctypes.POINTER(vp)]),
+        "fs_adjoint_destroy": (ip, [vp]),
+        "fs_adjoint_forward": (ip, [vp, ctypes.c_double, ip]),
+        "fs_adjoint_seed": (ip, [vp, ip, ip, vp, sz]),
+        "fs_adjoint_seed_mismatch": (ip, [vp, fp, fp, dp]),
+        "fs_adjoint_backward": (ip, [vp, ctypes.c_double, ip]),
+        "fs_adjoint_read": (ip, [vp, ip, ip, vp, sz]),
+        "fs_adjoint_grads": (ip, [vp, dp, ip, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -386,6 +395,22 @@ class FireSim:
         _check(lib().fs_get_clock(self._h, None, ctypes.byref(f)))
         return f.value
 
+    def set_clock(self, time, frame):
+        _check(lib().fs_set_clock(self._h, float(time), int(frame)))
+
+    def save_state(self):
+        """(vel, S, S2, p, time, frame): everything a later step reads besides the
+        parameters, spots and geometry (the FP64 pressure is the solve's starting guess)."""
+        return (self.read("vel"), self.read("S"), self.read("S2"), self.read("p", np.float64), self.time, self.frame)
+
+    def load_state(self, st):
+        vel, S_, S2_, p, t, f = st
+        self.write("vel", vel)
+        self.write("S", S_)
+        self.write("S2", S2_)
+        self.write("p", p)
+        self.set_clock(t, f)
+
     def grid_info(self):
         r = (ctypes.c_int * 3)()
         dx = ctypes.c_double()
@@ -418,6 +443,71 @@ class FireSim:
         ms = np.zeros(len(STAGE_LABELS), np.float32)
         _check(lib().fs_timings(self._h, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return dict(zip(STAGE_LABELS, ms.tolist()))
+
+
+class Adjoint:
+    """Reverse-mode derivative of a run of steps of one FireSim (fs_adjoint_*).
+
+    forward(dt, n) runs the steps (the sim ends in the final state) keeping the state every
+    `checkpoint_every` steps; seed the adjoint of the final state (seed / seed_mismatch), then
+    backward() gives the adjoint of the initial state (read) and the gradients of the spot
+    fluxes and the floor flux (grads).  Needs FP32 storage, the FP64-refined solve and visc 0.
+    """
+
+    def __init__(self, sim, max_steps, checkpoint_every=20):
+        self.sim = sim
+        self._h = ctypes.c_void_p()
+        _check(lib().fs_adjoint_create(sim._h, int(max_steps), int(checkpoint_every), ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().fs_adjoint_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, dt, n):
+        _check(lib().fs_adjoint_forward(self._h, float(dt), int(n)))
+
+    def seed(self, field, arr):
+        field = FIELD_NAMES.get(field, field)
+        a = np.ascontiguousarray(arr, np.float32)
+        _check(lib().fs_adjoint_seed(self._h, field, DT_F32, a.ctypes.data, a.nbytes))
+
+    def seed_mismatch(self, target_theta, target_soot):
+        """Seeds L = sum (theta - theta*)^2 + (soot - soot*)^2 over the final state; returns L."""
+        t = np.ascontiguousarray(target_theta, np.float32)
+        s = np.ascontiguousarray(target_soot, np.float32)
+        loss = ctypes.c_double()
+        _check(lib().fs_adjoint_seed_mismatch(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                              s.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(loss)))
+        return loss.value
+
+    def backward(self, solve_tol=0.0, max_cycles=0):
+        _check(lib().fs_adjoint_backward(self._h, float(solve_tol), int(max_cycles)))
+
+    def read(self, field):
+        """Adjoint of the initial state's field, shape (nz, ny, nx, 4)."""
+        field = FIELD_NAMES.get(field, field)
+        nx, ny, nz = self.sim.res
+        out = np.empty((nz, ny, nx, 4), np.float32)
+        _check(lib().fs_adjoint_read(self._h, field, DT_F32, out.ctypes.data, out.nbytes))
+        return out
+
+    def grads(self, n_spots=None, face=False):
+        """(dL/d spot flux [n_spots], dL/d floor flux [nz, nx] or None), FP64."""
+        if n_spots is None:
+            n_spots = len(self.sim.spots())
+        g = np.zeros(max(n_spots, 1), np.float64)
+        nx, _, nz = self.sim.res
+        gf = np.zeros((nz, nx), np.float64) if face else None
+        _check(lib().fs_adjoint_grads(self._h, g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(n_spots),
+                                      gf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if face else None))
+        return g[:n_spots], gf
 
 
 class Poisson:

@@ -96,17 +96,23 @@ def scene_size(name):
     return tuple(float(np.float32(n * sc["dx"])) for n in sc["res"])
 
 
-def scene_face_flux(name):
-    """Floor fuel flux (nz, nx) of a scene, or None."""
+def scene_face_flux(name, res=None, seed=11):
+    """Floor fuel flux (nz, nx) of a scene (on grid `res`, same dx), or None: within the
+    disc, FACE_FLUX times a per-cell factor U(0.5, 1.5) from the reference LCG seeded with
+    `seed` (row-major over the floor), zero outside.  The texture keeps neighbouring floor
+    cells from holding identical fuel values: with a uniform flux every limiter bound on
+    the floor is a tie between texels, where the step has no derivative."""
     sc = SCENES[name]
     if not sc.get("face"):
         return None
-    nx, _, nz = sc["res"]
+    nx, _, nz = res or sc["res"]
     dx = sc["dx"]
     x = (np.arange(nx) + 0.5) * dx - 0.5 * nx * dx
     z = (np.arange(nz) + 0.5) * dx - 0.5 * nz * dx
     r = np.hypot(x[None, :], z[:, None])
-    return np.where(r < 0.5 * DISC_DIAMETER, FACE_FLUX, 0.0).astype(np.float32)
+    R = lcg(seed)
+    u = np.array([R() for _ in range(nx * nz)], np.float64).reshape(nz, nx)
+    return np.where(r < 0.5 * DISC_DIAMETER, FACE_FLUX * (0.5 + u), 0.0).astype(np.float32)
 
 
 def scene_params(name):
