@@ -309,6 +309,164 @@ def flame_statistics(args):
     print(json.dumps(out))
 
 
+def adjoint_bench(args):
+    """Gradients of a temperature/soot mismatch with respect to the pyrolysis-spot fluxes and the
+    floor fuel flux: the adjoint scene (64 x 96 x 64 at 1 cm, dt 2 ms, 8 spots of U(1.5, 3) g/s
+    plus a 64 x 64 floor-flux texture) run 200 steps from rest, checkpoints every 20 steps.  The
+    target is the same run with every flux scaled by (1 + N(0, 0.2)).
+
+    Checks reported: the tangent-linear (forward-mode) derivative along a spot + floor direction
+    on the same trajectory (dot-product test of the adjoint); FP64 central differences of the
+    FP64 oracle (fluxes perturbed in double) at 200 steps and, where differences are a stable
+    reference, over the first 50 steps; FP32 central differences on the GPU (h = 1e-3); and
+    the conditioning of the 200-step derivative (FP64 differences after a 1e-9 change of the
+    initial temperature field)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import fsoracle as O
+    name = "adjoint8_64"
+    sc = scenes.SCENES[name]
+    steps = args.steps if args.steps != 200 else sc["steps"]
+    every = 20
+    dt = sc["dt"]
+    tol = 1e-8
+    sim = scenes.make_scene(name, precision=F.PREC_FP32, solve=F.SOLVE_REFINE64, div_tol=tol, max_vcycles=40,
+                            noise_n=128)
+    rows = sim.spots()[:, :6].astype(np.float32)
+    face = scenes.scene_face_flux(name)
+    base = sim.save_state()
+    rng = np.random.default_rng(2024)
+    pr = rows.copy()
+    pr[:, 4] *= (1.0 + rng.normal(0.0, 0.2, len(rows))).astype(np.float32)
+    pf = (face * (1.0 + rng.normal(0.0, 0.2, face.shape))).astype(np.float32)
+
+    def make_target(n):
+        sim.set_spots(pr)
+        sim.set_face_flux(pf)
+        sim.load_state(base)
+        sim.step(dt, n)
+        S = sim.read("S")
+        sim.set_spots(rows)
+        sim.set_face_flux(face)
+        return np.ascontiguousarray(S[..., 0]), np.ascontiguousarray(S[..., 2])
+
+    def loss_of(S, target):
+        S = np.asarray(S, np.float64)
+        return float(np.sum((S[..., 0] - target[0]) ** 2) + np.sum((S[..., 2] - target[1]) ** 2))
+
+    target = make_target(steps)
+    adj = F.Adjoint(sim, steps, every)
+    tf, tb = [], []
+    for rep in range(3):
+        sim.load_state(base)
+        sim.sync()
+        t0 = time.perf_counter()
+        adj.forward(dt, steps)
+        t1 = time.perf_counter()
+        loss = adj.seed_mismatch(*target)
+        t2 = time.perf_counter()
+        adj.backward(solve_tol=tol, max_cycles=40)
+        t3 = time.perf_counter()
+        tf.append(t1 - t0)
+        tb.append(t3 - t2)
+        print(f"adjoint rep {rep}: forward {t1 - t0:.3f} s, backward {t3 - t2:.3f} s, loss {loss:.6e}", flush=True)
+    g_spot, g_face = adj.grads(len(rows), face=True)
+    out = {"workload": f"{name}: 64x96x64 dx=1cm dt=2ms, {steps} steps from rest, 8 spots + 64x64 floor flux, "
+                       f"checkpoints every {every}, solve refined to {tol:g}",
+           "loss": loss, "forward_ms_per_step": float(np.median(tf)) / steps * 1e3,
+           "backward_ms_per_step": float(np.median(tb)) / steps * 1e3,
+           "backward_over_forward": float(np.median(tb) / np.median(tf)),
+           "grad_spot_flux": g_spot.tolist()}
+    # tangent along a spot + floor direction on the recorded trajectory: <seed, J d> = <J^T seed, d>
+    d_spot = rng.normal(0.0, 1.0, len(rows)) * 1e-3
+    d_face = (np.where(face > 0, rng.normal(0.0, 1.0, face.shape), 0.0) * 1e-3).astype(np.float32)
+    sim.load_state(base)
+    sim.sync()
+    t0 = time.perf_counter()
+    tan = adj.tangent(dt, steps, spot_dir=d_spot, face_dir=d_face, solve_tol=tol, max_cycles=40)
+    t_tan = time.perf_counter() - t0
+    S_fin = sim.read("S").astype(np.float64)
+    jd = float(np.sum(2.0 * (S_fin[..., 0] - target[0]) * tan["S"][..., 0]) +
+               np.sum(2.0 * (S_fin[..., 2] - target[1]) * tan["S"][..., 2]))
+    jtw = float(np.sum(g_spot * d_spot) + np.sum(g_face * d_face))
+    out["tangent"] = {"ms_per_step": t_tan / steps * 1e3, "seed_dot_tangent": jd, "grad_dot_direction": jtw,
+                      "rel_diff": abs(jd - jtw) / max(abs(jd), abs(jtw))}
+    print(f"dot-product test: {jd:.9e} vs {jtw:.9e}", flush=True)
+    # FP32 central differences on the GPU, h = 1e-3 relative (float fluxes)
+    fd32 = np.zeros(len(rows))
+    for q in range(len(rows)):
+        f0 = float(rows[q, 4])
+        v = {}
+        for sgn in (1, -1):
+            r = rows.copy()
+            r[q, 4] = np.float32(f0 * (1 + sgn * 1e-3))
+            sim.set_spots(r)
+            sim.load_state(base)
+            sim.step(dt, steps)
+            v[sgn] = (loss_of(sim.read("S"), target), float(r[q, 4]))
+        fd32[q] = (v[1][0] - v[-1][0]) / (v[1][1] - v[-1][1])
+    sim.set_spots(rows)
+    out["fd32_spot_flux"] = fd32.tolist()
+    out["rel_err_vs_fd32"] = float(np.linalg.norm(g_spot - fd32) / np.linalg.norm(fd32))
+    print(f"spot gradient vs FD32: {out['rel_err_vs_fd32']:.3e}", flush=True)
+    # FP64 oracle from the identical state, fluxes perturbed in double precision
+    orc = O.OracleSim(res=sc["res"], size=scenes.scene_size(name), logs=np.zeros(8, np.float32), n_logs=0,
+                      precision=O.FP64, vcycles=sc["vcycles"], max_levels=sc["max_levels"], solve=O.SOLVE_REFINE64,
+                      div_tol=1e-11, max_vcycles=40, noise_n=128)
+    orc.set_params(**scenes.scene_params(name))
+    for f in (F.GEO, F.NOISE, F.CURL):
+        orc.write(f, sim.read(f, np.float64))
+    orc.set_spots(rows.astype(np.float64))
+    orc.set_face_flux(face)
+    vel0, S0, S20, p0, t00, fr0 = base
+
+    def fd64(n, d, h, tgt, dS=None):
+        v = {}
+        for sgn in (1, -1):
+            orc.set_flux64(spot=rows[:, 4].astype(np.float64) + sgn * h * d)
+            orc.write(O.VEL, vel0.astype(np.float64))
+            orc.write(O.S, S0.astype(np.float64) + (0.0 if dS is None else dS))
+            orc.write(O.S2, S20.astype(np.float64))
+            orc.write(O.P, p0)
+            orc.set_clock(t00, fr0)
+            t0 = time.perf_counter()
+            for k in range(0, n, 50):
+                orc.step(dt, min(50, n - k))
+                print(f"  FP64 oracle: {min(k + 50, n)}/{n} steps ({time.perf_counter() - t0:.0f} s)", flush=True)
+            v[sgn] = loss_of(orc.read(O.S), tgt)
+        return (v[1] - v[-1]) / (2 * h)
+
+    dg = g_spot / np.linalg.norm(g_spot) * np.linalg.norm(rows[:, 4])
+    pred = float(np.sum(g_spot * dg))
+    out["fp64_200"] = {"adjoint": pred}
+    for h in (1e-6, 1e-7):
+        out["fp64_200"][f"fd64_h{h:g}"] = fd64(steps, dg, h, target)
+        print(f"FP64 {steps} steps along the gradient, h={h:g}: {out['fp64_200'][f'fd64_h{h:g}']:.6e} "
+              f"(adjoint {pred:.6e})", flush=True)
+    # conditioning: the same derivative from initial temperature fields changed by N(0, eps^2)
+    # (theta units, 1 = 1000 K): eps 1e-7 is one float rounding of theta ~ 1
+    for eps in (1e-9, 1e-7, 1e-5):
+        nudge = np.zeros_like(S0, np.float64)
+        nudge[..., 0] = eps * np.random.default_rng(7).normal(0.0, 1.0, S0.shape[:3])
+        key = f"fd64_h1e-07_theta_nudged_{eps:g}"
+        out["fp64_200"][key] = fd64(steps, dg, 1e-7, target, nudge)
+        print(f"FP64 {steps} steps, initial theta nudged by {eps:g}: {out['fp64_200'][key]:.6e}", flush=True)
+    # 50 steps: adjoint against converged FP64 differences
+    n50 = 50
+    t50 = make_target(n50)
+    adj50 = F.Adjoint(sim, n50, every)
+    sim.load_state(base)
+    adj50.forward(dt, n50)
+    adj50.seed_mismatch(*t50)
+    adj50.backward(solve_tol=tol, max_cycles=40)
+    g50 = adj50.grads(len(rows))[0]
+    d50 = g50 / np.linalg.norm(g50) * np.linalg.norm(rows[:, 4])
+    p50 = float(np.sum(g50 * d50))
+    f50 = fd64(n50, d50, 1e-7, t50)
+    out["fp64_50"] = {"adjoint": p50, "fd64": f50, "rel_err": abs(p50 - f50) / abs(f50)}
+    print(f"FP64 50 steps: adjoint {p50:.6e} FD64 {f50:.6e} rel err {abs(p50 - f50) / abs(f50):.3e}", flush=True)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,6 +487,9 @@ def main():
     ap.add_argument("--errors", action="store_true",
                     help="instead of timing: per-field error of the mixed and FP32 steps vs the FP64 oracle "
                          "after 1 and 10 steps")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="instead of the step: the adjoint workload (200 steps at 64x96x64, gradients of a "
+                         "temperature/soot mismatch w.r.t. spot and floor fluxes, checked by FD32 and FD64)")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB copy, then 3 steps of each step form")
     args = ap.parse_args()
@@ -348,6 +509,9 @@ def main():
         return
     if args.flames:
         flame_statistics(args)
+        return
+    if args.adjoint:
+        adjoint_bench(args)
         return
     ncell = float(np.prod(RES))
     graph = not args.no_graph

@@ -298,9 +298,22 @@ int  fs_adjoint_seed(fs_adjoint* a, int field, int dtype, const void* host, size
  * floats, x fastest); returns L (FP64).  Replaces every seed. */
 int  fs_adjoint_seed_mismatch(fs_adjoint* a, const float* target_theta, const float* target_soot, double* loss);
 /* The reverse sweep; each adjoint pressure solve runs FP64 refinement to solve_tol
- * (<= 0: the sim's div_tol) within max_cycles (<= 0: the sim's max_vcycles). */
+ * (<= 0: the sim's div_tol) within max_cycles (<= 0: the sim's max_vcycles).  The seed is
+ * consumed: a later backward needs a new seed (unseeded = zero). */
 int  fs_adjoint_backward(fs_adjoint* a, double solve_tol, int max_cycles);
-/* The adjoint of the initial state (FS_VEL, FS_S or FS_S2), layout as fs_adjoint_seed. */
+/* Forward mode: runs nsteps steps of dt like fs_adjoint_forward (recording the same
+ * checkpoints) and carries the tangent-linear derivative along the direction given by
+ * spot_dir (one value per spot, kg/s), face_dir (nz*nx floats, kg/m^2/s) and the initial
+ * state's direction vel_dir / S_dir / S2_dir (nx*ny*nz*4 floats each); NULL = zero.  The
+ * tangent of the final state is then read with fs_adjoint_read.  Each tangent pressure
+ * solve runs FP64 refinement to solve_tol / max_cycles (<= 0: the sim's settings).  The
+ * kernels mirror the adjoint's branch for branch, so the dot products of a seed with the
+ * tangent and of the gradients with the direction agree to rounding. */
+int  fs_adjoint_tangent(fs_adjoint* a, double dt, int nsteps, const double* spot_dir, const float* face_dir,
+                        const float* vel_dir, const float* S_dir, const float* S2_dir, double solve_tol,
+                        int max_cycles);
+/* After fs_adjoint_backward: the adjoint of the initial state (FS_VEL, FS_S or FS_S2);
+ * after fs_adjoint_tangent: the tangent of the final state.  Layout as fs_adjoint_seed. */
 int  fs_adjoint_read(fs_adjoint* a, int field, int dtype, void* host, size_t bytes);
 /* dL/d(flux) of each pyrolysis spot (kg/s; up to max_spots entries) and, when a floor
  * flux is set (fs_set_face_flux), dL/d(face flux) per floor cell (nz*nx, x fastest;

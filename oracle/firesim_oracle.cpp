@@ -390,6 +390,7 @@ struct OracleBase {
     virtual double residual(int* used) const = 0;
     virtual int set_spots(const fs_spot* sp, int n, fs_spot* out) = 0;
     virtual void set_face(const float* f) = 0;
+    virtual void set_flux64(const double* spot, int n, const double* face) = 0;
 };
 
 template <class R, class S, class M>
@@ -411,6 +412,9 @@ struct Oracle : OracleBase {
     std::vector<double> x64;   // FP64 pressure iterate (FS_SOLVE_REFINE64)
     std::vector<fs_spot> spots;          // pyrolysis spots (build extension)
     std::vector<float> face;             // floor fuel flux, nx*nz (empty = none)
+    // double-precision overrides of the spot and floor fluxes (finite differences of the FP64
+    // oracle with steps below float resolution); empty = the float values above
+    std::vector<double> spot64, face64;
     double last_rel = -1; int last_used = 0;
 
     Oracle(const fs_config* c, const float* lg, int nl) {
@@ -673,19 +677,25 @@ struct Oracle : OracleBase {
                         s2.y = wmax(s2.y, R(0.05) * bw * wclamp<R>(s.y + s2.x, R(0), R(1)));
                     }
                     // pyrolysis spots and floor fuel flux (build extension, mass-flux sources)
-                    for (const fs_spot& sp : spots) {
+                    for (size_t q = 0; q < spots.size(); q++) {
+                        const fs_spot& sp = spots[q];
                         V3<R> dp = p - V3<R>((R)sp.pos[0], (R)sp.pos[1], (R)sp.pos[2]);
                         R d = length(dp);
                         if (d < (R)sp.radius) {
                             R w = R(1) - smoothstep<R>(R(0.5) * (R)sp.radius, (R)sp.radius, d);
-                            s.y = wmin(s.y + (R)sp.flux * dt * (R)sp.norm * w, R(1));
+                            const R flux = q < spot64.size() ? (R)spot64[q] : (R)sp.flux;
+                            s.y = wmin(s.y + flux * dt * (R)sp.norm * w, R(1));
                             s.x = wmax(s.x, w * (R)sp.temp);
                             s2.w = mix(s2.w, R(0), w);
                             s2.y = s2.y * (R(1) - w);
                             srcAll = wmax(srcAll, w);
                         }
                     }
-                    if (!face.empty() && y == 0) s.y = wmin(s.y + (R)face[(size_t)z * nx + x] * dt / ((R)FS_RHO0 * DX), R(1));
+                    if (!face.empty() && y == 0) {
+                        const size_t fi = (size_t)z * nx + x;
+                        const R fv = face64.empty() ? (R)face[fi] : (R)face64[fi];
+                        s.y = wmin(s.y + fv * dt / ((R)FS_RHO0 * DX), R(1));
+                    }
                     R ageF = s2.y / wmax(s.y + s2.x, R(1e-4));
                     // mixing-limited combustion (sim.js:300-309)
                     R T0 = R(293) + R(1000) * s.x;
@@ -949,6 +959,10 @@ struct Oracle : OracleBase {
     void set_face(const float* f) override {
         if (f) face.assign(f, f + (size_t)nx * nz); else face.clear();
     }
+    void set_flux64(const double* spot, int n, const double* f) override {
+        if (spot) spot64.assign(spot, spot + n); else spot64.clear();
+        if (f) face64.assign(f, f + (size_t)nx * nz); else face64.clear();
+    }
     double residual(int* used) const override { if (used) *used = last_used; return last_rel; }
 };
 
@@ -1111,6 +1125,8 @@ double orc_residual(orc_sim* s, int* used) { return s->o->residual(used); }
 int orc_build_campfire(int seed, float* out, int max_logs) { return build_campfire(seed, out, max_logs); }
 int orc_set_spots(orc_sim* s, const fs_spot* sp, int n, fs_spot* out) { return s->o->set_spots(sp, n, out); }
 void orc_set_face_flux(orc_sim* s, const float* f) { s->o->set_face(f); }
+// Double-precision spot / floor fluxes replacing the float ones (NULL: back to the float values).
+void orc_set_flux64(orc_sim* s, const double* spot, int n, const double* face) { s->o->set_flux64(spot, n, face); }
 
 // Noise bake alone (for the noise-texture golden fixtures).
 void orc_bake_noise(int n, int seed, float* noise_out, float* curl_out) {

@@ -133,6 +133,7 @@ def lib():
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_spots.argtypes = [vp, ctypes.POINTER(FsSpot), ctypes.c_int, ctypes.POINTER(FsSpot)]
         L.orc_set_face_flux.argtypes = [vp, fp]
+        L.orc_set_flux64.argtypes = [vp, dp, ctypes.c_int, dp]
         _lib = L
     return _lib
 
@@ -259,6 +260,16 @@ class OracleSim:
         a = np.ascontiguousarray(flux, np.float32)
         assert a.shape == (nz, nx)
         lib().orc_set_face_flux(self.h, _fp(a))
+
+    def set_flux64(self, spot=None, face=None):
+        """Double-precision spot fluxes (kg/s, one per spot) and/or floor flux (nz, nx) in place of
+        the float values given to set_spots / set_face_flux (None: back to those)."""
+        sp = None if spot is None else np.ascontiguousarray(spot, np.float64)
+        fc = None if face is None else np.ascontiguousarray(face, np.float64)
+        if fc is not None:
+            assert fc.shape == (self.res[2], self.res[0])
+        lib().orc_set_flux64(self.h, None if sp is None else _dp(sp), 0 if sp is None else sp.size,
+                             None if fc is None else _dp(fc))
 
     def reset(self):
         lib().orc_reset(self.h)

@@ -301,8 +301,8 @@ __global__ __launch_bounds__(256) void k_forces_adj(const DevParams* __restrict_
         Dual s[4] = {x[3], x[4], x[5], x[6]}, s2[4] = {x[7], x[8], x[9], x[10]};
         const Dual w[3] = {x[11], x[12], x[13]};
         const Dual gx = x[14] - x[15], gy = x[16] - x[17], gz = x[18] - x[19];
-        forces_cell<Dual>(P->f, k, spots, P->n_spots, sp, face, Dual(fv, seed_face ? 1.f : 0.f), vx, vy, vz, s, s2, w,
-                          gx, gy, gz, phi);
+        forces_cell<Dual>(P->f, k, spots, P->n_spots, sp, nullptr, face, Dual(fv, seed_face ? 1.f : 0.f), vx, vy, vz,
+                          s, s2, w, gx, gy, gz, phi);
         return Lv.x * vx.d + Lv.y * vy.d + Lv.z * vz.d + Lv.w * phi.d + Ls.x * s[0].d + Ls.y * s[1].d + Ls.z * s[2].d +
                Ls.w * s[3].d + Ls2.x * s2[0].d + Ls2.y * s2[1].d + Ls2.z * s2[2].d + Ls2.w * s2[3].d;
     };
@@ -411,6 +411,245 @@ __global__ __launch_bounds__(256) void k_loss_seed(Dims g, const float4* __restr
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
+
+// ---- tangent-linear (forward-mode) kernels: the same Jacobians applied to a direction ----
+// Each mirrors its adjoint above branch for branch (the same frozen active sets and tie
+// rules), so <adjoint(w), d> = <w, tangent(d)> holds to rounding on any run of steps.
+
+// adv1 (src/sim.js:142-157)
+__global__ __launch_bounds__(256) void k_adv1_tan(const DevParams* __restrict__ P, Dims g, const float4* __restrict__ vel,
+                                                  const float4* __restrict__ sF, const float4* __restrict__ s2F,
+                                                  const float4* __restrict__ tvel, const float4* __restrict__ tS,
+                                                  const float4* __restrict__ tS2, float4* __restrict__ tvX,
+                                                  float4* __restrict__ tsX, float4* __restrict__ ts2X) {
+    Cell c;
+    if (!cell_of(g, c)) return;
+    const float dt = (float)P->dt, dx = P->f[P_dx];
+    const float cx = (float)c.x + 0.5f, cy = (float)c.y + 0.5f, cz = (float)c.z + 0.5f;
+    const float4 v0 = vel[c.i], tv0 = tvel[c.i];
+    const float hx = cx - v0.x * (0.5f * dt) / dx, hy = cy - v0.y * (0.5f * dt) / dx, hz = cz - v0.z * (0.5f * dt) / dx;
+    const float th[3] = {-tv0.x * (0.5f * dt) / dx, -tv0.y * (0.5f * dt) / dx, -tv0.z * (0.5f * dt) / dx};
+    const Tri t1 = tri_at(g, hx, hy, hz);
+    float4 vh[8], tvh[8];
+    tri_load(t1, vel, vh);
+    tri_load(t1, tvel, tvh);
+    float v1[3], tv1[3];
+    for (int ch = 0; ch < 3; ch++) {
+        float cv[8], ct = 0.f, gx, gy, gz;
+        for (int k = 0; k < 8; k++) { cv[k] = chan(vh[k], ch); ct += t1.w[k] * chan(tvh[k], ch); }
+        tri_eval(t1, cv, v1[ch], gx, gy, gz);
+        tv1[ch] = ct + gx * th[0] + gy * th[1] + gz * th[2];
+    }
+    const float bx = cx - v1[0] * dt / dx, by = cy - v1[1] * dt / dx, bz = cz - v1[2] * dt / dx;
+    const float tb[3] = {-tv1[0] * dt / dx, -tv1[1] * dt / dx, -tv1[2] * dt / dx};
+    float fgx, fgy, fgz;
+    const float f = fade_grad(g, bx, by, bz, fgx, fgy, fgz);
+    const float tf = fgx * tb[0] + fgy * tb[1] + fgz * tb[2];
+    const float ws = sstep(0.03f, 0.4f, by * dx);
+    const float tws = dsstep(0.03f, 0.4f, by * dx) * dx * tb[1];
+    const float W[3] = {P->f[P_windX], 0.f, P->f[P_windZ]};
+    const Tri t2 = tri_at(g, bx, by, bz);
+    const float4* fields[3] = {vel, sF, s2F};
+    const float4* tfields[3] = {tvel, tS, tS2};
+    float4* outs[3] = {tvX, tsX, ts2X};
+    for (int fi = 0; fi < 3; fi++) {
+        float4 tv[8], tt[8];
+        tri_load(t2, fields[fi], tv);
+        tri_load(t2, tfields[fi], tt);
+        float o[4] = {0.f, 0.f, 0.f, 0.f};
+        const int nch = fi == 0 ? 3 : 4;
+        for (int ch = 0; ch < nch; ch++) {
+            float cv[8], ct = 0.f, val, gx, gy, gz;
+            for (int k = 0; k < 8; k++) { cv[k] = chan(tv[k], ch); ct += t2.w[k] * chan(tt[k], ch); }
+            tri_eval(t2, cv, val, gx, gy, gz);
+            const float tsv = ct + gx * tb[0] + gy * tb[1] + gz * tb[2];
+            if (fi == 0) o[ch] = (val - W[ch] * ws) * tf + f * tsv + (1.f - f) * W[ch] * tws;
+            else o[ch] = val * tf + f * tsv;
+        }
+        outs[fi][c.i] = make_float4(o[0], o[1], o[2], o[3]);
+    }
+}
+
+// adv2 (src/sim.js:159-197), limiter frozen
+__global__ __launch_bounds__(256) void k_adv2_tan(const DevParams* __restrict__ P, Dims g, const float4* __restrict__ vel,
+                                                  const float4* __restrict__ sF, const float4* __restrict__ s2F,
+                                                  const float4* __restrict__ vH, const float4* __restrict__ sH,
+                                                  const float4* __restrict__ s2H, const float4* __restrict__ tvel,
+                                                  const float4* __restrict__ tS, const float4* __restrict__ tS2,
+                                                  const float4* __restrict__ tvH, const float4* __restrict__ tsH,
+                                                  const float4* __restrict__ ts2H, float4* __restrict__ tvY,
+                                                  float4* __restrict__ tsY, float4* __restrict__ ts2Y) {
+    Cell c;
+    if (!cell_of(g, c)) return;
+    const float dt = (float)P->dt, dx = P->f[P_dx];
+    const float cx = (float)c.x + 0.5f, cy = (float)c.y + 0.5f, cz = (float)c.z + 0.5f;
+    const float4 v = vel[c.i], tv = tvel[c.i];
+    const float fwx = cx + dt * v.x / dx, fwy = cy + dt * v.y / dx, fwz = cz + dt * v.z / dx;
+    const float tfw[3] = {dt * tv.x / dx, dt * tv.y / dx, dt * tv.z / dx};
+    const Tri tf = tri_at(g, fwx, fwy, fwz);
+    const float3 bt = backtrace<S>(vel, g, cx, cy, cz, dt, dx);
+    const float bx = bt.x - 0.5f, by = bt.y - 0.5f, bz = bt.z - 0.5f;
+    const int b0x = (int)floorf(wclamp(bx, -2.f, (float)g.nx + 1.f));
+    const int b0y = (int)floorf(wclamp(by, -2.f, (float)g.ny + 1.f));
+    const int b0z = (int)floorf(wclamp(bz, -2.f, (float)g.nz + 1.f));
+    int lim[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int qx = clampi(b0x + (k & 1), 0, g.nx - 1), qy = clampi(b0y + ((k >> 1) & 1), 0, g.ny - 1);
+        const int qz = clampi(b0z + ((k >> 2) & 1), 0, g.nz - 1);
+        lim[k] = (qz * g.ny + qy) * g.nx + qx;
+    }
+    const bool reverted = outside_fade(g, bx + 0.5f, by + 0.5f, bz + 0.5f) < 1.f;
+    const float4* orig[3] = {vel, sF, s2F};
+    const float4* hf[3] = {vH, sH, s2H};
+    const float4* torig[3] = {tvel, tS, tS2};
+    const float4* th_[3] = {tvH, tsH, ts2H};
+    float4* outs[3] = {tvY, tsY, ts2Y};
+    for (int fi = 0; fi < 3; fi++) {
+        const int nch = fi == 0 ? 3 : 4;
+        const float4 hc = hf[fi][c.i], oc = orig[fi][c.i], thc = th_[fi][c.i], toc = torig[fi][c.i];
+        float4 tv8[8], tt8[8], lv[8];
+        tri_load(tf, hf[fi], tv8);
+        tri_load(tf, th_[fi], tt8);
+        for (int k = 0; k < 8; k++) lv[k] = orig[fi][lim[k]];
+        float o[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int ch = 0; ch < nch; ch++) {
+            float cv[8], ct = 0.f, hs, gx, gy, gz;
+            for (int k = 0; k < 8; k++) { cv[k] = chan(tv8[k], ch); ct += tf.w[k] * chan(tt8[k], ch); }
+            tri_eval(tf, cv, hs, gx, gy, gz);
+            const float h = chan(hc, ch), oo = chan(oc, ch), thh = chan(thc, ch);
+            float N = h + (oo - hs) * 0.5f;
+            if (reverted) N = h;
+            float mn = 1e9f, mx = -1e9f;
+            int kmn = -1, kmx = -1;
+            for (int k = 0; k < 8; k++) {
+                const float t = chan(lv[k], ch);
+                if (kmn < 0 || t < mn) { mn = t; kmn = k; }
+                if (kmx < 0 || t > mx) { mx = t; kmx = k; }
+            }
+            const float lo = fminf(mn, h), hi = fmaxf(mx, h);
+            const float out = wclamp(N, lo, hi);
+            if (fi > 0 && !(out > 0.f)) continue;
+            float t;
+            if (N < lo) t = mn <= h ? chan(torig[fi][lim[kmn]], ch) : thh;
+            else if (N > hi) t = mx >= h ? chan(torig[fi][lim[kmx]], ch) : thh;
+            else if (reverted) t = thh;
+            else t = thh + 0.5f * chan(toc, ch) - 0.5f * (ct + gx * tfw[0] + gy * tfw[1] + gz * tfw[2]);
+            o[ch] = t;
+        }
+        outs[fi][c.i] = make_float4(o[0], o[1], o[2], o[3]);
+    }
+}
+
+// curl (src/sim.js:200-211)
+__global__ __launch_bounds__(256) void k_curl_tan(Dims g, const float4* __restrict__ v, const float4* __restrict__ tv,
+                                                  float4* __restrict__ tvort) {
+    Cell c;
+    if (!cell_of(g, c)) return;
+    const int iL = (c.z * g.ny + c.y) * g.nx + clampi(c.x - 1, 0, g.nx - 1), iR = (c.z * g.ny + c.y) * g.nx + clampi(c.x + 1, 0, g.nx - 1);
+    const int iB = (c.z * g.ny + clampi(c.y - 1, 0, g.ny - 1)) * g.nx + c.x, iT = (c.z * g.ny + clampi(c.y + 1, 0, g.ny - 1)) * g.nx + c.x;
+    const int iK = (clampi(c.z - 1, 0, g.nz - 1) * g.ny + c.y) * g.nx + c.x, iF = (clampi(c.z + 1, 0, g.nz - 1) * g.ny + c.y) * g.nx + c.x;
+    const float4 L = v[iL], R = v[iR], B = v[iB], T = v[iT], K = v[iK], F = v[iF];
+    const float4 tL = tv[iL], tR = tv[iR], tB = tv[iB], tT = tv[iT], tK = tv[iK], tF = tv[iF];
+    const float wx = ((T.z - B.z) - (F.y - K.y)) * 0.5f;
+    const float wy = ((F.x - K.x) - (R.z - L.z)) * 0.5f;
+    const float wz = ((R.y - L.y) - (T.x - B.x)) * 0.5f;
+    const float twx = ((tT.z - tB.z) - (tF.y - tK.y)) * 0.5f;
+    const float twy = ((tF.x - tK.x) - (tR.z - tL.z)) * 0.5f;
+    const float twz = ((tR.y - tL.y) - (tT.x - tB.x)) * 0.5f;
+    const float wl = sqrtf(wx * wx + wy * wy + wz * wz);
+    tvort[c.i] = make_float4(twx, twy, twz, wl > 0.f ? (wx * twx + wy * twy + wz * twz) / wl : 0.f);
+}
+
+// the source kernel (src/sim.js:214-396): one dual evaluation with the direction as tangents
+__global__ __launch_bounds__(256) void k_forces_tan(const DevParams* __restrict__ P, Dims g, const float4* __restrict__ velY,
+                                                    const float4* __restrict__ sY, const float4* __restrict__ s2Y,
+                                                    const float4* __restrict__ vort, const float4* __restrict__ geo,
+                                                    const half4_t* __restrict__ noise, const half4_t* __restrict__ curlT, int nn,
+                                                    const fs_spot* __restrict__ spots, const float* __restrict__ faceFlux,
+                                                    const float* __restrict__ spot_dir, const float* __restrict__ face_dir,
+                                                    const float4* __restrict__ tvY, const float4* __restrict__ tsY,
+                                                    const float4* __restrict__ ts2Y, const float4* __restrict__ tvort,
+                                                    float4* __restrict__ tvF, float4* __restrict__ tS, float4* __restrict__ tS2) {
+    Cell c;
+    if (!cell_of(g, c)) return;
+    const ForcesCell k = forces_setup<S>(P, g, c, velY, geo, noise, curlT, nn);
+    const float4 v4 = velY[c.i], s4 = sY[c.i], t4 = s2Y[c.i], w4 = vort[c.i];
+    const float4 dv = tvY[c.i], ds = tsY[c.i], dt2 = ts2Y[c.i], dw = tvort[c.i];
+    int nb[6];
+    nb[0] = (c.z * g.ny + c.y) * g.nx + clampi(c.x + 1, 0, g.nx - 1);
+    nb[1] = (c.z * g.ny + c.y) * g.nx + clampi(c.x - 1, 0, g.nx - 1);
+    nb[2] = (c.z * g.ny + clampi(c.y + 1, 0, g.ny - 1)) * g.nx + c.x;
+    nb[3] = (c.z * g.ny + clampi(c.y - 1, 0, g.ny - 1)) * g.nx + c.x;
+    nb[4] = (clampi(c.z + 1, 0, g.nz - 1) * g.ny + c.y) * g.nx + c.x;
+    nb[5] = (clampi(c.z - 1, 0, g.nz - 1) * g.ny + c.y) * g.nx + c.x;
+    Dual wn[6];
+    for (int j = 0; j < 6; j++) wn[j] = Dual(vort[nb[j]].w, tvort[nb[j]].w);
+    Dual vx(v4.x, dv.x), vy(v4.y, dv.y), vz(v4.z, dv.z), phi;
+    Dual s[4] = {Dual(s4.x, ds.x), Dual(s4.y, ds.y), Dual(s4.z, ds.z), Dual(s4.w, ds.w)};
+    Dual s2[4] = {Dual(t4.x, dt2.x), Dual(t4.y, dt2.y), Dual(t4.z, dt2.z), Dual(t4.w, dt2.w)};
+    const Dual w[3] = {Dual(w4.x, dw.x), Dual(w4.y, dw.y), Dual(w4.z, dw.z)};
+    const Dual gx = wn[0] - wn[1], gy = wn[2] - wn[3], gz = wn[4] - wn[5];
+    const bool face = P->has_face_flux != 0;
+    const bool floor_face = face && c.y == 0;
+    const Dual fv(floor_face ? faceFlux[c.z * g.nx + c.x] : 0.f, floor_face ? face_dir[c.z * g.nx + c.x] : 0.f);
+    forces_cell<Dual>(P->f, k, spots, P->n_spots, -1, spot_dir, face, fv, vx, vy, vz, s, s2, w, gx, gy, gz, phi);
+    tvF[c.i] = make_float4(vx.d, vy.d, vz.d, phi.d);
+    tS[c.i] = make_float4(s[0].d, s[1].d, s[2].d, s[3].d);
+    tS2[c.i] = make_float4(s2[0].d, s2[1].d, s2[2].d, s2[3].d);
+}
+
+// div (src/sim.js:399-413) into the parity-split right-hand side
+__global__ __launch_bounds__(256) void k_div_tan(const DevParams* __restrict__ P, Dims g, const float4* __restrict__ tv,
+                                                 float* __restrict__ tb) {
+    Cell c;
+    if (!cell_of(g, c)) return;
+    const float dx = P->f[P_dx];
+    const float4 me = tv[c.i];
+    const float L = tv[(c.z * g.ny + c.y) * g.nx + clampi(c.x - 1, 0, g.nx - 1)].x;
+    const float R = tv[(c.z * g.ny + c.y) * g.nx + clampi(c.x + 1, 0, g.nx - 1)].x;
+    const float B = c.y == 0 ? -me.y : tv[(c.z * g.ny + c.y - 1) * g.nx + c.x].y;
+    const float T = tv[(c.z * g.ny + clampi(c.y + 1, 0, g.ny - 1)) * g.nx + c.x].y;
+    const float K = tv[(clampi(c.z - 1, 0, g.nz - 1) * g.ny + c.y) * g.nx + c.x].z;
+    const float F = tv[(clampi(c.z + 1, 0, g.nz - 1) * g.ny + c.y) * g.nx + c.x].z;
+    tb[rb_index(c.x, c.y, c.z, g.nx, g.ny)] = ((R - L) + (T - B) + (F - K)) / (2.f * dx) - me.w;
+}
+
+// project (src/sim.js:526-550): p the step's pressure, tp its tangent (both FP64, parity-split)
+__global__ __launch_bounds__(256) void k_project_tan(const DevParams* __restrict__ P, Dims g, const float4* __restrict__ vIn,
+                                                     const double* __restrict__ p, const float4* __restrict__ geo,
+                                                     const float4* __restrict__ tvIn, const double* __restrict__ tp,
+                                                     float4* __restrict__ tvOut) {
+    Cell c;
+    if (!cell_of(g, c)) return;
+    const float dx = P->f[P_dx];
+    if (geo[c.i].z > 0.5f) { tvOut[c.i] = make_float4(0.f, 0.f, 0.f, 0.f); return; }
+    auto grad = [&](const double* q, float& gx, float& gy, float& gz) {
+        const int io = rb_index(c.x, c.y, c.z, g.nx, g.ny);
+        const float me = (float)q[io];
+        const float pxp = c.x + 1 < g.nx ? (float)q[rb_index(c.x + 1, c.y, c.z, g.nx, g.ny)] : -me;
+        const float pxm = c.x > 0 ? (float)q[rb_index(c.x - 1, c.y, c.z, g.nx, g.ny)] : -me;
+        const float pyp = c.y + 1 < g.ny ? (float)q[rb_index(c.x, c.y + 1, c.z, g.nx, g.ny)] : -me;
+        const float pym = c.y > 0 ? (float)q[rb_index(c.x, c.y - 1, c.z, g.nx, g.ny)] : me;
+        const float pzp = c.z + 1 < g.nz ? (float)q[rb_index(c.x, c.y, c.z + 1, g.nx, g.ny)] : -me;
+        const float pzm = c.z > 0 ? (float)q[rb_index(c.x, c.y, c.z - 1, g.nx, g.ny)] : -me;
+        gx = (pxp - pxm) / (2.f * dx); gy = (pyp - pym) / (2.f * dx); gz = (pzp - pzm) / (2.f * dx);
+    };
+    float gx, gy, gz, tgx, tgy, tgz;
+    grad(p, gx, gy, gz);
+    grad(tp, tgx, tgy, tgz);
+    const float4 vi = vIn[c.i], ti = tvIn[c.i];
+    float vx = vi.x - gx, vy = vi.y - gy, vz = vi.z - gz;
+    float tx = ti.x - tgx, ty = ti.y - tgy, tz = ti.z - tgz;
+    if (c.y == 0 && vy < 0.f) { vy = 0.f; ty = 0.f; }
+    const float vl = sqrtf(vx * vx + vy * vy + vz * vz);
+    if (vl > 20.f) {
+        const float d = (tx * vx + ty * vy + tz * vz) / vl;
+        const float sc = 20.f / vl;
+        tx = sc * (tx - d * vx / vl); ty = sc * (ty - d * vy / vl); tz = sc * (tz - d * vz / vl);
+    }
+    tvOut[c.i] = make_float4(tx, ty, tz, 0.f);
+}
 }  // namespace
 
 void launch_adv1_adj(const Dims& g, hipStream_t s, const DevParams* P, const float4* vel, const float4* sF,
@@ -444,4 +683,33 @@ void launch_project_adj(const Dims& g, hipStream_t s, const DevParams* P, const 
 void launch_loss_seed(const Dims& g, hipStream_t s, const float4* Sf, const float* tT, const float* tSoot, float4* lS,
                       double* partial) {
     k_loss_seed<<<g.nblocks, 256, 0, s>>>(g, Sf, tT, tSoot, lS, partial);
+}
+
+void launch_adv1_tan(const Dims& g, hipStream_t s, const DevParams* P, const float4* vel, const float4* sF,
+                     const float4* s2F, const float4* tvel, const float4* tS, const float4* tS2, float4* tvX, float4* tsX,
+                     float4* ts2X) {
+    k_adv1_tan<<<g.nblocks, 256, 0, s>>>(P, g, vel, sF, s2F, tvel, tS, tS2, tvX, tsX, ts2X);
+}
+void launch_adv2_tan(const Dims& g, hipStream_t s, const DevParams* P, const float4* vel, const float4* sF,
+                     const float4* s2F, const float4* vH, const float4* sH, const float4* s2H, const float4* tvel,
+                     const float4* tS, const float4* tS2, const float4* tvH, const float4* tsH, const float4* ts2H,
+                     float4* tvY, float4* tsY, float4* ts2Y) {
+    k_adv2_tan<<<g.nblocks, 256, 0, s>>>(P, g, vel, sF, s2F, vH, sH, s2H, tvel, tS, tS2, tvH, tsH, ts2H, tvY, tsY, ts2Y);
+}
+void launch_curl_tan(const Dims& g, hipStream_t s, const float4* v, const float4* tv, float4* tvort) {
+    k_curl_tan<<<g.nblocks, 256, 0, s>>>(g, v, tv, tvort);
+}
+void launch_forces_tan(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, const float4* velY,
+                       const float4* sY, const float4* s2Y, const float4* vort, const float* spot_dir,
+                       const float* face_dir, const float4* tvY, const float4* tsY, const float4* ts2Y,
+                       const float4* tvort, float4* tvF, float4* tS, float4* tS2) {
+    k_forces_tan<<<g.nblocks, 256, 0, s>>>(P, g, velY, sY, s2Y, vort, (const float4*)F.geo, F.noise, F.curl, F.noise_n,
+                                           F.spots, F.face_flux, spot_dir, face_dir, tvY, tsY, ts2Y, tvort, tvF, tS, tS2);
+}
+void launch_div_tan(const Dims& g, hipStream_t s, const DevParams* P, const float4* tv, float* tb) {
+    k_div_tan<<<g.nblocks, 256, 0, s>>>(P, g, tv, tb);
+}
+void launch_project_tan(const Dims& g, hipStream_t s, const DevParams* P, const float4* vIn, const double* p,
+                        const float4* geo, const float4* tvIn, const double* tp, float4* tvOut) {
+    k_project_tan<<<g.nblocks, 256, 0, s>>>(P, g, vIn, p, geo, tvIn, tp, tvOut);
 }

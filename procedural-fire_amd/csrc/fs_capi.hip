@@ -1165,6 +1165,8 @@ struct fs_adjoint {
     int64_t frame_end = 0;
     int nsteps = 0;
     bool have_forward = false, have_grads = false;
+    bool have_tangent = false;       // l[L_VEL..L_S2] hold the final state's tangent
+    bool seeds_stale = false;        // the seed buffers hold something else: clear on the first seed
     bool last_interval_held = false;   // seg holds the last interval's states (true after the forward)
     // adjoint fields: the state (l_vel, l_S, l_S2 = seed, then the result), the
     // intermediates of one step, and the forces kernel's velocity output (vF)
@@ -1172,6 +1174,8 @@ struct fs_adjoint {
     float4* l[L_COUNT] = {};
     double* gspot = nullptr;         // FS_MAX_SPOTS
     double* gface = nullptr;         // nx * nz
+    float* spot_dir = nullptr;       // FS_MAX_SPOTS (tangent runs)
+    float* face_dir = nullptr;       // nx * nz (tangent runs)
     float* target = nullptr;         // 2 * ncell (mismatch targets)
     double* partial = nullptr;       // per-block loss partials
     std::vector<void*> owned;
@@ -1319,6 +1323,8 @@ int fs_adjoint_create(fs_sim* h, int max_steps, int checkpoint_every, fs_adjoint
     }
     if (adj_alloc(a, (void**)&a->gspot, FS_MAX_SPOTS * sizeof(double)) ||
         adj_alloc(a, (void**)&a->gface, (size_t)h->nx * h->nz * sizeof(double)) ||
+        adj_alloc(a, (void**)&a->spot_dir, FS_MAX_SPOTS * sizeof(float)) ||
+        adj_alloc(a, (void**)&a->face_dir, (size_t)h->nx * h->nz * sizeof(float)) ||
         adj_alloc(a, (void**)&a->target, 2 * h->ncell * sizeof(float)) ||
         adj_alloc(a, (void**)&a->partial, (size_t)h->g.nblocks * sizeof(double)))
         return bail(FS_ENOMEM);
@@ -1339,16 +1345,44 @@ int fs_adjoint_destroy(fs_adjoint* a) {
     return FS_OK;
 }
 
-int fs_adjoint_forward(fs_adjoint* a, double dt, int nsteps) {
-    if (!a) return fail(FS_EINVAL, "null handle");
-    if (nsteps < 1 || nsteps > a->max_steps) return fail(FS_EINVAL, "nsteps must be in 1..max_steps");
-    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(FS_EINVAL, "dt must be positive and finite");
+}  // extern "C"
+
+namespace {
+
+// The tangent of step k, after replay_step(k, in, &out): l[L_VEL/L_S/L_S2] hold the state
+// tangent before the step on entry and after it on return.
+void tangent_step(fs_adjoint* a, const AdjSlot& in, const AdjSlot& out, double tol, int maxc) {
     fs_sim* h = a->h;
-    if (adj_check(h)) return FS_ESTATE;
-    if (set_device(h)) return FS_EDEVICE;
+    hipStream_t s = h->stream;
+    const Dims& g = h->g;
+    float4** l = a->l;
+    const FieldSet F = h->fields();
+    const size_t nd = mg_level_floats(h->mg, 0);
+    launch_adv1_tan(g, s, h->P_d, in.vel, in.S, in.S2, l[fs_adjoint::L_VEL], l[fs_adjoint::L_S], l[fs_adjoint::L_S2],
+                    l[fs_adjoint::L_VH], l[fs_adjoint::L_SH], l[fs_adjoint::L_S2H]);
+    launch_adv2_tan(g, s, h->P_d, in.vel, in.S, in.S2, (const float4*)F.velX, (const float4*)F.SX, (const float4*)F.S2X,
+                    l[fs_adjoint::L_VEL], l[fs_adjoint::L_S], l[fs_adjoint::L_S2], l[fs_adjoint::L_VH],
+                    l[fs_adjoint::L_SH], l[fs_adjoint::L_S2H], l[fs_adjoint::L_VY], l[fs_adjoint::L_SY],
+                    l[fs_adjoint::L_S2Y]);
+    launch_curl_tan(g, s, (const float4*)F.velY, l[fs_adjoint::L_VY], l[fs_adjoint::L_VORT]);
+    launch_forces_tan(g, s, h->P_d, F, (const float4*)F.velY, (const float4*)F.SY, (const float4*)F.S2Y,
+                      (const float4*)F.vort, a->spot_dir, a->face_dir, l[fs_adjoint::L_VY], l[fs_adjoint::L_SY],
+                      l[fs_adjoint::L_S2Y], l[fs_adjoint::L_VORT], l[fs_adjoint::L_VF], l[fs_adjoint::L_S],
+                      l[fs_adjoint::L_S2]);
+    launch_div_tan(g, s, h->P_d, l[fs_adjoint::L_VF], h->mg.bkeep);
+    launch_fill_zero(s, h->mg.x64, nd * sizeof(double));
+    mg_refine(h->mg, s, tol, maxc);
+    launch_project_tan(g, s, h->P_d, l[fs_adjoint::VF], out.p, (const float4*)F.geo, l[fs_adjoint::L_VF], h->mg.x64,
+                       l[fs_adjoint::L_VEL]);
+}
+
+// nsteps steps from the sim's state, recording the checkpoints; with `tangent`, the state
+// tangent in l[L_VEL/L_S/L_S2] and the flux directions are carried along
+int run_forward(fs_adjoint* a, double dt, int nsteps, bool tangent, double tol, int maxc) {
+    fs_sim* h = a->h;
     a->dt = dt;
     a->nsteps = nsteps;
-    a->have_forward = a->have_grads = false;
+    a->have_forward = a->have_grads = a->have_tangent = false;
     sync_params(h, dt);
     if (!h->geo_ready) {   // sim.js:632
         launch_geo(h->prec, h->g, h->stream, h->P_d, h->fields());
@@ -1359,6 +1393,7 @@ int fs_adjoint_forward(fs_adjoint* a, double dt, int nsteps) {
         a->times[k] = h->time;
         a->frames[k] = h->frame;
         replay_step(a, k, a->slot(k), &a->slot(k + 1));
+        if (tangent) tangent_step(a, a->slot(k), a->slot(k + 1), tol, maxc);
         h->time = a->times[k] + dt;   // sim.js:646-647
         h->frame = a->frames[k] + 1;
     }
@@ -1367,13 +1402,54 @@ int fs_adjoint_forward(fs_adjoint* a, double dt, int nsteps) {
     slot_copy(h, a->slot(nsteps), false);
     slot_copy(h, a->fin, true);
     h->params_dirty = true;
-    const size_t fb = h->ncell * sizeof(float4);
-    for (int i : {fs_adjoint::L_VEL, fs_adjoint::L_S, fs_adjoint::L_S2}) launch_fill_zero(h->stream, a->l[i], fb);
     FS_HIP_CHECK(hipGetLastError());
     FS_HIP_CHECK(hipStreamSynchronize(h->stream));
     a->have_forward = true;
+    a->have_tangent = tangent;
+    a->seeds_stale = true;
     a->last_interval_held = true;
     return FS_OK;
+}
+
+int forward_args(fs_adjoint* a, double dt, int nsteps) {
+    if (!a) return fail(FS_EINVAL, "null handle");
+    if (nsteps < 1 || nsteps > a->max_steps) return fail(FS_EINVAL, "nsteps must be in 1..max_steps");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(FS_EINVAL, "dt must be positive and finite");
+    if (adj_check(a->h)) return FS_ESTATE;
+    if (set_device(a->h)) return FS_EDEVICE;
+    return FS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fs_adjoint_forward(fs_adjoint* a, double dt, int nsteps) {
+    if (int rc = forward_args(a, dt, nsteps)) return rc;
+    return run_forward(a, dt, nsteps, false, 0.0, 0);
+}
+
+int fs_adjoint_tangent(fs_adjoint* a, double dt, int nsteps, const double* spot_dir, const float* face_dir,
+                       const float* vel_dir, const float* S_dir, const float* S2_dir, double solve_tol, int max_cycles) {
+    if (int rc = forward_args(a, dt, nsteps)) return rc;
+    fs_sim* h = a->h;
+    const size_t n = h->ncell, fb = n * sizeof(float4);
+    const int ns = (int)h->spots_h.size();
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    std::vector<float> sd(FS_MAX_SPOTS, 0.f);
+    if (spot_dir) for (int q = 0; q < ns; q++) sd[q] = (float)spot_dir[q];
+    FS_HIP_CHECK(hipMemcpy(a->spot_dir, sd.data(), FS_MAX_SPOTS * sizeof(float), hipMemcpyHostToDevice));
+    if (face_dir) FS_HIP_CHECK(hipMemcpy(a->face_dir, face_dir, (size_t)h->nx * h->nz * sizeof(float), hipMemcpyHostToDevice));
+    else launch_fill_zero(h->stream, a->face_dir, (size_t)h->nx * h->nz * sizeof(float));
+    const float* dirs[3] = {vel_dir, S_dir, S2_dir};
+    const int slots[3] = {fs_adjoint::L_VEL, fs_adjoint::L_S, fs_adjoint::L_S2};
+    for (int i = 0; i < 3; i++) {
+        if (dirs[i]) FS_HIP_CHECK(hipMemcpy(a->l[slots[i]], dirs[i], fb, hipMemcpyHostToDevice));
+        else launch_fill_zero(h->stream, a->l[slots[i]], fb);
+    }
+    const double tol = solve_tol > 0.0 ? solve_tol : h->cfg.div_tol;
+    const int maxc = std::min(max_cycles > 0 ? max_cycles : h->cfg.max_vcycles, MG_MAX_REFINE);
+    return run_forward(a, dt, nsteps, true, tol, maxc);
 }
 
 int fs_adjoint_seed(fs_adjoint* a, int field, int dtype, const void* host, size_t bytes) {
@@ -1389,9 +1465,13 @@ int fs_adjoint_seed(fs_adjoint* a, int field, int dtype, const void* host, size_
     std::vector<float> tmp(nval);
     if (dtype == FS_DT_F32) std::memcpy(tmp.data(), host, nval * 4);
     else for (size_t i = 0; i < nval; i++) tmp[i] = (float)((const double*)host)[i];
+    if (a->seeds_stale) {
+        for (int i : {fs_adjoint::L_VEL, fs_adjoint::L_S, fs_adjoint::L_S2}) launch_fill_zero(h->stream, a->l[i], nval * 4);
+        a->seeds_stale = false;
+    }
     FS_HIP_CHECK(hipStreamSynchronize(h->stream));
     FS_HIP_CHECK(hipMemcpy(dst, tmp.data(), nval * 4, hipMemcpyHostToDevice));
-    a->have_grads = false;
+    a->have_grads = a->have_tangent = false;
     return FS_OK;
 }
 
@@ -1413,7 +1493,8 @@ int fs_adjoint_seed_mismatch(fs_adjoint* a, const float* target_theta, const flo
     double sum = 0.0;
     for (double v : part) sum += v;
     if (loss) *loss = sum;
-    a->have_grads = false;
+    a->have_grads = a->have_tangent = false;
+    a->seeds_stale = false;
     return FS_OK;
 }
 
@@ -1427,6 +1508,12 @@ int fs_adjoint_backward(fs_adjoint* a, double solve_tol, int max_cycles) {
     const int maxc = std::min(max_cycles > 0 ? max_cycles : h->cfg.max_vcycles, MG_MAX_REFINE);
     launch_fill_zero(h->stream, a->gspot, FS_MAX_SPOTS * sizeof(double));
     launch_fill_zero(h->stream, a->gface, (size_t)h->nx * h->nz * sizeof(double));
+    if (a->seeds_stale) {   // nothing seeded since the forward: the seed is zero
+        for (int i : {fs_adjoint::L_VEL, fs_adjoint::L_S, fs_adjoint::L_S2})
+            launch_fill_zero(h->stream, a->l[i], h->ncell * sizeof(float4));
+        a->seeds_stale = false;
+    }
+    a->have_tangent = false;
     const int N = a->nsteps, E = a->every;
     const int nseg = (N + E - 1) / E;
     for (int j = nseg - 1; j >= 0; j--) {
@@ -1447,6 +1534,7 @@ int fs_adjoint_backward(fs_adjoint* a, double solve_tol, int max_cycles) {
     FS_HIP_CHECK(hipGetLastError());
     FS_HIP_CHECK(hipStreamSynchronize(h->stream));
     a->have_grads = true;
+    a->seeds_stale = true;   // the seed buffers now hold the initial state's adjoint
     return FS_OK;
 }
 
@@ -1454,7 +1542,7 @@ int fs_adjoint_read(fs_adjoint* a, int field, int dtype, void* host, size_t byte
     if (!a || !host) return fail(FS_EINVAL, "null argument");
     float4* src = adj_field(a, field);
     if (!src) return fail(FS_EINVAL, "the field must be FS_VEL, FS_S or FS_S2");
-    if (!a->have_grads) return fail(FS_ESTATE, "run fs_adjoint_backward first");
+    if (!a->have_grads && !a->have_tangent) return fail(FS_ESTATE, "run fs_adjoint_backward (or fs_adjoint_tangent) first");
     fs_sim* h = a->h;
     const size_t nval = h->ncell * 4;
     if (dtype != FS_DT_F32 && dtype != FS_DT_F64) return fail(FS_EINVAL, "dtype must be FS_DT_F32 or FS_DT_F64");

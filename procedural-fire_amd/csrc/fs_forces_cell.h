@@ -62,10 +62,14 @@ __device__ __forceinline__ Dual gsqrt(Dual a) {
     const float r = sqrtf(a.v);
     return Dual(r, r > 0.f ? a.d / (2.f * r) : 0.f);
 }
-// a mass-flux source as T: the derivative seed is 1 for the spot whose gradient is sought
-template <class T> __device__ __forceinline__ T flux_as(float f, bool seed);
-template <> __device__ __forceinline__ float flux_as<float>(float f, bool) { return f; }
-template <> __device__ __forceinline__ Dual flux_as<Dual>(float f, bool seed) { return Dual(f, seed ? 1.f : 0.f); }
+// a spot's mass flux as T: its derivative is spot_dir[q] when a direction is given (tangent
+// runs), else 1 for the seeded spot (one column of the adjoint's local Jacobian)
+template <class T>
+__device__ __forceinline__ T flux_as(float f, int q, int seed_spot, const float* spot_dir);
+template <> __device__ __forceinline__ float flux_as<float>(float f, int, int, const float*) { return f; }
+template <> __device__ __forceinline__ Dual flux_as<Dual>(float f, int q, int seed_spot, const float* spot_dir) {
+    return Dual(f, spot_dir ? spot_dir[q] : (q == seed_spot ? 1.f : 0.f));
+}
 __device__ __forceinline__ bool gisnan(float a) { return isnan(a); }
 __device__ __forceinline__ bool gisnan(Dual a) { return isnan(a.v); }
 
@@ -83,11 +87,13 @@ struct ForcesCell {
 
 // One cell: v (3), s (4: T, fuel, soot, q), s2 (4: vitiated, age, smoke, blue) in place;
 // w = own vorticity (3), gx/gy/gz = differences of |w| of the neighbours; phi out.
-// seed_spot: the spot whose flux carries a unit derivative (Dual only; -1: none);
+// seed_spot: the spot whose flux carries a unit derivative (Dual only; -1: none), or
+// spot_dir: the derivative of every spot's flux (Dual only; null: use seed_spot);
 // face: the floor flux of this cell as T.
 template <class T>
 __device__ __forceinline__ void forces_cell(const float* U, const ForcesCell& k, const fs_spot* __restrict__ spots,
-                                            int n_spots, int seed_spot, bool has_face, T face, T& vx, T& vy,
+                                            int n_spots, int seed_spot, const float* spot_dir, bool has_face,
+                                            T face, T& vx, T& vy,
                                             T& vz, T* s, T* s2, const T* w, T gx, T gy, T gz, T& phi_out) {
     const float dt = k.dt, I = k.I;
     // ---- sources (sim.js:246-297)
@@ -120,7 +126,7 @@ __device__ __forceinline__ void forces_cell(const float* U, const ForcesCell& k,
         const float d = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
         if (d < sp.radius) {
             const float wq = 1.f - sstep(0.5f * sp.radius, sp.radius, d);
-            s[1] = gmin(s[1] + flux_as<T>(sp.flux, q == seed_spot) * dt * sp.norm * wq, 1.f);
+            s[1] = gmin(s[1] + flux_as<T>(sp.flux, q, seed_spot, spot_dir) * dt * sp.norm * wq, 1.f);
             s[0] = gmax(s[0], wq * sp.temp);
             s2[3] = gmix(s2[3], 0.f, wq);
             s2[1] = s2[1] * (1.f - wq);

@@ -182,6 +182,22 @@ void launch_project_adj(const Dims& g, hipStream_t s, const DevParams* P, const 
                         const float4* geo, const float4* lvOut, float4* lvIn, float* lp);
 void launch_loss_seed(const Dims& g, hipStream_t s, const float4* Sf, const float* tT, const float* tSoot, float4* lS,
                       double* partial);
+// tangent-linear counterparts (direction in, direction out; every output is stored)
+void launch_adv1_tan(const Dims& g, hipStream_t s, const DevParams* P, const float4* vel, const float4* sF,
+                     const float4* s2F, const float4* tvel, const float4* tS, const float4* tS2, float4* tvX, float4* tsX,
+                     float4* ts2X);
+void launch_adv2_tan(const Dims& g, hipStream_t s, const DevParams* P, const float4* vel, const float4* sF,
+                     const float4* s2F, const float4* vH, const float4* sH, const float4* s2H, const float4* tvel,
+                     const float4* tS, const float4* tS2, const float4* tvH, const float4* tsH, const float4* ts2H,
+                     float4* tvY, float4* tsY, float4* ts2Y);
+void launch_curl_tan(const Dims& g, hipStream_t s, const float4* v, const float4* tv, float4* tvort);
+void launch_forces_tan(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, const float4* velY,
+                       const float4* sY, const float4* s2Y, const float4* vort, const float* spot_dir,
+                       const float* face_dir, const float4* tvY, const float4* tsY, const float4* ts2Y,
+                       const float4* tvort, float4* tvF, float4* tS, float4* tS2);
+void launch_div_tan(const Dims& g, hipStream_t s, const DevParams* P, const float4* tv, float* tb);
+void launch_project_tan(const Dims& g, hipStream_t s, const DevParams* P, const float4* vIn, const double* p,
+                        const float4* geo, const float4* tvIn, const double* tp, float4* tvOut);
 // ---- diagnostics (fs_diag.hip) -------------------------------------------------
 int  stats_blocks(size_t n);
 void launch_stats(int prec, hipStream_t s, const void* field, size_t n, double* part);

@@ -141,6 +141,7 @@ def lib():
         "fs_adjoint_seed": (ip, [vp, ip, ip, vp, sz]),
         "fs_adjoint_seed_mismatch": (ip, [vp, fp, fp, dp]),
         "fs_adjoint_backward": (ip, [vp, ctypes.c_double, ip]),
+        "fs_adjoint_tangent": (ip, [vp, ctypes.c_double, ip, dp, fp, fp, fp, fp, ctypes.c_double, ip]),
         "fs_adjoint_read": (ip, [vp, ip, ip, vp, sz]),
         "fs_adjoint_grads": (ip, [vp, dp, ip, dp]),
     }
@@ -490,8 +491,30 @@ class Adjoint:
     def backward(self, solve_tol=0.0, max_cycles=0):
         _check(lib().fs_adjoint_backward(self._h, float(solve_tol), int(max_cycles)))
 
+    def tangent(self, dt, n, spot_dir=None, face_dir=None, state_dir=None, solve_tol=0.0, max_cycles=0):
+        """Forward mode: n steps (recorded like forward()) carrying the derivative along the
+        direction (spot fluxes [n_spots], floor flux (nz, nx), initial (vel, S, S2)); returns the
+        final state's tangent {"vel", "S", "S2"}."""
+        P = ctypes.POINTER(ctypes.c_float)
+        keep = []
+
+        def fptr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float32)
+            keep.append(a)
+            return a.ctypes.data_as(P)
+        sd = None if spot_dir is None else np.ascontiguousarray(spot_dir, np.float64)
+        vd, Sd, S2d = state_dir if state_dir is not None else (None, None, None)
+        _check(lib().fs_adjoint_tangent(self._h, float(dt), int(n),
+                                        None if sd is None else sd.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                        fptr(face_dir), fptr(vd), fptr(Sd), fptr(S2d), float(solve_tol),
+                                        int(max_cycles)))
+        return {f: self.read(f) for f in ("vel", "S", "S2")}
+
     def read(self, field):
-        """Adjoint of the initial state's field, shape (nz, ny, nx, 4)."""
+        """After backward(): the adjoint of the initial state's field; after tangent(): the final
+        state's tangent.  Shape (nz, ny, nx, 4)."""
         field = FIELD_NAMES.get(field, field)
         nx, ny, nz = self.sim.res
         out = np.empty((nz, ny, nx, 4), np.float32)

@@ -1,16 +1,21 @@
-"""Gradients of the GPU adjoint (fs_adjoint_*) against finite differences.
+"""Gradients of the GPU adjoint (fs_adjoint_*) against finite differences and the tangent.
 
 Workload: the adjoint scene (8 pyrolysis spots of U(1.5, 3) g/s on a 0.3 m disc plus a
-floor fuel flux of 5 g/m^2/s under the disc, 1 cm cells, dt 2 ms, FP32 storage, pressure
-solve refined to 1e-9) on a 40x48x40 grid after a 30-step warm-up; the loss is the
+floor fuel-flux texture under the disc, 1 cm cells, dt 2 ms, FP32 storage, pressure solve
+refined to 1e-9) on a 40x48x40 grid after a 30-step warm-up; the loss is the
 temperature/soot mismatch L = sum (theta - theta*)^2 + (soot - soot*)^2 after N steps
-against a target made by the same N steps with every flux scaled by (1 + N(0, 0.2)).
+against a target made by the same N steps with every flux scaled by (1 + N(0, 0.2)), or a
+random weighting L = w . (vel, S, S2).
 
-References for dL/d(flux):
+References:
   * FP64: central differences of the FP64 oracle (float64 storage and arithmetic, solve
-    converged to 1e-11) from the identical state and target, h = 1e-4 relative: the FP64
-    reference gradient.  Bar: relative error <= 1e-3.
+    converged to 1e-11) from the identical state, the fluxes perturbed in double precision:
+    the FP64 reference derivative.  Bar: relative error <= 1e-3.
   * FD32: central differences of the GPU's own FP32 steps, h = 1e-3 relative.
+  * Tangent: the forward-mode derivative along the same FP32 trajectory; over 200 steps the
+    adjoint must be its transpose (dot-product test).  Beyond ~50 steps differences stop
+    being a reference: the FP64 derivative moves by tens of percent under a 1e-7 change of
+    the initial state, while the FP32 and FP64 trajectories drift apart by ~1e-2.
 """
 import numpy as np
 import pytest
@@ -130,14 +135,15 @@ def test_spot_flux_gradient_against_fp64_and_fp32_differences(case):
     for q in range(len(rows)):
         f0 = float(rows[q, 4])
         vals = {}
+        orc.set_spots(rows.astype(np.float64))
         for sgn in (1, -1):
-            r = rows.astype(np.float64).copy()
-            r[q, 4] = f0 * (1 + sgn * 1e-4)
-            orc.set_spots(r)
+            r = rows[:, 4].astype(np.float64)
+            r[q] = f0 * (1 + sgn * 1e-6)
+            orc.set_flux64(spot=r)
             _oracle_load(orc, case["base"])
             orc.step(DT, n)
             vals[sgn] = _loss(orc.read(O.S), target)
-        g64[q] = (vals[1] - vals[-1]) / (2e-4 * f0)
+        g64[q] = (vals[1] - vals[-1]) / (2e-6 * f0)
         for sgn in (1, -1):
             r = rows.copy()
             r[q, 4] = np.float32(f0 * (1 + sgn * 1e-3))
@@ -194,11 +200,13 @@ def test_face_flux_gradient_against_fp64_differences(case):
                      ("uniform", np.where(face > 0, 1e-3, 0.0))):
         pred = float(np.sum(gf * d))
         vals = {}
+        orc.set_face_flux(face)
         for sgn in (1, -1):
-            orc.set_face_flux(face.astype(np.float64) + sgn * 1e-3 * d)
+            orc.set_flux64(face=face.astype(np.float64) + sgn * 1e-3 * d)
             _oracle_load(orc, case["base"])
             orc.step(DT, n)
             vals[sgn] = _weighted(orc, w)
+        orc.set_flux64()
         fd = (vals[1] - vals[-1]) / 2e-3
         err = abs(pred - fd) / abs(fd)
         print(f"face direction {label}: adjoint {pred:.6e} FP64 FD {fd:.6e} rel err {err:.3e}", flush=True)
@@ -258,6 +266,79 @@ def test_initial_state_gradient_one_step(case):
             err = abs(pred - fd) / abs(fd)
             print(f"direction {label}: adjoint {pred:.6e} FP64 FD(h={h:g}) {fd:.6e} rel err {err:.3e}", flush=True)
             assert err <= 1e-3
+
+
+def _directions(case, seed):
+    """A direction in every input: spot fluxes, floor flux, initial velocity (4 cells in from
+    every face) and initial scalars (where the channel is positive)."""
+    rng = np.random.default_rng(seed)
+    rows, face = case["rows"], case["face"]
+    vel, S, S2 = case["base"][:3]
+    d_spot = rng.normal(0.0, 1.0, len(rows)) * 1e-3
+    d_face = np.where(face > 0, rng.normal(0.0, 1.0, face.shape), 0.0) * 1e-3
+    d_vel = np.zeros_like(vel)
+    d_vel[4:-4, 4:-4, 4:-4, :3] = 1e-2 * rng.normal(0.0, 1.0, d_vel[4:-4, 4:-4, 4:-4, :3].shape)
+    d_S = np.where(S > 1e-3, 1e-3 * rng.normal(0.0, 1.0, S.shape), 0.0)
+    d_S2 = np.where(S2 > 1e-3, 1e-3 * rng.normal(0.0, 1.0, S2.shape), 0.0)
+    return d_spot, d_face.astype(np.float32), d_vel.astype(np.float32), d_S.astype(np.float32), d_S2.astype(np.float32)
+
+
+def test_tangent_against_fp64_differences(case):
+    """Forward mode over 6 steps along a spot-flux + floor-flux direction vs FP64 differences.
+    (Per-cell random directions in the scalars are left to the dot-product test: they move
+    cells off exact ties, e.g. products saturated at 1 in neighbouring texels, where the step
+    has one-sided derivatives only.)"""
+    print("tangent vs FP64 FD, 6 steps", flush=True)
+    sim, n, rows, face = case["sim"], case["n"], case["rows"], case["face"]
+    w = _random_seeds(11)
+    d_spot, d_face = _directions(case, 3)[:2]
+    d_S = d_S2 = np.zeros(1, np.float32)
+    sim.load_state(case["base"])
+    adj = F.Adjoint(sim, n, 4)
+    t = adj.tangent(DT, n, spot_dir=d_spot, face_dir=d_face, solve_tol=TOL, max_cycles=40)
+    pred = sum(float(np.sum(t[f].astype(np.float64) * w[f])) for f in ("vel", "S", "S2"))
+    orc = _oracle(sim)
+    orc.set_spots(rows.astype(np.float64))
+    orc.set_face_flux(face)
+    vel, S, S2, p, t0, fr = case["base"]
+    h = 1e-5
+    vals = {}
+    for sgn in (1, -1):
+        orc.set_flux64(spot=rows[:, 4].astype(np.float64) + sgn * h * d_spot,
+                       face=face.astype(np.float64) + sgn * h * d_face)
+        _oracle_load(orc, (vel, S + sgn * h * d_S.astype(np.float64), S2 + sgn * h * d_S2.astype(np.float64), p, t0, fr))
+        orc.step(DT, n)
+        vals[sgn] = _weighted(orc, w)
+    fd = (vals[1] - vals[-1]) / (2 * h)
+    err = abs(pred - fd) / abs(fd)
+    print(f"tangent {pred:.6e} FP64 FD {fd:.6e} rel err {err:.3e}", flush=True)
+    assert err <= 1e-3
+
+
+def test_adjoint_is_the_transpose_of_the_tangent_over_200_steps(case):
+    """Dot-product test along one recorded 200-step FP32 trajectory (checkpoints every 20):
+    <w, tangent(d)> = <adjoint(w), d> over spot fluxes, floor flux and the initial state."""
+    print("adjoint/tangent dot product, 200 steps", flush=True)
+    sim, rows = case["sim"], case["rows"]
+    n = 200
+    w = _random_seeds(13)
+    d_spot, d_face, d_vel, d_S, d_S2 = _directions(case, 4)
+    sim.load_state(case["base"])
+    adj = F.Adjoint(sim, n, 20)
+    t = adj.tangent(DT, n, spot_dir=d_spot, face_dir=d_face, state_dir=(d_vel, d_S, d_S2), solve_tol=TOL,
+                    max_cycles=40)
+    lhs = sum(float(np.sum(t[f].astype(np.float64) * w[f])) for f in ("vel", "S", "S2"))
+    for f in ("vel", "S", "S2"):
+        adj.seed(f, w[f])
+    adj.backward(solve_tol=TOL, max_cycles=40)
+    gs, gf = adj.grads(len(rows), face=True)
+    g = {f: adj.read(f).astype(np.float64) for f in ("vel", "S", "S2")}
+    parts = [float(np.sum(gs * d_spot)), float(np.sum(gf * d_face)), float(np.sum(g["vel"] * d_vel)),
+             float(np.sum(g["S"] * d_S)), float(np.sum(g["S2"] * d_S2))]
+    rhs = sum(parts)
+    err = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"<w, J d> {lhs:.9e}  <J^T w, d> {rhs:.9e}  rel {err:.2e}  parts {parts}", flush=True)
+    assert err <= 1e-4
 
 
 def test_autograd_function_matches_the_c_abi(case):
