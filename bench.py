@@ -391,6 +391,9 @@ def adjoint_bench(args):
     out["tangent"] = {"ms_per_step": t_tan / steps * 1e3, "seed_dot_tangent": jd, "grad_dot_direction": jtw,
                       "rel_diff": abs(jd - jtw) / max(abs(jd), abs(jtw))}
     print(f"dot-product test: {jd:.9e} vs {jtw:.9e}", flush=True)
+    if args.timing_only:
+        print(json.dumps(out))
+        return
     # FP32 central differences on the GPU, h = 1e-3 relative (float fluxes)
     fd32 = np.zeros(len(rows))
     for q in range(len(rows)):
@@ -490,6 +493,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="instead of the step: the adjoint workload (200 steps at 64x96x64, gradients of a "
                          "temperature/soot mismatch w.r.t. spot and floor fluxes, checked by FD32 and FD64)")
+    ap.add_argument("--timing-only", action="store_true",
+                    help="with --adjoint: forward, backward and tangent timings and the dot-product test only")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB copy, then 3 steps of each step form")
     args = ap.parse_args()

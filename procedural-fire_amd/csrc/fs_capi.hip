@@ -1152,6 +1152,13 @@ struct AdjSlot {                     // the state at the start of a step
     float4* S2 = nullptr;
     double* p = nullptr;             // FP64 pressure of the previous step (the solve's starting guess)
 };
+struct StepRec {                     // one step's intermediates, kept for its reverse and tangent
+    float4 *vX = nullptr, *sX = nullptr, *s2X = nullptr;   // adv1 out
+    float4 *vY = nullptr, *sY = nullptr, *s2Y = nullptr;   // adv2 out
+    float4* vort = nullptr;                                 // curl
+    float4* vF = nullptr;                                   // forces' velocity, before the projection
+    double* p = nullptr;                                    // the step's pressure (FP64, parity-split)
+};
 
 struct fs_adjoint {
     fs_sim* h = nullptr;
@@ -1159,6 +1166,7 @@ struct fs_adjoint {
     std::vector<AdjSlot> ckpt;       // states 0, every, 2 every, ...
     std::vector<AdjSlot> seg;        // states inside one checkpoint interval (index = step % every; 0 unused)
     AdjSlot fin;                     // the final state (seg is reused by the reverse sweep)
+    std::vector<StepRec> rec;        // intermediates of the steps of one interval (index = step % every)
     std::vector<double> times;       // clock at the start of each step
     std::vector<int64_t> frames;
     double dt = 0.0, t_end = 0.0;
@@ -1167,13 +1175,15 @@ struct fs_adjoint {
     bool have_forward = false, have_grads = false;
     bool have_tangent = false;       // l[L_VEL..L_S2] hold the final state's tangent
     bool seeds_stale = false;        // the seed buffers hold something else: clear on the first seed
-    bool last_interval_held = false;   // seg holds the last interval's states (true after the forward)
-    // adjoint fields: the state (l_vel, l_S, l_S2 = seed, then the result), the
-    // intermediates of one step, and the forces kernel's velocity output (vF)
-    enum { L_VEL, L_S, L_S2, L_VF, L_VY, L_SY, L_S2Y, L_VORT, L_VH, L_SH, L_S2H, VF, L_COUNT };
+    bool last_interval_held = false; // seg and rec hold the last interval (true after the forward)
+    int predict_step = 1, predict_lin = 1;   // V-cycles the last step / adjoint-or-tangent solve needed
+    // adjoint (or tangent) fields: the state (l_vel, l_S, l_S2 = seed, then the result) and
+    // the intermediates of one step
+    enum { L_VEL, L_S, L_S2, L_VF, L_VY, L_SY, L_S2Y, L_VORT, L_VH, L_SH, L_S2H, L_COUNT };
     float4* l[L_COUNT] = {};
     double* gspot = nullptr;         // FS_MAX_SPOTS
     double* gface = nullptr;         // nx * nz
+    double* xlin = nullptr;          // the last adjoint / tangent pressure solution: the next one's start
     float* spot_dir = nullptr;       // FS_MAX_SPOTS (tangent runs)
     float* face_dir = nullptr;       // nx * nz (tangent runs)
     float* target = nullptr;         // 2 * ncell (mismatch targets)
@@ -1201,12 +1211,39 @@ int adj_slot_alloc(fs_adjoint* a, AdjSlot& s) {
         return FS_ENOMEM;
     return FS_OK;
 }
+int adj_rec_alloc(fs_adjoint* a, StepRec& r) {
+    const size_t fb = a->h->ncell * sizeof(float4), pb = mg_level_floats(a->h->mg, 0) * sizeof(double);
+    float4** f[8] = {&r.vX, &r.sX, &r.s2X, &r.vY, &r.sY, &r.s2Y, &r.vort, &r.vF};
+    for (float4** q : f) if (adj_alloc(a, (void**)q, fb)) return FS_ENOMEM;
+    return adj_alloc(a, (void**)&r.p, pb);
+}
 int adj_check(const fs_sim* h) {
     if (h->prec != FS_PREC_FP32) return fail(FS_ESTATE, "the adjoint needs FS_PREC_FP32 storage");
     if (!h->refine()) return fail(FS_ESTATE, "the adjoint needs the converged solve FS_SOLVE_REFINE64");
     if (h->prm[P_visc] != 0.f) return fail(FS_ESTATE, "the adjoint needs visc = 0");
     return FS_OK;
 }
+// FP64-refined solve of A x = bkeep from the current x64, run from the host like the
+// adaptive step: as many iterations as the last solve of this kind needed, then the
+// device's stopping test is read and the solve continued if needed (the iterations past
+// convergence would only launch and exit).  The result equals mg_refine's.
+int refine_solve(fs_sim* h, double tol, int maxc, int* predict) {
+    hipStream_t s = h->stream;
+    maxc = std::min(maxc, MG_MAX_REFINE);
+    mg_refine_begin(h->mg, s, tol, maxc);
+    int k = 0;
+    for (; k < std::min(*predict, maxc); k++) mg_refine_iter(h->mg, s, tol, maxc);
+    while (true) {
+        FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        FS_HIP_CHECK(hipStreamSynchronize(s));
+        if (h->host_state[1] || k >= maxc) break;
+        mg_refine_iter(h->mg, s, tol, maxc);
+        k++;
+    }
+    *predict = std::max(1, h->host_state[0]);
+    return FS_OK;
+}
+
 // copies between the sim's own state and a slot
 void slot_copy(fs_sim* h, const AdjSlot& s, bool to_slot) {
     const size_t fb = h->ncell * sizeof(float4), pb = mg_level_floats(h->mg, 0) * sizeof(double);
@@ -1217,36 +1254,36 @@ void slot_copy(fs_sim* h, const AdjSlot& s, bool to_slot) {
         (void)hipMemcpyAsync(to_slot ? sl[i] : sim[i], to_slot ? sim[i] : sl[i], nb[i], hipMemcpyDeviceToDevice, h->stream);
 }
 
-// Step k from slot `in`, through the pressure solve (sim.js:633-642): adv1 -> the sim's
-// velX/SX/S2X, adv2 -> velY/SY/S2Y, curl -> vort, forces -> vF and (out's or the sim's)
-// S/S2, div -> bkeep, solve -> x64 (from in.p).  With `out`, the projection writes
-// out.vel and the pressure is kept in out.p.
-void replay_step(fs_adjoint* a, int k, const AdjSlot& in, const AdjSlot* out) {
+// Step k from slot `in` to slot `out` (sim.js:633-644), its intermediates written to
+// rec[k % every]: adv1 -> vX/sX/s2X, adv2 -> vY/sY/s2Y, curl -> vort, forces -> vF and
+// out.S/S2, div -> bkeep, solve (from in.p) -> p, projection -> out.vel, out.p.
+void replay_step(fs_adjoint* a, int k, const AdjSlot& in, const AdjSlot& out) {
     fs_sim* h = a->h;
     hipStream_t s = h->stream;
+    const StepRec& r = a->rec[k % a->every];
     h->time = a->times[k];
     h->frame = a->frames[k];
     push_params(h, a->dt);
     FieldSet A = h->fields();
     A.vel = in.vel; A.S = in.S; A.S2 = in.S2;
+    A.velX = r.vX; A.SX = r.sX; A.S2X = r.s2X;
+    A.velY = r.vY; A.SY = r.sY; A.S2Y = r.s2Y;
+    A.vort = r.vort;
     launch_adv1(h->prec, h->g, s, h->P_d, A);
     launch_adv2(h->prec, h->g, s, h->P_d, A);
     launch_curl(h->prec, h->g, s, A);
-    FieldSet B = h->fields();
-    B.velX = a->l[fs_adjoint::VF];
-    if (out) { B.S = out->S; B.S2 = out->S2; }
+    FieldSet B = A;
+    B.velX = r.vF; B.S = out.S; B.S2 = out.S2;
     launch_forces(h->prec, h->g, s, h->P_d, B);
     launch_div(h->prec, h->g, s, h->P_d, B, h->mg.bkeep, 0);
     const size_t pb = mg_level_floats(h->mg, 0) * sizeof(double);
     (void)hipMemcpyAsync(h->mg.x64, in.p, pb, hipMemcpyDeviceToDevice, s);
-    h->solve(s);
-    if (out) {
-        FieldSet C = h->fields();
-        C.velX = a->l[fs_adjoint::VF];
-        C.vel = out->vel;
-        launch_project(h->prec, h->g, s, h->P_d, C, nullptr, h->mg.x64);
-        (void)hipMemcpyAsync(out->p, h->mg.x64, pb, hipMemcpyDeviceToDevice, s);
-    }
+    refine_solve(h, h->cfg.div_tol, h->cfg.max_vcycles, &a->predict_step);
+    FieldSet C = B;
+    C.vel = out.vel;
+    launch_project(h->prec, h->g, s, h->P_d, C, nullptr, h->mg.x64);
+    (void)hipMemcpyAsync(r.p, h->mg.x64, pb, hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(out.p, h->mg.x64, pb, hipMemcpyDeviceToDevice, s);
 }
 
 // The adjoint of step k: l_vel/l_S/l_S2 hold the adjoint of the state after the step on
@@ -1256,32 +1293,35 @@ void reverse_step(fs_adjoint* a, int k, double tol, int maxc) {
     hipStream_t s = h->stream;
     const Dims& g = h->g;
     const AdjSlot& in = a->slot(k);
-    replay_step(a, k, in, nullptr);
+    const StepRec& r = a->rec[k % a->every];
+    h->time = a->times[k];
+    h->frame = a->frames[k];
+    push_params(h, a->dt);
     float4** l = a->l;
     const size_t fb = h->ncell * sizeof(float4), nd = mg_level_floats(h->mg, 0);
     const FieldSet F = h->fields();
     // projection: vF, p -> vel (sim.js:526-550); the pressure adjoint goes to bkeep
     launch_fill_zero(s, h->mg.bkeep, nd * sizeof(float));
-    launch_project_adj(g, s, h->P_d, l[fs_adjoint::VF], h->mg.x64, (const float4*)F.geo, l[fs_adjoint::L_VEL],
-                       l[fs_adjoint::L_VF], h->mg.bkeep);
-    // the pressure solve: A lb = lp, from zero
-    launch_fill_zero(s, h->mg.x64, nd * sizeof(double));
-    mg_refine(h->mg, s, tol, maxc);
+    launch_project_adj(g, s, h->P_d, r.vF, r.p, (const float4*)F.geo, l[fs_adjoint::L_VEL], l[fs_adjoint::L_VF],
+                       h->mg.bkeep);
+    // the pressure solve: A lb = lp, started from the previous step's solution (the
+    // adjoint pressure changes little from step to step)
+    (void)hipMemcpyAsync(h->mg.x64, a->xlin, nd * sizeof(double), hipMemcpyDeviceToDevice, s);
+    refine_solve(h, tol, maxc, &a->predict_lin);
+    (void)hipMemcpyAsync(a->xlin, h->mg.x64, nd * sizeof(double), hipMemcpyDeviceToDevice, s);
     // divergence (sim.js:399-413)
     launch_div_adj(g, s, h->P_d, h->mg.x64, l[fs_adjoint::L_VF]);
     // sources (sim.js:214-396)
     launch_fill_zero(s, l[fs_adjoint::L_VORT], fb);
-    launch_forces_adj(g, s, h->P_d, F, (const float4*)F.velY, (const float4*)F.SY, (const float4*)F.S2Y,
-                      (const float4*)F.vort, l[fs_adjoint::L_VF], l[fs_adjoint::L_S], l[fs_adjoint::L_S2],
-                      l[fs_adjoint::L_VY], l[fs_adjoint::L_SY], l[fs_adjoint::L_S2Y], l[fs_adjoint::L_VORT], a->gspot,
-                      a->gface);
+    launch_forces_adj(g, s, h->P_d, F, r.vY, r.sY, r.s2Y, r.vort, l[fs_adjoint::L_VF], l[fs_adjoint::L_S],
+                      l[fs_adjoint::L_S2], l[fs_adjoint::L_VY], l[fs_adjoint::L_SY], l[fs_adjoint::L_S2Y],
+                      l[fs_adjoint::L_VORT], a->gspot, a->gface);
     // curl (sim.js:200-211)
-    launch_curl_adj(g, s, (const float4*)F.velY, l[fs_adjoint::L_VORT], l[fs_adjoint::L_VY]);
+    launch_curl_adj(g, s, r.vY, l[fs_adjoint::L_VORT], l[fs_adjoint::L_VY]);
     // MacCormack correction and semi-Lagrangian pass (sim.js:142-197)
     for (int i : {fs_adjoint::L_VEL, fs_adjoint::L_S, fs_adjoint::L_S2, fs_adjoint::L_VH, fs_adjoint::L_SH, fs_adjoint::L_S2H})
         launch_fill_zero(s, l[i], fb);
-    launch_adv2_adj(g, s, h->P_d, in.vel, in.S, in.S2, (const float4*)F.velX, (const float4*)F.SX, (const float4*)F.S2X,
-                    l[fs_adjoint::L_VY], l[fs_adjoint::L_SY], l[fs_adjoint::L_S2Y], l[fs_adjoint::L_VEL],
+    launch_adv2_adj(g, s, h->P_d, in.vel, in.S, in.S2, r.vX, r.sX, r.s2X, l[fs_adjoint::L_VY], l[fs_adjoint::L_SY], l[fs_adjoint::L_S2Y], l[fs_adjoint::L_VEL],
                     l[fs_adjoint::L_S], l[fs_adjoint::L_S2], l[fs_adjoint::L_VH], l[fs_adjoint::L_SH], l[fs_adjoint::L_S2H]);
     launch_adv1_adj(g, s, h->P_d, in.vel, in.S, in.S2, l[fs_adjoint::L_VH], l[fs_adjoint::L_SH], l[fs_adjoint::L_S2H],
                     l[fs_adjoint::L_VEL], l[fs_adjoint::L_S], l[fs_adjoint::L_S2]);
@@ -1316,6 +1356,8 @@ int fs_adjoint_create(fs_sim* h, int max_steps, int checkpoint_every, fs_adjoint
     for (auto& s : a->ckpt) if (adj_slot_alloc(a, s)) return bail(FS_ENOMEM);
     for (int i = 1; i < a->every; i++) if (adj_slot_alloc(a, a->seg[i])) return bail(FS_ENOMEM);
     if (adj_slot_alloc(a, a->fin)) return bail(FS_ENOMEM);
+    a->rec.resize(a->every);
+    for (auto& r : a->rec) if (adj_rec_alloc(a, r)) return bail(FS_ENOMEM);
     const size_t fb = h->ncell * sizeof(float4);
     for (int i = 0; i < fs_adjoint::L_COUNT; i++) {
         if (adj_alloc(a, (void**)&a->l[i], fb)) return bail(FS_ENOMEM);
@@ -1324,6 +1366,7 @@ int fs_adjoint_create(fs_sim* h, int max_steps, int checkpoint_every, fs_adjoint
     if (adj_alloc(a, (void**)&a->gspot, FS_MAX_SPOTS * sizeof(double)) ||
         adj_alloc(a, (void**)&a->gface, (size_t)h->nx * h->nz * sizeof(double)) ||
         adj_alloc(a, (void**)&a->spot_dir, FS_MAX_SPOTS * sizeof(float)) ||
+        adj_alloc(a, (void**)&a->xlin, mg_level_floats(h->mg, 0) * sizeof(double)) ||
         adj_alloc(a, (void**)&a->face_dir, (size_t)h->nx * h->nz * sizeof(float)) ||
         adj_alloc(a, (void**)&a->target, 2 * h->ncell * sizeof(float)) ||
         adj_alloc(a, (void**)&a->partial, (size_t)h->g.nblocks * sizeof(double)))
@@ -1351,28 +1394,28 @@ namespace {
 
 // The tangent of step k, after replay_step(k, in, &out): l[L_VEL/L_S/L_S2] hold the state
 // tangent before the step on entry and after it on return.
-void tangent_step(fs_adjoint* a, const AdjSlot& in, const AdjSlot& out, double tol, int maxc) {
+void tangent_step(fs_adjoint* a, int k, const AdjSlot& in, double tol, int maxc) {
     fs_sim* h = a->h;
     hipStream_t s = h->stream;
     const Dims& g = h->g;
     float4** l = a->l;
     const FieldSet F = h->fields();
+    const StepRec& r = a->rec[k % a->every];
     const size_t nd = mg_level_floats(h->mg, 0);
     launch_adv1_tan(g, s, h->P_d, in.vel, in.S, in.S2, l[fs_adjoint::L_VEL], l[fs_adjoint::L_S], l[fs_adjoint::L_S2],
                     l[fs_adjoint::L_VH], l[fs_adjoint::L_SH], l[fs_adjoint::L_S2H]);
-    launch_adv2_tan(g, s, h->P_d, in.vel, in.S, in.S2, (const float4*)F.velX, (const float4*)F.SX, (const float4*)F.S2X,
-                    l[fs_adjoint::L_VEL], l[fs_adjoint::L_S], l[fs_adjoint::L_S2], l[fs_adjoint::L_VH],
+    launch_adv2_tan(g, s, h->P_d, in.vel, in.S, in.S2, r.vX, r.sX, r.s2X, l[fs_adjoint::L_VEL], l[fs_adjoint::L_S], l[fs_adjoint::L_S2], l[fs_adjoint::L_VH],
                     l[fs_adjoint::L_SH], l[fs_adjoint::L_S2H], l[fs_adjoint::L_VY], l[fs_adjoint::L_SY],
                     l[fs_adjoint::L_S2Y]);
-    launch_curl_tan(g, s, (const float4*)F.velY, l[fs_adjoint::L_VY], l[fs_adjoint::L_VORT]);
-    launch_forces_tan(g, s, h->P_d, F, (const float4*)F.velY, (const float4*)F.SY, (const float4*)F.S2Y,
-                      (const float4*)F.vort, a->spot_dir, a->face_dir, l[fs_adjoint::L_VY], l[fs_adjoint::L_SY],
+    launch_curl_tan(g, s, r.vY, l[fs_adjoint::L_VY], l[fs_adjoint::L_VORT]);
+    launch_forces_tan(g, s, h->P_d, F, r.vY, r.sY, r.s2Y, r.vort, a->spot_dir, a->face_dir, l[fs_adjoint::L_VY], l[fs_adjoint::L_SY],
                       l[fs_adjoint::L_S2Y], l[fs_adjoint::L_VORT], l[fs_adjoint::L_VF], l[fs_adjoint::L_S],
                       l[fs_adjoint::L_S2]);
     launch_div_tan(g, s, h->P_d, l[fs_adjoint::L_VF], h->mg.bkeep);
-    launch_fill_zero(s, h->mg.x64, nd * sizeof(double));
-    mg_refine(h->mg, s, tol, maxc);
-    launch_project_tan(g, s, h->P_d, l[fs_adjoint::VF], out.p, (const float4*)F.geo, l[fs_adjoint::L_VF], h->mg.x64,
+    (void)hipMemcpyAsync(h->mg.x64, a->xlin, nd * sizeof(double), hipMemcpyDeviceToDevice, s);
+    refine_solve(h, tol, maxc, &a->predict_lin);
+    (void)hipMemcpyAsync(a->xlin, h->mg.x64, nd * sizeof(double), hipMemcpyDeviceToDevice, s);
+    launch_project_tan(g, s, h->P_d, r.vF, r.p, (const float4*)F.geo, l[fs_adjoint::L_VF], h->mg.x64,
                        l[fs_adjoint::L_VEL]);
 }
 
@@ -1392,8 +1435,8 @@ int run_forward(fs_adjoint* a, double dt, int nsteps, bool tangent, double tol, 
     for (int k = 0; k < nsteps; k++) {
         a->times[k] = h->time;
         a->frames[k] = h->frame;
-        replay_step(a, k, a->slot(k), &a->slot(k + 1));
-        if (tangent) tangent_step(a, a->slot(k), a->slot(k + 1), tol, maxc);
+        replay_step(a, k, a->slot(k), a->slot(k + 1));
+        if (tangent) tangent_step(a, k, a->slot(k), tol, maxc);
         h->time = a->times[k] + dt;   // sim.js:646-647
         h->frame = a->frames[k] + 1;
     }
@@ -1447,6 +1490,7 @@ int fs_adjoint_tangent(fs_adjoint* a, double dt, int nsteps, const double* spot_
         if (dirs[i]) FS_HIP_CHECK(hipMemcpy(a->l[slots[i]], dirs[i], fb, hipMemcpyHostToDevice));
         else launch_fill_zero(h->stream, a->l[slots[i]], fb);
     }
+    launch_fill_zero(h->stream, a->xlin, mg_level_floats(h->mg, 0) * sizeof(double));
     const double tol = solve_tol > 0.0 ? solve_tol : h->cfg.div_tol;
     const int maxc = std::min(max_cycles > 0 ? max_cycles : h->cfg.max_vcycles, MG_MAX_REFINE);
     return run_forward(a, dt, nsteps, true, tol, maxc);
@@ -1508,6 +1552,7 @@ int fs_adjoint_backward(fs_adjoint* a, double solve_tol, int max_cycles) {
     const int maxc = std::min(max_cycles > 0 ? max_cycles : h->cfg.max_vcycles, MG_MAX_REFINE);
     launch_fill_zero(h->stream, a->gspot, FS_MAX_SPOTS * sizeof(double));
     launch_fill_zero(h->stream, a->gface, (size_t)h->nx * h->nz * sizeof(double));
+    launch_fill_zero(h->stream, a->xlin, mg_level_floats(h->mg, 0) * sizeof(double));
     if (a->seeds_stale) {   // nothing seeded since the forward: the seed is zero
         for (int i : {fs_adjoint::L_VEL, fs_adjoint::L_S, fs_adjoint::L_S2})
             launch_fill_zero(h->stream, a->l[i], h->ncell * sizeof(float4));
@@ -1520,7 +1565,8 @@ int fs_adjoint_backward(fs_adjoint* a, double solve_tol, int max_cycles) {
         const int k0 = j * E, k1 = std::min(k0 + E, N);
         // the states inside the interval; the forward left the last interval's in place
         if (j != nseg - 1 || !a->last_interval_held) {
-            for (int k = k0; k < k1 - 1; k++) replay_step(a, k, a->slot(k), &a->slot(k + 1));
+            // deterministic kernels: the states and checkpoints rewritten here are bit-identical
+            for (int k = k0; k < k1; k++) replay_step(a, k, a->slot(k), a->slot(k + 1));
             a->last_interval_held = j == nseg - 1;
         }
         for (int k = k1 - 1; k >= k0; k--) reverse_step(a, k, tol, maxc);
