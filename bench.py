@@ -470,6 +470,59 @@ def adjoint_bench(args):
     print(json.dumps(out))
 
 
+def sparks_bench(args):
+    """Spark particle update (src/sparks.js:55-135) over the step's velocity: the 8-spot scene at
+    128x192x128 (fp16 fields) after 100 steps from rest; N sparks spawned by one burst over the fire,
+    then timed spark steps on the frozen velocity field.  Algorithmic bytes: one read and one write
+    of the 64-B particle record per update (the velocity and curl samples hit the caches)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import fsoracle as O
+    sim = scenes.make_scene("spots8_128", precision=F.PREC_FP16, solve=F.SOLVE_REFINE64, div_tol=1e-5, max_vcycles=8,
+                            noise_n=128, use_graph=True)
+    sim.step(DT, 100)
+    sim.sync()
+    out = {"workload": "sparks over the 8-spot 128x192x128 fire (fp16 fields, 100 steps from rest), dt 1 ms"}
+    steps = max(args.steps // 10, 20)
+    for count in (8192, 1 << 22):
+        sp = F.Sparks(sim, count)
+        sp.set(rate=0.0)
+        sp.burst(n=count, pos=(0.0, 0.25, 0.0), speed=1.0, radius=0.2, dur=1e-4, fan=0.3)
+        sp.step(DT)
+        for _ in range(3):
+            sp.step(DT)
+        sim.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sp.step(DT)
+        sim.sync()
+        t = (time.perf_counter() - t0) / steps
+        alive = int((sp.read()[:, 14] > 0.5).sum())
+        gbs = count * 128 / t / 1e9
+        out[f"n{count}"] = {"us_per_step": t * 1e6, "particle_updates_per_s": count / t, "alive": alive,
+                            "achieved_GBs": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+        print(f"{count} sparks: {t * 1e6:.1f} us/step, {count / t / 1e9:.2f} G updates/s, {gbs:.0f} GB/s, "
+              f"{alive} alive", flush=True)
+        sp.close()
+    out["roofline"] = {"bound": "hbm", "kernel": "k_spark_update (4M sparks)", "achieved": out[f"n{1 << 22}"]["achieved_GBs"],
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": out[f"n{1 << 22}"]["frac_of_hbm_peak"]}
+    # CPU baseline: the oracle's update (one thread) on 65536 sparks over the same velocity field
+    res = RES
+    orc = O.OracleSim(res=res, size=SIZE, logs=np.zeros(8, np.float32), n_logs=0, precision=O.FP16, noise_n=128)
+    for f in (F.VEL, F.CURL):
+        orc.write(f, sim.read(f, np.float64))
+    so = O.OracleSparks(orc, 65536)
+    so.set(rate=0.0)
+    so.burst(n=65536, pos=(0.0, 0.25, 0.0), speed=1.0, radius=0.2, dur=1e-4, fan=0.3)
+    so.step(DT)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        so.step(DT)
+    tc = (time.perf_counter() - t0) / 5
+    out["cpu_baseline"] = {"value": 65536 / tc, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+                           "sample": f"5 oracle spark steps of 65536 sparks; {tc * 1e3:.1f} ms per step"}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -493,6 +546,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="instead of the step: the adjoint workload (200 steps at 64x96x64, gradients of a "
                          "temperature/soot mismatch w.r.t. spot and floor fluxes, checked by FD32 and FD64)")
+    ap.add_argument("--sparks", action="store_true",
+                    help="instead of the step: spark-particle update throughput (8192 and 4M sparks) on the step's velocity")
     ap.add_argument("--timing-only", action="store_true",
                     help="with --adjoint: forward, backward and tangent timings and the dot-product test only")
     ap.add_argument("--probe", action="store_true",
@@ -517,6 +572,9 @@ def main():
         return
     if args.adjoint:
         adjoint_bench(args)
+        return
+    if args.sparks:
+        sparks_bench(args)
         return
     ncell = float(np.prod(RES))
     graph = not args.no_graph

@@ -323,6 +323,44 @@ int  fs_adjoint_read(fs_adjoint* a, int field, int dtype, void* host, size_t byt
  * may be NULL). */
 int  fs_adjoint_grads(fs_adjoint* a, double* spot_flux, int max_spots, double* face_flux);
 
+/* ---- spark (ember) particles carried by the velocity field ---------------------
+ * Replaces `new Sparks(device, {count, sim, curlTex})` (src/sparks.js:17-36), its
+ * reset/burst/step (src/sparks.js:38-46, 241-271) and the update kernel
+ * (src/sparks.js:55-135).  Drawing (src/sparks.js:273-289) stays with the renderer.
+ * A particle is FS_SPARK_FLOATS floats: position + age, velocity + burn time left,
+ * previous position + temperature (K), diameter (mm) / seed / alive / flicker (Hz).
+ * fs_sparks_step reads the sim's current velocity: call it right after fs_step with
+ * the same dt, on the sim's stream.  Dead particles take the due spawn slots in index
+ * order (the reference takes them in atomic-counter order), so runs are reproducible. */
+#define FS_SPARK_FLOATS 16
+typedef struct fs_sparks fs_sparks;
+typedef struct fs_spark_burst {          /* Sparks.burst({...}) options (src/sparks.js:44-46) */
+    int   n;                             /* sparks (60)                                      */
+    float pos[3];                        /* centre (0, 0.1, 0) m                             */
+    float speed;                         /* launch speed (3 m/s)                             */
+    float radius;                        /* spawn box half-size (0.05 m)                     */
+    double dur;                          /* emission time (0.12 s; double like the JS number) */
+    float fan;                           /* outward fan (0.6)                                */
+    float size;                          /* size factor of burst sparks (1.3)                */
+    int   has_aim;                       /* 0 = no aim (the reference's aim: null)           */
+    float aim[3];                        /* aim direction (x, z used)                        */
+} fs_spark_burst;
+
+void fs_spark_burst_default(fs_spark_burst* b);
+int  fs_sparks_create(fs_sim* h, int count, fs_sparks** out);          /* count <= 2^24 */
+/* May be called after the sim was destroyed (it touches only the particle memory). */
+int  fs_sparks_destroy(fs_sparks* sp);
+int  fs_sparks_reset(fs_sparks* sp);
+/* "rate" (sparks/s, 3), "turb" (1), "lifeK" (1), "sizeK" (1), "upK" (1), "brightness" (13) */
+int  fs_sparks_set(fs_sparks* sp, const char* name, float value);
+int  fs_sparks_burst(fs_sparks* sp, const fs_spark_burst* b);
+int  fs_sparks_step(fs_sparks* sp, double dt);
+int  fs_sparks_read(fs_sparks* sp, float* host, size_t bytes);         /* count * 16 floats */
+int  fs_sparks_write(fs_sparks* sp, const float* host, size_t bytes);
+int  fs_sparks_clock(fs_sparks* sp, double* time, int64_t* frame);
+/* Device address of the particle array (count * 64 B), for renderers on the same device. */
+void* fs_sparks_device_ptr(fs_sparks* sp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -370,6 +370,13 @@ int refine64(Multigrid<M>& mg, std::vector<double>& x, const std::vector<M>& b_k
     return used;
 }
 
+// ---- spark particles (src/sparks.js) ------------------------------------------
+// The update kernel's uniforms, SPARK_FIELDS order (src/sparks.js:10-14), f32 as uploaded.
+struct SparkU {
+    float dt, time, frame, count, nBase, nBurst, burstX, burstY, burstZ, burstSpeed, burstRadius;
+    float ox, oy, oz, sx, sy, sz, turb, lifeK, bedRadius, seed, sizeK, upK, burstFan, burstSize, aimX, aimZ;
+};
+
 // ---- the simulation ---------------------------------------------------------
 struct OracleBase {
     virtual ~OracleBase() {}
@@ -391,6 +398,8 @@ struct OracleBase {
     virtual int set_spots(const fs_spot* sp, int n, fs_spot* out) = 0;
     virtual void set_face(const float* f) = 0;
     virtual void set_flux64(const double* spot, int n, const double* face) = 0;
+    virtual void spark_update(const struct SparkU& U, float* sp, int count) const = 0;
+    virtual void box(double org[3], float size[3]) const = 0;
 };
 
 template <class R, class S, class M>
@@ -399,6 +408,7 @@ struct Oracle : OracleBase {
     int nx, ny, nz;
     size_t ncell;
     double dx, ox, oy, oz;
+    float size_[3];
     int vcycles, solve_mode, max_vcycles;
     double div_tol;
     float prm[FS_NUM_PARAMS];
@@ -422,6 +432,7 @@ struct Oracle : OracleBase {
         ncell = (size_t)nx * ny * nz;
         dx = (double)c->size[0] / nx;                                 // sim.js:53
         ox = -(double)c->size[0] / 2; oy = 0; oz = -(double)c->size[2] / 2;   // sim.js:54
+        for (int k = 0; k < 3; k++) size_[k] = c->size[k];
         vcycles = c->vcycles; solve_mode = c->solve; max_vcycles = c->max_vcycles; div_tol = c->div_tol;
         set_defaults(prm);
         for (auto* f : {&vel, &velX, &velY, &Sf, &SX, &SY, &S2, &S2X, &S2Y, &vort, &geo})
@@ -959,6 +970,92 @@ struct Oracle : OracleBase {
     void set_face(const float* f) override {
         if (f) face.assign(f, f + (size_t)nx * nz); else face.clear();
     }
+    void box(double org[3], float size[3]) const override {
+        org[0] = ox; org[1] = oy; org[2] = oz;
+        for (int k = 0; k < 3; k++) size[k] = size_[k];
+    }
+    // One launch of the spark update (src/sparks.js:80-135) over the particle array, in f32.
+    // Dead particles take the spawn slots in index order.
+    void spark_update(const SparkU& U, float* sp, int count) const override {
+        const uint32_t want = (uint32_t)(U.nBase + U.nBurst);
+        uint32_t next = 0;
+        const float dt = U.dt;
+        for (int i = 0; i < count; i++) {
+            float* P = sp + 16 * i; float* Vv = P + 4; float* Q = P + 8; float* Mm = P + 12;
+            if (Mm[2] < 0.5f) {
+                const uint32_t slot = next++;
+                if (want == 0u || slot >= want) continue;
+                const uint32_t seed = (uint32_t)U.seed, frame = (uint32_t)U.frame;
+                auto r = [&](uint32_t k) { return rand2(slot * 7919u + k * 104729u + seed, frame); };
+                auto gauss = [&](uint32_t k) {
+                    const float u1 = std::fmax(r(k), 1e-6f);
+                    const float u2 = r(k + 1u);
+                    return std::sqrt(-2.0f * flog(u1)) * fcos(6.2831853f * u2);
+                };
+                const bool burst = slot >= (uint32_t)U.nBase;
+                V3<float> p, v;
+                if (burst) {
+                    p = V3<float>(U.burstX, U.burstY, U.burstZ) +
+                        (V3<float>(r(7u), r(8u), r(9u)) - V3<float>(0.5f, 0.5f, 0.5f)) * U.burstRadius * 2.0f;
+                    V3<float> radial(p.x, 0.0f, p.z);
+                    radial = length(radial) > 1e-3f ? radial / length(radial) : V3<float>();
+                    const V3<float> aim(U.aimX, 0.0f, U.aimZ);
+                    V3<float> d = V3<float>(gauss(1u), std::fabs(gauss(3u)) * 1.2f + 0.7f, gauss(5u)) + (radial + aim) * U.burstFan;
+                    d = d / length(d);
+                    v = d * U.burstSpeed * (0.5f + 0.8f * r(10u));
+                } else {
+                    const float a = r(1u) * 6.2831853f;
+                    const float rr = std::sqrt(r(2u)) * U.bedRadius;
+                    p = V3<float>(fcos(a) * rr, 0.05f + r(3u) * 0.12f, fsin(a) * rr);
+                    v = V3<float>(gauss(4u) * 0.4f, 0.6f + r(6u) * 1.8f, gauss(11u) * 0.4f);
+                }
+                const float dmm = wclamp<float>(0.5f * fexp(0.55f * gauss(13u)) * U.sizeK * (burst ? U.burstSize : 1.0f), 0.15f, 2.5f);
+                const float tb = (0.25f + 2.0f * dmm * dmm) * (0.6f + 0.8f * r(15u)) * U.lifeK;
+                const float T0 = 1230.0f + 70.0f * gauss(16u) + (burst ? 80.0f : 0.0f);
+                const float fl = r(20u) < 0.45f ? 5.0f + 15.0f * r(18u) : 0.0f;
+                P[0] = p.x; P[1] = p.y; P[2] = p.z; P[3] = 0.0f;
+                Vv[0] = v.x; Vv[1] = v.y; Vv[2] = v.z; Vv[3] = tb;
+                Q[0] = p.x; Q[1] = p.y; Q[2] = p.z; Q[3] = T0;
+                Mm[0] = dmm; Mm[1] = r(19u); Mm[2] = 1.0f; Mm[3] = fl;
+                continue;
+            }
+            const float dmm = Mm[0];
+            V3<float> p(P[0], P[1], P[2]), v(Vv[0], Vv[1], Vv[2]);
+            const float tau = std::fmax(0.4f * dmm * dmm, 0.01f);
+            V3<float> u;
+            const float gx = (p.x - U.ox) / U.sx, gy = (p.y - U.oy) / U.sy, gz = (p.z - U.oz) / U.sz;
+            if (gx < 0.f || gy < 0.f || gz < 0.f || gx > 1.f || gy > 1.f || gz > 1.f) {
+                if (gy > 1.0f && std::sqrt(p.x * p.x + p.z * p.z) < 0.5f) u = V3<float>(0.0f, 1.6f / (1.0f + (gy - 1.0f) * 1.5f), 0.0f);
+            } else {
+                const V4<R> s4 = sample(vel, V3<R>((R)(gx * (float)nx), (R)(gy * (float)ny), (R)(gz * (float)nz)));
+                u = V3<float>((float)s4.x, (float)s4.y, (float)s4.z);
+            }
+            u = u * U.upK;
+            const float tqx = p.x * 0.9f + 0.0f, tqy = p.y * 0.9f + -U.time * 0.5f, tqz = p.z * 0.9f + (U.time * 0.13f + Mm[1] * 3.0f);
+            const float turbAmp = U.turb * (0.3f + 3.0f * smoothstep<float>(0.4f, 1.2f, p.y)) / 12.0f;
+            const V4<R> cn = sample_rep(curl, V3<R>((R)tqx, (R)tqy, (R)tqz));
+            u = u + V3<float>((float)cn.x, (float)cn.y, (float)cn.z) * turbAmp;
+            const float k = 1.0f - fexp(-dt / tau);
+            v = V3<float>(mix(v.x, u.x, k), mix(v.y, u.y, k), mix(v.z, u.z, k));
+            v.y = v.y - 9.77076f * dt * (dmm < 0.3f ? 0.2f : 1.0f);
+            const V3<float> prev = p;
+            p = p + v * dt;
+            float T = Q[3];
+            const float tb = Vv[3] - dt;
+            if (tb > 0.0f) {
+                const float rel = length(u - v);
+                T = mix(T, 1180.0f + 60.0f * std::fmin(rel, 3.0f) + 60.0f * (Mm[1] - 0.5f), 1.0f - fexp(-dt / 0.3f));
+            } else {
+                T = 293.0f + (T - 293.0f) * fexp(-dt / (0.06f + 0.08f * dmm));
+            }
+            const float age = P[3] + dt;
+            const bool kill = T < 760.0f || p.y < 0.0f || p.y > 4.5f || std::sqrt(p.x * p.x + p.z * p.z) > 3.0f || age > 12.0f;
+            P[0] = p.x; P[1] = p.y; P[2] = p.z; P[3] = age;
+            Vv[0] = v.x; Vv[1] = v.y; Vv[2] = v.z; Vv[3] = tb;
+            Q[0] = prev.x; Q[1] = prev.y; Q[2] = prev.z; Q[3] = T;
+            Mm[2] = kill ? 0.0f : 1.0f;
+        }
+    }
     void set_flux64(const double* spot, int n, const double* f) override {
         if (spot) spot64.assign(spot, spot + n); else spot64.clear();
         if (f) face64.assign(f, f + (size_t)nx * nz); else face64.clear();
@@ -1241,5 +1338,77 @@ double orc_mjs_study(int N, int face, int mode, double omega, int pre, int post,
 
 const char* orc_last_error(void) { return g_err.c_str(); }
 int orc_num_threads(void) { return omp_get_max_threads(); }
+
+
+// ---- spark particles: the host side of src/sparks.js:17-46, 241-271 ------------------
+struct orc_sparks {
+    orc_sim* sim;
+    int count;
+    std::vector<float> sp;
+    double rate = 3.0, carry = 0.0;
+    float turb = 1.f, lifeK = 1.f, sizeK = 1.f, upK = 1.f;
+    struct Burst { double t0; int n; float pos[3]; float speed, radius; double dur; float fan, size; int has_aim; float aim[3]; int emitted; };
+    std::vector<Burst> bursts;
+    double time = 0.0;
+    long frame = 0;
+};
+orc_sparks* orc_sparks_create(orc_sim* s, int count) {
+    orc_sparks* p = new orc_sparks();
+    p->sim = s; p->count = count; p->sp.assign((size_t)count * 16, 0.f);
+    return p;
+}
+void orc_sparks_destroy(orc_sparks* p) { delete p; }
+void orc_sparks_reset(orc_sparks* p) {
+    std::fill(p->sp.begin(), p->sp.end(), 0.f);
+    p->carry = 0.0; p->bursts.clear(); p->time = 0.0; p->frame = 0;
+}
+int orc_sparks_set(orc_sparks* p, const char* name, float v) {
+    const std::string n(name);
+    if (n == "rate") p->rate = v; else if (n == "turb") p->turb = v; else if (n == "lifeK") p->lifeK = v;
+    else if (n == "sizeK") p->sizeK = v; else if (n == "upK") p->upK = v; else if (n != "brightness") return -1;
+    return 0;
+}
+void orc_sparks_burst(orc_sparks* p, int n, const float* pos, float speed, float radius, double dur, float fan, float size,
+                      int has_aim, const float* aim) {
+    orc_sparks::Burst b;
+    b.t0 = p->time; b.n = n; b.speed = speed; b.radius = radius; b.dur = dur; b.fan = fan; b.size = size;
+    b.has_aim = has_aim; b.emitted = 0;
+    for (int k = 0; k < 3; k++) { b.pos[k] = pos[k]; b.aim[k] = aim ? aim[k] : 0.f; }
+    p->bursts.push_back(b);
+}
+void orc_sparks_step(orc_sparks* p, double dt) {
+    p->carry += p->rate * dt;
+    const double nBase = std::floor(p->carry);
+    p->carry -= nBase;
+    int nBurst = 0;
+    orc_sparks::Burst B{};
+    bool have = false;
+    for (auto& b : p->bursts) {
+        const double frac = std::min((p->time + dt - b.t0) / b.dur, 1.0);
+        const int due = (int)std::min((double)b.n, std::floor((double)b.n * frac + 0.5));
+        const int k = due - b.emitted;
+        if (k > 0 && !have) { have = true; nBurst = k; b.emitted += k; B = b; }
+    }
+    p->bursts.erase(std::remove_if(p->bursts.begin(), p->bursts.end(), [](const orc_sparks::Burst& b) { return b.emitted >= b.n; }),
+                    p->bursts.end());
+    double org[3]; float size[3];
+    p->sim->o->box(org, size);
+    SparkU U;
+    U.dt = (float)dt; U.time = (float)p->time; U.frame = (float)p->frame; U.count = (float)p->count;
+    U.nBase = (float)nBase; U.nBurst = (float)nBurst;
+    U.burstX = have ? B.pos[0] : 0.f; U.burstY = have ? B.pos[1] : 0.f; U.burstZ = have ? B.pos[2] : 0.f;
+    U.burstSpeed = have ? B.speed : 0.f; U.burstRadius = have ? B.radius : 0.f;
+    U.burstFan = have ? B.fan : 0.f; U.burstSize = have ? B.size : 1.f;
+    U.aimX = have && B.has_aim ? B.aim[0] : 0.f; U.aimZ = have && B.has_aim ? B.aim[2] : 0.f;
+    U.ox = (float)org[0]; U.oy = (float)org[1]; U.oz = (float)org[2];
+    U.sx = size[0]; U.sy = size[1]; U.sz = size[2];
+    U.turb = p->turb; U.lifeK = p->lifeK; U.bedRadius = 0.2f; U.seed = 1234.f; U.sizeK = p->sizeK; U.upK = p->upK;
+    p->sim->o->spark_update(U, p->sp.data(), p->count);
+    p->time += dt;
+    p->frame += 1;
+}
+void orc_sparks_read(orc_sparks* p, float* out) { std::copy(p->sp.begin(), p->sp.end(), out); }
+void orc_sparks_write(orc_sparks* p, const float* in) { std::copy(in, in + p->sp.size(), p->sp.begin()); }
+double orc_sparks_time(orc_sparks* p) { return p->time; }
 
 }  // extern "C"

@@ -134,6 +134,18 @@ def lib():
         L.orc_set_spots.argtypes = [vp, ctypes.POINTER(FsSpot), ctypes.c_int, ctypes.POINTER(FsSpot)]
         L.orc_set_face_flux.argtypes = [vp, fp]
         L.orc_set_flux64.argtypes = [vp, dp, ctypes.c_int, dp]
+        L.orc_sparks_create.restype = vp
+        L.orc_sparks_create.argtypes = [vp, ctypes.c_int]
+        L.orc_sparks_destroy.argtypes = [vp]
+        L.orc_sparks_reset.argtypes = [vp]
+        L.orc_sparks_set.argtypes = [vp, ctypes.c_char_p, ctypes.c_float]
+        L.orc_sparks_burst.argtypes = [vp, ctypes.c_int, fp, ctypes.c_float, ctypes.c_float, ctypes.c_double,
+                                       ctypes.c_float, ctypes.c_float, ctypes.c_int, fp]
+        L.orc_sparks_step.argtypes = [vp, ctypes.c_double]
+        L.orc_sparks_read.argtypes = [vp, fp]
+        L.orc_sparks_write.argtypes = [vp, fp]
+        L.orc_sparks_time.restype = ctypes.c_double
+        L.orc_sparks_time.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -322,6 +334,54 @@ class OracleSim:
         u = ctypes.c_int()
         r = lib().orc_residual(self.h, ctypes.byref(u))
         return r, u.value
+
+
+class OracleSparks:
+    """CPU restatement of the spark particles (src/sparks.js) over an OracleSim's velocity and curl
+    texture; dead particles take the spawn slots in index order, like the library."""
+
+    def __init__(self, sim, count=8192):
+        self.sim = sim
+        self.count = int(count)
+        self.h = lib().orc_sparks_create(sim.h, self.count)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_sparks_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def reset(self):
+        lib().orc_sparks_reset(self.h)
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            if lib().orc_sparks_set(self.h, k.encode(), float(v)):
+                raise ValueError(k)
+
+    def burst(self, n=60, pos=(0.0, 0.1, 0.0), speed=3.0, radius=0.05, dur=0.12, fan=0.6, size=1.3, aim=None):
+        p = np.asarray(pos, np.float32)
+        a = np.asarray(aim if aim is not None else (0, 0, 0), np.float32)
+        lib().orc_sparks_burst(self.h, int(n), _fp(p), float(speed), float(radius), float(dur), float(fan), float(size),
+                               1 if aim is not None else 0, _fp(a))
+
+    def step(self, dt):
+        lib().orc_sparks_step(self.h, float(dt))
+
+    def read(self):
+        out = np.zeros((self.count, 16), np.float32)
+        lib().orc_sparks_read(self.h, _fp(out))
+        return out
+
+    def write(self, arr):
+        a = np.ascontiguousarray(arr, np.float32)
+        lib().orc_sparks_write(self.h, _fp(a))
+
+    @property
+    def time(self):
+        return lib().orc_sparks_time(self.h)
 
 
 class OracleMG:

@@ -137,6 +137,37 @@ static inline float fpow(float a, float b) {
     if (a == 0.0f) return b > 0.0f ? 0.0f : INFINITY;
     return fexp(b * flog(a));
 }
+// sin / cos: Cody-Waite reduction by k = rint(x 2/pi), pi/2 split into 12 + 12 + 24
+// significant bits, Taylor polynomials of degree 9 (sin) and 10 (cos) on |r| <= pi/4.
+static inline void fsincos_red(float x, float& sr, float& cr, int& q) {
+    const float k = std::rint(x * 0.636619747f);
+    const float r = ((x - k * 1.5703125f) - k * 4.83751297e-4f) - k * 7.54978995e-8f;
+    const float r2 = r * r;
+    float ps = 1.0f / 362880.0f;
+    ps = -1.0f / 5040.0f + r2 * ps;
+    ps = 1.0f / 120.0f + r2 * ps;
+    ps = -1.0f / 6.0f + r2 * ps;
+    sr = r + r * (r2 * ps);
+    float pc = -1.0f / 3628800.0f;
+    pc = 1.0f / 40320.0f + r2 * pc;
+    pc = -1.0f / 720.0f + r2 * pc;
+    pc = 1.0f / 24.0f + r2 * pc;
+    pc = -0.5f + r2 * pc;
+    cr = 1.0f + r2 * pc;
+    q = ((int)k) & 3;
+}
+static inline float fsin(float x) {
+    float s, c;
+    int q;
+    fsincos_red(x, s, c, q);
+    return q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
+}
+static inline float fcos(float x) {
+    float s, c;
+    int q;
+    fsincos_red(x, s, c, q);
+    return q == 0 ? c : q == 1 ? -s : q == 2 ? -c : s;
+}
 template <class R> static inline R wexp(R x);
 template <> inline float wexp<float>(float x) { return fexp(x); }
 template <> inline double wexp<double>(double x) { return std::exp(x); }
@@ -170,6 +201,15 @@ template <class R> struct V4 {
 template <class R> static inline V4<R> mix4(const V4<R>& a, const V4<R>& b, R t) {
     return V4<R>(mix(a.x, b.x, t), mix(a.y, b.y, t), mix(a.z, b.z, t), mix(a.w, b.w, t));
 }
+
+// ---- hashing: pcg (src/shaders/lib.js:11-17) --------------------------------
+static inline uint32_t pcg1(uint32_t n) {
+    const uint32_t s = n * 747796405u + 2891336453u;
+    const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+// rand2(n, m) = f32(pcg(n ^ pcg(m))) * f32(1/4294967295); the constant rounds to 2^-32
+static inline float rand2(uint32_t n, uint32_t m) { return (float)pcg1(n ^ pcg1(m)) * 2.3283064365386963e-10f; }
 
 // ---- hashing: pcg3d + hash3f (src/shaders/lib.js:4-16) -----------------------
 static inline void pcg3d(uint32_t& x, uint32_t& y, uint32_t& z) {

@@ -1618,3 +1618,172 @@ int fs_adjoint_grads(fs_adjoint* a, double* spot_flux, int max_spots, double* fa
 }
 
 }  // extern "C"
+
+// ---- spark particles (src/sparks.js) -----------------------------------------------
+struct fs_sparks {
+    fs_sim* h = nullptr;
+    int device = 0;                  // for destroy, which must not touch the sim (it may be gone)
+    int count = 0;
+    float4* sp = nullptr;            // count x 4 float4
+    int* dead = nullptr;             // per-block dead counts (kept current by the update kernel)
+    int* off = nullptr;              // their exclusive scan: the blocks' first spawn slots
+    bool counts_valid = false;       // false after a host write / reset: count before the next step
+    double rate = 3.0;               // baseline sparks per second (src/sparks.js:24)
+    double carry = 0.0;
+    float turb = 1.f, lifeK = 1.f, sizeK = 1.f, upK = 1.f, brightness = 13.f;   // src/sparks.js:29
+    struct Burst { double t0; int n; float pos[3]; float speed, radius; double dur; float fan, size; bool has_aim; float aim[3]; int emitted; };
+    std::vector<Burst> bursts;
+    double time = 0.0;
+    int64_t frame = 0;
+};
+
+extern "C" {
+
+void fs_spark_burst_default(fs_spark_burst* b) {   // src/sparks.js:44 defaults
+    if (!b) return;
+    b->n = 60; b->pos[0] = 0.f; b->pos[1] = 0.1f; b->pos[2] = 0.f;
+    b->speed = 3.f; b->radius = 0.05f; b->dur = 0.12; b->fan = 0.6f; b->size = 1.3f;
+    b->has_aim = 0; b->aim[0] = b->aim[1] = b->aim[2] = 0.f;
+}
+
+int fs_sparks_create(fs_sim* h, int count, fs_sparks** out) {
+    if (!h || !out) return fail(FS_EINVAL, "null argument");
+    *out = nullptr;
+    if (count < 1 || count > (1 << 24)) return fail(FS_EINVAL, "count must be in 1..2^24");
+    if (set_device(h)) return FS_EDEVICE;
+    fs_sparks* s = new fs_sparks();
+    s->h = h;
+    s->device = h->cfg.device;
+    s->count = count;
+    const size_t nb = (size_t)spark_blocks(count) * sizeof(int);
+    if (hipMalloc(&s->sp, (size_t)count * 64) != hipSuccess || hipMalloc(&s->dead, nb) != hipSuccess ||
+        hipMalloc(&s->off, nb) != hipSuccess) {
+        if (s->sp) (void)hipFree(s->sp);
+        if (s->dead) (void)hipFree(s->dead);
+        delete s;
+        return fail(FS_ENOMEM, "hipMalloc failed for sparks");
+    }
+    launch_fill_zero(h->stream, s->sp, (size_t)count * 64);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) {
+        (void)hipFree(s->sp); (void)hipFree(s->dead); (void)hipFree(s->off); delete s;
+        return fail(FS_EDEVICE, "device error in sparks create");
+    }
+    *out = s;
+    return FS_OK;
+}
+
+int fs_sparks_destroy(fs_sparks* s) {
+    if (!s) return FS_OK;
+    (void)hipSetDevice(s->device);   // hipFree waits for the device's work
+    (void)hipFree(s->sp);
+    (void)hipFree(s->dead);
+    (void)hipFree(s->off);
+    delete s;
+    return FS_OK;
+}
+
+int fs_sparks_reset(fs_sparks* s) {   // src/sparks.js:38-41
+    if (!s) return fail(FS_EINVAL, "null handle");
+    if (set_device(s->h)) return FS_EDEVICE;
+    launch_fill_zero(s->h->stream, s->sp, (size_t)s->count * 64);
+    s->counts_valid = false;
+    s->carry = 0.0; s->bursts.clear(); s->time = 0.0; s->frame = 0;
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int fs_sparks_set(fs_sparks* s, const char* name, float value) {
+    if (!s || !name) return fail(FS_EINVAL, "null argument");
+    const std::string n(name);
+    if (n == "rate") s->rate = value;
+    else if (n == "turb") s->turb = value;
+    else if (n == "lifeK") s->lifeK = value;
+    else if (n == "sizeK") s->sizeK = value;
+    else if (n == "upK") s->upK = value;
+    else if (n == "brightness") s->brightness = value;
+    else return fail(FS_EINVAL, "unknown spark parameter: " + n);
+    return FS_OK;
+}
+
+int fs_sparks_burst(fs_sparks* s, const fs_spark_burst* b) {   // src/sparks.js:44-46
+    if (!s || !b) return fail(FS_EINVAL, "null argument");
+    if (b->n < 0 || !(b->dur > 0.0)) return fail(FS_EINVAL, "burst needs n >= 0 and dur > 0");
+    fs_sparks::Burst q;
+    q.t0 = s->time; q.n = b->n;
+    for (int k = 0; k < 3; k++) { q.pos[k] = b->pos[k]; q.aim[k] = b->aim[k]; }
+    q.speed = b->speed; q.radius = b->radius; q.dur = b->dur; q.fan = b->fan; q.size = b->size;
+    q.has_aim = b->has_aim != 0;
+    q.emitted = 0;
+    s->bursts.push_back(q);
+    return FS_OK;
+}
+
+int fs_sparks_step(fs_sparks* s, double dt) {   // src/sparks.js:241-271
+    if (!s) return fail(FS_EINVAL, "null handle");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(FS_EINVAL, "dt must be positive and finite");
+    fs_sim* h = s->h;
+    if (set_device(h)) return FS_EDEVICE;
+    s->carry += s->rate * dt;
+    const double nBase = std::floor(s->carry);
+    s->carry -= nBase;
+    int nBurst = 0;
+    const fs_sparks::Burst* B = nullptr;
+    for (auto& b : s->bursts) {
+        const double frac = std::min((s->time + dt - b.t0) / b.dur, 1.0);
+        const int due = (int)std::min((double)b.n, std::floor((double)b.n * frac + 0.5));   // Math.round
+        const int k = due - b.emitted;
+        if (k > 0 && !B) { B = &b; nBurst = k; b.emitted += k; }
+    }
+    fs_sparks::Burst Bc{};
+    if (B) Bc = *B;
+    s->bursts.erase(std::remove_if(s->bursts.begin(), s->bursts.end(),
+                                   [](const fs_sparks::Burst& b) { return b.emitted >= b.n; }),
+                    s->bursts.end());
+    SparkUniform U;
+    U.dt = (float)dt; U.time = (float)s->time; U.frame = (float)s->frame; U.count = (float)s->count;
+    U.nBase = (float)nBase; U.nBurst = (float)nBurst;
+    U.burstX = B ? Bc.pos[0] : 0.f; U.burstY = B ? Bc.pos[1] : 0.f; U.burstZ = B ? Bc.pos[2] : 0.f;
+    U.burstSpeed = B ? Bc.speed : 0.f; U.burstRadius = B ? Bc.radius : 0.f;
+    U.burstFan = B ? Bc.fan : 0.f; U.burstSize = B ? Bc.size : 1.f;
+    U.aimX = B && Bc.has_aim ? Bc.aim[0] : 0.f; U.aimZ = B && Bc.has_aim ? Bc.aim[2] : 0.f;
+    U.ox = (float)h->origin[0]; U.oy = (float)h->origin[1]; U.oz = (float)h->origin[2];
+    U.sx = h->cfg.size[0]; U.sy = h->cfg.size[1]; U.sz = h->cfg.size[2];
+    U.turb = s->turb; U.lifeK = s->lifeK; U.bedRadius = 0.2f; U.seed = 1234.f; U.sizeK = s->sizeK; U.upK = s->upK;
+    if (!s->counts_valid) launch_spark_count(h->stream, s->sp, s->count, s->dead);
+    launch_spark_step(h->prec, h->stream, U, s->sp, s->dead, s->off, h->f[F_VEL], h->g, h->curl, h->cfg.noise_n);
+    s->counts_valid = true;
+    s->time += dt;
+    s->frame += 1;
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int fs_sparks_read(fs_sparks* s, float* host, size_t bytes) {
+    if (!s || !host) return fail(FS_EINVAL, "null argument");
+    if (bytes != (size_t)s->count * 64) return fail(FS_EINVAL, "size mismatch: expected count * 64 bytes");
+    if (set_device(s->h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipStreamSynchronize(s->h->stream));
+    FS_HIP_CHECK(hipMemcpy(host, s->sp, bytes, hipMemcpyDeviceToHost));
+    return FS_OK;
+}
+
+int fs_sparks_write(fs_sparks* s, const float* host, size_t bytes) {
+    if (!s || !host) return fail(FS_EINVAL, "null argument");
+    if (bytes != (size_t)s->count * 64) return fail(FS_EINVAL, "size mismatch: expected count * 64 bytes");
+    if (set_device(s->h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipStreamSynchronize(s->h->stream));
+    FS_HIP_CHECK(hipMemcpy(s->sp, host, bytes, hipMemcpyHostToDevice));
+    s->counts_valid = false;
+    return FS_OK;
+}
+
+int fs_sparks_clock(fs_sparks* s, double* time, int64_t* frame) {
+    if (!s) return fail(FS_EINVAL, "null handle");
+    if (time) *time = s->time;
+    if (frame) *frame = s->frame;
+    return FS_OK;
+}
+
+void* fs_sparks_device_ptr(fs_sparks* s) { return s ? (void*)s->sp : nullptr; }
+
+}  // extern "C"

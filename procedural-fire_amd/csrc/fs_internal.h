@@ -198,6 +198,20 @@ void launch_forces_tan(const Dims& g, hipStream_t s, const DevParams* P, const F
 void launch_div_tan(const Dims& g, hipStream_t s, const DevParams* P, const float4* tv, float* tb);
 void launch_project_tan(const Dims& g, hipStream_t s, const DevParams* P, const float4* vIn, const double* p,
                         const float4* geo, const float4* tvIn, const double* tp, float4* tvOut);
+// ---- sparks (fs_sparks.hip) --------------------------------------------------------
+// The update kernel's uniform block: SPARK_FIELDS of src/sparks.js:10-14, all f32 as the
+// reference uploads them.
+struct SparkUniform {
+    float dt, time, frame, count, nBase, nBurst, burstX, burstY, burstZ, burstSpeed, burstRadius;
+    float ox, oy, oz, sx, sy, sz, turb, lifeK, bedRadius, seed, sizeK, upK, burstFan, burstSize, aimX, aimZ;
+};
+int  spark_blocks(int count);
+// dead particles per block (after a host write or reset)
+void launch_spark_count(hipStream_t s, const float4* sp, int count, int* dead);
+// scan `dead` into `off`, update (sp: count x 4 float4) and write the blocks' dead counts
+// after the update back into `dead` (both arrays spark_blocks(count) ints)
+void launch_spark_step(int prec, hipStream_t s, const SparkUniform& U, float4* sp, int* dead, int* off, const void* vel,
+                       const Dims& g, const half4_t* curl, int nn);
 // ---- diagnostics (fs_diag.hip) -------------------------------------------------
 int  stats_blocks(size_t n);
 void launch_stats(int prec, hipStream_t s, const void* field, size_t n, double* part);

@@ -69,6 +69,40 @@ __device__ __forceinline__ float fs_pow(float a, float b) {
     return fs_exp(b * fs_log(a));
 }
 
+// sin / cos (WGSL leaves them to the implementation too): Cody-Waite reduction by
+// k = rint(x 2/pi) with pi/2 split into 12 + 12 + 24 significant bits (k*P1, k*P2 exact
+// for |k| < 2048), Taylor polynomials of degree 9 (sin) and 10 (cos) on |r| <= pi/4,
+// quadrant from k mod 4.  Identical to the oracle's.
+__device__ __forceinline__ void fs_sincos_red(float x, float& sr, float& cr, int& q) {
+    const float k = rintf(x * 0.636619747f);
+    const float r = ((x - k * 1.5703125f) - k * 4.83751297e-4f) - k * 7.54978995e-8f;
+    const float r2 = r * r;
+    float ps = 1.0f / 362880.0f;
+    ps = -1.0f / 5040.0f + r2 * ps;
+    ps = 1.0f / 120.0f + r2 * ps;
+    ps = -1.0f / 6.0f + r2 * ps;
+    sr = r + r * (r2 * ps);
+    float pc = -1.0f / 3628800.0f;
+    pc = 1.0f / 40320.0f + r2 * pc;
+    pc = -1.0f / 720.0f + r2 * pc;
+    pc = 1.0f / 24.0f + r2 * pc;
+    pc = -0.5f + r2 * pc;
+    cr = 1.0f + r2 * pc;
+    q = ((int)k) & 3;
+}
+__device__ __forceinline__ float fs_sin(float x) {
+    float s, c;
+    int q;
+    fs_sincos_red(x, s, c, q);
+    return q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
+}
+__device__ __forceinline__ float fs_cos(float x) {
+    float s, c;
+    int q;
+    fs_sincos_red(x, s, c, q);
+    return q == 0 ? c : q == 1 ? -s : q == 2 ? -c : s;
+}
+
 // ---- tiling -----------------------------------------------------------------
 __device__ __forceinline__ int xcd_remap(int hw, int total) {
     const int xcd = hw & 7, q = total >> 3, r = total & 7;

@@ -61,6 +61,14 @@ class FsSpot(ctypes.Structure):
                 ("temp", ctypes.c_float), ("norm", ctypes.c_float), ("pad", ctypes.c_float)]
 
 
+class FsSparkBurst(ctypes.Structure):
+    """Sparks.burst({...}) options (src/sparks.js:44-46)."""
+    _fields_ = [("n", ctypes.c_int), ("pos", ctypes.c_float * 3), ("speed", ctypes.c_float),
+                ("radius", ctypes.c_float), ("dur", ctypes.c_double), ("fan", ctypes.c_float),
+                ("size", ctypes.c_float), ("has_aim", ctypes.c_int), ("aim", ctypes.c_float * 3)]
+
+
+SPARK_FLOATS = 16
 MAX_SPOTS = 64
 RHO0 = 1.204
 
@@ -135,6 +143,17 @@ def lib():
         "fs_diag_read": (ip, [vp, dp, ip, ctypes.POINTER(ctypes.c_int)]),
         "fs_poisson_info": (ip, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "fs_set_clock": (ip, [vp, ctypes.c_double, ctypes.c_int64]),
+        "fs_spark_burst_default": (None, [ctypes.POINTER(FsSparkBurst)]),
+        "fs_sparks_create": (ip, [vp, ip, ctypes.POINTER(vp)]),
+        "fs_sparks_destroy": (ip, [vp]),
+        "fs_sparks_reset": (ip, [vp]),
+        "fs_sparks_set": (ip, [vp, ctypes.c_char_p, ctypes.c_float]),
+        "fs_sparks_burst": (ip, [vp, ctypes.POINTER(FsSparkBurst)]),
+        "fs_sparks_step": (ip, [vp, ctypes.c_double]),
+        "fs_sparks_read": (ip, [vp, vp, sz]),
+        "fs_sparks_write": (ip, [vp, vp, sz]),
+        "fs_sparks_clock": (ip, [vp, dp, ctypes.POINTER(ctypes.c_int64)]),
+        "fs_sparks_device_ptr": (vp, [vp]),
         "fs_adjoint_create": (ip, [vp, ip, ip, ctypes.POINTER(vp)]),
         "fs_adjoint_destroy": (ip, [vp]),
         "fs_adjoint_forward": (ip, [vp, ctypes.c_double, ip]),
@@ -531,6 +550,66 @@ class Adjoint:
         _check(lib().fs_adjoint_grads(self._h, g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(n_spots),
                                       gf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if face else None))
         return g[:n_spots], gf
+
+
+class Sparks:
+    """Spark particles carried by a FireSim's velocity (the reference's Sparks, src/sparks.js:17-271):
+    step(dt) right after the sim's step with the same dt; read() gives (count, 16) float32 rows
+    (position, age | velocity, burn time | previous position, T | diameter mm, seed, alive, flicker)."""
+
+    def __init__(self, sim, count=8192):
+        self.sim = sim
+        self.count = int(count)
+        self._h = ctypes.c_void_p()
+        _check(lib().fs_sparks_create(sim._h, self.count, ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().fs_sparks_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        _check(lib().fs_sparks_reset(self._h))
+
+    def set(self, **kw):
+        """rate, turb, lifeK, sizeK, upK, brightness"""
+        for k, v in kw.items():
+            _check(lib().fs_sparks_set(self._h, k.encode(), float(v)))
+
+    def burst(self, n=60, pos=(0.0, 0.1, 0.0), speed=3.0, radius=0.05, dur=0.12, fan=0.6, size=1.3, aim=None):
+        b = FsSparkBurst()
+        lib().fs_spark_burst_default(ctypes.byref(b))
+        b.n = int(n)
+        b.pos[:] = [float(v) for v in pos]
+        b.speed, b.radius, b.dur, b.fan, b.size = float(speed), float(radius), float(dur), float(fan), float(size)
+        if aim is not None:
+            b.has_aim = 1
+            b.aim[:] = [float(v) for v in aim]
+        _check(lib().fs_sparks_burst(self._h, ctypes.byref(b)))
+
+    def step(self, dt):
+        _check(lib().fs_sparks_step(self._h, float(dt)))
+
+    def read(self):
+        out = np.empty((self.count, SPARK_FLOATS), np.float32)
+        _check(lib().fs_sparks_read(self._h, out.ctypes.data, out.nbytes))
+        return out
+
+    def write(self, arr):
+        a = np.ascontiguousarray(arr, np.float32)
+        _check(lib().fs_sparks_write(self._h, a.ctypes.data, a.nbytes))
+
+    @property
+    def time(self):
+        t = ctypes.c_double()
+        _check(lib().fs_sparks_clock(self._h, ctypes.byref(t), None))
+        return t.value
 
 
 class Poisson:

@@ -155,4 +155,26 @@ class FireSim {
   }
 }
 
-module.exports = { FireSim, SIM_FIELDS, SIM_DEFAULTS, PRECISION, SOLVE, FIELD, STAGE_LABELS, buildCampfireLogs, addon };
+// Spark particles carried by a FireSim's velocity: the reference's Sparks (src/sparks.js:17-271) minus
+// drawing.  step(dt) right after the sim's step with the same dt; read() returns count x 16 floats
+// (position+age, velocity+burn time, previous position+T, diameter/seed/alive/flicker).
+class Sparks {
+  constructor(sim, { count = 8192 } = {}) {
+    this.sim = sim;              // kept alive while the sparks are
+    this.count = count;
+    this.h = addon.sparksCreate(sim.h, count);
+    this._params = { rate: 3.0, turb: 1.0, lifeK: 1.0, sizeK: 1.0, upK: 1.0, brightness: 13.0 };
+  }
+  get rate() { return this._params.rate; }
+  set rate(v) { this.setParams({ rate: v }); }
+  get params() { return { ...this._params }; }
+  setParams(p) {
+    for (const [k, v] of Object.entries(p)) { addon.sparksSet(this.h, k, v); this._params[k] = v; }
+  }
+  reset() { addon.sparksReset(this.h); }                                  // src/sparks.js:38-41
+  burst(opts = {}) { addon.sparksBurst(this.h, opts); }                    // src/sparks.js:44-46
+  step(dt) { addon.sparksStep(this.h, dt); }                               // src/sparks.js:241-271
+  read() { return addon.sparksRead(this.h, this.count); }
+}
+
+module.exports = { FireSim, Sparks, SIM_FIELDS, SIM_DEFAULTS, PRECISION, SOLVE, FIELD, STAGE_LABELS, buildCampfireLogs, addon };

@@ -640,6 +640,120 @@ static napi_value js_build_campfire(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+/* ---- sparks (src/sparks.js) ---------------------------------------------------- */
+static void finalize_sparks(napi_env env, void* data, void* hint) {
+    (void)env; (void)hint;
+    fs_sparks_destroy((fs_sparks*)data);
+}
+static fs_sparks* get_sparks(napi_env env, napi_value v) {
+    void* p = NULL;
+    if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+        napi_throw_type_error(env, NULL, "expected a Sparks handle");
+        return NULL;
+    }
+    return (fs_sparks*)p;
+}
+/* sparksCreate(h, count) */
+static napi_value js_sparks_create(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sim* h = get_sim(env, argv[0]);
+    if (!h) return NULL;
+    double count = 8192;
+    if (argc >= 2) napi_get_value_double(env, argv[1], &count);
+    fs_sparks* sp = NULL;
+    int rc = fs_sparks_create(h, (int)count, &sp);
+    if (rc) return throw_fs(env, rc);
+    napi_value ext;
+    CHECK_NAPI(env, napi_create_external(env, sp, finalize_sparks, NULL, &ext));
+    return ext;
+}
+static napi_value js_sparks_reset(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sparks* sp = get_sparks(env, argv[0]);
+    if (!sp) return NULL;
+    int rc = fs_sparks_reset(sp);
+    if (rc) return throw_fs(env, rc);
+    return NULL;
+}
+/* sparksSet(s, name, value) */
+static napi_value js_sparks_set(napi_env env, napi_callback_info info) {
+    size_t argc = 3;
+    napi_value argv[3];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sparks* sp = get_sparks(env, argv[0]);
+    if (!sp) return NULL;
+    char name[64];
+    size_t len = 0;
+    if (napi_get_value_string_utf8(env, argv[1], name, sizeof name, &len) != napi_ok) {
+        napi_throw_type_error(env, NULL, "expected a parameter name");
+        return NULL;
+    }
+    double v = 0;
+    napi_get_value_double(env, argv[2], &v);
+    int rc = fs_sparks_set(sp, name, (float)v);
+    if (rc) return throw_fs(env, rc);
+    return NULL;
+}
+/* sparksBurst(s, {n, pos, speed, radius, dur, fan, size, aim}) (src/sparks.js:44-46) */
+static napi_value js_sparks_burst(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sparks* sp = get_sparks(env, argv[0]);
+    if (!sp) return NULL;
+    fs_spark_burst b;
+    fs_spark_burst_default(&b);
+    if (argc >= 2) {
+        double v, a3[3];
+        if (get_prop_num(env, argv[1], "n", &v)) b.n = (int)v;
+        if (get_prop_array3(env, argv[1], "pos", a3)) for (int k = 0; k < 3; k++) b.pos[k] = (float)a3[k];
+        if (get_prop_num(env, argv[1], "speed", &v)) b.speed = (float)v;
+        if (get_prop_num(env, argv[1], "radius", &v)) b.radius = (float)v;
+        if (get_prop_num(env, argv[1], "dur", &v)) b.dur = v;
+        if (get_prop_num(env, argv[1], "fan", &v)) b.fan = (float)v;
+        if (get_prop_num(env, argv[1], "size", &v)) b.size = (float)v;
+        if (get_prop_array3(env, argv[1], "aim", a3)) { b.has_aim = 1; for (int k = 0; k < 3; k++) b.aim[k] = (float)a3[k]; }
+    }
+    int rc = fs_sparks_burst(sp, &b);
+    if (rc) return throw_fs(env, rc);
+    return NULL;
+}
+/* sparksStep(s, dt) */
+static napi_value js_sparks_step(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sparks* sp = get_sparks(env, argv[0]);
+    if (!sp) return NULL;
+    double dt = 0;
+    napi_get_value_double(env, argv[1], &dt);
+    int rc = fs_sparks_step(sp, dt);
+    if (rc) return throw_fs(env, rc);
+    return NULL;
+}
+/* sparksRead(s, count) -> Float32Array(count * 16) */
+static napi_value js_sparks_read(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sparks* sp = get_sparks(env, argv[0]);
+    if (!sp) return NULL;
+    double count = 0;
+    napi_get_value_double(env, argv[1], &count);
+    const size_t n = (size_t)count * FS_SPARK_FLOATS;
+    void* data = NULL;
+    napi_value ab, arr;
+    CHECK_NAPI(env, napi_create_arraybuffer(env, n * 4, &data, &ab));
+    int rc = fs_sparks_read(sp, (float*)data, n * 4);
+    if (rc) return throw_fs(env, rc);
+    CHECK_NAPI(env, napi_create_typedarray(env, napi_float32_array, n, ab, 0, &arr));
+    return arr;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
     (void)info;
     napi_value s;
@@ -680,6 +794,12 @@ static napi_value init(napi_env env, napi_value exports) {
     EXPORT("timings", js_timings);
     EXPORT("buildCampfire", js_build_campfire);
     EXPORT("version", js_version);
+    EXPORT("sparksCreate", js_sparks_create);
+    EXPORT("sparksReset", js_sparks_reset);
+    EXPORT("sparksSet", js_sparks_set);
+    EXPORT("sparksBurst", js_sparks_burst);
+    EXPORT("sparksStep", js_sparks_step);
+    EXPORT("sparksRead", js_sparks_read);
     return exports;
 }
 

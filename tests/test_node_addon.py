@@ -93,3 +93,27 @@ def test_node_spot_scene_matches_the_c_abi_bit_for_bit(tmp_path):
     for f, fid in (("vel", F.VEL), ("S", F.S), ("S2", F.S2)):
         a = np.fromfile(tmp_path / f"{f}.bin", np.uint16)
         assert np.array_equal(a, sim.read(fid, "native").ravel()), f
+
+
+@pytest.mark.gpu
+@needs_node
+def test_node_sparks_match_the_c_abi_bit_for_bit(tmp_path):
+    """The Node Sparks class (rate, params, bursts, step) gives the C ABI's particle array."""
+    print("node sparks 32x48x32 x20", flush=True)
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "run_sparks.js"), str(tmp_path), "20"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    meta = json.loads((tmp_path / "meta.json").read_text())
+    assert meta["threwOnUnknown"] and meta["alive"] > 50
+    sim = F.FireSim(res=(32, 48, 32), size=(1.2, 1.8, 1.2), noise_n=32, use_graph=True)
+    sp = F.Sparks(sim, 1024)
+    sp.set(rate=300.0, turb=1.2)
+    for k in range(20):
+        if k == 0:
+            sp.burst(n=140, pos=(0, 0.15, 0), speed=3.2, radius=0.12, dur=0.5)
+        if k == 7:
+            sp.burst(n=40, pos=(0.05, 0.1, 0), speed=5, radius=0.03, dur=0.05, fan=0.9, size=1.4, aim=(0.3, 0, -0.2))
+        sim.step(1 / 90)
+        sp.step(1 / 90)
+    a = np.fromfile(tmp_path / "sparks.bin", np.float32).reshape(1024, 16)
+    assert np.array_equal(a.view(np.uint32), sp.read().view(np.uint32))
