@@ -523,6 +523,45 @@ def sparks_bench(args):
     print(json.dumps(out))
 
 
+def capture_bench(args):
+    """The timeline-driven capture loop (per-step parameter overrides, log shift, event bursts,
+    sparks; firesim.driver) against plain stepping, on the reference app's default grid
+    (80x120x80 in 1.2x1.8x1.2 m, fp16 fields, 2 V-cycles, campfire seed 7) at the capture's
+    dt = 1/240 s: 1 s of pre-roll, then frames 0-59 (0-2 s of the cinematic, the hero pop at 0.3 s)."""
+    from firesim import driver as D
+    logs, _ = F.build_campfire(7)
+    out = {"workload": "capture loop, 80x120x80 campfire (fp16, 2 V-cycles), dt 1/240 s, 8 steps per 30-fps frame"}
+    for label, with_sparks in (("driver+sparks", True), ("driver", False)):
+        sim = F.FireSim(res=(80, 120, 80), size=(1.2, 1.8, 1.2), logs=logs, vcycles=2, precision=F.PREC_FP16,
+                        noise_n=128, use_graph=True)
+        sparks = F.Sparks(sim, 8192) if with_sparks else None
+        cap = D.Capture(sim, sparks, logs, sim_dt=1 / 240)
+        cap.begin(preroll=1.0)
+        sim.sync()
+        t0 = time.perf_counter()
+        for i in range(60):
+            cap.frame(i)
+        sim.sync()
+        t = (time.perf_counter() - t0) / 60
+        out[label] = {"ms_per_frame": t * 1e3, "ms_per_step": t * 1e3 / cap.n, "log_uploads": cap.log_uploads}
+        if sparks is not None:
+            out[label]["sparks_alive"] = int((sparks.read()[:, 14] > 0.5).sum())
+            sparks.close()
+        sim.close()
+        print(f"{label}: {t * 1e3:.3f} ms per frame", flush=True)
+    sim = F.FireSim(res=(80, 120, 80), size=(1.2, 1.8, 1.2), logs=logs, vcycles=2, precision=F.PREC_FP16,
+                    noise_n=128, use_graph=True)
+    sim.step(1 / 240, 240)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(1 / 240, 480)
+    sim.sync()
+    t = (time.perf_counter() - t0) / 60
+    out["plain"] = {"ms_per_frame": t * 1e3, "ms_per_step": t * 1e3 / 8}
+    out["driver_overhead_ms_per_step"] = out["driver"]["ms_per_step"] - out["plain"]["ms_per_step"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -548,6 +587,8 @@ def main():
                          "temperature/soot mismatch w.r.t. spot and floor fluxes, checked by FD32 and FD64)")
     ap.add_argument("--sparks", action="store_true",
                     help="instead of the step: spark-particle update throughput (8192 and 4M sparks) on the step's velocity")
+    ap.add_argument("--capture", action="store_true",
+                    help="instead of the step: the timeline-driven capture loop (firesim.driver) vs plain stepping")
     ap.add_argument("--timing-only", action="store_true",
                     help="with --adjoint: forward, backward and tangent timings and the dot-product test only")
     ap.add_argument("--probe", action="store_true",
@@ -575,6 +616,9 @@ def main():
         return
     if args.sparks:
         sparks_bench(args)
+        return
+    if args.capture:
+        capture_bench(args)
         return
     ncell = float(np.prod(RES))
     graph = not args.no_graph
