@@ -361,6 +361,50 @@ int  fs_sparks_clock(fs_sparks* sp, double* time, int64_t* frame);
 /* Device address of the particle array (count * 64 B), for renderers on the same device. */
 void* fs_sparks_device_ptr(fs_sparks* sp);
 
+/* ---- render-volume consumers of the step's fields -------------------------------
+ * Replaces the Renderer passes that read sim.S / sim.S2 (bind groups at
+ * src/renderer.js:603-610, dispatch at 668-690): the fire-light extraction
+ * (src/renderer.js:104-164), the flame-field prefilter (173-191) and the volume
+ * ray-march (194-326), with the uniform block of uploadParams (634-648) and the
+ * blackbody LUT of setWhiteBalance (57-63, src/blackbody.js:78-88).  The SDF surface
+ * pass, composite, bloom and tone map stay with the caller's renderer: the surface
+ * depth comes in through fs_render_set_depth (without one, every pixel sees the
+ * surface pass's sky depth, 1e6 m), and the volume (L, Tr) and aux (haze, depth)
+ * images go out through fs_render_read / fs_render_device_ptr.
+ * A frame runs on the sim's stream and reads the sim's current S and S2. */
+#define FS_RP_COUNT 88                   /* RP_FIELDS entries (src/renderer.js:8-17) */
+#define FS_RENDER_LIGHTS 8               /* extracted point lights (src/renderer.js:27) */
+enum { FS_RENDER_VOL = 0,                /* vw*vh x (L.rgb, Tr), binary16 on the device, read as float */
+       FS_RENDER_AUX = 1,                /* vw*vh x (haze, surface depth), float */
+       FS_RENDER_LIGHTS_BUF = 2,         /* 8 x (pos.xyz, luminance, colour.rgb, spread), float */
+       FS_RENDER_SHARP = 3,              /* the prefiltered S (binary16 on the device, read as float) */
+       FS_RENDER_LUT = 4 };              /* 2 x bbN x 4 floats: blackbody row, soot row */
+typedef struct fs_render fs_render;
+int  fs_render_param_count(void);
+int  fs_render_param_index(const char* name);          /* -1 if unknown */
+int  fs_render_param_defaults(float* out);             /* RP_DEFAULTS, FS_RP_COUNT floats */
+/* Renderer(...) + resize(width, height, vol_scale): vw = max(8, round(width * vol_scale)). */
+int  fs_render_create(fs_sim* h, int width, int height, double vol_scale, fs_render** out);
+int  fs_render_destroy(fs_render* r);
+/* this.params[name] = value, for the user-settable RP_FIELDS entries (exposure, stepMul, ...). */
+int  fs_render_set(fs_render* r, const char* name, float value);
+/* The uniform block the last fs_render_frame uploaded (FS_RP_COUNT floats). */
+int  fs_render_get_uniforms(fs_render* r, float* out);
+/* setCamera(lookAt(...)): position, unit forward / right / up, vertical field of view (rad). */
+int  fs_render_set_camera(fs_render* r, const double pos[3], const double fwd[3], const double right[3],
+                          const double up[3], double fov_y);
+int  fs_render_set_white_balance(fs_render* r, double twb);   /* blackbodyLUT(1024, 400, 3000, twb) */
+/* Surface depth (m along the ray) of the surface pass, dw x dh floats; NULL = sky everywhere. */
+int  fs_render_set_depth(fs_render* r, const float* depth, int dw, int dh);
+/* uploadParams(time, frame, jitter) then the lights, prefilter (flameSharp > 0 and cubic,
+ * once per sim state) and volume passes of renderHDR. */
+int  fs_render_frame(fs_render* r, double time, double frame, float jit_x, float jit_y);
+int  fs_render_size(fs_render* r, int* vw, int* vh);
+int  fs_render_read(fs_render* r, int what, float* host, size_t bytes);
+void* fs_render_device_ptr(fs_render* r, int what);
+/* blackbodyLUT({n, Tmin, Tmax, Twb}) (src/blackbody.js:78-88): 2 x n x 4 floats. */
+int  fs_blackbody_lut(int n, double tmin, double tmax, double twb, float* out);
+
 #ifdef __cplusplus
 }
 #endif

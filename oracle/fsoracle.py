@@ -456,3 +456,106 @@ def mjs_study(N, face=True, mode=2, omega=1.0, pre=2, post=2):
     hist = np.zeros(8, np.float64)
     r0 = lib().orc_mjs_study(N, 1 if face else 0, mode, omega, pre, post, _dp(hist))
     return r0, hist[: (6 if mode in (0, 1) else 8)]
+
+
+# ---- render-volume consumers (oracle/render_oracle.cpp) --------------------------------
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def render_lights(U, S, lut, lstate):
+    """One light-extraction dispatch (src/renderer.js:112-164).  U: the 88-float uniform block;
+    S: (nz, ny, nx, 4) decoded floats; lut: (2, n, 4); lstate: (3, 8, 4) filter state, updated
+    in place.  Returns the (8, 8) lights."""
+    L = lib()
+    L.orc_render_lights.argtypes = [ctypes.POINTER(ctypes.c_float)] * 5
+    U, S, lut = _f32(U), _f32(S), _f32(lut)
+    assert lstate.dtype == np.float32 and lstate.flags.c_contiguous and lstate.size == 96
+    out = np.zeros((8, 8), np.float32)
+    L.orc_render_lights(_fptr(U), _fptr(S), _fptr(lut), _fptr(lstate), _fptr(out))
+    return out
+
+
+def render_sharpen(k, S):
+    """The flame-field prefilter (src/renderer.js:176-191) of S (nz, ny, nx, 4), rounded to binary16."""
+    L = lib()
+    L.orc_render_sharpen.argtypes = [ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    S = _f32(S)
+    nz, ny, nx, _ = S.shape
+    out = np.zeros_like(S)
+    L.orc_render_sharpen(float(k), nx, ny, nz, _fptr(S), _fptr(out))
+    return out
+
+
+def render_volume(U, S, S2, lut, lights, curl, depth=None):
+    """The volume march (src/renderer.js:235-326) over every pixel: returns vol (vh, vw, 4) (L, Tr)
+    rounded to binary16 and aux (vh, vw, 2) (haze, depth).  S is the field marched (the sim's S
+    or the prefiltered copy), curl the (n, n, n, 4) curl texture; depth None = sky (1e6)."""
+    L = lib()
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.orc_render_volume.argtypes = [fp, fp, fp, fp, fp, fp, ctypes.c_int, ctypes.c_int, fp, ctypes.c_int, fp, fp]
+    U, S, S2, lut, lights, curl = _f32(U), _f32(S), _f32(S2), _f32(lut), _f32(lights), _f32(curl)
+    depth = np.full((1, 1), 1e6, np.float32) if depth is None else _f32(depth)
+    vw, vh = int(U[2]), int(U[3])
+    vol = np.zeros((vh, vw, 4), np.float32)
+    aux = np.zeros((vh, vw, 2), np.float32)
+    L.orc_render_volume(_fptr(U), _fptr(S), _fptr(S2), _fptr(lut), _fptr(lights), _fptr(depth), depth.shape[1],
+                        depth.shape[0], _fptr(curl), curl.shape[0], _fptr(vol), _fptr(aux))
+    return vol, aux
+
+
+# ---- blackbody LUT (src/blackbody.js:4-88), restated with numpy in float64 ----------------
+_H, _C, _KB = 6.62607015e-34, 2.99792458e8, 1.380649e-23
+_XYZ2RGB = np.array([[3.2404542, -1.5371385, -0.4985314], [-0.9692660, 1.8760108, 0.0415560],
+                     [0.0556434, -0.2040259, 1.0572252]])
+_BRADFORD = np.array([[0.8951, 0.2664, -0.1614], [-0.7502, 1.7135, 0.0367], [0.0389, -0.0685, 1.0296]])
+_D65 = np.array([0.95047, 1.0, 1.08883])
+
+
+def cie_fit(lnm):
+    """The multi-lobe piecewise-Gaussian fit of the CIE 1931 2-degree observer (Wyman, Sloan &
+    Shirley 2013), as src/blackbody.js:20-25 writes it."""
+    l = np.asarray(lnm, np.float64)
+
+    def g(mu, s1, s2):
+        t = (l - mu) / np.where(l < mu, s1, s2)
+        return np.exp(-0.5 * t * t)
+    x = 1.056 * g(599.8, 37.9, 31.0) + 0.362 * g(442.0, 16.0, 26.7) - 0.065 * g(501.1, 20.4, 26.2)
+    y = 0.821 * g(568.8, 46.9, 40.5) + 0.286 * g(530.9, 16.3, 31.1)
+    z = 1.217 * g(437.0, 11.8, 36.0) + 0.681 * g(459.0, 26.0, 13.8)
+    return np.stack([x, y, z], -1)
+
+
+def planck(lnm, T):
+    """Planck's law, spectral radiance per metre of wavelength."""
+    l = np.asarray(lnm, np.float64) * 1e-9
+    return (2 * _H * _C * _C) / (l ** 5 * np.expm1((_H * _C) / (l * _KB * T)))
+
+
+def blackbody_xyz(T, soot=False):
+    l = np.arange(380, 781, 2, dtype=np.float64)
+    p = planck(l, T) * ((l / 550.0) ** -1.39 if soot else 1.0)
+    return (p[:, None] * cie_fit(l)).sum(0)
+
+
+def blackbody_lut(n=1024, tmin=400.0, tmax=3000.0, twb=5000.0):
+    ref = blackbody_xyz(1500.0)[1]
+    if twb:
+        w = blackbody_xyz(twb)
+        src, dst = _BRADFORD @ (w / w[1]), _BRADFORD @ _D65
+        M = _XYZ2RGB @ np.linalg.inv(_BRADFORD) @ np.diag(dst / src) @ _BRADFORD
+    else:
+        M = _XYZ2RGB
+    out = np.zeros((2, n, 4), np.float32)
+    for row in range(2):
+        for i in range(n):
+            T = tmin + (tmax - tmin) * i / (n - 1)
+            xyz = blackbody_xyz(T, row == 1) / ref
+            out[row, i, :3] = np.maximum(M @ xyz, 0.0)
+            out[row, i, 3] = xyz[1]
+    return out

@@ -104,6 +104,7 @@ struct fs_sim {
     int* ring_count_d = nullptr;
     int diag_cap = 0;
     double diag_tflame = 1000.0;
+    uint64_t gen = 0;                // bumped by every call that may change the fields
 
     FieldSet fields() const {
         FieldSet F;
@@ -585,6 +586,7 @@ int fs_reset(fs_sim* h) {   // sim.js:606-618
     h->time = 0.0;
     h->frame = 0;
     h->params_dirty = true;
+    h->gen++;
     if (hipGetLastError() != hipSuccess) return fail(FS_EDEVICE, "launch failed in reset");
     return FS_OK;
 }
@@ -593,6 +595,7 @@ int fs_step(fs_sim* h, double dt, int nsteps) {   // sim.js:627-648
     if (!h) return fail(FS_EINVAL, "null handle");
     if (nsteps < 0 || !(dt > 0.0) || !std::isfinite(dt)) return fail(FS_EINVAL, "dt must be positive and finite, nsteps >= 0");
     if (set_device(h)) return FS_EDEVICE;
+    h->gen++;
     for (int k = 0; k < nsteps; k++) {
         sync_params(h, dt);
         if (!h->geo_ready) {   // sim.js:632
@@ -620,6 +623,7 @@ int fs_run_stage(fs_sim* h, int stage, double dt) {
     if (!h) return fail(FS_EINVAL, "null handle");
     if (stage < 0 || stage >= FS_NUM_STAGES) return fail(FS_EINVAL, "bad stage");
     if (set_device(h)) return FS_EDEVICE;
+    h->gen++;
     sync_params(h, dt);
     const FieldSet F = h->fields();
     hipStream_t s = h->stream;
@@ -722,6 +726,7 @@ int fs_write_field(fs_sim* h, int field, int dtype, const void* src, size_t byte
     if (dtype != FS_DT_F32 && dtype != FS_DT_F64 && dtype != FS_DT_NATIVE) return fail(FS_EINVAL, "bad dtype");
     if (bytes != want) return fail(FS_EINVAL, "source size mismatch: expected " + std::to_string(want) + " bytes");
     FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->gen++;
     if (is_mg_field(field)) {   // reorder into the device's parity-split layout (padding stays zero)
         const size_t nd = mg_level_floats(h->mg, 0);
         const bool src64 = dtype == FS_DT_F64 || (dtype == FS_DT_NATIVE && d.kind == 3);
@@ -1250,6 +1255,7 @@ void slot_copy(fs_sim* h, const AdjSlot& s, bool to_slot) {
     void* sim[4] = {h->f[F_VEL], h->f[F_S], h->f[F_S2], h->mg.x64};
     void* sl[4] = {s.vel, s.S, s.S2, s.p};
     const size_t nb[4] = {fb, fb, fb, pb};
+    if (!to_slot) h->gen++;
     for (int i = 0; i < 4; i++)
         (void)hipMemcpyAsync(to_slot ? sl[i] : sim[i], to_slot ? sim[i] : sl[i], nb[i], hipMemcpyDeviceToDevice, h->stream);
 }
@@ -1785,5 +1791,416 @@ int fs_sparks_clock(fs_sparks* s, double* time, int64_t* frame) {
 }
 
 void* fs_sparks_device_ptr(fs_sparks* s) { return s ? (void*)s->sp : nullptr; }
+
+}  // extern "C"
+
+// ---- render-volume consumers (src/renderer.js) ---------------------------------------
+namespace {
+// RP_FIELDS (src/renderer.js:8-17)
+const char* const kRenderNames[FS_RP_COUNT] = {
+    "width", "height", "vw", "vh", "time", "frame", "nLogs", "nStones",
+    "camX", "camY", "camZ", "tanFov", "fwdX", "fwdY", "fwdZ", "aspect", "rgtX", "rgtY", "rgtZ", "jitX", "upX", "upY", "upZ", "jitY",
+    "ox", "oy", "oz", "dx", "sx", "sy", "sz", "tempScale", "nx", "ny", "nz", "nLights",
+    "exposure", "stepMul", "sootAbs", "sootEmit", "emitK", "blueK", "smokeScat", "smokeG",
+    "ambient", "detailAmt", "detailFreq", "detailSpeed", "hazeK", "glowK", "bloomK", "tonemap",
+    "grain", "vignette", "saturation", "contrast", "bedGlow", "logGlow", "lightK", "accumN",
+    "bbTmin", "bbTmax", "bbN", "emberFlicker", "sootEmitMin", "flameSharp", "moon", "fogK",
+    "shadowSoft", "lift", "emitTempLo", "glowTemp", "inset", "knee", "surfDetail", "matA", "matB", "matC", "sootScat",
+    "chroma", "lensX", "lensY", "focus", "lightDt", "flickerK", "cubic", "aperture", "sootEdge",
+};
+// RP_DEFAULTS (src/renderer.js:19-25); entries without a default start at 0
+void render_defaults(float* p) {
+    std::fill(p, p + FS_RP_COUNT, 0.f);
+    p[RP_exposure] = 15.0f; p[RP_stepMul] = 0.7f; p[RP_sootAbs] = 4.0f; p[RP_sootEmit] = 10.0f; p[RP_emitK] = 4.0f;
+    p[RP_blueK] = 0.05f; p[RP_smokeScat] = 0.02f; p[RP_smokeG] = 0.5f;
+    p[RP_ambient] = 1.0f; p[RP_detailAmt] = 0.6f; p[RP_detailFreq] = 2.2f; p[RP_detailSpeed] = 0.9f; p[RP_hazeK] = 1.0f;
+    p[RP_glowK] = 1.0f; p[RP_bloomK] = 0.04f; p[RP_tonemap] = 2.f;
+    p[RP_grain] = 0.02f; p[RP_vignette] = 0.6f; p[RP_saturation] = 1.0f; p[RP_contrast] = 1.0f; p[RP_bedGlow] = 1.0f;
+    p[RP_logGlow] = 1.0f; p[RP_lightK] = 1.0f;
+    p[RP_emberFlicker] = 1.0f; p[RP_sootEmitMin] = 0.0f; p[RP_flameSharp] = 0.0f; p[RP_moon] = 1.0f; p[RP_fogK] = 1.0f;
+    p[RP_shadowSoft] = 6.0f; p[RP_lift] = 0.0f;
+    p[RP_emitTempLo] = 900.f; p[RP_glowTemp] = 1.0f; p[RP_tempScale] = 1000.f; p[RP_inset] = 0.06f; p[RP_knee] = 0.6f;
+    p[RP_surfDetail] = 1.f; p[RP_sootScat] = 0.015f; p[RP_chroma] = 0.0025f; p[RP_lightDt] = -1.f; p[RP_flickerK] = 0.35f;
+}
+// entries uploadParams owns (src/renderer.js:640-647); setting them through this.params has no effect
+bool render_owned(int i) {
+    switch (i) {
+        case RP_width: case RP_height: case RP_vw: case RP_vh: case RP_time: case RP_frame: case RP_nLogs: case RP_nStones:
+        case RP_nLights: case RP_lightDt: case RP_bbTmin: case RP_bbTmax: case RP_bbN: case RP_tanFov: case RP_aspect:
+        case RP_jitX: case RP_jitY: case RP_ox: case RP_oy: case RP_oz: case RP_dx: case RP_sx: case RP_sy: case RP_sz:
+        case RP_nx: case RP_ny: case RP_nz:
+            return true;
+    }
+    return (i >= RP_camX && i <= RP_camZ) || (i >= RP_fwdX && i <= RP_fwdZ) || (i >= RP_rgtX && i <= RP_rgtZ) ||
+           (i >= RP_upX && i <= RP_upZ);
+}
+
+// ---- blackbody LUT (src/blackbody.js): Planck's law against the multi-lobe Gaussian fit of the
+// CIE 1931 2-degree observer (Wyman, Sloan & Shirley 2013), XYZ -> linear sRGB, Bradford white
+// balance from a blackbody at Twb to D65.  All in double, like the reference's JS numbers.
+namespace bb {
+const double kH = 6.62607015e-34, kC = 2.99792458e8, kB = 1.380649e-23;
+double lobe(double x, double mu, double s1, double s2) {
+    const double t = (x - mu) / (x < mu ? s1 : s2);
+    return std::exp(-0.5 * t * t);
+}
+void cie(double l, double o[3]) {
+    o[0] = 1.056 * lobe(l, 599.8, 37.9, 31.0) + 0.362 * lobe(l, 442.0, 16.0, 26.7) - 0.065 * lobe(l, 501.1, 20.4, 26.2);
+    o[1] = 0.821 * lobe(l, 568.8, 46.9, 40.5) + 0.286 * lobe(l, 530.9, 16.3, 31.1);
+    o[2] = 1.217 * lobe(l, 437.0, 11.8, 36.0) + 0.681 * lobe(l, 459.0, 26.0, 13.8);
+}
+double planck(double lnm, double T) {   // spectral radiance, W sr^-1 m^-3
+    const double l = lnm * 1e-9;
+    return (2 * kH * kC * kC) / (std::pow(l, 5) * (std::exp((kH * kC) / (l * kB * T)) - 1));
+}
+// 380..780 nm in 2 nm steps; soot: emissivity ~ (lambda / 550 nm)^-1.39 (Hottel-Broughton)
+void xyz_of(double T, bool soot, double o[3]) {
+    double X = 0, Y = 0, Z = 0;
+    for (int l = 380; l <= 780; l += 2) {
+        const double p = planck(l, T) * (soot ? std::pow(l / 550.0, -1.39) : 1.0);
+        double c[3];
+        cie(l, c);
+        X += p * c[0]; Y += p * c[1]; Z += p * c[2];
+    }
+    o[0] = X; o[1] = Y; o[2] = Z;
+}
+const double kXYZ2RGB[3][3] = {{3.2404542, -1.5371385, -0.4985314}, {-0.9692660, 1.8760108, 0.0415560},
+                               {0.0556434, -0.2040259, 1.0572252}};
+const double kBradford[3][3] = {{0.8951, 0.2664, -0.1614}, {-0.7502, 1.7135, 0.0367}, {0.0389, -0.0685, 1.0296}};
+const double kD65[3] = {0.95047, 1.0, 1.08883};
+void mul(const double M[3][3], const double v[3], double o[3]) {
+    for (int r = 0; r < 3; r++) o[r] = M[r][0] * v[0] + M[r][1] * v[1] + M[r][2] * v[2];
+}
+void inverse(const double m[3][3], double o[3][3]) {   // adjugate / determinant
+    const double a = m[0][0], b = m[0][1], c = m[0][2], d = m[1][0], e = m[1][1], f = m[1][2];
+    const double g = m[2][0], h = m[2][1], i = m[2][2];
+    const double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    const double det = a * A + b * B + c * C;
+    o[0][0] = A / det; o[0][1] = -(b * i - c * h) / det; o[0][2] = (b * f - c * e) / det;
+    o[1][0] = B / det; o[1][1] = (a * i - c * g) / det; o[1][2] = -(a * f - c * d) / det;
+    o[2][0] = C / det; o[2][1] = -(a * h - b * g) / det; o[2][2] = (a * e - b * d) / det;
+}
+}  // namespace bb
+}  // namespace
+
+struct fs_render {
+    fs_sim* h = nullptr;
+    int device = 0;
+    int w = 0, hgt = 0, vw = 0, vh = 0;
+    float params[FS_RP_COUNT];         // this.params
+    RenderU U{};                       // the block of the last frame
+    bool have_U = false;
+    bool have_cam = false;
+    double cam_pos[3], cam_fwd[3], cam_rgt[3], cam_up[3], fov_y = 0.0;
+    int bbN = 1024;
+    double bbTmin = 400.0, bbTmax = 3000.0;
+    float4* lut = nullptr;             // 2 x bbN
+    float4* lstate = nullptr;          // 3 x 8 light filter state
+    float4* lights = nullptr;          // 8 x (pos, col)
+    float* depth = nullptr;            // dw x dh
+    int dw = 1, dh = 1;
+    size_t depth_cap = 1;
+    half4_t* sharp = nullptr;          // prefiltered S
+    half4_t* vol = nullptr;            // vw x vh
+    float2* aux = nullptr;
+    bool have_last_light = false;
+    double last_light_time = 0.0;
+    bool sharp_valid = false;          // prefilter stamp (src/renderer.js:679)
+    uint64_t sharp_gen = 0;
+    int64_t sharp_frame = 0;
+    double sharp_time = 0.0;
+    float sharp_k = 0.f;
+};
+
+namespace {
+void render_free(fs_render* r) {
+    void* p[] = {r->lut, r->lstate, r->lights, r->depth, r->sharp, r->vol, r->aux};
+    for (void* q : p) if (q) (void)hipFree(q);
+}
+size_t render_bytes(const fs_render* r, int what) {
+    switch (what) {
+        case FS_RENDER_VOL: return (size_t)r->vw * r->vh * 4 * sizeof(float);
+        case FS_RENDER_AUX: return (size_t)r->vw * r->vh * 2 * sizeof(float);
+        case FS_RENDER_LIGHTS_BUF: return (size_t)FS_NLIGHTS * 8 * sizeof(float);
+        case FS_RENDER_SHARP: return r->h->ncell * 4 * sizeof(float);
+        case FS_RENDER_LUT: return (size_t)2 * r->bbN * 4 * sizeof(float);
+    }
+    return 0;
+}
+float half_bits_to_float(uint16_t u) {   // binary16 -> binary32, exact
+    const uint32_t s = (uint32_t)(u & 0x8000u) << 16, e = (u >> 10) & 0x1fu, m = u & 0x3ffu;
+    uint32_t bits;
+    if (e == 0) {
+        if (m == 0) bits = s;
+        else {   // subnormal: m * 2^-24
+            float f = (float)m * 5.9604644775390625e-8f;
+            std::memcpy(&bits, &f, 4);
+            bits |= s;
+        }
+    } else if (e == 31) bits = s | 0x7f800000u | (m << 13);
+    else bits = s | ((e + 112u) << 23) | (m << 13);
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+}  // namespace
+
+extern "C" {
+
+int fs_blackbody_lut(int n, double tmin, double tmax, double twb, float* out) {   // src/blackbody.js:57-88
+    if (!out) return fail(FS_EINVAL, "null output");
+    if (n < 2 || n > (1 << 16)) return fail(FS_EINVAL, "n must be in 2..65536");
+    if (!(tmin > 0.0) || !(tmax > tmin) || !(twb >= 0.0)) return fail(FS_EINVAL, "need 0 < tmin < tmax and twb >= 0");
+    double lms2xyz[3][3];
+    bb::inverse(bb::kBradford, lms2xyz);
+    double src[3] = {1, 1, 1}, dst[3] = {1, 1, 1};
+    if (twb > 0.0) {   // whiteBalance(Twb); Twb = 0 is the reference's `!Twb`: plain XYZ -> sRGB
+        double w[3], wn[3];
+        bb::xyz_of(twb, false, w);
+        for (int k = 0; k < 3; k++) wn[k] = w[k] / w[1];
+        bb::mul(bb::kBradford, wn, src);
+        bb::mul(bb::kBradford, bb::kD65, dst);
+    }
+    double ref[3];
+    bb::xyz_of(1500.0, false, ref);
+    for (int row = 0; row < 2; row++) {
+        for (int i = 0; i < n; i++) {
+            const double T = tmin + (tmax - tmin) * i / (n - 1);
+            double xyz[3], rgb[3];
+            bb::xyz_of(T, row == 1, xyz);
+            for (int k = 0; k < 3; k++) xyz[k] = xyz[k] / ref[1];
+            if (twb > 0.0) {
+                double lms[3], x2[3];
+                bb::mul(bb::kBradford, xyz, lms);
+                for (int k = 0; k < 3; k++) lms[k] = lms[k] * dst[k] / src[k];
+                bb::mul(lms2xyz, lms, x2);
+                bb::mul(bb::kXYZ2RGB, x2, rgb);
+            } else {
+                bb::mul(bb::kXYZ2RGB, xyz, rgb);
+            }
+            float* o = out + ((size_t)row * n + i) * 4;
+            for (int k = 0; k < 3; k++) o[k] = (float)std::max(0.0, rgb[k]);
+            o[3] = (float)xyz[1];
+        }
+    }
+    return FS_OK;
+}
+
+int fs_render_param_count(void) { return FS_RP_COUNT; }
+int fs_render_param_index(const char* name) {
+    if (!name) return -1;
+    for (int i = 0; i < FS_RP_COUNT; i++) if (std::strcmp(name, kRenderNames[i]) == 0) return i;
+    return -1;
+}
+int fs_render_param_defaults(float* out) {
+    if (!out) return fail(FS_EINVAL, "null output");
+    render_defaults(out);
+    return FS_OK;
+}
+
+int fs_render_set_white_balance(fs_render* r, double twb) {   // src/renderer.js:57-63
+    if (!r) return fail(FS_EINVAL, "null handle");
+    std::vector<float> lut((size_t)2 * r->bbN * 4);
+    if (int rc = fs_blackbody_lut(r->bbN, r->bbTmin, r->bbTmax, twb, lut.data())) return rc;
+    if (set_device(r->h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipStreamSynchronize(r->h->stream));
+    FS_HIP_CHECK(hipMemcpy(r->lut, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice));
+    return FS_OK;
+}
+
+int fs_render_create(fs_sim* h, int width, int height, double vol_scale, fs_render** out) {
+    if (!h || !out) return fail(FS_EINVAL, "null argument");
+    *out = nullptr;
+    if (!(vol_scale > 0.0) || !std::isfinite(vol_scale)) return fail(FS_EINVAL, "vol_scale must be positive");
+    if (h->ny / FS_NLIGHTS < 2 || h->nx < 2 || h->nz < 2)
+        return fail(FS_EINVAL, "light extraction needs ny >= 16 (two rows per band) and nx, nz >= 2");
+    // resize (src/renderer.js:575-581)
+    const int w = std::max(16, width), hh = std::max(16, height);
+    const int vw = std::max(8, (int)std::floor(w * vol_scale + 0.5)), vh = std::max(8, (int)std::floor(hh * vol_scale + 0.5));
+    if ((long long)vw * vh > (1LL << 28)) return fail(FS_EINVAL, "volume image too large");
+    if (set_device(h)) return FS_EDEVICE;
+    fs_render* r = new fs_render();
+    r->h = h;
+    r->device = h->cfg.device;
+    r->w = w; r->hgt = hh; r->vw = vw; r->vh = vh;
+    render_defaults(r->params);
+    const float sky = 1e6f;   // depth the surface pass writes where no surface is hit
+    bool ok = hipMalloc(&r->lut, (size_t)2 * r->bbN * sizeof(float4)) == hipSuccess &&
+              hipMalloc(&r->lstate, 3 * FS_NLIGHTS * sizeof(float4)) == hipSuccess &&
+              hipMalloc(&r->lights, 2 * FS_NLIGHTS * sizeof(float4)) == hipSuccess &&
+              hipMalloc(&r->depth, sizeof(float)) == hipSuccess &&
+              hipMalloc(&r->sharp, h->ncell * sizeof(half4_t)) == hipSuccess &&
+              hipMalloc(&r->vol, (size_t)vw * vh * sizeof(half4_t)) == hipSuccess &&
+              hipMalloc(&r->aux, (size_t)vw * vh * sizeof(float2)) == hipSuccess;
+    if (!ok) {
+        render_free(r);
+        delete r;
+        return fail(FS_ENOMEM, "hipMalloc failed for the renderer");
+    }
+    launch_fill_zero(h->stream, r->lstate, 3 * FS_NLIGHTS * sizeof(float4));
+    launch_fill_zero(h->stream, r->lights, 2 * FS_NLIGHTS * sizeof(float4));
+    if (hipStreamSynchronize(h->stream) != hipSuccess ||
+        hipMemcpy(r->depth, &sky, sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        render_free(r);
+        delete r;
+        return fail(FS_EDEVICE, "device error in renderer create");
+    }
+    if (int rc = fs_render_set_white_balance(r, 5000.0)) {   // src/renderer.js:53
+        render_free(r);
+        delete r;
+        return rc;
+    }
+    *out = r;
+    return FS_OK;
+}
+
+int fs_render_destroy(fs_render* r) {
+    if (!r) return FS_OK;
+    (void)hipSetDevice(r->device);
+    render_free(r);
+    delete r;
+    return FS_OK;
+}
+
+int fs_render_set(fs_render* r, const char* name, float value) {
+    if (!r || !name) return fail(FS_EINVAL, "null argument");
+    const int i = fs_render_param_index(name);
+    if (i < 0) return fail(FS_EINVAL, std::string("unknown render parameter: ") + name);
+    if (render_owned(i)) return fail(FS_EINVAL, std::string("render parameter is set per frame: ") + name);
+    r->params[i] = value;
+    return FS_OK;
+}
+
+int fs_render_get_uniforms(fs_render* r, float* out) {
+    if (!r || !out) return fail(FS_EINVAL, "null argument");
+    if (!r->have_U) return fail(FS_ESTATE, "no frame rendered yet");
+    std::memcpy(out, r->U.f, sizeof(r->U.f));
+    return FS_OK;
+}
+
+int fs_render_set_camera(fs_render* r, const double pos[3], const double fwd[3], const double right[3], const double up[3],
+                         double fov_y) {
+    if (!r || !pos || !fwd || !right || !up) return fail(FS_EINVAL, "null argument");
+    if (!(fov_y > 0.0 && fov_y < 3.14159)) return fail(FS_EINVAL, "fov_y must be in (0, pi)");
+    for (int k = 0; k < 3; k++) {
+        r->cam_pos[k] = pos[k]; r->cam_fwd[k] = fwd[k]; r->cam_rgt[k] = right[k]; r->cam_up[k] = up[k];
+    }
+    r->fov_y = fov_y;
+    r->have_cam = true;
+    return FS_OK;
+}
+
+int fs_render_set_depth(fs_render* r, const float* depth, int dw, int dh) {
+    if (!r) return fail(FS_EINVAL, "null handle");
+    if (set_device(r->h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipStreamSynchronize(r->h->stream));
+    if (!depth) {
+        const float sky = 1e6f;
+        FS_HIP_CHECK(hipMemcpy(r->depth, &sky, sizeof(float), hipMemcpyHostToDevice));
+        r->dw = r->dh = 1;
+        return FS_OK;
+    }
+    if (dw < 1 || dh < 1 || (long long)dw * dh > (1LL << 28)) return fail(FS_EINVAL, "bad depth size");
+    const size_t n = (size_t)dw * dh;
+    if (n > r->depth_cap) {
+        float* nd = nullptr;
+        if (hipMalloc(&nd, n * sizeof(float)) != hipSuccess) return fail(FS_ENOMEM, "hipMalloc failed for depth");
+        (void)hipFree(r->depth);
+        r->depth = nd;
+        r->depth_cap = n;
+    }
+    FS_HIP_CHECK(hipMemcpy(r->depth, depth, n * sizeof(float), hipMemcpyHostToDevice));
+    r->dw = dw;
+    r->dh = dh;
+    return FS_OK;
+}
+
+int fs_render_frame(fs_render* r, double time, double frame, float jit_x, float jit_y) {
+    if (!r) return fail(FS_EINVAL, "null handle");
+    if (!r->have_cam) return fail(FS_ESTATE, "set a camera first");
+    fs_sim* h = r->h;
+    if (set_device(h)) return FS_EDEVICE;
+    // uploadParams (src/renderer.js:634-648): this.params plus the per-frame entries, each a
+    // JS number rounded to f32
+    const double lightDt = (!r->have_last_light || time < r->last_light_time || time - r->last_light_time > 0.5)
+                               ? -1.0 : time - r->last_light_time;
+    r->have_last_light = true;
+    r->last_light_time = time;
+    RenderU& U = r->U;
+    std::memcpy(U.f, r->params, sizeof(U.f));
+    U.f[RP_lightDt] = (float)lightDt;
+    U.f[RP_width] = (float)r->w; U.f[RP_height] = (float)r->hgt; U.f[RP_vw] = (float)r->vw; U.f[RP_vh] = (float)r->vh;
+    U.f[RP_time] = (float)time; U.f[RP_frame] = (float)frame;
+    U.f[RP_nLogs] = (float)(h->logs_h.size() / 8); U.f[RP_nStones] = 0.f; U.f[RP_nLights] = (float)FS_NLIGHTS;
+    U.f[RP_camX] = (float)r->cam_pos[0]; U.f[RP_camY] = (float)r->cam_pos[1]; U.f[RP_camZ] = (float)r->cam_pos[2];
+    U.f[RP_fwdX] = (float)r->cam_fwd[0]; U.f[RP_fwdY] = (float)r->cam_fwd[1]; U.f[RP_fwdZ] = (float)r->cam_fwd[2];
+    U.f[RP_rgtX] = (float)r->cam_rgt[0]; U.f[RP_rgtY] = (float)r->cam_rgt[1]; U.f[RP_rgtZ] = (float)r->cam_rgt[2];
+    U.f[RP_upX] = (float)r->cam_up[0]; U.f[RP_upY] = (float)r->cam_up[1]; U.f[RP_upZ] = (float)r->cam_up[2];
+    U.f[RP_tanFov] = (float)std::tan(r->fov_y / 2); U.f[RP_aspect] = (float)((double)r->w / r->hgt);
+    U.f[RP_jitX] = jit_x; U.f[RP_jitY] = jit_y;
+    U.f[RP_ox] = (float)h->origin[0]; U.f[RP_oy] = (float)h->origin[1]; U.f[RP_oz] = (float)h->origin[2];
+    U.f[RP_dx] = (float)h->dx;
+    U.f[RP_sx] = h->cfg.size[0]; U.f[RP_sy] = h->cfg.size[1]; U.f[RP_sz] = h->cfg.size[2];
+    U.f[RP_nx] = (float)h->nx; U.f[RP_ny] = (float)h->ny; U.f[RP_nz] = (float)h->nz;
+    U.f[RP_bbTmin] = (float)r->bbTmin; U.f[RP_bbTmax] = (float)r->bbTmax; U.f[RP_bbN] = (float)r->bbN;
+    r->have_U = true;
+    // renderHDR's passes on the step's fields (src/renderer.js:670-690)
+    const FieldSet F = h->fields();
+    launch_lights(h->prec, h->stream, U, h->g, F.S, r->lut, r->lstate, r->lights);
+    const bool sharp = r->params[RP_flameSharp] > 0.f && r->params[RP_cubic] > 0.5f;
+    if (sharp) {   // once per simulation state
+        const bool fresh = r->sharp_valid && r->sharp_gen == h->gen && r->sharp_frame == h->frame &&
+                           r->sharp_time == h->time && r->sharp_k == r->params[RP_flameSharp];
+        if (!fresh) {
+            launch_sharpen(h->prec, h->stream, U.f[RP_flameSharp], h->g, F.S, r->sharp);
+            r->sharp_valid = true;
+            r->sharp_gen = h->gen; r->sharp_frame = h->frame; r->sharp_time = h->time; r->sharp_k = r->params[RP_flameSharp];
+        }
+    }
+    launch_volume(h->prec, sharp, h->stream, U, h->g, sharp ? (const void*)r->sharp : F.S, F.S2, r->lut, r->lights, r->depth,
+                  r->dw, r->dh, h->curl, h->cfg.noise_n, r->vol, r->aux);
+    FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int fs_render_size(fs_render* r, int* vw, int* vh) {
+    if (!r) return fail(FS_EINVAL, "null handle");
+    if (vw) *vw = r->vw;
+    if (vh) *vh = r->vh;
+    return FS_OK;
+}
+
+int fs_render_read(fs_render* r, int what, float* host, size_t bytes) {
+    if (!r || !host) return fail(FS_EINVAL, "null argument");
+    const size_t want = render_bytes(r, what);
+    if (!want) return fail(FS_EINVAL, "unknown render buffer");
+    if (bytes != want) return fail(FS_EINVAL, "size mismatch: expected " + std::to_string(want) + " bytes");
+    if (what == FS_RENDER_SHARP && !r->sharp_valid) return fail(FS_ESTATE, "no prefiltered field yet");
+    if (set_device(r->h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipStreamSynchronize(r->h->stream));
+    if (what == FS_RENDER_VOL || what == FS_RENDER_SHARP) {   // binary16 on the device
+        const size_t nval = want / sizeof(float);
+        std::vector<uint16_t> raw(nval);
+        FS_HIP_CHECK(hipMemcpy(raw.data(), what == FS_RENDER_VOL ? (void*)r->vol : (void*)r->sharp, nval * 2, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nval; i++) host[i] = half_bits_to_float(raw[i]);
+        return FS_OK;
+    }
+    const void* src = what == FS_RENDER_AUX ? (const void*)r->aux : what == FS_RENDER_LIGHTS_BUF ? (const void*)r->lights
+                                                                                                  : (const void*)r->lut;
+    FS_HIP_CHECK(hipMemcpy(host, src, want, hipMemcpyDeviceToHost));
+    return FS_OK;
+}
+
+void* fs_render_device_ptr(fs_render* r, int what) {
+    if (!r) return nullptr;
+    switch (what) {
+        case FS_RENDER_VOL: return r->vol;
+        case FS_RENDER_AUX: return r->aux;
+        case FS_RENDER_LIGHTS_BUF: return r->lights;
+        case FS_RENDER_SHARP: return r->sharp;
+        case FS_RENDER_LUT: return r->lut;
+    }
+    return nullptr;
+}
 
 }  // extern "C"

@@ -212,6 +212,35 @@ void launch_spark_count(hipStream_t s, const float4* sp, int count, int* dead);
 // after the update back into `dead` (both arrays spark_blocks(count) ints)
 void launch_spark_step(int prec, hipStream_t s, const SparkUniform& U, float4* sp, int* dead, int* off, const void* vel,
                        const Dims& g, const half4_t* curl, int nn);
+// ---- render-volume consumers (fs_render.hip) --------------------------------------
+// The renderer's uniform block: RP_FIELDS of src/renderer.js:8-17, all f32 as uploaded.
+enum {
+    RP_width, RP_height, RP_vw, RP_vh, RP_time, RP_frame, RP_nLogs, RP_nStones,
+    RP_camX, RP_camY, RP_camZ, RP_tanFov, RP_fwdX, RP_fwdY, RP_fwdZ, RP_aspect, RP_rgtX, RP_rgtY, RP_rgtZ, RP_jitX,
+    RP_upX, RP_upY, RP_upZ, RP_jitY,
+    RP_ox, RP_oy, RP_oz, RP_dx, RP_sx, RP_sy, RP_sz, RP_tempScale, RP_nx, RP_ny, RP_nz, RP_nLights,
+    RP_exposure, RP_stepMul, RP_sootAbs, RP_sootEmit, RP_emitK, RP_blueK, RP_smokeScat, RP_smokeG,
+    RP_ambient, RP_detailAmt, RP_detailFreq, RP_detailSpeed, RP_hazeK, RP_glowK, RP_bloomK, RP_tonemap,
+    RP_grain, RP_vignette, RP_saturation, RP_contrast, RP_bedGlow, RP_logGlow, RP_lightK, RP_accumN,
+    RP_bbTmin, RP_bbTmax, RP_bbN, RP_emberFlicker, RP_sootEmitMin, RP_flameSharp, RP_moon, RP_fogK,
+    RP_shadowSoft, RP_lift, RP_emitTempLo, RP_glowTemp, RP_inset, RP_knee, RP_surfDetail, RP_matA, RP_matB, RP_matC,
+    RP_sootScat, RP_chroma, RP_lensX, RP_lensY, RP_focus, RP_lightDt, RP_flickerK, RP_cubic, RP_aperture, RP_sootEdge,
+    RP_COUNT
+};
+static_assert(RP_COUNT == FS_RP_COUNT, "uniform block layout");
+struct RenderU { float f[RP_COUNT]; };
+constexpr int FS_NLIGHTS = 8;
+// light extraction: one block per height band; S is the sim's field (its storage), lut 2 x bbN
+// float4 texels, lstate 3 x 8 float4 of filter state, lights 8 x 2 float4 (pos, col)
+void launch_lights(int prec, hipStream_t s, const RenderU& R, const Dims& g, const void* S_field, const float4* lut,
+                   float4* lstate, float4* lights);
+// flame-field prefilter of S into a binary16 copy (same cell order)
+void launch_sharpen(int prec, hipStream_t s, float k, const Dims& g, const void* S_field, half4_t* out);
+// volume march: S_field in the sim's storage, or the binary16 prefiltered copy when `sharpened`;
+// depth is dw x dh floats; vol is vw x vh binary16 (L, Tr), aux vw x vh float2 (haze, depth)
+void launch_volume(int prec, bool sharpened, hipStream_t s, const RenderU& R, const Dims& g, const void* S_field,
+                   const void* S2_field, const float4* lut, const float4* lights, const float* depth, int dw, int dh,
+                   const half4_t* curl, int nn, half4_t* vol, float2* aux);
 // ---- diagnostics (fs_diag.hip) -------------------------------------------------
 int  stats_blocks(size_t n);
 void launch_stats(int prec, hipStream_t s, const void* field, size_t n, double* part);

@@ -10,6 +10,7 @@ There is no CPU fallback: if libfiresim.so is missing or no GPU is present, the
 constructors raise.
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -163,6 +164,21 @@ def lib():
         "fs_adjoint_tangent": (ip, [vp, ctypes.c_double, ip, dp, fp, fp, fp, fp, ctypes.c_double, ip]),
         "fs_adjoint_read": (ip, [vp, ip, ip, vp, sz]),
         "fs_adjoint_grads": (ip, [vp, dp, ip, dp]),
+        "fs_render_param_count": (ip, []),
+        "fs_render_param_index": (ip, [ctypes.c_char_p]),
+        "fs_render_param_defaults": (ip, [fp]),
+        "fs_render_create": (ip, [vp, ip, ip, ctypes.c_double, ctypes.POINTER(vp)]),
+        "fs_render_destroy": (ip, [vp]),
+        "fs_render_set": (ip, [vp, ctypes.c_char_p, ctypes.c_float]),
+        "fs_render_get_uniforms": (ip, [vp, fp]),
+        "fs_render_set_camera": (ip, [vp, dp, dp, dp, dp, ctypes.c_double]),
+        "fs_render_set_white_balance": (ip, [vp, ctypes.c_double]),
+        "fs_render_set_depth": (ip, [vp, fp, ip, ip]),
+        "fs_render_frame": (ip, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_float, ctypes.c_float]),
+        "fs_render_size": (ip, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "fs_render_read": (ip, [vp, ip, vp, sz]),
+        "fs_render_device_ptr": (vp, [vp, ip]),
+        "fs_blackbody_lut": (ip, [ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, fp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -610,6 +626,134 @@ class Sparks:
         t = ctypes.c_double()
         _check(lib().fs_sparks_clock(self._h, ctypes.byref(t), None))
         return t.value
+
+
+RP_COUNT = 88
+RENDER_VOL, RENDER_AUX, RENDER_LIGHTS, RENDER_SHARP, RENDER_LUT = range(5)
+RENDER_NLIGHTS = 8
+
+
+def render_param_index(name):
+    return lib().fs_render_param_index(name.encode())
+
+
+def render_param_defaults():
+    out = np.zeros(RP_COUNT, np.float32)
+    _check(lib().fs_render_param_defaults(out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return out
+
+
+def blackbody_lut(n=1024, tmin=400.0, tmax=3000.0, twb=5000.0):
+    """blackbodyLUT({n, Tmin, Tmax, Twb}) (src/blackbody.js:78-88): (2, n, 4) float32, row 0 the
+    blackbody, row 1 soot; columns linear sRGB radiance relative to a 1500 K blackbody's luminance,
+    then the unbalanced luminance Y."""
+    out = np.zeros((2, n, 4), np.float32)
+    _check(lib().fs_blackbody_lut(int(n), float(tmin), float(tmax), float(twb),
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return out
+
+
+def _vnorm(a):
+    l = math.hypot(*a) or 1.0
+    return [a[0] / l, a[1] / l, a[2] / l]
+
+
+def look_at(pos, target, fov_y_deg=40.0, roll=0.0):
+    """lookAt (src/camera.js:6-20): dict with pos, target, fwd, right, up, fovY (rad)."""
+    fwd = _vnorm([target[i] - pos[i] for i in range(3)])
+    right = _vnorm([fwd[1] * 0 - fwd[2] * 1, fwd[2] * 0 - fwd[0] * 0, fwd[0] * 1 - fwd[1] * 0])
+    up = [right[1] * fwd[2] - right[2] * fwd[1], right[2] * fwd[0] - right[0] * fwd[2], right[0] * fwd[1] - right[1] * fwd[0]]
+    if roll:
+        c, s_ = math.cos(roll), math.sin(roll)
+        right, up = [right[i] * c + up[i] * s_ for i in range(3)], [up[i] * c - right[i] * s_ for i in range(3)]
+    return dict(pos=list(pos), target=list(target), fwd=fwd, right=right, up=up, fovY=fov_y_deg * math.pi / 180)
+
+
+def orbit(yaw=0.0, pitch=0.15, dist=2.4, target=(0.0, 0.45, 0.0), fov=40.0):
+    """orbit (src/camera.js:22-29)."""
+    pos = [target[0] + dist * math.cos(pitch) * math.sin(yaw), target[1] + dist * math.sin(pitch),
+           target[2] + dist * math.cos(pitch) * math.cos(yaw)]
+    return look_at(pos, target, fov)
+
+
+class Render:
+    """The Renderer passes that consume a FireSim's S / S2 (src/renderer.js:104-326): fire-light
+    extraction, flame-field prefilter and the volume ray-march, on the sim's GPU and stream.
+    frame() renders the sim's current fields; read_vol() gives (vh, vw, 4) float32 (L.rgb, Tr),
+    read_aux() (vh, vw, 2) (haze, surface depth), read_lights() (8, 8) (pos, luminance, colour,
+    spread)."""
+
+    def __init__(self, sim, width=960, height=1080, vol_scale=0.5):
+        self.sim = sim
+        self._h = ctypes.c_void_p()
+        _check(lib().fs_render_create(sim._h, int(width), int(height), float(vol_scale), ctypes.byref(self._h)))
+        vw, vh = ctypes.c_int(), ctypes.c_int()
+        _check(lib().fs_render_size(self._h, ctypes.byref(vw), ctypes.byref(vh)))
+        self.vw, self.vh = vw.value, vh.value
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().fs_render_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, **kw):
+        """this.params entries: exposure, stepMul, sootAbs, cubic, flameSharp, detailAmt, ..."""
+        for k, v in kw.items():
+            _check(lib().fs_render_set(self._h, k.encode(), float(v)))
+
+    def set_camera(self, cam):
+        """cam: a look_at / orbit dict (pos, fwd, right, up, fovY)."""
+        arr = [(ctypes.c_double * 3)(*[float(x) for x in cam[k]]) for k in ("pos", "fwd", "right", "up")]
+        _check(lib().fs_render_set_camera(self._h, *arr, float(cam["fovY"])))
+
+    def set_white_balance(self, twb):
+        _check(lib().fs_render_set_white_balance(self._h, float(twb)))
+
+    def set_depth(self, depth=None):
+        """Surface depth (dh, dw) float32 in metres along the ray; None = no surfaces (sky)."""
+        if depth is None:
+            _check(lib().fs_render_set_depth(self._h, None, 0, 0))
+            return
+        d = np.ascontiguousarray(depth, np.float32)
+        _check(lib().fs_render_set_depth(self._h, d.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), d.shape[1], d.shape[0]))
+
+    def frame(self, time, frame, jitter=(0.0, 0.0)):
+        _check(lib().fs_render_frame(self._h, float(time), float(frame), float(jitter[0]), float(jitter[1])))
+
+    def uniforms(self):
+        out = np.zeros(RP_COUNT, np.float32)
+        _check(lib().fs_render_get_uniforms(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def _read(self, what, shape):
+        out = np.empty(shape, np.float32)
+        _check(lib().fs_render_read(self._h, what, out.ctypes.data, out.nbytes))
+        return out
+
+    def read_vol(self):
+        return self._read(RENDER_VOL, (self.vh, self.vw, 4))
+
+    def read_aux(self):
+        return self._read(RENDER_AUX, (self.vh, self.vw, 2))
+
+    def read_lights(self):
+        return self._read(RENDER_LIGHTS, (RENDER_NLIGHTS, 8))
+
+    def read_sharp(self):
+        nx, ny, nz = self.sim.res
+        return self._read(RENDER_SHARP, (nz, ny, nx, 4))
+
+    def read_lut(self):
+        return self._read(RENDER_LUT, (2, 1024, 4))
+
+    def device_ptr(self, what):
+        return lib().fs_render_device_ptr(self._h, what)
 
 
 class Poisson:
