@@ -562,6 +562,137 @@ def capture_bench(args):
     print(json.dumps(out))
 
 
+def march_samples(U):
+    """Upper bound of the march samples of one frame: (tf - tn) / ds per ray through the box with no
+    surfaces, ignoring the early exit at Tr < 0.003 (src/renderer.js:245-259), in float64."""
+    vw, vh = int(U[2]), int(U[3])
+    px = np.arange(vw) + 0.5
+    py = np.arange(vh) + 0.5
+    ndx = (px / vw * 2 - 1)[None, :, None]
+    ndy = (1 - py / vh * 2)[:, None, None]
+    fwd, rgt, up = (np.array(U[i:i + 3], np.float64) for i in (12, 16, 20))
+    d = fwd + rgt * ndx * U[11] * U[15] + up * ndy * U[11]
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    ro = np.array(U[8:11], np.float64)
+    bmin = np.array(U[24:27], np.float64)
+    bmax = bmin + np.array(U[28:31], np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t0, t1 = (bmin - ro) / d, (bmax - ro) / d
+    tn = np.maximum(np.minimum(t0, t1).max(-1), 0)
+    tf = np.maximum(t0, t1).min(-1)
+    ds = U[27] * U[37]
+    n = np.where(tf > tn, np.minimum((tf - tn) / ds + 0.5, 1200), 0)
+    return float(n.sum()), int((tf > tn).sum())
+
+
+def volume_bench(args):
+    """Render-volume consumers of the step's fields (src/renderer.js:104-326): the fire-light extraction,
+    flame-field prefilter and volume ray-march of one 1080 x 1920 frame at volume scale 1 (2.07 M rays,
+    stepMul 0.5: the reference's quality preset for a 128-cell-wide grid, src/main.js:19; look of
+    tools/render_final.mjs:23: cubic, flameSharp 1, detailAmt 0.3, sootEdge 0.12, lightK 0.6), on the
+    mixed 128x192x128 8-spot fire after 1000 steps (1 s) from rest, when the plume fills the box.  One sim
+    step between frames, so every frame runs all three passes (as the capture loop renders each sub-step).
+    The march runs with its empty-space map (the default) and, for comparison, without it."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import fsoracle as O
+    sim = make_sim(F.PREC_FP16, F.SOLVE_REFINE64, True)
+    sim.step(DT, 1000)
+    sim.sync()
+    W, H = 1080, 1920
+    look = dict(stepMul=0.5, cubic=1, flameSharp=1, detailAmt=0.3, sootEdge=0.12, lightK=0.6)
+    cam = F.orbit(yaw=0.35, pitch=0.12, dist=1.6, target=(0.0, 0.42, 0.0), fov=38.0)
+    r = F.Render(sim, W, H, 1.0)
+    r.set(**look)
+    r.set_camera(cam)
+    frames = max(10, args.steps // 10)
+    for i in range(3):
+        sim.step(DT, 1)
+        r.frame(sim.time, i)
+    sim.sync()
+    t0 = time.perf_counter()
+    for i in range(frames):
+        sim.step(DT, 1)
+    sim.sync()
+    t_steps = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for i in range(frames):
+        sim.step(DT, 1)
+        r.frame(sim.time, 3 + i)
+    sim.sync()
+    t_frame = (time.perf_counter() - t0 - t_steps) / frames
+    r.profile(True)
+    acc = {"lights": 0.0, "sharpen": 0.0, "volume": 0.0}
+    for i in range(frames):
+        sim.step(DT, 1)
+        r.frame(sim.time, 100 + i)
+        for k, v in r.timings().items():
+            acc[k] += v / frames
+    r.empty_skip(False)
+    ms_noskip = 0.0
+    for i in range(frames):
+        sim.step(DT, 1)
+        r.frame(sim.time, 200 + i)
+        ms_noskip += r.timings()["volume"] / frames
+    r.empty_skip(True)
+    r.profile(False)
+    U = r.uniforms()
+    vol = r.read_vol()
+    samples, hit = march_samples(U)
+    ncell = float(np.prod(RES))
+    npix = W * H
+    nn = 128
+    # algorithmic bytes of one volume launch: the prefiltered S, S2 and the curl texture read once
+    # (binary16 x 4), the image written once (binary16 x 4 + 2 x float)
+    alg = ncell * 8 + ncell * 8 + nn ** 3 * 8 + npix * 16
+    ms_v = acc["volume"]
+    out = {"metric": "render-volume frame (lights + prefilter + ray-march) on the 128x192x128 fire",
+           "workload": f"{W}x{H} frame, volume scale 1 ({npix} rays), stepMul 0.5, cubic + flameSharp 1, detailAmt 0.3, "
+                       "sootEdge 0.12, lightK 0.6; mixed 8-spot fire after 1000 steps",
+           "ms_per_frame": t_frame * 1e3, "frames_per_s": 1.0 / t_frame, "rays_per_s": npix / t_frame,
+           "kernel_ms": acc, "volume_ms_without_empty_skip": ms_noskip, "rays_hitting_box": hit,
+           "march_samples_upper": samples,
+           "samples_per_s_upper": samples / (ms_v * 1e-3),
+           "texel_bytes_requested_per_sample": 64 * 8 + 16 * 8 + 8 * 8,
+           "requested_GBs_upper": samples * 704 / (ms_v * 1e-3) / 1e9,
+           "mean_L": float(vol[..., :3].mean()), "min_Tr": float(vol[..., 3].min()),
+           "roofline": {"bound": "hbm", "kernel": "k_volume", "achieved": alg / (ms_v * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": alg / (ms_v * 1e-3) / 1e9 / HBM_PEAK_GBS, "alg_bytes_per_launch": alg,
+                        "avg_launch_ms": ms_v, "traffic": None},
+           "sharpen_GBs": ncell * 16 / (acc["sharpen"] * 1e-3) / 1e9 if acc["sharpen"] > 0 else None}
+    print(f"volume frame: {t_frame * 1e3:.3f} ms ({acc}); march without the empty-space map {ms_noskip:.3f} ms",
+          flush=True)
+    if args.no_cpu:
+        r.close()
+        sim.close()
+        print(json.dumps(out))
+        return
+    # CPU baseline: the oracle's lights + prefilter + march of a 270 x 480 frame (1/16 of the rays, same
+    # camera and look) from the GPU's fields, on this host's cores
+    rs = F.Render(sim, W // 4, H // 4, 1.0)
+    rs.set(**look)
+    rs.set_camera(cam)
+    rs.frame(sim.time, 0)
+    Us = rs.uniforms()
+    S, S2 = sim.read(F.S, np.float32), sim.read(F.S2, np.float32)
+    curl = sim.read(F.CURL, np.float32)
+    lut = rs.read_lut()
+    t0 = time.perf_counter()
+    lst = np.zeros((3, 8, 4), np.float32)
+    lights = O.render_lights(Us, S, lut, lst)
+    sharp = O.render_sharpen(1.0, S)
+    O.render_volume(Us, sharp, S2, lut, lights, curl)
+    tc = time.perf_counter() - t0
+    nps = (W // 4) * (H // 4)
+    out["cpu_baseline"] = {"value": nps / tc, "unit": "rays/s", "cores": O.num_threads(), "kind": "port",
+                           "sample": f"oracle lights + prefilter + march of a {W // 4}x{H // 4} frame ({nps} rays) "
+                                     f"from the GPU's fields; {tc:.2f} s"}
+    out["vs_cpu"] = (npix / t_frame) / out["cpu_baseline"]["value"]
+    rs.close()
+    r.close()
+    sim.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -589,6 +720,8 @@ def main():
                     help="instead of the step: spark-particle update throughput (8192 and 4M sparks) on the step's velocity")
     ap.add_argument("--capture", action="store_true",
                     help="instead of the step: the timeline-driven capture loop (firesim.driver) vs plain stepping")
+    ap.add_argument("--volume", action="store_true",
+                    help="instead of the step: the render-volume consumers (lights, prefilter, ray-march) of a 1080x1920 frame")
     ap.add_argument("--timing-only", action="store_true",
                     help="with --adjoint: forward, backward and tangent timings and the dot-product test only")
     ap.add_argument("--probe", action="store_true",
@@ -619,6 +752,9 @@ def main():
         return
     if args.capture:
         capture_bench(args)
+        return
+    if args.volume:
+        volume_bench(args)
         return
     ncell = float(np.prod(RES))
     graph = not args.no_graph

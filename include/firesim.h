@@ -400,6 +400,14 @@ int  fs_render_set_depth(fs_render* r, const float* depth, int dw, int dh);
  * once per sim state) and volume passes of renderHDR. */
 int  fs_render_frame(fs_render* r, double time, double frame, float jit_x, float jit_y);
 int  fs_render_size(fs_render* r, int* vw, int* vh);
+/* Empty-space skipping of the march (default on; no counterpart in the reference): samples
+ * whose texels cannot make them contribute are passed over.  Results are bit-identical
+ * either way; off is for timing comparisons. */
+int  fs_render_empty_skip(fs_render* r, int on);
+/* HIP events around the passes of each frame on the sim's stream; fs_render_timings gives the
+ * last frame's device time of the lights, prefilter (0 if it did not run) and volume passes (ms). */
+int  fs_render_profile_enable(fs_render* r, int on);
+int  fs_render_timings(fs_render* r, float ms[3]);
 int  fs_render_read(fs_render* r, int what, float* host, size_t bytes);
 void* fs_render_device_ptr(fs_render* r, int what);
 /* blackbodyLUT({n, Tmin, Tmax, Twb}) (src/blackbody.js:78-88): 2 x n x 4 floats. */

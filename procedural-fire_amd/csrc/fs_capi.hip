@@ -1910,12 +1910,22 @@ struct fs_render {
     int64_t sharp_frame = 0;
     double sharp_time = 0.0;
     float sharp_k = 0.f;
+    bool skip = true;                  // empty-space map of the march
+    int bx = 0, by = 0, bz = 0;        // bricks of 4^3 cells
+    unsigned char* raw = nullptr;      // bricks holding an active cell
+    unsigned char* flags = nullptr;    // ... within the march's reach
+    float curl_max = 0.f;              // largest |component| of the curl texture's xyz
+    bool profile = false;              // HIP events around the passes of each frame
+    hipEvent_t ev[4] = {};
+    bool timings_valid = false;
+    bool sharp_ran = false;            // the last frame ran the prefilter
 };
 
 namespace {
 void render_free(fs_render* r) {
-    void* p[] = {r->lut, r->lstate, r->lights, r->depth, r->sharp, r->vol, r->aux};
+    void* p[] = {r->lut, r->lstate, r->lights, r->depth, r->sharp, r->vol, r->aux, r->raw, r->flags};
     for (void* q : p) if (q) (void)hipFree(q);
+    for (hipEvent_t& e : r->ev) if (e) (void)hipEventDestroy(e);
 }
 size_t render_bytes(const fs_render* r, int what) {
     switch (what) {
@@ -2023,6 +2033,7 @@ int fs_render_create(fs_sim* h, int width, int height, double vol_scale, fs_rend
     r->h = h;
     r->device = h->cfg.device;
     r->w = w; r->hgt = hh; r->vw = vw; r->vh = vh;
+    r->bx = (h->nx + 3) / 4; r->by = (h->ny + 3) / 4; r->bz = (h->nz + 3) / 4;
     render_defaults(r->params);
     const float sky = 1e6f;   // depth the surface pass writes where no surface is hit
     bool ok = hipMalloc(&r->lut, (size_t)2 * r->bbN * sizeof(float4)) == hipSuccess &&
@@ -2031,7 +2042,9 @@ int fs_render_create(fs_sim* h, int width, int height, double vol_scale, fs_rend
               hipMalloc(&r->depth, sizeof(float)) == hipSuccess &&
               hipMalloc(&r->sharp, h->ncell * sizeof(half4_t)) == hipSuccess &&
               hipMalloc(&r->vol, (size_t)vw * vh * sizeof(half4_t)) == hipSuccess &&
-              hipMalloc(&r->aux, (size_t)vw * vh * sizeof(float2)) == hipSuccess;
+              hipMalloc(&r->aux, (size_t)vw * vh * sizeof(float2)) == hipSuccess &&
+              hipMalloc(&r->raw, (size_t)r->bx * r->by * r->bz) == hipSuccess &&
+              hipMalloc(&r->flags, (size_t)r->bx * r->by * r->bz) == hipSuccess;
     if (!ok) {
         render_free(r);
         delete r;
@@ -2039,12 +2052,17 @@ int fs_render_create(fs_sim* h, int width, int height, double vol_scale, fs_rend
     }
     launch_fill_zero(h->stream, r->lstate, 3 * FS_NLIGHTS * sizeof(float4));
     launch_fill_zero(h->stream, r->lights, 2 * FS_NLIGHTS * sizeof(float4));
+    const size_t nn3 = (size_t)h->cfg.noise_n * h->cfg.noise_n * h->cfg.noise_n;
+    std::vector<uint16_t> curl(nn3 * 4);
     if (hipStreamSynchronize(h->stream) != hipSuccess ||
-        hipMemcpy(r->depth, &sky, sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(r->depth, &sky, sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(curl.data(), h->curl, curl.size() * 2, hipMemcpyDeviceToHost) != hipSuccess) {
         render_free(r);
         delete r;
         return fail(FS_EDEVICE, "device error in renderer create");
     }
+    for (size_t i = 0; i < nn3; i++)   // the curl warp moves a sample by at most detail * 0.12 * this
+        for (int c = 0; c < 3; c++) r->curl_max = std::max(r->curl_max, std::fabs(half_bits_to_float(curl[i * 4 + c])));
     if (int rc = fs_render_set_white_balance(r, 5000.0)) {   // src/renderer.js:53
         render_free(r);
         delete r;
@@ -2146,20 +2164,66 @@ int fs_render_frame(fs_render* r, double time, double frame, float jit_x, float 
     r->have_U = true;
     // renderHDR's passes on the step's fields (src/renderer.js:670-690)
     const FieldSet F = h->fields();
+    if (r->profile) FS_HIP_CHECK(hipEventRecord(r->ev[0], h->stream));
     launch_lights(h->prec, h->stream, U, h->g, F.S, r->lut, r->lstate, r->lights);
+    if (r->profile) FS_HIP_CHECK(hipEventRecord(r->ev[1], h->stream));
     const bool sharp = r->params[RP_flameSharp] > 0.f && r->params[RP_cubic] > 0.5f;
+    r->sharp_ran = false;
     if (sharp) {   // once per simulation state
         const bool fresh = r->sharp_valid && r->sharp_gen == h->gen && r->sharp_frame == h->frame &&
                            r->sharp_time == h->time && r->sharp_k == r->params[RP_flameSharp];
         if (!fresh) {
             launch_sharpen(h->prec, h->stream, U.f[RP_flameSharp], h->g, F.S, r->sharp);
             r->sharp_valid = true;
+            r->sharp_ran = true;
             r->sharp_gen = h->gen; r->sharp_frame = h->frame; r->sharp_time = h->time; r->sharp_k = r->params[RP_flameSharp];
         }
     }
-    launch_volume(h->prec, sharp, h->stream, U, h->g, sharp ? (const void*)r->sharp : F.S, F.S2, r->lut, r->lights, r->depth,
-                  r->dw, r->dh, h->curl, h->cfg.noise_n, r->vol, r->aux);
+    if (r->profile) FS_HIP_CHECK(hipEventRecord(r->ev[2], h->stream));
+    const void* marched = sharp ? (const void*)r->sharp : F.S;
+    const float detail = U.f[RP_detailAmt];
+    // reach of a sample's texels from its unwarped cell: the cubic footprint (2), the warp and
+    // one cell for the rounding of the map lookup
+    const double warp = detail > 0.f ? (double)detail * 0.12 * r->curl_max : 0.0;
+    const bool skip = r->skip && std::isfinite(warp) && warp < 64.0;
+    if (skip) {
+        const int reach = 3 + (int)std::ceil(warp);
+        launch_occupancy(h->prec, sharp, h->stream, h->g, marched, F.S2, U.f[RP_smokeScat] > 0.f, r->raw, r->flags, r->bx,
+                         r->by, r->bz, (reach + 3) / 4);
+    }
+    launch_volume(h->prec, sharp, h->stream, U, h->g, marched, F.S2, r->lut, r->lights, r->depth, r->dw, r->dh, h->curl,
+                  h->cfg.noise_n, skip ? r->flags : nullptr, r->bx, r->by, r->bz, (float)(1.0 / h->dx), r->vol, r->aux);
+    if (r->profile) FS_HIP_CHECK(hipEventRecord(r->ev[3], h->stream));
+    r->timings_valid = r->profile;
     FS_HIP_CHECK(hipGetLastError());
+    return FS_OK;
+}
+
+int fs_render_empty_skip(fs_render* r, int on) {
+    if (!r) return fail(FS_EINVAL, "null handle");
+    r->skip = on != 0;
+    return FS_OK;
+}
+
+int fs_render_profile_enable(fs_render* r, int on) {
+    if (!r) return fail(FS_EINVAL, "null handle");
+    if (set_device(r->h)) return FS_EDEVICE;
+    if (on && !r->ev[0])
+        for (hipEvent_t& e : r->ev) FS_HIP_CHECK(hipEventCreate(&e));
+    r->profile = on != 0;
+    r->timings_valid = false;
+    return FS_OK;
+}
+
+int fs_render_timings(fs_render* r, float ms[3]) {
+    if (!r || !ms) return fail(FS_EINVAL, "null argument");
+    if (!r->timings_valid) return fail(FS_ESTATE, "enable profiling and render a frame first");
+    if (set_device(r->h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipEventSynchronize(r->ev[3]));
+    FS_HIP_CHECK(hipEventElapsedTime(&ms[0], r->ev[0], r->ev[1]));
+    FS_HIP_CHECK(hipEventElapsedTime(&ms[1], r->ev[1], r->ev[2]));
+    FS_HIP_CHECK(hipEventElapsedTime(&ms[2], r->ev[2], r->ev[3]));
+    if (!r->sharp_ran) ms[1] = 0.f;
     return FS_OK;
 }
 

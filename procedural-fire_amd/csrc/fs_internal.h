@@ -238,9 +238,15 @@ void launch_lights(int prec, hipStream_t s, const RenderU& R, const Dims& g, con
 void launch_sharpen(int prec, hipStream_t s, float k, const Dims& g, const void* S_field, half4_t* out);
 // volume march: S_field in the sim's storage, or the binary16 prefiltered copy when `sharpened`;
 // depth is dw x dh floats; vol is vw x vh binary16 (L, Tr), aux vw x vh float2 (haze, depth)
+// empty-space map: raw / flags are bx*by*bz bytes (bricks of 4^3 cells); a brick is flagged
+// when a cell within rb bricks may make a march sample contribute
+void launch_occupancy(int prec, bool sharpened, hipStream_t s, const Dims& g, const void* S_field, const void* S2_field,
+                      bool smoke_on, unsigned char* raw, unsigned char* flags, int bx, int by, int bz, int rb);
+// flags null: march every sample
 void launch_volume(int prec, bool sharpened, hipStream_t s, const RenderU& R, const Dims& g, const void* S_field,
                    const void* S2_field, const float4* lut, const float4* lights, const float* depth, int dw, int dh,
-                   const half4_t* curl, int nn, half4_t* vol, float2* aux);
+                   const half4_t* curl, int nn, const unsigned char* flags, int bx, int by, int bz, float inv_dx,
+                   half4_t* vol, float2* aux);
 // ---- diagnostics (fs_diag.hip) -------------------------------------------------
 int  stats_blocks(size_t n);
 void launch_stats(int prec, hipStream_t s, const void* field, size_t n, double* part);

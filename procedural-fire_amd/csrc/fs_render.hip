@@ -142,6 +142,39 @@ __global__ __launch_bounds__(256) void k_sharpen(float k, Dims g, const typename
     Codec<StF16>::st(sOut, c.i, o);
 }
 
+// ---- empty-space map of the march ---------------------------------------------------------
+// A march sample changes nothing (L, Tr and haze stay bit for bit) when its temperature,
+// soot and heat release after the fade, and its smoke, are at or below the thresholds of
+// src/renderer.js:292 and :320.  The cubic and trilinear weights are non-negative and sum to
+// 1 and the fade is at most 1, so a sample whose texel footprint holds no texel above
+// (1 - 1e-5) x threshold is such a no-op (the margin covers the rounding of the weighted sum).
+// Bricks of 4^3 cells are flagged when any cell within `reach` cells (the cubic footprint, the
+// largest curl-warp displacement and one cell of slack) exceeds that bound.
+constexpr float kActiveMargin = 0.99999f;
+
+template <class SS, class S2S>
+__global__ __launch_bounds__(256) void k_active(Dims g, const typename SS::raw* __restrict__ sT,
+                                                const typename S2S::raw* __restrict__ s2T, int smoke_on,
+                                                unsigned char* __restrict__ raw, int bx, int by) {
+    Cell c;
+    if (!cell_of(g, c)) return;
+    const float4 s = Codec<SS>::ld(sT, c.i);
+    bool on = s.x > 0.05f * kActiveMargin || s.z > 1e-4f * kActiveMargin || s.w > 1e-3f * kActiveMargin;
+    if (smoke_on) on = on || Codec<S2S>::ld(s2T, c.i).z > 1e-4f * kActiveMargin;
+    if (on) raw[((c.z >> 2) * by + (c.y >> 2)) * bx + (c.x >> 2)] = 1;   // every writer stores 1
+}
+__global__ __launch_bounds__(256) void k_dilate(const unsigned char* __restrict__ raw, unsigned char* __restrict__ flags,
+                                                int bx, int by, int bz, int rb) {
+    const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (i >= bx * by * bz) return;
+    const int x = i % bx, y = (i / bx) % by, z = i / (bx * by);
+    unsigned char f = 0;
+    for (int dz = max(z - rb, 0); dz <= min(z + rb, bz - 1); dz++)
+        for (int dy = max(y - rb, 0); dy <= min(y + rb, by - 1); dy++)
+            for (int dx = max(x - rb, 0); dx <= min(x + rb, bx - 1); dx++) f |= raw[(dz * by + dy) * bx + dx];
+    flags[i] = f;
+}
+
 // ---- volume ray-march (src/renderer.js:207-326) -----------------------------------------
 struct VolIn {
     const float4* lut;     // 2 rows x bbN texels: blackbody, soot
@@ -150,6 +183,9 @@ struct VolIn {
     int dw, dh;
     const half4_t* curl;
     int nn;
+    const unsigned char* flags;   // empty-space map (bricks of 4^3 cells), or null
+    int bx, by, bz;
+    float inv_dx;                 // for the map lookup only (the march itself divides by dx)
 };
 
 __device__ __forceinline__ float hg(float c, float g) {   // Henyey-Greenstein phase function
@@ -238,6 +274,15 @@ __global__ __launch_bounds__(256) void k_volume(RenderU R, Dims d, const typenam
         for (int i = 0; i < 1200; i++) {
             if (t > tf || Tr < 0.003f) break;
             const F3 p = vadd(ro, vmul(rd, t));
+            if (in.flags) {   // a sample in an empty brick is a no-op: advance only
+                const int cx = clampi((int)floorf((p.x - bmin.x) * in.inv_dx), 0, in.bx * 4 - 1);
+                const int cy = clampi((int)floorf((p.y - bmin.y) * in.inv_dx), 0, in.by * 4 - 1);
+                const int cz = clampi((int)floorf((p.z - bmin.z) * in.inv_dx), 0, in.bz * 4 - 1);
+                if (!in.flags[((cz >> 2) * in.by + (cy >> 2)) * in.bx + (cx >> 2)]) {
+                    t += ds;
+                    continue;
+                }
+            }
             F3 g = v3((p.x - bmin.x) / dx, (p.y - bmin.y) / dx, (p.z - bmin.z) / dx);
             float mottle = 0.0f;
             if (detail > 0.0f) {   // divergence-free warp of the lookup in two cross-faded phases
@@ -287,6 +332,7 @@ __global__ __launch_bounds__(256) void k_volume(RenderU R, Dims d, const typenam
                 if (sigS * Tr > 1e-6f) {
                     for (int k = 0; k < nlit; k++) {
                         const float4 lp = in.lights[2 * k], lc = in.lights[2 * k + 1];
+                        if (lp.w == 0.0f) continue;   // a dark light (colour 0) adds exactly +0
                         const F3 toL = vsub(v3(lp.x, lp.y, lp.z), p);
                         const float r2 = vdot(toL, toL) + lc.w * 0.6f + 0.004f;
                         const float ph_ = hg(vdot(rd, vmul(toL, 1.0f / sqrtf(r2))), R.f[RP_smokeG]);
@@ -335,10 +381,32 @@ void launch_sharpen(int prec, hipStream_t s, float k, const Dims& g, const void*
     }
 }
 
+void launch_occupancy(int prec, bool sharpened, hipStream_t s, const Dims& g, const void* S_field, const void* S2_field,
+                      bool smoke_on, unsigned char* raw, unsigned char* flags, int bx, int by, int bz, int rb) {
+    const int nb = bx * by * bz;
+    launch_fill_zero(s, raw, (size_t)nb);
+    const int so = smoke_on ? 1 : 0;
+    if (sharpened) {
+        switch (prec) {
+            case FS_PREC_FP32: k_active<StF16, StF32><<<g.nblocks, 256, 0, s>>>(g, (const half4_t*)S_field, (const float4*)S2_field, so, raw, bx, by); break;
+            case FS_PREC_FP16: k_active<StF16, StF16><<<g.nblocks, 256, 0, s>>>(g, (const half4_t*)S_field, (const half4_t*)S2_field, so, raw, bx, by); break;
+            default: k_active<StF16, StBF16><<<g.nblocks, 256, 0, s>>>(g, (const half4_t*)S_field, (const StBF16::raw*)S2_field, so, raw, bx, by); break;
+        }
+    } else {
+        switch (prec) {
+            case FS_PREC_FP32: k_active<StF32, StF32><<<g.nblocks, 256, 0, s>>>(g, (const float4*)S_field, (const float4*)S2_field, so, raw, bx, by); break;
+            case FS_PREC_FP16: k_active<StF16, StF16><<<g.nblocks, 256, 0, s>>>(g, (const half4_t*)S_field, (const half4_t*)S2_field, so, raw, bx, by); break;
+            default: k_active<StBF16, StBF16><<<g.nblocks, 256, 0, s>>>(g, (const StBF16::raw*)S_field, (const StBF16::raw*)S2_field, so, raw, bx, by); break;
+        }
+    }
+    k_dilate<<<(nb + 255) / 256, 256, 0, s>>>(raw, flags, bx, by, bz, rb);
+}
+
 void launch_volume(int prec, bool sharpened, hipStream_t s, const RenderU& R, const Dims& g, const void* S_field,
                    const void* S2_field, const float4* lut, const float4* lights, const float* depth, int dw, int dh,
-                   const half4_t* curl, int nn, half4_t* vol, float2* aux) {
-    const VolIn in{lut, lights, depth, dw, dh, curl, nn};
+                   const half4_t* curl, int nn, const unsigned char* flags, int bx, int by, int bz, float inv_dx,
+                   half4_t* vol, float2* aux) {
+    const VolIn in{lut, lights, depth, dw, dh, curl, nn, flags, bx, by, bz, inv_dx};
     if (sharpened) {
         switch (prec) {
             case FS_PREC_FP32: volume_t<StF16, StF32>(s, R, g, S_field, S2_field, in, vol, aux); break;

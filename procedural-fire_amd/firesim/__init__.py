@@ -179,6 +179,9 @@ def lib():
         "fs_render_read": (ip, [vp, ip, vp, sz]),
         "fs_render_device_ptr": (vp, [vp, ip]),
         "fs_blackbody_lut": (ip, [ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, fp]),
+        "fs_render_profile_enable": (ip, [vp, ip]),
+        "fs_render_empty_skip": (ip, [vp, ip]),
+        "fs_render_timings": (ip, [vp, fp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -725,6 +728,19 @@ class Render:
 
     def frame(self, time, frame, jitter=(0.0, 0.0)):
         _check(lib().fs_render_frame(self._h, float(time), float(frame), float(jitter[0]), float(jitter[1])))
+
+    def profile(self, on=True):
+        _check(lib().fs_render_profile_enable(self._h, int(bool(on))))
+
+    def empty_skip(self, on=True):
+        """Skip march samples that provably contribute nothing (default on; bit-identical results)."""
+        _check(lib().fs_render_empty_skip(self._h, int(bool(on))))
+
+    def timings(self):
+        """Device ms of the last frame's passes: {"lights", "sharpen", "volume"} (sharpen 0 if skipped)."""
+        ms = (ctypes.c_float * 3)()
+        _check(lib().fs_render_timings(self._h, ms))
+        return {"lights": ms[0], "sharpen": ms[1], "volume": ms[2]}
 
     def uniforms(self):
         out = np.zeros(RP_COUNT, np.float32)

@@ -116,3 +116,22 @@ def test_render_sharpen_once_per_state_and_errors():
     small = F.FireSim(res=(16, 8, 16), size=(0.3, 0.15, 0.3), noise_n=16)
     with pytest.raises(F.FsError, match="ny >= 16"):
         F.Render(small, 32, 32)
+
+
+@pytest.mark.parametrize("params", [dict(FINAL), dict(FINAL, cubic=0, flameSharp=0, smokeScat=0.5), dict(FINAL, detailAmt=6.0)],
+                         ids=["final", "trilinear_smoke", "wide_warp"])
+def test_empty_space_skip_is_exact(params):
+    print(f"empty-space skip {params}", flush=True)
+    sim, _ = fire(F.PREC_FP16, steps=90)
+    r = F.Render(sim, width=96, height=128, vol_scale=0.5)
+    r.set(**params)
+    r.set_camera(CAM)
+    out = {}
+    for on in (True, False):
+        r.empty_skip(on)
+        r.frame(sim.time, 3)
+        out[on] = (r.read_vol(), r.read_aux())
+    for a, b in zip(out[True], out[False]):
+        nd, info = bits_equal(a, b)
+        assert nd == 0, info
+    assert out[True][0][..., 3].min() < 1
