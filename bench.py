@@ -595,7 +595,7 @@ def volume_bench(args):
     The march runs with its empty-space map (the default) and, for comparison, without it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import fsoracle as O
-    sim = make_sim(F.PREC_FP16, F.SOLVE_REFINE64, True)
+    sim = make_sim(F.PREC_FP16, F.SOLVE_REFINE64, not args.no_graph)
     sim.step(DT, 1000)
     sim.sync()
     W, H = 1080, 1920

@@ -177,4 +177,57 @@ class Sparks {
   read() { return addon.sparksRead(this.h, this.count); }
 }
 
-module.exports = { FireSim, Sparks, SIM_FIELDS, SIM_DEFAULTS, PRECISION, SOLVE, FIELD, STAGE_LABELS, buildCampfireLogs, addon };
+// Look-at camera (src/camera.js:1-29), for Render.setCamera.
+const vsub = (a, b) => [a[0] - b[0], a[1] - b[1], a[2] - b[2]];
+const vcross = (a, b) => [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]];
+const vnorm = (a) => { const l = Math.hypot(a[0], a[1], a[2]) || 1; return [a[0] / l, a[1] / l, a[2] / l]; };
+function lookAt(pos, target, fovYdeg = 40, roll = 0) {
+  const fwd = vnorm(vsub(target, pos));
+  let right = vnorm(vcross(fwd, [0, 1, 0]));
+  let up = vcross(right, fwd);
+  if (roll) {
+    const c = Math.cos(roll), s = Math.sin(roll);
+    const r2 = right.map((v, i) => v * c + up[i] * s);
+    const u2 = up.map((v, i) => v * c - right[i] * s);
+    right = r2; up = u2;
+  }
+  return { pos, target, fwd, right, up, fovY: (fovYdeg * Math.PI) / 180 };
+}
+function orbit({ yaw = 0, pitch = 0.15, dist = 2.4, target = [0, 0.45, 0], fov = 40 } = {}) {
+  const pos = [target[0] + dist * Math.cos(pitch) * Math.sin(yaw), target[1] + dist * Math.sin(pitch),
+    target[2] + dist * Math.cos(pitch) * Math.cos(yaw)];
+  return lookAt(pos, target, fov);
+}
+
+// The Renderer passes that read a FireSim's S / S2 (src/renderer.js:104-326): fire-light extraction,
+// flame-field prefilter and the volume ray-march.  params / setParams mirror `renderer.params`;
+// frame(time, frame, jitter) is uploadParams (src/renderer.js:634-648) followed by the lights,
+// prefilter and volume passes of renderHDR (src/renderer.js:668-690).  The surface pass, composite
+// and post stay with the caller: pass its depth with setDepth, take vol (L.rgb, Tr) and aux
+// (haze, depth) back with readVol / readAux.
+const RP_COUNT = 88;
+const RENDER_BUF = { vol: 0, aux: 1, lights: 2, sharp: 3, lut: 4 };
+class Render {
+  constructor(sim, { width = 960, height = 1080, volScale = 0.5 } = {}) {
+    this.sim = sim;              // kept alive while the renderer is
+    this.h = addon.renderCreate(sim.h, width, height, volScale);
+    [this.vw, this.vh] = addon.renderSize(this.h);
+    this._params = {};
+  }
+  get params() { return { ...this._params }; }
+  setParams(p) {
+    for (const [k, v] of Object.entries(p)) { addon.renderSet(this.h, k, v); this._params[k] = v; }
+  }
+  setCamera(cam) { addon.renderSetCamera(this.h, cam); }                  // src/renderer.js:632
+  setWhiteBalance(twb) { addon.renderSetWhiteBalance(this.h, twb); }      // src/renderer.js:57-63
+  setDepth(depth, dw, dh) { addon.renderSetDepth(this.h, depth || null, dw || 0, dh || 0); }
+  frame(time, frame, jitter = [0, 0]) { addon.renderFrame(this.h, time, frame, jitter[0], jitter[1]); }
+  uniforms() { return addon.renderUniforms(this.h); }
+  readVol() { return addon.renderRead(this.h, RENDER_BUF.vol, this.vw * this.vh * 4); }
+  readAux() { return addon.renderRead(this.h, RENDER_BUF.aux, this.vw * this.vh * 2); }
+  readLights() { return addon.renderRead(this.h, RENDER_BUF.lights, 64); }
+  readSharp() { const [x, y, z] = this.sim.res; return addon.renderRead(this.h, RENDER_BUF.sharp, x * y * z * 4); }
+}
+
+module.exports = { FireSim, Sparks, Render, lookAt, orbit, RP_COUNT, SIM_FIELDS, SIM_DEFAULTS, PRECISION, SOLVE, FIELD,
+  STAGE_LABELS, buildCampfireLogs, addon };

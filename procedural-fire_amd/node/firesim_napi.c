@@ -754,6 +754,203 @@ static napi_value js_sparks_read(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+/* ---- render-volume consumers (src/renderer.js:104-326) -------------------------- */
+static void finalize_render(napi_env env, void* data, void* hint) {
+    (void)env; (void)hint;
+    fs_render_destroy((fs_render*)data);
+}
+static fs_render* get_render(napi_env env, napi_value v) {
+    void* p = NULL;
+    if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+        napi_throw_type_error(env, NULL, "expected a Render handle");
+        return NULL;
+    }
+    return (fs_render*)p;
+}
+/* renderCreate(h, width, height, volScale) */
+static napi_value js_render_create(napi_env env, napi_callback_info info) {
+    size_t argc = 4;
+    napi_value argv[4];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_sim* h = get_sim(env, argv[0]);
+    if (!h) return NULL;
+    double w = 960, ht = 1080, vs = 0.5;
+    if (argc >= 2) napi_get_value_double(env, argv[1], &w);
+    if (argc >= 3) napi_get_value_double(env, argv[2], &ht);
+    if (argc >= 4) napi_get_value_double(env, argv[3], &vs);
+    fs_render* r = NULL;
+    int rc = fs_render_create(h, (int)w, (int)ht, vs, &r);
+    if (rc) return throw_fs(env, rc);
+    napi_value ext;
+    CHECK_NAPI(env, napi_create_external(env, r, finalize_render, NULL, &ext));
+    return ext;
+}
+/* renderSet(r, name, value) */
+static napi_value js_render_set(napi_env env, napi_callback_info info) {
+    size_t argc = 3;
+    napi_value argv[3];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_render* r = get_render(env, argv[0]);
+    if (!r) return NULL;
+    char name[64];
+    size_t len = 0;
+    if (napi_get_value_string_utf8(env, argv[1], name, sizeof name, &len) != napi_ok) {
+        napi_throw_type_error(env, NULL, "expected a parameter name");
+        return NULL;
+    }
+    double v = 0;
+    napi_get_value_double(env, argv[2], &v);
+    int rc = fs_render_set(r, name, (float)v);
+    if (rc) return throw_fs(env, rc);
+    return NULL;
+}
+/* renderSetCamera(r, {pos, fwd, right, up, fovY}) (src/camera.js lookAt) */
+static napi_value js_render_set_camera(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_render* r = get_render(env, argv[0]);
+    if (!r) return NULL;
+    double pos[3], fwd[3], rgt[3], up[3], fov = 0;
+    if (argc < 2 || !get_prop_array3(env, argv[1], "pos", pos) || !get_prop_array3(env, argv[1], "fwd", fwd) ||
+        !get_prop_array3(env, argv[1], "right", rgt) || !get_prop_array3(env, argv[1], "up", up) ||
+        !get_prop_num(env, argv[1], "fovY", &fov)) {
+        napi_throw_type_error(env, NULL, "expected a camera {pos, fwd, right, up, fovY}");
+        return NULL;
+    }
+    int rc = fs_render_set_camera(r, pos, fwd, rgt, up, fov);
+    if (rc) return throw_fs(env, rc);
+    return NULL;
+}
+/* renderSetWhiteBalance(r, twb) */
+static napi_value js_render_set_wb(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_render* r = get_render(env, argv[0]);
+    if (!r) return NULL;
+    double t = 5000;
+    napi_get_value_double(env, argv[1], &t);
+    int rc = fs_render_set_white_balance(r, t);
+    if (rc) return throw_fs(env, rc);
+    return NULL;
+}
+/* renderSetDepth(r, Float32Array | null, dw, dh) */
+static napi_value js_render_set_depth(napi_env env, napi_callback_info info) {
+    size_t argc = 4;
+    napi_value argv[4];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_render* r = get_render(env, argv[0]);
+    if (!r) return NULL;
+    napi_valuetype t = napi_null;
+    if (argc >= 2) napi_typeof(env, argv[1], &t);
+    if (argc < 2 || t == napi_null || t == napi_undefined) {
+        int rc = fs_render_set_depth(r, NULL, 0, 0);
+        if (rc) return throw_fs(env, rc);
+        return NULL;
+    }
+    napi_typedarray_type ty;
+    size_t n = 0;
+    void* data = NULL;
+    if (napi_get_typedarray_info(env, argv[1], &ty, &n, &data, NULL, NULL) != napi_ok || ty != napi_float32_array) {
+        napi_throw_type_error(env, NULL, "depth must be a Float32Array");
+        return NULL;
+    }
+    double dw = 0, dh = 0;
+    if (argc >= 4) { napi_get_value_double(env, argv[2], &dw); napi_get_value_double(env, argv[3], &dh); }
+    if ((size_t)dw * (size_t)dh != n) {
+        napi_throw_range_error(env, NULL, "depth length must be dw * dh");
+        return NULL;
+    }
+    int rc = fs_render_set_depth(r, (const float*)data, (int)dw, (int)dh);
+    if (rc) return throw_fs(env, rc);
+    return NULL;
+}
+/* renderFrame(r, time, frame, jitX, jitY) */
+static napi_value js_render_frame(napi_env env, napi_callback_info info) {
+    size_t argc = 5;
+    napi_value argv[5];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_render* r = get_render(env, argv[0]);
+    if (!r) return NULL;
+    double a[4] = {0, 0, 0, 0};
+    for (size_t k = 1; k < argc && k < 5; k++) napi_get_value_double(env, argv[k], &a[k - 1]);
+    int rc = fs_render_frame(r, a[0], a[1], (float)a[2], (float)a[3]);
+    if (rc) return throw_fs(env, rc);
+    return NULL;
+}
+/* renderSize(r) -> [vw, vh] */
+static napi_value js_render_size(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_render* r = get_render(env, argv[0]);
+    if (!r) return NULL;
+    int vw = 0, vh = 0;
+    int rc = fs_render_size(r, &vw, &vh);
+    if (rc) return throw_fs(env, rc);
+    napi_value arr, a, b;
+    CHECK_NAPI(env, napi_create_array_with_length(env, 2, &arr));
+    napi_create_int32(env, vw, &a);
+    napi_create_int32(env, vh, &b);
+    napi_set_element(env, arr, 0, a);
+    napi_set_element(env, arr, 1, b);
+    return arr;
+}
+/* renderRead(r, what, nFloats) -> Float32Array; renderUniforms(r) -> Float32Array(88) */
+static napi_value js_render_read(napi_env env, napi_callback_info info) {
+    size_t argc = 3;
+    napi_value argv[3];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_render* r = get_render(env, argv[0]);
+    if (!r) return NULL;
+    double what = 0, nf = 0;
+    napi_get_value_double(env, argv[1], &what);
+    napi_get_value_double(env, argv[2], &nf);
+    const size_t n = (size_t)nf;
+    void* data = NULL;
+    napi_value ab, arr;
+    CHECK_NAPI(env, napi_create_arraybuffer(env, n * 4, &data, &ab));
+    int rc = fs_render_read(r, (int)what, (float*)data, n * 4);
+    if (rc) return throw_fs(env, rc);
+    CHECK_NAPI(env, napi_create_typedarray(env, napi_float32_array, n, ab, 0, &arr));
+    return arr;
+}
+static napi_value js_render_uniforms(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    fs_render* r = get_render(env, argv[0]);
+    if (!r) return NULL;
+    float u[FS_RP_COUNT];
+    int rc = fs_render_get_uniforms(r, u);
+    if (rc) return throw_fs(env, rc);
+    return float32_array(env, u, FS_RP_COUNT);
+}
+/* renderParamDefaults() -> Float32Array(88), renderParamIndex(name) */
+static napi_value js_render_param_defaults(napi_env env, napi_callback_info info) {
+    (void)info;
+    float u[FS_RP_COUNT];
+    fs_render_param_defaults(u);
+    return float32_array(env, u, FS_RP_COUNT);
+}
+/* blackbodyLut(n, tmin, tmax, twb) -> Float32Array(2 n 4) (src/blackbody.js:78-88) */
+static napi_value js_blackbody_lut(napi_env env, napi_callback_info info) {
+    size_t argc = 4;
+    napi_value argv[4];
+    CHECK_NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    double a[4] = {1024, 400, 3000, 5000};
+    for (size_t k = 0; k < argc && k < 4; k++) napi_get_value_double(env, argv[k], &a[k]);
+    const size_t n = (size_t)a[0] * 8;
+    void* data = NULL;
+    napi_value ab, arr;
+    CHECK_NAPI(env, napi_create_arraybuffer(env, n * 4, &data, &ab));
+    int rc = fs_blackbody_lut((int)a[0], a[1], a[2], a[3], (float*)data);
+    if (rc) return throw_fs(env, rc);
+    CHECK_NAPI(env, napi_create_typedarray(env, napi_float32_array, n, ab, 0, &arr));
+    return arr;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
     (void)info;
     napi_value s;
@@ -800,6 +997,17 @@ static napi_value init(napi_env env, napi_value exports) {
     EXPORT("sparksBurst", js_sparks_burst);
     EXPORT("sparksStep", js_sparks_step);
     EXPORT("sparksRead", js_sparks_read);
+    EXPORT("renderCreate", js_render_create);
+    EXPORT("renderSet", js_render_set);
+    EXPORT("renderSetCamera", js_render_set_camera);
+    EXPORT("renderSetWhiteBalance", js_render_set_wb);
+    EXPORT("renderSetDepth", js_render_set_depth);
+    EXPORT("renderFrame", js_render_frame);
+    EXPORT("renderSize", js_render_size);
+    EXPORT("renderRead", js_render_read);
+    EXPORT("renderUniforms", js_render_uniforms);
+    EXPORT("renderParamDefaults", js_render_param_defaults);
+    EXPORT("blackbodyLut", js_blackbody_lut);
     return exports;
 }
 

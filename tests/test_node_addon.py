@@ -117,3 +117,45 @@ def test_node_sparks_match_the_c_abi_bit_for_bit(tmp_path):
         sp.step(1 / 90)
     a = np.fromfile(tmp_path / "sparks.bin", np.float32).reshape(1024, 16)
     assert np.array_equal(a.view(np.uint32), sp.read().view(np.uint32))
+
+
+@pytest.mark.gpu
+@needs_node
+def test_node_render_matches_the_c_abi_bit_for_bit(tmp_path):
+    """The Node Render class (params, orbit camera, depth, jitter, two frames) gives the C ABI's images."""
+    print("node render 32x48x32", flush=True)
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "run_render.js"), str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    meta = json.loads((tmp_path / "meta.json").read_text())
+    assert meta["threwOnUnknown"] and (meta["vw"], meta["vh"]) == (48, 64)
+    logs, _ = F.build_campfire(7)
+    sim = F.FireSim(res=(32, 48, 32), size=(0.6, 0.9, 0.6), logs=logs, noise_n=32)
+    sim.step(1 / 90, 40)
+    rd = F.Render(sim, 96, 128, 0.5)
+    rd.set(cubic=1, flameSharp=1, detailAmt=0.3, sootEdge=0.12, lightK=0.6, focus=2.2, lensX=0.01)
+    rd.set_camera(F.orbit(yaw=0.35, pitch=0.12, dist=2.2, target=(0, 0.5, 0), fov=38))
+    depth = (1.8 + (np.arange(600) % 7) * 0.1).astype(np.float32).reshape(30, 20)
+    rd.set_depth(depth)
+    for f in range(2):
+        rd.frame(sim.time, 5 + f, (0.25, -0.125))
+        sim.step(1 / 90)
+    for name, want in (("vol", rd.read_vol()), ("aux", rd.read_aux()), ("lights", rd.read_lights())):
+        a = np.fromfile(tmp_path / f"{name}.bin", np.float32).reshape(want.shape)
+        assert np.array_equal(a.view(np.uint32), want.view(np.uint32)), name
+    assert rd.read_vol()[..., 3].min() < 1
+
+
+@needs_node
+def test_node_camera_and_lut_helpers():
+    r = _node(f"const m = require({JS!r}); const c = m.orbit({{yaw: 0.35, pitch: 0.12, dist: 2.2, target: [0, 0.5, 0], "
+              f"fov: 38}}); console.log(JSON.stringify({{c, lut: Array.from(m.addon.blackbodyLut(16, 500, 2500, 4000)), "
+              f"d: Array.from(m.addon.renderParamDefaults())}}))")
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    c = F.orbit(yaw=0.35, pitch=0.12, dist=2.2, target=(0, 0.5, 0), fov=38)
+    for k in ("pos", "fwd", "right", "up"):
+        np.testing.assert_allclose(d["c"][k], c[k], rtol=0, atol=1e-15)
+    assert abs(d["c"]["fovY"] - c["fovY"]) < 1e-15
+    np.testing.assert_array_equal(np.array(d["lut"], np.float32), F.blackbody_lut(16, 500, 2500, 4000).ravel())
+    np.testing.assert_array_equal(np.array(d["d"], np.float32), F.render_param_defaults())
