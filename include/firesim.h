@@ -371,7 +371,8 @@ void* fs_sparks_device_ptr(fs_sparks* sp);
  * depth comes in through fs_render_set_depth (without one, every pixel sees the
  * surface pass's sky depth, 1e6 m), and the volume (L, Tr) and aux (haze, depth)
  * images go out through fs_render_read / fs_render_device_ptr.
- * A frame runs on the sim's stream and reads the sim's current S and S2. */
+ * A frame runs on the sim's stream and reads the sim's current S and S2.  Every call but
+ * fs_render_destroy needs the sim alive; fs_render_destroy touches only the renderer's memory. */
 #define FS_RP_COUNT 88                   /* RP_FIELDS entries (src/renderer.js:8-17) */
 #define FS_RENDER_LIGHTS 8               /* extracted point lights (src/renderer.js:27) */
 enum { FS_RENDER_VOL = 0,                /* vw*vh x (L.rgb, Tr), binary16 on the device, read as float */

@@ -42,7 +42,7 @@ def fire(prec, steps=60):
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("prec", [F.PREC_FP32, F.PREC_FP16], ids=["fp32", "fp16"])
+@pytest.mark.parametrize("prec", [F.PREC_FP32, F.PREC_FP16, F.PREC_BF16], ids=["fp32", "fp16", "bf16"])
 def test_render_bit_exact_with_oracle(prec, case):
     c = CASES[case]
     print(f"render parity prec {prec} case {case}", flush=True)
@@ -135,3 +135,39 @@ def test_empty_space_skip_is_exact(params):
         nd, info = bits_equal(a, b)
         assert nd == 0, info
     assert out[True][0][..., 3].min() < 1
+
+
+EDGE_CAMS = {
+    "inside_box": F.look_at([0.05, 0.3, 0.02], [0.0, 0.5, 0.0], 70.0),        # rays start inside the volume
+    "looking_away": F.look_at([0.0, 0.4, 2.0], [0.0, 0.4, 5.0], 38.0),        # every ray misses the box
+    "straight_down": F.look_at([0.001, 2.5, 0.0], [0.0, 0.0, 0.0], 30.0),     # near-vertical rays
+}
+
+
+@pytest.mark.parametrize("cam", list(EDGE_CAMS))
+def test_render_edge_cases_bit_exact(cam):
+    """Odd volume sizes (vol_scale 0.37 of 101x77), a zero-depth surface patch, and cameras inside the box,
+    facing away from it and looking straight down."""
+    print(f"render edge case {cam}", flush=True)
+    sim, _ = fire(F.PREC_FP16, steps=60)
+    r = F.Render(sim, width=101, height=77, vol_scale=0.37)
+    assert (r.vw, r.vh) == (37, 28)
+    r.set(**FINAL)
+    r.set_camera(EDGE_CAMS[cam])
+    depth = np.full((9, 11), 1e6, np.float32)
+    depth[3:6, 4:7] = 0.0                                  # a surface at the eye: no march there
+    r.set_depth(depth)
+    r.frame(sim.time, 63, (0.5, -0.5))
+    U = r.uniforms()
+    S, S2 = sim.read(F.S, np.float32), sim.read(F.S2, np.float32)
+    lut = r.read_lut()
+    L = r.read_lights()
+    field = O.render_sharpen(U[RP["flameSharp"]], S)
+    vol_o, aux_o = O.render_volume(U, field, S2, lut, L, sim.read(F.CURL, np.float32), depth)
+    vol_g, aux_g = r.read_vol(), r.read_aux()
+    for a, b in ((vol_g, vol_o), (aux_g, aux_o)):
+        nd, info = bits_equal(a, b)
+        assert nd == 0, info
+    if cam == "looking_away":
+        assert np.all(vol_g[..., 3] == 1) and np.all(vol_g[..., :3] == 0)
+    assert np.all(np.isfinite(vol_g)) and np.all(np.isfinite(aux_g))
