@@ -41,6 +41,10 @@ extern "C" {
 /* ---- pressure solve ------------------------------------------------------- */
 #define FS_SOLVE_VCYCLES    0  /* `vcycles` V(2,2) cycles on p, warm-started (src/sim.js:642) */
 #define FS_SOLVE_REFINE64   1  /* FP32 V-cycles inside FP64 defect correction to div_tol  */
+#define FS_SOLVE_SPECTRAL64 3  /* build extension: the exact separable inner solve (DST-II / DCT-IV
+                                  transforms as FP32 matrix-core GEMMs, one division by the
+                                  eigenvalues) inside the FP64 defect correction to div_tol, in place
+                                  of the V-cycles; nx even; max_vcycles caps the corrections */
 #define FS_SOLVE_TOL32      2  /* FP32 V-cycles on p until the FP64-evaluated ||b - A p||/||b||
                                   <= div_tol or max_vcycles (no defect correction)        */
 
@@ -411,6 +415,10 @@ int  fs_render_profile_enable(fs_render* r, int on);
 int  fs_render_timings(fs_render* r, float ms[3]);
 int  fs_render_read(fs_render* r, int what, float* host, size_t bytes);
 void* fs_render_device_ptr(fs_render* r, int what);
+/* Bases of the exact pressure solve: kind 0 = DST-II (Dirichlet faces at both ends), kind 1 =
+ * DCT-IV (Neumann low end, Dirichlet high end).  T is n x n row-major (row k = mode k, orthonormal),
+ * lam[k] the eigenvalue of the 1D second difference with that boundary rule (h = 1).  Host only. */
+int  fs_spectral_basis(int n, int kind, double* T, double* lam);
 /* blackbodyLUT({n, Tmin, Tmax, Twb}) (src/blackbody.js:78-88): 2 x n x 4 floats. */
 int  fs_blackbody_lut(int n, double tmin, double tmax, double twb, float* out);
 

@@ -118,21 +118,30 @@ struct fs_sim {
         F.face_flux = face_d;
         return F;
     }
-    bool refine() const { return cfg.solve == FS_SOLVE_REFINE64; }
+    Spectral* spec = nullptr;        // FS_SOLVE_SPECTRAL64: the exact separable inner solve
+    bool refine() const { return cfg.solve == FS_SOLVE_REFINE64 || cfg.solve == FS_SOLVE_SPECTRAL64; }
     bool tracked() const { return cfg.solve != FS_SOLVE_VCYCLES; }
     bool adaptive() const { return tracked() && cfg.adaptive; }
     void solve_begin(hipStream_t s) {
-        if (cfg.solve == FS_SOLVE_REFINE64) mg_refine_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
+        if (cfg.solve == FS_SOLVE_SPECTRAL64) mg_spectral_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
+        else if (cfg.solve == FS_SOLVE_REFINE64) mg_refine_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
         else mg_tol32_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
     }
-    void solve_iter(hipStream_t s) {
+    int solve_iter(hipStream_t s) {
+        if (cfg.solve == FS_SOLVE_SPECTRAL64) return mg_spectral_iter(mg, spec, s, cfg.div_tol, cfg.max_vcycles);
         if (cfg.solve == FS_SOLVE_REFINE64) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
         else mg_tol32_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
+        return FS_OK;
     }
-    void solve(hipStream_t s) {   // the pressure solve of one step (sim.js:642)
-        if (cfg.solve == FS_SOLVE_REFINE64) mg_refine(mg, s, cfg.div_tol, cfg.max_vcycles);
+    int solve(hipStream_t s) {   // the pressure solve of one step (sim.js:642)
+        if (cfg.solve == FS_SOLVE_SPECTRAL64) {
+            mg_spectral_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
+            for (int k = 0; k < std::min(cfg.max_vcycles, MG_MAX_REFINE); k++)
+                if (int rc = mg_spectral_iter(mg, spec, s, cfg.div_tol, cfg.max_vcycles)) return rc;
+        } else if (cfg.solve == FS_SOLVE_REFINE64) mg_refine(mg, s, cfg.div_tol, cfg.max_vcycles);
         else if (cfg.solve == FS_SOLVE_TOL32) mg_solve_tol32(mg, s, cfg.div_tol, cfg.max_vcycles);
         else for (int i = 0; i < cfg.vcycles; i++) mg_vcycle(mg, s, nullptr);
+        return FS_OK;
     }
 };
 
@@ -198,22 +207,29 @@ void enqueue_post(fs_sim* h, bool prof) {
                       h->ring_count_d);
     }
 }
-void enqueue_step(fs_sim* h, bool prof) {
+int enqueue_step(fs_sim* h, bool prof) {
     enqueue_pre(h, prof, false);
-    h->solve(h->stream);
+    if (int rc = h->solve(h->stream)) return rc;
     enqueue_post(h, prof);
+    return FS_OK;
 }
 
 int capture(fs_sim* h, int which) {
     if (h->gexec[which]) return FS_OK;
     FS_HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = FS_OK;
     switch (which) {
-        case fs_sim::G_FULL: enqueue_step(h, false); break;
+        case fs_sim::G_FULL: rc = enqueue_step(h, false); break;
         case fs_sim::G_PRE: enqueue_pre(h, false, true); break;
-        case fs_sim::G_ITER: h->solve_iter(h->stream); break;
+        case fs_sim::G_ITER: rc = h->solve_iter(h->stream); break;
         case fs_sim::G_POST: enqueue_post(h, false); break;
     }
     hipGraph_t gr = nullptr;
+    if (rc) {   // end the capture before reporting the failure
+        (void)hipStreamEndCapture(h->stream, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+    }
     FS_HIP_CHECK(hipStreamEndCapture(h->stream, &gr));
     h->graph[which] = gr;
     FS_HIP_CHECK(hipGraphInstantiate(&h->gexec[which], gr, nullptr, nullptr, 0));
@@ -227,9 +243,9 @@ int run_part(fs_sim* h, int which, bool prof) {
         return FS_OK;
     }
     switch (which) {
-        case fs_sim::G_FULL: enqueue_step(h, prof); break;
+        case fs_sim::G_FULL: return enqueue_step(h, prof);
         case fs_sim::G_PRE: enqueue_pre(h, prof, true); break;
-        case fs_sim::G_ITER: h->solve_iter(h->stream); break;
+        case fs_sim::G_ITER: return h->solve_iter(h->stream);
         case fs_sim::G_POST: enqueue_post(h, prof); break;
     }
     return FS_OK;
@@ -277,6 +293,7 @@ void free_sim(fs_sim* h) {
     if (h->ring_d) (void)hipFree(h->ring_d);
     if (h->ring_count_d) (void)hipFree(h->ring_count_d);
     mg_free(h->mg);
+    spectral_free(h->spec);
     for (auto& e : h->ev) if (e) (void)hipEventDestroy(e);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -409,7 +426,8 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     if ((long long)nx * ny * nz * 16 > (1LL << 31) * 8LL) return fail(FS_EINVAL, "grid too large");
     if ((long long)nx * ny * nz >= (1LL << 31) / 4) return fail(FS_EINVAL, "grid exceeds 32-bit texel indexing");
     if (cfg->precision < FS_PREC_FP32 || cfg->precision > FS_PREC_BF16) return fail(FS_EINVAL, "bad precision");
-    if (cfg->solve < FS_SOLVE_VCYCLES || cfg->solve > FS_SOLVE_TOL32) return fail(FS_EINVAL, "bad solve mode");
+    if (cfg->solve < FS_SOLVE_VCYCLES || cfg->solve > FS_SOLVE_SPECTRAL64) return fail(FS_EINVAL, "bad solve mode");
+    if (cfg->solve == FS_SOLVE_SPECTRAL64 && nx % 2 != 0) return fail(FS_EINVAL, "FS_SOLVE_SPECTRAL64 needs an even nx");
     if (!(cfg->omega == 0.f || (cfg->omega > 0.f && cfg->omega < 2.f))) return fail(FS_EINVAL, "omega must be 0 or in (0, 2)");
     if (cfg->pre_sweeps < 0 || cfg->pre_sweeps > 8 || cfg->post_sweeps < 0 || cfg->post_sweeps > 8)
         return fail(FS_EINVAL, "pre/post sweeps must be in 0..8");
@@ -456,6 +474,10 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
         h->mg.omega = cfg->omega > 0.f ? cfg->omega : 1.25f;
         h->mg.pre = cfg->pre_sweeps > 0 ? cfg->pre_sweeps : 1;
         h->mg.post = cfg->post_sweeps > 0 ? cfg->post_sweeps : 2;
+    }
+    if (cfg->solve == FS_SOLVE_SPECTRAL64) {
+        rc = spectral_build(&h->spec, nx, ny, nz, (double)h->mg.h[0], h->stream, h->mg.b[0], h->mg.p[0], err);
+        if (rc) return bail(fail(rc, err));
     }
     for (auto& e : h->ev) if (hipEventCreate(&e) != hipSuccess) return bail(fail(FS_EDEVICE, "event create failed"));
     if (hipHostMalloc((void**)&h->host_state, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
@@ -634,7 +656,7 @@ int fs_run_stage(fs_sim* h, int stage, double dt) {
         case FS_STAGE_CURL: launch_curl(h->prec, h->g, s, F); break;
         case FS_STAGE_FORCES: launch_forces(h->prec, h->g, s, h->P_d, F); break;
         case FS_STAGE_DIV: launch_div(h->prec, h->g, s, h->P_d, F, h->refine() ? h->mg.bkeep : h->mg.b[0], 0); break;
-        case FS_STAGE_MG: h->solve(s); break;
+        case FS_STAGE_MG: if (int rc = h->solve(s)) return rc; break;
         case FS_STAGE_PROJECT:
             launch_project(h->prec, h->g, s, h->P_d, F, h->refine() ? nullptr : h->mg.p[0], h->refine() ? h->mg.x64 : nullptr);
             break;
@@ -1224,7 +1246,7 @@ int adj_rec_alloc(fs_adjoint* a, StepRec& r) {
 }
 int adj_check(const fs_sim* h) {
     if (h->prec != FS_PREC_FP32) return fail(FS_ESTATE, "the adjoint needs FS_PREC_FP32 storage");
-    if (!h->refine()) return fail(FS_ESTATE, "the adjoint needs the converged solve FS_SOLVE_REFINE64");
+    if (h->cfg.solve != FS_SOLVE_REFINE64) return fail(FS_ESTATE, "the adjoint needs the converged solve FS_SOLVE_REFINE64");
     if (h->prm[P_visc] != 0.f) return fail(FS_ESTATE, "the adjoint needs visc = 0");
     return FS_OK;
 }

@@ -121,6 +121,18 @@ void mg_tol32_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 void mg_tol32_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 // relative FP64 residual of the FP32 level-0 p against level-0 b -> st->rel[0]
 void mg_residual_of_p(GpuMG& mg, hipStream_t s);
+// ---- exact separable inner solve (fs_spectral.hip) ----------------------------------
+struct Spectral;
+// transform bases, eigenvalues and a rocBLAS handle for an nx x ny x nz level 0 of cell size h
+// (nx even); runs one eager solve on scratch_b / scratch_p (level-0 sized) to load the library
+int  spectral_build(Spectral** out, int nx, int ny, int nz, double h, hipStream_t s, float* scratch_b, float* scratch_p,
+                    std::string& err);
+void spectral_free(Spectral* sp);
+// p = A^-1 b on the parity-split level-0 layout (FP32 transforms; b is kept)
+int  spectral_solve(Spectral* sp, hipStream_t s, float* b, float* p);
+// FP64 defect correction with the spectral inner solve: begin = residual of x, iter = one correction
+void mg_spectral_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
+int  mg_spectral_iter(GpuMG& mg, Spectral* sp, hipStream_t s, double tol, int max_cycles);
 // single-operator launches (tests)
 void mg_smooth(GpuMG& mg, hipStream_t s, int level, int iters, const int* skip, double* xupd = nullptr,
                bool first_red_done = false);

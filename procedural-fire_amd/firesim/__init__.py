@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libfiresim.so")
 
 PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
-SOLVE_VCYCLES, SOLVE_REFINE64, SOLVE_TOL32 = 0, 1, 2
+SOLVE_VCYCLES, SOLVE_REFINE64, SOLVE_TOL32, SOLVE_SPECTRAL64 = 0, 1, 2, 3
 (VEL, S, S2, VORT, GEO, VELX, SX, S2X, VELY, SY, S2Y, P, B, NOISE, CURL) = range(15)
 FIELD_NAMES = {"vel": VEL, "S": S, "S2": S2, "vort": VORT, "geo": GEO, "velX": VELX, "SX": SX, "S2X": S2X,
                "velY": VELY, "SY": SY, "S2Y": S2Y, "p": P, "b": B, "noise": NOISE, "curl": CURL}
@@ -179,6 +179,7 @@ def lib():
         "fs_render_read": (ip, [vp, ip, vp, sz]),
         "fs_render_device_ptr": (vp, [vp, ip]),
         "fs_blackbody_lut": (ip, [ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, fp]),
+        "fs_spectral_basis": (ip, [ip, ip, dp, dp]),
         "fs_render_profile_enable": (ip, [vp, ip]),
         "fs_render_empty_skip": (ip, [vp, ip]),
         "fs_render_timings": (ip, [vp, fp]),
@@ -629,6 +630,17 @@ class Sparks:
         t = ctypes.c_double()
         _check(lib().fs_sparks_clock(self._h, ctypes.byref(t), None))
         return t.value
+
+
+def spectral_basis(n, kind):
+    """Bases of the exact pressure solve (host only): kind 0 = DST-II (Dirichlet faces at both ends),
+    1 = DCT-IV (Neumann low end, Dirichlet high end).  Returns (T, lam): T[k] the orthonormal mode k,
+    lam[k] the eigenvalue of the 1D second difference with that boundary rule (h = 1)."""
+    T = np.zeros((n, n), np.float64)
+    lam = np.zeros(n, np.float64)
+    _check(lib().fs_spectral_basis(int(n), int(kind), T.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   lam.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return T, lam
 
 
 RP_COUNT = 88

@@ -23,7 +23,7 @@ const SIM_DEFAULTS = (() => {                                 // src/sim.js:25-4
 })();
 
 const PRECISION = { fp32: 0, fp16: 1, bf16: 2 };
-const SOLVE = { vcycles: 0, refine64: 1, tol32: 2 };
+const SOLVE = { vcycles: 0, refine64: 1, tol32: 2, spectral64: 3 };
 const FIELD = { vel: 0, S: 1, S2: 2, vort: 3, geo: 4, velX: 5, SX: 6, S2X: 7, velY: 8, SY: 9, S2Y: 10,
   p: 11, b: 12, noise: 13, curl: 14 };
 const STAGE_LABELS = ['adv1', 'adv2', 'curl', 'forces', 'div', 'mg', 'project'];

@@ -5,6 +5,7 @@ pressure solve converged to ||b - A p|| / ||b|| <= 1e-10; it starts from the GPU
 state (the spot-fed 1 cm scene after a warm-up), so the comparison isolates what the
 GPU's precision choices do over 1 and 10 steps:
   * mixed: fp16 storage, f32 math, FP32 V-cycles in FP64 defect correction to 1e-5;
+  * mixed_spectral: the same with the exact separable inner solve (FS_SOLVE_SPECTRAL64);
   * plain FP32: float4 storage, the reference's 2 x V(2,2) per step.
 Bar for the mixed step: temperature relative L2 <= 1e-3 after 1 and after 10 steps,
 and the divergence residual <= 1e-5 every step.
@@ -45,14 +46,14 @@ def _fp64_oracle(sc, res):
     return orc
 
 
-@pytest.mark.parametrize("form", ["mixed", "fp32"])
+@pytest.mark.parametrize("form", ["mixed", "mixed_spectral", "fp32"])
 def test_step_error_against_fp64_oracle(form):
     print(f"fp64 error {NAME} {form}", flush=True)
     sc = scenes.SCENES[NAME]
     res = sc["res"]
-    if form == "mixed":
-        gpu = scenes.make_scene(NAME, precision=F.PREC_FP16, solve=F.SOLVE_REFINE64, div_tol=1e-5, max_vcycles=12,
-                                noise_n=32)
+    if form in ("mixed", "mixed_spectral"):
+        solve = F.SOLVE_REFINE64 if form == "mixed" else F.SOLVE_SPECTRAL64
+        gpu = scenes.make_scene(NAME, precision=F.PREC_FP16, solve=solve, div_tol=1e-5, max_vcycles=12, noise_n=32)
     else:
         gpu = scenes.make_scene(NAME, precision=F.PREC_FP32, solve=F.SOLVE_VCYCLES, noise_n=32)
     gpu.step(sc["dt"], 40)
@@ -64,14 +65,14 @@ def test_step_error_against_fp64_oracle(form):
         for _ in range(n):
             gpu.step(sc["dt"], 1)
             orc.step(sc["dt"], 1)
-            if form == "mixed":
+            if form != "fp32":
                 rel, used = gpu.residual()
                 assert rel <= 1e-5, (rel, used)
         gpu.sync()
         errs[n_done] = _errors(gpu, orc)
     print(form, {k: {f: f"{v:.2e}" for f, v in e.items()} for k, e in errs.items()}, flush=True)
     assert _temperature(orc.read(O.S)).max() > 600.0          # a burning state
-    if form == "mixed":
+    if form != "fp32":
         assert errs[1]["T"] <= 1e-3 and errs[10]["T"] <= 1e-3, errs
     else:
         assert errs[1]["T"] <= 1e-2, errs                       # reported; the reference's solve is not converged
