@@ -74,7 +74,10 @@ def measured_traffic(kernel):
 
 
 SCENE = "spots8_128"
-SWEEPS = (0, 0)        # red-black sweeps before / after the coarse correction in the tolerance solves (0: default)
+SWEEPS = (0, 0)
+# the step forms: storage precision and pressure solve
+FORMS = {"fp32": (F.PREC_FP32, F.SOLVE_VCYCLES), "fp32_tol": (F.PREC_FP32, F.SOLVE_TOL32),
+         "mixed_spectral": (F.PREC_FP16, F.SOLVE_SPECTRAL64), "mixed": (F.PREC_FP16, F.SOLVE_REFINE64)}        # red-black sweeps before / after the coarse correction in the tolerance solves (0: default)
 
 
 def make_sim(precision, solve, use_graph=True):
@@ -724,6 +727,8 @@ def main():
                     help="instead of the step: the render-volume consumers (lights, prefilter, ray-march) of a 1080x1920 frame")
     ap.add_argument("--timing-only", action="store_true",
                     help="with --adjoint: forward, backward and tangent timings and the dot-product test only")
+    ap.add_argument("--trace-form", default=None, choices=list(FORMS),
+                    help="short run for kernel traces: --warmup then --steps steps of one step form, nothing else")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: a 256 MiB copy, then 3 steps of each step form")
     args = ap.parse_args()
@@ -758,6 +763,17 @@ def main():
         return
     ncell = float(np.prod(RES))
     graph = not args.no_graph
+    if args.trace_form:
+        prec, solve = FORMS[args.trace_form]
+        sim = make_sim(prec, solve, graph)
+        sim.step(DT, args.warmup)
+        sim.sync()
+        print(f"{args.trace_form}: {args.warmup} warm-up steps, residual {sim.residual()}", flush=True)
+        sim.step(DT, args.steps)
+        sim.sync()
+        print(f"{args.trace_form}: {args.steps} steps, residual {sim.residual()}", flush=True)
+        sim.close()
+        return
     if args.probe:
         ms = F.bench_copy(256 << 20, 3)
         print(f"copy probe: 256 MiB in {ms:.4f} ms -> {2 * (256 << 20) / ms / 1e6:.0f} GB/s (read+write)", flush=True)
@@ -772,8 +788,7 @@ def main():
     runs = {}
     # fp32: the reference's solve (2 x V(2,2)) on float4 fields; fp32_tol: plain FP32 V-cycles until the
     # residual reaches the same 1e-5 as the mixed step (or 8 cycles); mixed: the headline
-    for name, prec, solve in (("fp32", F.PREC_FP32, F.SOLVE_VCYCLES), ("fp32_tol", F.PREC_FP32, F.SOLVE_TOL32),
-                              ("mixed_spectral", F.PREC_FP16, F.SOLVE_SPECTRAL64), ("mixed", F.PREC_FP16, F.SOLVE_REFINE64)):
+    for name, (prec, solve) in FORMS.items():
         sim = make_sim(prec, solve, graph)
         sim.step(DT, args.warmup)
         t = time_steps(sim, args.steps)
