@@ -21,23 +21,148 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "fs_internal.h"
 
 struct Spectral {
     int nx = 0, ny = 0, nz = 0;
+    bool fused_z = false;     // the z pass runs as one fused kernel (nz = 32 k <= 160, nx ny = 32 m)
+    int zgrid = 0;            // its blocks: one per CU, at most one per panel
     rocblas_handle blas = nullptr;
     float* Mx = nullptr;      // nx x nx, columns in parity-split storage order
     float* My = nullptr;      // ny x ny
     float* Mz = nullptr;      // nz x nz
+    float* Tz = nullptr;      // T_z row-major (the fused z pass)
     double* lam = nullptr;    // nx + ny + nz eigenvalues of the 1D operators (h = 1)
     float* tmp = nullptr;     // nx ny nz work array
     double ih2 = 1.0;
 };
 
 namespace {
+
+// The z transform, the division by the eigenvalues and the inverse z transform, fused, over panels of 32
+// columns (kx, ky) of the array seen as nz rows z of nx ny columns.  Persistent: each block (one per CU)
+// reads T_z into LDS once, then walks its panels, prefetching the next panel into registers while the
+// current one is on the matrix cores.  v_mfma_f32_16x16x4f32: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16], and gets D[4 (l / 16) + j][l % 16].  Wave w owns the panel's columns 16 (w & 1) ..
+// +15 and row blocks (w >> 1) NRB/2 .. +NRB/2-1, so the four SIMDs get equal work; the forward result is
+// divided in registers and written to LDS over the panel, and the inverse result is stored from registers.
+typedef float f4v __attribute__((ext_vector_type(4)));
+constexpr int ZC = 32;            // panel width (columns)
+constexpr int ZLDP = 48;          // panel row stride in LDS (rows 4 apart start 16 banks apart)
+template <int NRB>
+__global__ __launch_bounds__(256) void k_spec_z(float* __restrict__ f, const float* __restrict__ Tz,
+                                                const double* __restrict__ lam, int nx, int ny, int npanels, double ih2) {
+    constexpr int nz = NRB * 16, ldm = nz + 4, HR = NRB / 2;
+    constexpr int NM = nz * nz / 1024, NV = nz * ZC / 1024;   // float4 per thread: T_z, one panel
+    extern __shared__ float4 sm4[];
+    float* sM = (float*)sm4;          // sM[k ldm + z] = T_z[k][z]
+    float* sP = sM + nz * ldm;        // sP[z ZLDP + c]: the panel, then the divided forward result
+    const int P = nx * ny, t = threadIdx.x;
+    {
+        float4 m[NM];
+#pragma unroll
+        for (int q = 0; q < NM; q++) m[q] = ((const float4*)Tz)[q * 256 + t];
+#pragma unroll
+        for (int q = 0; q < NM; q++) {
+            const int i = (q * 256 + t) * 4;
+            *(float4*)&sM[(i / nz) * ldm + i % nz] = m[q];
+        }
+    }
+    const int w = t >> 6, l = t & 63, lr = l & 15, lk = l >> 4;
+    const int col = (w & 1) * 16 + lr, rb0 = (w >> 1) * HR;
+    double lz[HR][4];
+#pragma unroll
+    for (int h = 0; h < HR; h++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) lz[h][j] = lam[nx + ny + (rb0 + h) * 16 + 4 * lk + j];
+    float4 pre[NV];
+    int pnl = blockIdx.x;
+    if (pnl < npanels)
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            const int v = q * 256 + t;
+            pre[q] = *(const float4*)&f[(size_t)(v / (ZC / 4)) * P + pnl * ZC + (v % (ZC / 4)) * 4];
+        }
+    for (; pnl < npanels; pnl += gridDim.x) {
+        const int c0 = pnl * ZC;
+        __syncthreads();              // T_z is in LDS; the previous panel's inverse is done reading sP
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            const int v = q * 256 + t;
+            *(float4*)&sP[(v / (ZC / 4)) * ZLDP + (v % (ZC / 4)) * 4] = pre[q];
+        }
+        __syncthreads();
+        const int nxt = pnl + gridDim.x;
+        if (nxt < npanels)
+#pragma unroll
+            for (int q = 0; q < NV; q++) {
+                const int v = q * 256 + t;
+                pre[q] = *(const float4*)&f[(size_t)(v / (ZC / 4)) * P + nxt * ZC + (v % (ZC / 4)) * 4];
+            }
+        const int kxy = c0 + col;
+        const double lxy = lam[kxy % nx] + lam[nx + kxy / nx];
+        f4v acc[HR];
+#pragma unroll
+        for (int h = 0; h < HR; h++) acc[h] = f4v{0.f, 0.f, 0.f, 0.f};
+        // forward: C[kz][c] = sum_z T_z[kz][z] panel[z][c]
+#pragma unroll 4
+        for (int k0 = 0; k0 < nz; k0 += 4) {
+            const float bv = sP[(k0 + lk) * ZLDP + col];
+#pragma unroll
+            for (int h = 0; h < HR; h++)
+                acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(sM[((rb0 + h) * 16 + lr) * ldm + k0 + lk], bv, acc[h], 0, 0, 0);
+        }
+        __syncthreads();              // every wave is done reading the panel
+#pragma unroll
+        for (int h = 0; h < HR; h++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                sP[((rb0 + h) * 16 + 4 * lk + j) * ZLDP + col] = (float)((double)acc[h][j] / ((lxy + lz[h][j]) * ih2));
+        __syncthreads();
+        // inverse: out[z][c] = sum_kz T_z[kz][z] C[kz][c]
+#pragma unroll
+        for (int h = 0; h < HR; h++) acc[h] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+        for (int k0 = 0; k0 < nz; k0 += 4) {
+            const float bv = sP[(k0 + lk) * ZLDP + col];
+#pragma unroll
+            for (int h = 0; h < HR; h++)
+                acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(sM[(k0 + lk) * ldm + (rb0 + h) * 16 + lr], bv, acc[h], 0, 0, 0);
+        }
+#pragma unroll
+        for (int h = 0; h < HR; h++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) f[(size_t)((rb0 + h) * 16 + 4 * lk + j) * P + c0 + col] = acc[h][j];
+    }
+}
+
+template <int NRB>
+constexpr size_t spec_z_lds() { return (size_t)(NRB * 16 * (NRB * 16 + 4) + NRB * 16 * ZLDP) * sizeof(float); }
+
+template <int NRB>
+void spec_z_t(hipStream_t s, int grid, float* f, const float* Tz, const double* lam, int nx, int ny, double ih2) {
+    k_spec_z<NRB><<<grid, 256, spec_z_lds<NRB>(), s>>>(f, Tz, lam, nx, ny, nx * ny / ZC, ih2);
+}
+
+// Allows the fused z pass its LDS (above the 64 KiB default for nz >= 96); false if the device refuses.
+bool spec_z_prepare(int nrb) {
+    hipError_t e = hipErrorInvalidValue;
+    switch (nrb) {
+        case 2: e = hipFuncSetAttribute((const void*)k_spec_z<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spec_z_lds<2>()); break;
+        case 4: e = hipFuncSetAttribute((const void*)k_spec_z<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spec_z_lds<4>()); break;
+        case 6: e = hipFuncSetAttribute((const void*)k_spec_z<6>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spec_z_lds<6>()); break;
+        case 8: e = hipFuncSetAttribute((const void*)k_spec_z<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spec_z_lds<8>()); break;
+        case 10: e = hipFuncSetAttribute((const void*)k_spec_z<10>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)spec_z_lds<10>()); break;
+        default: break;
+    }
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e == hipSuccess;
+}
 
 __global__ __launch_bounds__(256) void k_spec_scale(float* __restrict__ f, const double* __restrict__ lam, int nx, int ny,
                                                     int nz, double ih2) {
@@ -86,7 +211,7 @@ extern "C" int fs_spectral_basis(int n, int kind, double* T, double* lam) {
 
 void spectral_free(Spectral* sp) {
     if (!sp) return;
-    void* p[] = {sp->Mx, sp->My, sp->Mz, sp->lam, sp->tmp};
+    void* p[] = {sp->Mx, sp->My, sp->Mz, sp->Tz, sp->lam, sp->tmp};
     for (void* q : p) if (q) (void)hipFree(q);
     if (sp->blas) rocblas_destroy_handle(sp->blas);
     delete sp;
@@ -102,6 +227,20 @@ int spectral_solve(Spectral* sp, hipStream_t s, float* b, float* p) {
     BLAS_CHECK(rocblas_sgemm(h, N, N, nx, R, nx, &one, sp->Mx, nx, b, nx, &zero, sp->tmp, nx));
     BLAS_CHECK(rocblas_sgemm_strided_batched(h, N, T, nx, ny, ny, &one, sp->tmp, nx, (rocblas_stride)P, sp->My, ny, 0, &zero, p,
                                              nx, (rocblas_stride)P, nz));
+    if (sp->fused_z) {   // z, division and inverse z in place on p, then y (p -> tmp), x (tmp -> p)
+        switch (nz / 16) {
+            case 2: spec_z_t<2>(s, sp->zgrid, p, sp->Tz, sp->lam, nx, ny, sp->ih2); break;
+            case 4: spec_z_t<4>(s, sp->zgrid, p, sp->Tz, sp->lam, nx, ny, sp->ih2); break;
+            case 6: spec_z_t<6>(s, sp->zgrid, p, sp->Tz, sp->lam, nx, ny, sp->ih2); break;
+            case 8: spec_z_t<8>(s, sp->zgrid, p, sp->Tz, sp->lam, nx, ny, sp->ih2); break;
+            default: spec_z_t<10>(s, sp->zgrid, p, sp->Tz, sp->lam, nx, ny, sp->ih2); break;
+        }
+        BLAS_CHECK(rocblas_sgemm_strided_batched(h, N, N, nx, ny, ny, &one, p, nx, (rocblas_stride)P, sp->My, ny, 0, &zero,
+                                                 sp->tmp, nx, (rocblas_stride)P, nz));
+        BLAS_CHECK(rocblas_sgemm(h, T, N, nx, R, nx, &one, sp->Mx, nx, sp->tmp, nx, &zero, p, nx));
+        FS_HIP_CHECK(hipGetLastError());
+        return FS_OK;
+    }
     BLAS_CHECK(rocblas_sgemm(h, N, T, P, nz, nz, &one, p, P, sp->Mz, nz, &zero, sp->tmp, P));
     const int n = nx * ny * nz;
     k_spec_scale<<<(n + 255) / 256, 256, 0, s>>>(sp->tmp, sp->lam, nx, ny, nz, sp->ih2);
@@ -140,13 +279,22 @@ int spectral_build(Spectral** out, int nx, int ny, int nz, double h, hipStream_t
     lam.insert(lam.end(), ly.begin(), ly.end());
     lam.insert(lam.end(), lz.begin(), lz.end());
     const size_t n = (size_t)nx * ny * nz;
+    std::vector<float> tz(Tz.begin(), Tz.end());
+    const int nrb = nz / 16;
+    int dev = 0, ncu = 0;
+    sp->fused_z = nz % 32 == 0 && nrb >= 2 && nrb <= 10 && (nx * ny) % ZC == 0 && !getenv("FS_SPECTRAL_BLAS_Z") &&
+                  hipGetDevice(&dev) == hipSuccess &&
+                  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0 &&
+                  spec_z_prepare(nrb);
+    sp->zgrid = std::min(ncu, nx * ny / ZC);
     bool ok = hipMalloc(&sp->Mx, mx.size() * 4) == hipSuccess && hipMalloc(&sp->My, my.size() * 4) == hipSuccess &&
-              hipMalloc(&sp->Mz, mz.size() * 4) == hipSuccess && hipMalloc(&sp->lam, lam.size() * 8) == hipSuccess &&
-              hipMalloc(&sp->tmp, n * 4) == hipSuccess;
+              hipMalloc(&sp->Mz, mz.size() * 4) == hipSuccess && hipMalloc(&sp->Tz, tz.size() * 4) == hipSuccess &&
+              hipMalloc(&sp->lam, lam.size() * 8) == hipSuccess && hipMalloc(&sp->tmp, n * 4) == hipSuccess;
     if (!ok) { spectral_free(sp); err = "hipMalloc failed for the spectral solve"; return FS_ENOMEM; }
     ok = hipMemcpy(sp->Mx, mx.data(), mx.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(sp->My, my.data(), my.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(sp->Mz, mz.data(), mz.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(sp->Tz, tz.data(), tz.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(sp->lam, lam.data(), lam.size() * 8, hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) { spectral_free(sp); err = "device copy failed for the spectral solve"; return FS_EDEVICE; }
     if (rocblas_create_handle(&sp->blas) != rocblas_status_success) {

@@ -81,6 +81,25 @@ def test_spectral_graph_replay_equals_eager():
         assert a.residual() == b.residual()
 
 
+def test_fused_z_pass_matches_the_blas_z_pass(monkeypatch):
+    """The fused MFMA z pass (z transform, eigenvalue division, inverse z in one kernel) against the three
+    rocBLAS calls it replaces (FS_SPECTRAL_BLAS_Z at create): the same solve up to FP32 summation order."""
+    print("spectral fused z vs rocBLAS z", flush=True)
+    sc = scenes.SCENES[NAME]
+    fused = make(False, F.PREC_FP32)
+    monkeypatch.setenv("FS_SPECTRAL_BLAS_Z", "1")
+    blas = make(False, F.PREC_FP32)
+    monkeypatch.delenv("FS_SPECTRAL_BLAS_Z")
+    for k in range(3):
+        for sim in (fused, blas):
+            sim.step(sc["dt"], 1)
+        pf, pb = fused.read(F.P, np.float64), blas.read(F.P, np.float64)
+        rel = np.linalg.norm(pf - pb) / np.linalg.norm(pb)
+        print(f"step {k}: pressure rel diff {rel:.3e}, residuals {fused.residual()} {blas.residual()}", flush=True)
+        assert rel <= 1e-5
+        assert fused.residual()[0] <= 1e-5 and blas.residual()[0] <= 1e-5
+
+
 def test_spectral_mode_errors():
     print("spectral errors", flush=True)
     with pytest.raises(F.FsError, match="even nx"):
