@@ -44,7 +44,9 @@ extern "C" {
 #define FS_SOLVE_SPECTRAL64 3  /* build extension: the exact separable inner solve (DST-II / DCT-IV
                                   transforms as FP32 matrix-core GEMMs, one division by the
                                   eigenvalues) inside the FP64 defect correction to div_tol, in place
-                                  of the V-cycles; nx even; max_vcycles caps the corrections */
+                                  of the V-cycles; nx even; max_vcycles caps the corrections.
+                                  FS_SPECTRAL_BLAS_Z set in the environment at create runs the z
+                                  transforms through rocBLAS instead of the fused kernel */
 #define FS_SOLVE_TOL32      2  /* FP32 V-cycles on p until the FP64-evaluated ||b - A p||/||b||
                                   <= div_tol or max_vcycles (no defect correction)        */
 
