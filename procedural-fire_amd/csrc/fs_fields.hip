@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_adv1(const DevParams* __restrict__ P, D
     if (!cell_of(g, c)) return;
     const float dt = (float)P->dt, dx = P->f[P_dx];
     const float cx = (float)c.x + 0.5f, cy = (float)c.y + 0.5f, cz = (float)c.z + 0.5f;
-    const float3 b = backtrace<S>(vel, g, cx, cy, cz, dt, dx);
+    const float3 b = backtrace_c<S>(vel, g, cx, cy, cz, dt, dx, Codec<S>::ld(vel, c.i));
     const float f = outside_fade(g, b.x, b.y, b.z);
     const float ws = sstep(0.03f, 0.4f, b.y * dx);
     const float wx = P->f[P_windX] * ws, wy = 0.f * ws, wz = P->f[P_windZ] * ws;
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, D
     float4 s2N = make_float4(s2Hc.x + (s20.x - s2F4.x) * 0.5f, s2Hc.y + (s20.y - s2F4.y) * 0.5f,
                              s2Hc.z + (s20.z - s2F4.z) * 0.5f, s2Hc.w + (s20.w - s2F4.w) * 0.5f);
     // Selle et al. 2008 limiter over the 8 texels at floor(backtrace - 0.5) (sim.js:180-193)
-    const float3 bt = backtrace<S>(vel, g, cx, cy, cz, dt, dx);
+    const float3 bt = backtrace_c<S>(vel, g, cx, cy, cz, dt, dx, v);
     const float bx = bt.x - 0.5f, by = bt.y - 0.5f, bz = bt.z - 0.5f;
     const int b0x = (int)floorf(wclamp(bx, -2.f, (float)g.nx + 1.f));
     const int b0y = (int)floorf(wclamp(by, -2.f, (float)g.ny + 1.f));
