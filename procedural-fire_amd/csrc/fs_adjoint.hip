@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void k_forces_adj(const DevParams* __restrict_
         Dual s[4] = {x[3], x[4], x[5], x[6]}, s2[4] = {x[7], x[8], x[9], x[10]};
         const Dual w[3] = {x[11], x[12], x[13]};
         const Dual gx = x[14] - x[15], gy = x[16] - x[17], gz = x[18] - x[19];
-        forces_cell<Dual>(P->f, k, spots, P->n_spots, sp, nullptr, face, Dual(fv, seed_face ? 1.f : 0.f), vx, vy, vz,
+        forces_cell<Dual>(P->f, P->d, k, spots, P->n_spots, sp, nullptr, face, Dual(fv, seed_face ? 1.f : 0.f), vx, vy, vz,
                           s, s2, w, gx, gy, gz, phi);
         return Lv.x * vx.d + Lv.y * vy.d + Lv.z * vz.d + Lv.w * phi.d + Ls.x * s[0].d + Ls.y * s[1].d + Ls.z * s[2].d +
                Ls.w * s[3].d + Ls2.x * s2[0].d + Ls2.y * s2[1].d + Ls2.z * s2[2].d + Ls2.w * s2[3].d;
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(256) void k_forces_tan(const DevParams* __restrict_
     const bool face = P->has_face_flux != 0;
     const bool floor_face = face && c.y == 0;
     const Dual fv(floor_face ? faceFlux[c.z * g.nx + c.x] : 0.f, floor_face ? face_dir[c.z * g.nx + c.x] : 0.f);
-    forces_cell<Dual>(P->f, k, spots, P->n_spots, -1, spot_dir, face, fv, vx, vy, vz, s, s2, w, gx, gy, gz, phi);
+    forces_cell<Dual>(P->f, P->d, k, spots, P->n_spots, -1, spot_dir, face, fv, vx, vy, vz, s, s2, w, gx, gy, gz, phi);
     tvF[c.i] = make_float4(vx.d, vy.d, vz.d, phi.d);
     tS[c.i] = make_float4(s[0].d, s[1].d, s[2].d, s[3].d);
     tS2[c.i] = make_float4(s2[0].d, s2[1].d, s2[2].d, s2[3].d);

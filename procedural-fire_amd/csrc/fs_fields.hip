@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_forces(const DevParams* __restrict__ P,
     const float wv[3] = {w.x, w.y, w.z};
     const bool face = P->has_face_flux != 0;
     const float fv = (face && c.y == 0) ? faceFlux[c.z * g.nx + c.x] : 0.f;
-    forces_cell<float>(P->f, k, spots, P->n_spots, -1, nullptr, face, fv, vx, vy, vz, s, s2, wv, gx, gy, gz, phi);
+    forces_cell<float>(P->f, P->d, k, spots, P->n_spots, -1, nullptr, face, fv, vx, vy, vz, s, s2, wv, gx, gy, gz, phi);
     Codec<S>::st(velOut, c.i, make_float4(vx, vy, vz, phi));
     Codec<S>::st(sOut, c.i, make_float4(s[0], s[1], s[2], s[3]));
     Codec<S>::st(s2Out, c.i, make_float4(s2[0], s2[1], s2[2], s2[3]));
@@ -337,7 +337,10 @@ __global__ __launch_bounds__(256) void k_curl_tex(const half4_t* __restrict__ sr
 }
 
 // ---- small utility kernels ------------------------------------------------------
-__global__ void k_set_params(DevParams* __restrict__ dst, DevParams src) { *dst = src; }
+__global__ void k_set_params(DevParams* __restrict__ dst, DevParams src) {
+    derive_params(src);
+    *dst = src;
+}
 __global__ __launch_bounds__(256) void k_fill_zero(uint4* __restrict__ p, size_t n16, unsigned char* __restrict__ tail, size_t ntail) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n16) p[i] = make_uint4(0u, 0u, 0u, 0u);

@@ -91,7 +91,7 @@ struct ForcesCell {
 // spot_dir: the derivative of every spot's flux (Dual only; null: use seed_spot);
 // face: the floor flux of this cell as T.
 template <class T>
-__device__ __forceinline__ void forces_cell(const float* U, const ForcesCell& k, const fs_spot* __restrict__ spots,
+__device__ __forceinline__ void forces_cell(const float* U, const float* D, const ForcesCell& k, const fs_spot* __restrict__ spots,
                                             int n_spots, int seed_spot, const float* spot_dir, bool has_face,
                                             T face, T& vx, T& vy,
                                             T& vz, T* s, T* s2, const T* w, T gx, T gy, T gz, T& phi_out) {
@@ -153,7 +153,7 @@ __device__ __forceinline__ void forces_cell(const float* U, const ForcesCell& k,
     s[2] = s[2] + U[P_sootForm] * b * gsstep(0.015f, 0.05f, ageF) * gsstep(1000.f, 1400.f, Tk) * (1.f - s2[3]);
     s[2] = s[2] * gexp(-U[P_sootOx] * (0.25f + o) * gsstep(1100.f, 1400.f, Tk) * dt);
     s[2] = s[2] * gexp(-U[P_sootOx] * 0.5f * o * gsstep(850.f, 1100.f, Tk) * (1.f - gsstep(1100.f, 1300.f, Tk)) * dt);
-    s[2] = s[2] * fs_exp(-U[P_sootDissip] * dt);
+    s[2] = s[2] * D[D_eSootDissip];                      // exp(-sootDissip dt)
 
     // ---- cooling (sim.js:321-330)
     const T th = Tk * 0.001f;
@@ -162,9 +162,9 @@ __device__ __forceinline__ void forces_cell(const float* U, const ForcesCell& k,
     Tk = 293.f + gmax(Tk - 293.f, 0.f) * gexp(-U[P_coolRate] * notBurning * dt);
     s[0] = (Tk - 293.f) * 0.001f;
     s[3] = q * 1e-4f;
-    s[1] = s[1] * fs_exp(-U[P_fuelDissip] * dt);
-    s2[0] = s2[0] * fs_exp(-U[P_vitDecay] * dt);
-    s2[2] = s2[2] * fs_exp(-U[P_smokeDissip] * dt);
+    s[1] = s[1] * D[D_eFuelDissip];                      // exp(-fuelDissip dt)
+    s2[0] = s2[0] * D[D_eVitDecay];                      // exp(-vitDecay dt)
+    s2[2] = s2[2] * D[D_eSmokeDissip];                   // exp(-smokeDissip dt)
     s2[1] = ageF * gclamp(s[1] + s2[0], 0.f, 1.f);
 
     // ---- buoyancy (sim.js:333)
@@ -194,7 +194,7 @@ __device__ __forceinline__ void forces_cell(const float* U, const ForcesCell& k,
     const T tgate = gmax(gsstep(500.f, 1100.f, Tk), gsstep(0.f, 3.f, s[3])) * (1.f - srcAll) *
                     mixf(0.6f, 1.8f, sstep(0.2f, 0.75f, k.py)) * lam;
     {
-        const T kn = (U[P_turbAmp] / 12.f) * tgate * dt * 10.f;
+        const T kn = D[D_turbAmp12] * tgate * dt * 10.f;   // turbAmp / 12
         vx = vx + k.cnx * kn; vy = vy + k.cny * kn; vz = vz + k.cnz * kn;
     }
     // ---- wind (sim.js:372-375)
@@ -202,7 +202,7 @@ __device__ __forceinline__ void forces_cell(const float* U, const ForcesCell& k,
     vx = vx + (U[P_windX] * windK - vx) * dt * 0.35f * windK;
     vz = vz + (U[P_windZ] * windK - vz) * dt * 0.35f * windK;
     {
-        const float e = fs_exp(-U[P_velDissip] * dt);
+        const float e = D[D_eVelDissip];                 // exp(-velDissip dt)
         vx = vx * e; vy = vy * e; vz = vz * e;
     }
     // ---- solids and sponge layers (sim.js:378-388)
@@ -278,13 +278,38 @@ __device__ __forceinline__ ForcesCell forces_setup(const DevParams* __restrict__
         k.lapy = a.y + bb.y + cc.y + dd.y + ee.y + ff.y - me.y * 6.f;
         k.lapz = a.z + bb.z + cc.z + dd.z + ee.z + ff.z - me.z * 6.f;
     }
-    const float fpx = (float)c.x + 0.5f, fpy = (float)c.y + 0.5f, fpz = (float)c.z + 0.5f;
+    const float fpx = (float)c.x + 0.5f, fpz = (float)c.z + 0.5f;
     const float side = fminf(fminf(fpx, (float)g.nx - fpx), fminf(fpz, (float)g.nz - fpz));
-    const float sp = (1.f - sstep(0.f, 5.f, side)) * U[P_sponge];
     const float wsg = sstep(0.03f, 0.4f, py);
     k.wSx = U[P_windX] * wsg; k.wSz = U[P_windZ] * wsg;
-    k.ks = 1.f - fs_pow(1.f - 0.15f * wclamp(sp, 0.f, 1.f), k.dt * 90.f);
-    const float top = sstep((float)g.ny - 10.f, (float)g.ny, fpy);
-    k.kt = fs_pow(1.f - 0.1f * top, k.dt * 90.f);
+    // side is j + 0.5 (j an integer); from 5 cells in, the sponge weight is 0 and the factor 1 - pow(1, .) = 0
+    k.ks = side < 5.f ? P->d[D_ks0 + (int)side] : 0.f;
+    // below the top 10 rows the smoothstep is 0 and the factor pow(1, .) = 1
+    k.kt = c.y >= g.ny - 10 ? P->d[D_kt0 + c.y - (g.ny - 10)] : 1.f;
     return k;
+}
+
+// The per-step derived values (see DevParams::d), from the same expressions the source kernel
+// evaluates per cell (src/sim.js:321-375 and the sponge rows of src/sim.js:380-388), so the
+// factors it reads are bit-identical to evaluating them in place.
+__device__ __forceinline__ void derive_params(DevParams& p) {
+    const float* U = p.f;
+    const float dt = (float)p.dt;
+    p.d[D_eSootDissip] = fs_exp(-U[P_sootDissip] * dt);
+    p.d[D_eFuelDissip] = fs_exp(-U[P_fuelDissip] * dt);
+    p.d[D_eVitDecay] = fs_exp(-U[P_vitDecay] * dt);
+    p.d[D_eSmokeDissip] = fs_exp(-U[P_smokeDissip] * dt);
+    p.d[D_eVelDissip] = fs_exp(-U[P_velDissip] * dt);
+    p.d[D_turbAmp12] = U[P_turbAmp] / 12.f;
+    for (int j = 0; j < 5; j++) {
+        const float side = (float)j + 0.5f;
+        const float sp = (1.f - sstep(0.f, 5.f, side)) * U[P_sponge];
+        p.d[D_ks0 + j] = 1.f - fs_pow(1.f - 0.15f * wclamp(sp, 0.f, 1.f), dt * 90.f);
+    }
+    const int ny = (int)U[P_ny];
+    for (int j = 0; j < 10; j++) {
+        const float fpy = (float)(ny - 10 + j) + 0.5f;
+        const float top = sstep((float)ny - 10.f, (float)ny, fpy);
+        p.d[D_kt0 + j] = fs_pow(1.f - 0.1f * top, dt * 90.f);
+    }
 }

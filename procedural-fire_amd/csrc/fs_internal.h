@@ -23,8 +23,17 @@ enum {
 // JS numbers are, and rounded to f32 where a kernel reads them.  Living in device
 // memory (not kernel arguments) keeps every launch of a step static, so the step
 // can be replayed from a captured graph.
+// Per-step values of the source kernel that depend on the parameters alone, computed once by the
+// parameter upload (derive_params) with the same expressions the kernel would evaluate per cell:
+// the five dissipation factors exp(-rate dt), turbAmp / 12, and the sponge factors of the five
+// side-distance rows (side = 0.5 .. 4.5 cells) and of the ten top rows.
+enum {
+    D_eSootDissip, D_eFuelDissip, D_eVitDecay, D_eSmokeDissip, D_eVelDissip, D_turbAmp12,
+    D_ks0, D_kt0 = D_ks0 + 5, D_COUNT = D_kt0 + 10,
+};
 struct DevParams {
     float f[64];
+    float d[D_COUNT];            // derived by the upload kernel; the host leaves them zero
     double time;
     double dt;
     long long frame;
