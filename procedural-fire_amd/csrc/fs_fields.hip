@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_adv1(const DevParams* __restrict__ P, D
     const float cx = (float)c.x + 0.5f, cy = (float)c.y + 0.5f, cz = (float)c.z + 0.5f;
     const float3 b = backtrace_c<S>(vel, g, cx, cy, cz, dt, dx, Codec<S>::ld(vel, c.i));
     const float f = outside_fade(g, b.x, b.y, b.z);
-    const float ws = sstep(0.03f, 0.4f, b.y * dx);
+    const float ws = sstep_c(0.03f, 0.4f, b.y * dx);
     const float wx = P->f[P_windX] * ws, wy = 0.f * ws, wz = P->f[P_windZ] * ws;
     const float4 sv = sample_clamp<S>(vel, g, b.x, b.y, b.z);
     Codec<S>::st(velX, c.i, make_float4(mixf(wx, sv.x, f), mixf(wy, sv.y, f), mixf(wz, sv.z, f), 0.f));

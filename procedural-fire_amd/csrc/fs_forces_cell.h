@@ -46,6 +46,10 @@ __device__ __forceinline__ Dual gsstep(float e0, float e1, Dual x) {
     const Dual t = gclamp((x - e0) / (e1 - e0), 0.f, 1.f);
     return t * t * (3.f - 2.f * t);
 }
+// literal edges: the float path divides through the reciprocal (sstep_c, bit-identical values); the
+// dual path keeps the plain division
+__device__ __forceinline__ float gsstep_c(float e0, float e1, float x) { return sstep_c(e0, e1, x); }
+__device__ __forceinline__ Dual gsstep_c(float e0, float e1, Dual x) { return gsstep(e0, e1, x); }
 template <class A, class B, class C>
 __device__ __forceinline__ auto gmix(A a, B b, C t) -> decltype(a * (1.f - t) + b * t) { return a * (1.f - t) + b * t; }
 __device__ __forceinline__ float gexp(float x) { return fs_exp(x); }
@@ -150,15 +154,15 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
 
     // ---- soot (sim.js:312-318)
     ageF = gmin(ageF + dt, 4.f);
-    s[2] = s[2] + U[P_sootForm] * b * gsstep(0.015f, 0.05f, ageF) * gsstep(1000.f, 1400.f, Tk) * (1.f - s2[3]);
-    s[2] = s[2] * gexp(-U[P_sootOx] * (0.25f + o) * gsstep(1100.f, 1400.f, Tk) * dt);
-    s[2] = s[2] * gexp(-U[P_sootOx] * 0.5f * o * gsstep(850.f, 1100.f, Tk) * (1.f - gsstep(1100.f, 1300.f, Tk)) * dt);
+    s[2] = s[2] + U[P_sootForm] * b * gsstep_c(0.015f, 0.05f, ageF) * gsstep_c(1000.f, 1400.f, Tk) * (1.f - s2[3]);
+    s[2] = s[2] * gexp(-U[P_sootOx] * (0.25f + o) * gsstep_c(1100.f, 1400.f, Tk) * dt);
+    s[2] = s[2] * gexp(-U[P_sootOx] * 0.5f * o * gsstep_c(850.f, 1100.f, Tk) * (1.f - gsstep_c(1100.f, 1300.f, Tk)) * dt);
     s[2] = s[2] * D[D_eSootDissip];                      // exp(-sootDissip dt)
 
     // ---- cooling (sim.js:321-330)
     const T th = Tk * 0.001f;
     Tk = Tk - gmin(U[P_radCool] * 970.f * s[2] * th * th * (th * th * th * th - 0.00737f), 6000.f) * dt;
-    const T notBurning = 1.f - gsstep(500.f, 5000.f, q);
+    const T notBurning = 1.f - gsstep_c(500.f, 5000.f, q);
     Tk = 293.f + gmax(Tk - 293.f, 0.f) * gexp(-U[P_coolRate] * notBurning * dt);
     s[0] = (Tk - 293.f) * 0.001f;
     s[3] = q * 1e-4f;
@@ -183,7 +187,7 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
     // ---- vorticity confinement (sim.js:353-361)
     const T gl = gsqrt(gx * gx + gy * gy + gz * gz);
     if (gval(gl) > 1e-6f) {
-        const T gate = gsstep(450.f, 900.f, Tk) * (1.f - srcAll) * sstep(0.02f, 0.08f, k.py) * lam;
+        const T gate = gsstep_c(450.f, 900.f, Tk) * (1.f - srcAll) * sstep_c(0.02f, 0.08f, k.py) * lam;
         const T nx_ = gx / gl, ny_ = gy / gl, nz_ = gz / gl;
         const T kc = U[P_vortEps] * gate * dt;
         vx = vx + (ny_ * w[2] - nz_ * w[1]) * kc;
@@ -191,14 +195,14 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
         vz = vz + (nx_ * w[1] - ny_ * w[0]) * kc;
     }
     // ---- rising curl noise (sim.js:364-368)
-    const T tgate = gmax(gsstep(500.f, 1100.f, Tk), gsstep(0.f, 3.f, s[3])) * (1.f - srcAll) *
-                    mixf(0.6f, 1.8f, sstep(0.2f, 0.75f, k.py)) * lam;
+    const T tgate = gmax(gsstep_c(500.f, 1100.f, Tk), gsstep_c(0.f, 3.f, s[3])) * (1.f - srcAll) *
+                    mixf(0.6f, 1.8f, sstep_c(0.2f, 0.75f, k.py)) * lam;
     {
         const T kn = D[D_turbAmp12] * tgate * dt * 10.f;   // turbAmp / 12
         vx = vx + k.cnx * kn; vy = vy + k.cny * kn; vz = vz + k.cnz * kn;
     }
     // ---- wind (sim.js:372-375)
-    const float windK = sstep(0.05f, 0.5f, k.py) * (1.f + U[P_gust]);
+    const float windK = sstep_c(0.05f, 0.5f, k.py) * (1.f + U[P_gust]);
     vx = vx + (U[P_windX] * windK - vx) * dt * 0.35f * windK;
     vz = vz + (U[P_windZ] * windK - vz) * dt * 0.35f * windK;
     {
@@ -247,7 +251,7 @@ __device__ __forceinline__ ForcesCell forces_setup(const DevParams* __restrict__
     if (gg.x > 0.001f) {
         const float af = U[P_anchorFreq];
         const float4 na = sample_rep(noise, nn, px * af + U[P_seed] * 0.37f, py * af + 0.f, pz * af + 0.11f);
-        anchor = sstep(-0.05f, 0.25f, na.z * 2.f - 1.f);
+        anchor = sstep_c(-0.05f, 0.25f, na.z * 2.f - 1.f);
         const float sf = U[P_srcFreq] * 0.06f;
         const float4 nq = sample_rep(noise, nn, px * sf + U[P_seed] * 0.131f, py * sf + 0.f, pz * sf + tm * U[P_srcSpeed]);
         const float pulse = sstep(U[P_srcThresh] - 0.25f, U[P_srcThresh] + 0.25f, nq.x * 2.f - 1.f);
@@ -258,7 +262,7 @@ __device__ __forceinline__ ForcesCell forces_setup(const DevParams* __restrict__
     float bed = gg.y * wclamp(I * 1.4f, 0.3f, 1.f);
     if (bed > 0.001f) {
         const float4 nb = sample_rep(noise, nn, px * 2.4f + 0.17f, py * 2.4f + -tm * 0.12f, pz * 2.4f + tm * 0.05f);
-        bed *= sstep(0.f, 0.3f, nb.y * 2.f - 1.f);
+        bed *= sstep_c(0.f, 0.3f, nb.y * 2.f - 1.f);
     }
     k.src = src; k.srcW = srcW; k.bed = bed; k.bedHot = gg.y; k.anchor = anchor;
     k.wsrc = 1.f - fs_pow(1.f - wclamp(srcW + bed * 0.4f, 0.f, 1.f) * 0.5f, k.dt * 90.f);
@@ -280,7 +284,7 @@ __device__ __forceinline__ ForcesCell forces_setup(const DevParams* __restrict__
     }
     const float fpx = (float)c.x + 0.5f, fpz = (float)c.z + 0.5f;
     const float side = fminf(fminf(fpx, (float)g.nx - fpx), fminf(fpz, (float)g.nz - fpz));
-    const float wsg = sstep(0.03f, 0.4f, py);
+    const float wsg = sstep_c(0.03f, 0.4f, py);
     k.wSx = U[P_windX] * wsg; k.wSz = U[P_windZ] * wsg;
     // side is j + 0.5 (j an integer); from 5 cells in, the sponge weight is 0 and the factor 1 - pow(1, .) = 0
     k.ks = side < 5.f ? P->d[D_ks0 + (int)side] : 0.f;

@@ -6,12 +6,22 @@
 #include <hip/hip_runtime.h>
 
 #include "fs_codec.h"
+#include "fs_divc.h"
 #include "fs_internal.h"
 
 // ---- WGSL built-in semantics (clamp = min(max()), smoothstep, mix) -------------
 __device__ __forceinline__ float wclamp(float e, float lo, float hi) { return fminf(fmaxf(e, lo), hi); }
 __device__ __forceinline__ float sstep(float e0, float e1, float x) {
     float t = wclamp((x - e0) / (e1 - e0), 0.f, 1.f);
+    return t * t * (3.f - 2.f * t);
+}
+// smoothstep whose edges e0 < e1 are literals: bit-identical to sstep.  The numerator is clamped to
+// [-d, 2d] first (t is the same for every value outside it, infinities and NaN included), then divided
+// by d = e1 - e0 through its reciprocal, which folds at compile time.
+__device__ __forceinline__ float sstep_c(float e0, float e1, float x) {
+    const float d = e1 - e0;
+    const float n = fminf(fmaxf(x - e0, -d), 2.f * d);
+    const float t = wclamp(fs_div_recip(n, d, 1.f / d), 0.f, 1.f);
     return t * t * (3.f - 2.f * t);
 }
 __device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.f - t) + b * t; }
