@@ -426,6 +426,8 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     if ((long long)nx * ny * nz * 16 > (1LL << 31) * 8LL) return fail(FS_EINVAL, "grid too large");
     if ((long long)nx * ny * nz >= (1LL << 31) / 4) return fail(FS_EINVAL, "grid exceeds 32-bit texel indexing");
     if (cfg->precision < FS_PREC_FP32 || cfg->precision > FS_PREC_BF16) return fail(FS_EINVAL, "bad precision");
+    if ((long long)nx * ny * nz * (long long)texel_bytes(cfg->precision) >= (1LL << 32))   // buffer-resource reads
+        return fail(FS_EINVAL, "a field must stay below 4 GiB (fp32 storage: fewer than 2^28 cells)");
     if (cfg->solve < FS_SOLVE_VCYCLES || cfg->solve > FS_SOLVE_SPECTRAL64) return fail(FS_EINVAL, "bad solve mode");
     if (cfg->solve == FS_SOLVE_SPECTRAL64 && nx % 2 != 0) return fail(FS_EINVAL, "FS_SOLVE_SPECTRAL64 needs an even nx");
     if (!(cfg->omega == 0.f || (cfg->omega > 0.f && cfg->omega < 2.f))) return fail(FS_EINVAL, "omega must be 0 or in (0, 2)");

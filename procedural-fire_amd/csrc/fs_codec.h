@@ -14,14 +14,27 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(unsigned short u) { return __uint_as_float((unsigned)u << 16); }
 
+// Word 3 of a raw (untyped) buffer resource on gfx9-class chips: 32-bit data format, no swizzle.
+constexpr int FS_BUF_WORD3 = 0x00020000;
+typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+
 template <class S> struct Codec;
 template <> struct Codec<StF32> {
     static __device__ __forceinline__ float4 ld(const float4* p, int i) { return p[i]; }
+    static __device__ __forceinline__ float4 ldb(__amdgpu_buffer_rsrc_t r, int off) {
+        const u32x4_t u = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        return make_float4(__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z), __uint_as_float(u.w));
+    }
     static __device__ __forceinline__ void st(float4* p, int i, float4 v) { p[i] = v; }
 };
 template <> struct Codec<StF16> {
     static __device__ __forceinline__ float4 ld(const half4_t* p, int i) {
         half4_t h = p[i];
+        return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
+    }
+    static __device__ __forceinline__ float4 ldb(__amdgpu_buffer_rsrc_t r, int off) {
+        const half4_t h = __builtin_bit_cast(half4_t, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
         return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
     }
     static __device__ __forceinline__ void st(half4_t* p, int i, float4 v) {
@@ -33,6 +46,10 @@ template <> struct Codec<StF16> {
 template <> struct Codec<StBF16> {
     static __device__ __forceinline__ float4 ld(const ushort4_t* p, int i) {
         ushort4_t u = p[i];
+        return make_float4(bf2f(u.x), bf2f(u.y), bf2f(u.z), bf2f(u.w));
+    }
+    static __device__ __forceinline__ float4 ldb(__amdgpu_buffer_rsrc_t r, int off) {
+        const ushort4_t u = __builtin_bit_cast(ushort4_t, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
         return make_float4(bf2f(u.x), bf2f(u.y), bf2f(u.z), bf2f(u.w));
     }
     static __device__ __forceinline__ void st(ushort4_t* p, int i, float4 v) {

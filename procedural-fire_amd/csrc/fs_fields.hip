@@ -69,14 +69,15 @@ __global__ __launch_bounds__(256) void k_adv1(const DevParams* __restrict__ P, D
     if (!cell_of(g, c)) return;
     const float dt = (float)P->dt, dx = P->f[P_dx];
     const float cx = (float)c.x + 0.5f, cy = (float)c.y + 0.5f, cz = (float)c.z + 0.5f;
-    const float3 b = backtrace_c<S>(vel, g, cx, cy, cz, dt, dx, Codec<S>::ld(vel, c.i));
+    const FBuf<S> V(vel, g), SF(sF, g), S2F(s2F, g);
+    const float3 b = backtrace_c<S>(V, g, cx, cy, cz, dt, dx, V.ld(c.i));
     const float f = outside_fade(g, b.x, b.y, b.z);
     const float ws = sstep_c(0.03f, 0.4f, b.y * dx);
     const float wx = P->f[P_windX] * ws, wy = 0.f * ws, wz = P->f[P_windZ] * ws;
-    const float4 sv = sample_clamp<S>(vel, g, b.x, b.y, b.z);
+    const float4 sv = sample_clamp_f<S>(V, g, b.x, b.y, b.z);
     Codec<S>::st(velX, c.i, make_float4(mixf(wx, sv.x, f), mixf(wy, sv.y, f), mixf(wz, sv.z, f), 0.f));
-    Codec<S>::st(sX, c.i, scale4(sample_clamp<S>(sF, g, b.x, b.y, b.z), f));
-    Codec<S>::st(s2X, c.i, scale4(sample_clamp<S>(s2F, g, b.x, b.y, b.z), f));
+    Codec<S>::st(sX, c.i, scale4(sample_clamp_f<S>(SF, g, b.x, b.y, b.z), f));
+    Codec<S>::st(s2X, c.i, scale4(sample_clamp_f<S>(S2F, g, b.x, b.y, b.z), f));
 }
 
 
@@ -91,23 +92,24 @@ __global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, D
     if (!cell_of(g, c)) return;
     const float dt = (float)P->dt, dx = P->f[P_dx];
     const float cx = (float)c.x + 0.5f, cy = (float)c.y + 0.5f, cz = (float)c.z + 0.5f;
-    const float4 v = Codec<S>::ld(vel, c.i);
+    const FBuf<S> V(vel, g), SF(sF, g), S2F(s2F, g), VH(velH, g), SH(sH, g), S2H(s2H, g);
+    const float4 v = V.ld(c.i);
     // reverse advection of the forward result (sim.js:172-178)
     const float fwx = cx + dt * v.x / dx, fwy = cy + dt * v.y / dx, fwz = cz + dt * v.z / dx;
-    const float4 vH = Codec<S>::ld(velH, c.i);
-    const float4 sHc = Codec<S>::ld(sH, c.i);
-    const float4 s2Hc = Codec<S>::ld(s2H, c.i);
-    const float4 vF = sample_clamp<S>(velH, g, fwx, fwy, fwz);
-    const float4 sF4 = sample_clamp<S>(sH, g, fwx, fwy, fwz);
-    const float4 s2F4 = sample_clamp<S>(s2H, g, fwx, fwy, fwz);
-    const float4 s0 = Codec<S>::ld(sF, c.i), s20 = Codec<S>::ld(s2F, c.i);
+    const float4 vH = VH.ld(c.i);
+    const float4 sHc = SH.ld(c.i);
+    const float4 s2Hc = S2H.ld(c.i);
+    const float4 vF = sample_clamp_f<S>(VH, g, fwx, fwy, fwz);
+    const float4 sF4 = sample_clamp_f<S>(SH, g, fwx, fwy, fwz);
+    const float4 s2F4 = sample_clamp_f<S>(S2H, g, fwx, fwy, fwz);
+    const float4 s0 = SF.ld(c.i), s20 = S2F.ld(c.i);
     float vNx = vH.x + (v.x - vF.x) * 0.5f, vNy = vH.y + (v.y - vF.y) * 0.5f, vNz = vH.z + (v.z - vF.z) * 0.5f;
     float4 sN = make_float4(sHc.x + (s0.x - sF4.x) * 0.5f, sHc.y + (s0.y - sF4.y) * 0.5f,
                             sHc.z + (s0.z - sF4.z) * 0.5f, sHc.w + (s0.w - sF4.w) * 0.5f);
     float4 s2N = make_float4(s2Hc.x + (s20.x - s2F4.x) * 0.5f, s2Hc.y + (s20.y - s2F4.y) * 0.5f,
                              s2Hc.z + (s20.z - s2F4.z) * 0.5f, s2Hc.w + (s20.w - s2F4.w) * 0.5f);
     // Selle et al. 2008 limiter over the 8 texels at floor(backtrace - 0.5) (sim.js:180-193)
-    const float3 bt = backtrace_c<S>(vel, g, cx, cy, cz, dt, dx, v);
+    const float3 bt = backtrace_c<S>(V, g, cx, cy, cz, dt, dx, v);
     const float bx = bt.x - 0.5f, by = bt.y - 0.5f, bz = bt.z - 0.5f;
     const int b0x = (int)floorf(wclamp(bx, -2.f, (float)g.nx + 1.f));
     const int b0y = (int)floorf(wclamp(by, -2.f, (float)g.ny + 1.f));
@@ -118,7 +120,7 @@ __global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, D
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int qx = b0x + (k & 1), qy = b0y + ((k >> 1) & 1), qz = b0z + ((k >> 2) & 1);
-        const float4 vv = ldc<S>(vel, g, qx, qy, qz), ss = ldc<S>(sF, g, qx, qy, qz), tt = ldc<S>(s2F, g, qx, qy, qz);
+        const float4 vv = ldc_f<S>(V, g, qx, qy, qz), ss = ldc_f<S>(SF, g, qx, qy, qz), tt = ldc_f<S>(S2F, g, qx, qy, qz);
         vmn.x = fminf(vmn.x, vv.x); vmn.y = fminf(vmn.y, vv.y); vmn.z = fminf(vmn.z, vv.z);
         vmx.x = fmaxf(vmx.x, vv.x); vmx.y = fmaxf(vmx.y, vv.y); vmx.z = fmaxf(vmx.z, vv.z);
         smn.x = fminf(smn.x, ss.x); smn.y = fminf(smn.y, ss.y); smn.z = fminf(smn.z, ss.z); smn.w = fminf(smn.w, ss.w);

@@ -130,16 +130,39 @@ __device__ __forceinline__ bool cell_of(const Dims& g, Cell& c) {
     c.i = (c.z * g.ny + c.y) * g.nx + c.x;
     return c.x < g.nx && c.y < g.ny;
 }
+// ---- field readers: texel i of a field through a global pointer, or through a raw buffer resource.
+// FBuf keeps the base and size in scalar registers and each load takes a 32-bit byte offset, so an
+// offset formed once per texel position serves every field read there, with no 64-bit address
+// arithmetic per load.  A field read this way stays below 4 GiB (checked when a sim is created).
+template <class S> struct FPtr {
+    const typename S::raw* p;
+    __device__ __forceinline__ float4 ld(int i) const { return Codec<S>::ld(p, i); }
+};
+template <class S> struct FBuf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ FBuf(const typename S::raw* p, const Dims& g)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0,
+                                              (int)((unsigned)g.nx * (unsigned)g.ny * (unsigned)g.nz * (unsigned)sizeof(typename S::raw)),
+                                              FS_BUF_WORD3)) {}
+    __device__ __forceinline__ float4 ld(int i) const {
+        return Codec<S>::ldb(r, (int)((unsigned)i * (unsigned)sizeof(typename S::raw)));
+    }
+};
+
 // clamped texel load (ld(), src/sim.js:114)
+template <class S, class F>
+__device__ __forceinline__ float4 ldc_f(const F& f, const Dims& g, int x, int y, int z) {
+    x = clampi(x, 0, g.nx - 1); y = clampi(y, 0, g.ny - 1); z = clampi(z, 0, g.nz - 1);
+    return f.ld((z * g.ny + y) * g.nx + x);
+}
 template <class S>
 __device__ __forceinline__ float4 ldc(const typename S::raw* f, const Dims& g, int x, int y, int z) {
-    x = clampi(x, 0, g.nx - 1); y = clampi(y, 0, g.ny - 1); z = clampi(z, 0, g.nz - 1);
-    return Codec<S>::ld(f, (z * g.ny + y) * g.nx + x);
+    return ldc_f<S>(FPtr<S>{f}, g, x, y, z);
 }
 // trilinear, clamp-to-edge, at cell-space position b (texel centres at i + 0.5).
 // Clamping the coordinate to [-1, n] first leaves the clamped-texel result unchanged.
-template <class S>
-__device__ __forceinline__ float4 sample_clamp(const typename S::raw* f, const Dims& g, float bx, float by, float bz) {
+template <class S, class F>
+__device__ __forceinline__ float4 sample_clamp_f(const F& f, const Dims& g, float bx, float by, float bz) {
     float tx = wclamp(bx - 0.5f, -1.f, (float)g.nx);
     float ty = wclamp(by - 0.5f, -1.f, (float)g.ny);
     float tz = wclamp(bz - 0.5f, -1.f, (float)g.nz);
@@ -151,13 +174,17 @@ __device__ __forceinline__ float4 sample_clamp(const typename S::raw* f, const D
     const int z0 = clampi(iz, 0, g.nz - 1), z1 = clampi(iz + 1, 0, g.nz - 1);
     const int r00 = (z0 * g.ny + y0) * g.nx, r10 = (z0 * g.ny + y1) * g.nx;
     const int r01 = (z1 * g.ny + y0) * g.nx, r11 = (z1 * g.ny + y1) * g.nx;
-    const float4 c000 = Codec<S>::ld(f, r00 + x0), c100 = Codec<S>::ld(f, r00 + x1);
-    const float4 c010 = Codec<S>::ld(f, r10 + x0), c110 = Codec<S>::ld(f, r10 + x1);
-    const float4 c001 = Codec<S>::ld(f, r01 + x0), c101 = Codec<S>::ld(f, r01 + x1);
-    const float4 c011 = Codec<S>::ld(f, r11 + x0), c111 = Codec<S>::ld(f, r11 + x1);
+    const float4 c000 = f.ld(r00 + x0), c100 = f.ld(r00 + x1);
+    const float4 c010 = f.ld(r10 + x0), c110 = f.ld(r10 + x1);
+    const float4 c001 = f.ld(r01 + x0), c101 = f.ld(r01 + x1);
+    const float4 c011 = f.ld(r11 + x0), c111 = f.ld(r11 + x1);
     const float4 x00 = mix4(c000, c100, fx), x10 = mix4(c010, c110, fx);
     const float4 x01 = mix4(c001, c101, fx), x11 = mix4(c011, c111, fx);
     return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+}
+template <class S>
+__device__ __forceinline__ float4 sample_clamp(const typename S::raw* f, const Dims& g, float bx, float by, float bz) {
+    return sample_clamp_f<S>(FPtr<S>{f}, g, bx, by, bz);
 }
 // trilinear with repeat addressing at normalised coordinate q (noise textures, n = 2^k)
 __device__ __forceinline__ float4 ld_h4(const half4_t* t, int i) {
@@ -191,11 +218,11 @@ __device__ __forceinline__ float3 backtrace(const typename S::raw* vel, const Di
 // trilinear weights are exactly 1 and 0, so for finite texels the first sample is v0 up to the sign of a
 // zero channel, and a zero only enters as c - (+-0) = c: the result is bit-identical to backtrace() and
 // skips eight texel loads and their blends.
-template <class S>
-__device__ __forceinline__ float3 backtrace_c(const typename S::raw* vel, const Dims& g, float cx, float cy, float cz,
+template <class S, class F>
+__device__ __forceinline__ float3 backtrace_c(const F& vel, const Dims& g, float cx, float cy, float cz,
                                               float dt, float dx, float4 v0) {
     const float hx = cx - v0.x * (0.5f * dt) / dx, hy = cy - v0.y * (0.5f * dt) / dx, hz = cz - v0.z * (0.5f * dt) / dx;
-    const float4 v1 = sample_clamp<S>(vel, g, hx, hy, hz);
+    const float4 v1 = sample_clamp_f<S>(vel, g, hx, hy, hz);
     return make_float3(cx - v1.x * dt / dx, cy - v1.y * dt / dx, cz - v1.z * dt / dx);
 }
 __device__ __forceinline__ float outside_fade(const Dims& g, float x, float y, float z) {   // sim.js:137-140

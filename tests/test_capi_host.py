@@ -83,3 +83,18 @@ def test_errors_are_status_codes_not_exceptions():
     assert rc == -1 and b"power of two" in L.fs_last_error()
     assert L.fs_step(None, 0.01, 1) == -1
     assert L.fs_destroy(None) == 0
+
+
+def test_field_size_limits_are_checked_before_any_device_call():
+    L = F.lib()
+    h = ctypes.c_void_p()
+    cfg = F.FsConfig()
+    L.fs_config_default(ctypes.byref(cfg))
+    cfg.precision = F.PREC_FP32
+    cfg.res[0], cfg.res[1], cfg.res[2] = 512, 1024, 512          # 2^28 cells x 16 B = 4 GiB per field
+    assert L.fs_create(ctypes.byref(cfg), None, 0, ctypes.byref(h)) == -1
+    assert b"4 GiB" in L.fs_last_error()
+    cfg.precision = F.PREC_FP16                                  # 2 GiB per field: past this check
+    cfg.noise_n = 100                                            # and stopped by the next one, still on the host
+    assert L.fs_create(ctypes.byref(cfg), None, 0, ctypes.byref(h)) == -1
+    assert b"power of two" in L.fs_last_error()
