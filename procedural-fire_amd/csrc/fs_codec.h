@@ -33,9 +33,12 @@ template <> struct Codec<StF16> {
         half4_t h = p[i];
         return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
     }
-    static __device__ __forceinline__ float4 ldb(__amdgpu_buffer_rsrc_t r, int off) {
-        const half4_t h = __builtin_bit_cast(half4_t, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    static __device__ __forceinline__ float4 dec(u32x2_t u) {
+        const half4_t h = __builtin_bit_cast(half4_t, u);
         return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
+    }
+    static __device__ __forceinline__ float4 ldb(__amdgpu_buffer_rsrc_t r, int off) {
+        return dec(__builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
     }
     static __device__ __forceinline__ void st(half4_t* p, int i, float4 v) {
         half4_t h;
@@ -48,9 +51,12 @@ template <> struct Codec<StBF16> {
         ushort4_t u = p[i];
         return make_float4(bf2f(u.x), bf2f(u.y), bf2f(u.z), bf2f(u.w));
     }
-    static __device__ __forceinline__ float4 ldb(__amdgpu_buffer_rsrc_t r, int off) {
-        const ushort4_t u = __builtin_bit_cast(ushort4_t, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    static __device__ __forceinline__ float4 dec(u32x2_t w) {
+        const ushort4_t u = __builtin_bit_cast(ushort4_t, w);
         return make_float4(bf2f(u.x), bf2f(u.y), bf2f(u.z), bf2f(u.w));
+    }
+    static __device__ __forceinline__ float4 ldb(__amdgpu_buffer_rsrc_t r, int off) {
+        return dec(__builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
     }
     static __device__ __forceinline__ void st(ushort4_t* p, int i, float4 v) {
         ushort4_t u;

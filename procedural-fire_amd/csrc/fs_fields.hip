@@ -117,10 +117,13 @@ __global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, D
     float4 vmn = make_float4(1e9f, 1e9f, 1e9f, 0.f), vmx = make_float4(-1e9f, -1e9f, -1e9f, 0.f);
     float4 smn = make_float4(1e9f, 1e9f, 1e9f, 1e9f), smx = make_float4(-1e9f, -1e9f, -1e9f, -1e9f);
     float4 tmn = smn, tmx = smx;
+    float4 cv[8], cs[8], ct[8];
+    corners8_f<S>(V, g, b0x, b0y, b0z, cv);
+    corners8_f<S>(SF, g, b0x, b0y, b0z, cs);
+    corners8_f<S>(S2F, g, b0x, b0y, b0z, ct);
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        const int qx = b0x + (k & 1), qy = b0y + ((k >> 1) & 1), qz = b0z + ((k >> 2) & 1);
-        const float4 vv = ldc_f<S>(V, g, qx, qy, qz), ss = ldc_f<S>(SF, g, qx, qy, qz), tt = ldc_f<S>(S2F, g, qx, qy, qz);
+        const float4 vv = cv[k], ss = cs[k], tt = ct[k];
         vmn.x = fminf(vmn.x, vv.x); vmn.y = fminf(vmn.y, vv.y); vmn.z = fminf(vmn.z, vv.z);
         vmx.x = fmaxf(vmx.x, vv.x); vmx.y = fmaxf(vmx.y, vv.y); vmx.z = fmaxf(vmx.z, vv.z);
         smn.x = fminf(smn.x, ss.x); smn.y = fminf(smn.y, ss.y); smn.z = fminf(smn.z, ss.z); smn.w = fminf(smn.w, ss.w);

@@ -134,7 +134,11 @@ __device__ __forceinline__ bool cell_of(const Dims& g, Cell& c) {
 // FBuf keeps the base and size in scalar registers and each load takes a 32-bit byte offset, so an
 // offset formed once per texel position serves every field read there, with no 64-bit address
 // arithmetic per load.  A field read this way stays below 4 GiB (checked when a sim is created).
+// With 8-byte texels (fp16, bf16) FBuf also reads two x-neighbours in one 16-byte load: the vector
+// memory path spends about the same cycles on a wave's 16-byte load as on its 8-byte load, and the
+// gather-heavy kernels are bound there.
 template <class S> struct FPtr {
+    static constexpr bool kPair = false;
     const typename S::raw* p;
     __device__ __forceinline__ float4 ld(int i) const { return Codec<S>::ld(p, i); }
 };
@@ -146,6 +150,16 @@ template <class S> struct FBuf {
                                               FS_BUF_WORD3)) {}
     __device__ __forceinline__ float4 ld(int i) const {
         return Codec<S>::ldb(r, (int)((unsigned)i * (unsigned)sizeof(typename S::raw)));
+    }
+    static constexpr bool kPair = sizeof(typename S::raw) == 8;
+    // The clamped texels x0 = clamp(ix, 0, n-1), x1 = clamp(ix+1, 0, n-1) of one row, from texels i and
+    // i + 1 where i is the row start + clamp(ix, 0, n-2) (n >= 2): both are the first texel when ix < 0
+    // (lo_both) and both the second when ix >= n - 1 (hi_both).  The selects act on the raw words.
+    __device__ __forceinline__ void ld_pair(int i, bool lo_both, bool hi_both, float4& a, float4& b) const {
+        const u32x4_t u = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((unsigned)i * 8u), 0, 0);
+        const u32x2_t lo = {u.x, u.y}, hi = {u.z, u.w};
+        a = Codec<S>::dec(hi_both ? hi : lo);
+        b = Codec<S>::dec(lo_both ? lo : hi);
     }
 };
 
@@ -169,18 +183,43 @@ __device__ __forceinline__ float4 sample_clamp_f(const F& f, const Dims& g, floa
     const float flx = floorf(tx), fly = floorf(ty), flz = floorf(tz);
     const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
     const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
-    const int x0 = clampi(ix, 0, g.nx - 1), x1 = clampi(ix + 1, 0, g.nx - 1);
     const int y0 = clampi(iy, 0, g.ny - 1), y1 = clampi(iy + 1, 0, g.ny - 1);
     const int z0 = clampi(iz, 0, g.nz - 1), z1 = clampi(iz + 1, 0, g.nz - 1);
     const int r00 = (z0 * g.ny + y0) * g.nx, r10 = (z0 * g.ny + y1) * g.nx;
     const int r01 = (z1 * g.ny + y0) * g.nx, r11 = (z1 * g.ny + y1) * g.nx;
-    const float4 c000 = f.ld(r00 + x0), c100 = f.ld(r00 + x1);
-    const float4 c010 = f.ld(r10 + x0), c110 = f.ld(r10 + x1);
-    const float4 c001 = f.ld(r01 + x0), c101 = f.ld(r01 + x1);
-    const float4 c011 = f.ld(r11 + x0), c111 = f.ld(r11 + x1);
+    float4 c000, c100, c010, c110, c001, c101, c011, c111;
+    if constexpr (F::kPair) {
+        const int xs = clampi(ix, 0, g.nx - 2);
+        const bool lo = ix < 0, hi = ix >= g.nx - 1;
+        f.ld_pair(r00 + xs, lo, hi, c000, c100);
+        f.ld_pair(r10 + xs, lo, hi, c010, c110);
+        f.ld_pair(r01 + xs, lo, hi, c001, c101);
+        f.ld_pair(r11 + xs, lo, hi, c011, c111);
+    } else {
+        const int x0 = clampi(ix, 0, g.nx - 1), x1 = clampi(ix + 1, 0, g.nx - 1);
+        c000 = f.ld(r00 + x0); c100 = f.ld(r00 + x1);
+        c010 = f.ld(r10 + x0); c110 = f.ld(r10 + x1);
+        c001 = f.ld(r01 + x0); c101 = f.ld(r01 + x1);
+        c011 = f.ld(r11 + x0); c111 = f.ld(r11 + x1);
+    }
     const float4 x00 = mix4(c000, c100, fx), x10 = mix4(c010, c110, fx);
     const float4 x01 = mix4(c001, c101, fx), x11 = mix4(c011, c111, fx);
     return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+}
+// the 8 clamped texels at (b0x + (k & 1), b0y + ((k >> 1) & 1), b0z + (k >> 2)), k = 0..7 (ld(), src/sim.js:114)
+template <class S, class F>
+__device__ __forceinline__ void corners8_f(const F& f, const Dims& g, int b0x, int b0y, int b0z, float4* c) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int y = clampi(b0y + (j & 1), 0, g.ny - 1), z = clampi(b0z + (j >> 1), 0, g.nz - 1);
+        const int row = (z * g.ny + y) * g.nx;
+        if constexpr (F::kPair) {
+            f.ld_pair(row + clampi(b0x, 0, g.nx - 2), b0x < 0, b0x >= g.nx - 1, c[2 * j], c[2 * j + 1]);
+        } else {
+            c[2 * j] = f.ld(row + clampi(b0x, 0, g.nx - 1));
+            c[2 * j + 1] = f.ld(row + clampi(b0x + 1, 0, g.nx - 1));
+        }
+    }
 }
 template <class S>
 __device__ __forceinline__ float4 sample_clamp(const typename S::raw* f, const Dims& g, float bx, float by, float bz) {
