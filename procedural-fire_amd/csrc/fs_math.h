@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fs_codec.h"
 #include "fs_divc.h"
 #include "fs_internal.h"
@@ -140,6 +142,7 @@ __device__ __forceinline__ bool cell_of(const Dims& g, Cell& c) {
 template <class S> struct FPtr {
     static constexpr bool kPair = false;
     const typename S::raw* p;
+    __device__ __forceinline__ FPtr(const typename S::raw* q, const Dims&) : p(q) {}
     __device__ __forceinline__ float4 ld(int i) const { return Codec<S>::ld(p, i); }
 };
 template <class S> struct FBuf {
@@ -171,7 +174,7 @@ __device__ __forceinline__ float4 ldc_f(const F& f, const Dims& g, int x, int y,
 }
 template <class S>
 __device__ __forceinline__ float4 ldc(const typename S::raw* f, const Dims& g, int x, int y, int z) {
-    return ldc_f<S>(FPtr<S>{f}, g, x, y, z);
+    return ldc_f<S>(FPtr<S>(f, g), g, x, y, z);
 }
 // trilinear, clamp-to-edge, at cell-space position b (texel centres at i + 0.5).
 // Clamping the coordinate to [-1, n] first leaves the clamped-texel result unchanged.
@@ -206,6 +209,9 @@ __device__ __forceinline__ float4 sample_clamp_f(const F& f, const Dims& g, floa
     const float4 x01 = mix4(c001, c101, fx), x11 = mix4(c011, c111, fx);
     return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
 }
+// FBuf where it pays in every kernel (8-byte texels, paired); 16-byte texels keep plain pointer reads
+template <class S> using FPair = typename std::conditional<sizeof(typename S::raw) == 8, FBuf<S>, FPtr<S>>::type;
+
 // the 8 clamped texels at (b0x + (k & 1), b0y + ((k >> 1) & 1), b0z + (k >> 2)), k = 0..7 (ld(), src/sim.js:114)
 template <class S, class F>
 __device__ __forceinline__ void corners8_f(const F& f, const Dims& g, int b0x, int b0y, int b0z, float4* c) {
@@ -223,7 +229,7 @@ __device__ __forceinline__ void corners8_f(const F& f, const Dims& g, int b0x, i
 }
 template <class S>
 __device__ __forceinline__ float4 sample_clamp(const typename S::raw* f, const Dims& g, float bx, float by, float bz) {
-    return sample_clamp_f<S>(FPtr<S>{f}, g, bx, by, bz);
+    return sample_clamp_f<S>(FPtr<S>(f, g), g, bx, by, bz);
 }
 // trilinear with repeat addressing at normalised coordinate q (noise textures, n = 2^k)
 __device__ __forceinline__ float4 ld_h4(const half4_t* t, int i) {

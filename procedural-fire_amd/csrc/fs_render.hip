@@ -194,8 +194,8 @@ __device__ __forceinline__ float hg(float c, float g) {   // Henyey-Greenstein p
 }
 // cubic B-spline reconstruction from 8 trilinear fetches (Sigg & Hadwiger 2005); g is the
 // cell-space position (texel centres at i + 0.5), and so are the fetch positions h0, h1
-template <class SS>
-__device__ __forceinline__ float4 sample_cubic(const typename SS::raw* sT, const Dims& d, F3 g) {
+template <class SS, class F>
+__device__ __forceinline__ float4 sample_cubic(const F& sT, const Dims& d, F3 g) {
     const float cs[3] = {g.x - 0.5f, g.y - 0.5f, g.z - 0.5f};
     float h0[3], h1[3], g0[3], g1[3];
 #pragma unroll
@@ -215,7 +215,7 @@ __device__ __forceinline__ float4 sample_cubic(const typename SS::raw* sT, const
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const bool ox = k & 1, oy = k & 2, oz = k & 4;
-        const float4 s = sample_clamp<SS>(sT, d, ox ? h1[0] : h0[0], oy ? h1[1] : h0[1], oz ? h1[2] : h0[2]);
+        const float4 s = sample_clamp_f<SS>(sT, d, ox ? h1[0] : h0[0], oy ? h1[1] : h0[1], oz ? h1[2] : h0[2]);
         const float wv = (ox ? g1[0] : g0[0]) * (oy ? g1[1] : g0[1]) * (oz ? g1[2] : g0[2]);
         acc.x += s.x * wv; acc.y += s.y * wv; acc.z += s.z * wv; acc.w += s.w * wv;
     }
@@ -230,6 +230,8 @@ __global__ __launch_bounds__(256) void k_volume(RenderU R, Dims d, const typenam
     const int tilesX = ((int)vw + 15) / 16, tilesY = ((int)vh + 15) / 16;
     const int tile = xcd_remap((int)blockIdx.x, tilesX * tilesY);
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const FPair<SS> ST(sT, d);        // x-neighbour texel pairs through buffer resources for 8-byte texels
+    const FPair<S2S> S2T(s2T, d);
     const int ix = (tile % tilesX) * 16 + (wv & 1) * 8 + (lane & 7);
     const int iy = (tile / tilesX) * 16 + (wv >> 1) * 8 + (lane >> 3);
     if ((float)ix >= vw || (float)iy >= vh) return;
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(256) void k_volume(RenderU R, Dims d, const typenam
                 g = v3(g.x + cwx * k12 * 1.0f, g.y + cwy * k12 * 0.6f, g.z + cwz * k12 * 1.0f);
                 mottle = cww * 0.08f - 0.6f;
             }
-            float4 s = R.f[RP_cubic] > 0.5f ? sample_cubic<SS>(sT, d, g) : sample_clamp<SS>(sT, d, g.x, g.y, g.z);
+            float4 s = R.f[RP_cubic] > 0.5f ? sample_cubic<SS>(ST, d, g) : sample_clamp_f<SS>(ST, d, g.x, g.y, g.z);
             // fade towards the open sides and top
             const float edge = fminf(fminf(g.x, nxf - g.x), fminf(g.z, nzf - g.z));
             const float fade = sstep(0.02f, 0.15f, edge * dx) * sstep(0.0f, 0.4f, (nyf - g.y) * dx);
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(256) void k_volume(RenderU R, Dims d, const typenam
             float soot = s.z;
             if (R.f[RP_sootEdge] > 0.0f) soot = soot * sstep(0.6f * R.f[RP_sootEdge], 1.4f * R.f[RP_sootEdge], soot);
             float smoke = 0.0f;
-            if (R.f[RP_smokeScat] > 0.0f) smoke = sample_clamp<S2S>(s2T, d, g.x, g.y, g.z).z * fade;
+            if (R.f[RP_smokeScat] > 0.0f) smoke = sample_clamp_f<S2S>(S2T, d, g.x, g.y, g.z).z * fade;
             if (soot > 1e-4f || s.w > 1e-3f || smoke > 1e-4f) {
                 const float sigSoot = soot * R.f[RP_sootAbs];
                 const float sigS = sigSoot * R.f[RP_sootScat] + smoke * R.f[RP_smokeScat];
