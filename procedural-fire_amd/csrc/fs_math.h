@@ -250,6 +250,42 @@ __device__ __forceinline__ float4 sample_rep(const half4_t* t, int n, float qx, 
     const float4 x11 = mix4(ld_h4(t, r11 + x0), ld_h4(t, r11 + x1), fx);
     return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
 }
+// The same sample from an n^3 binary16 texture read through a buffer resource: each row's texels x0, x0 + 1
+// come in one 16-byte load.  Where x0 = n - 1 wraps to x1 = 0, the second half of that load (the next row's
+// first texel, or zeros past the end of the buffer, whose range check returns 0) is replaced by a second load,
+// behind one branch that waves with no wrapping lane skip.  Bit-identical to sample_rep.
+struct RepTex {
+    __amdgpu_buffer_rsrc_t r;
+    int n;
+    __device__ __forceinline__ RepTex(const half4_t* t, int nn)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void*)t, (short)0, (int)((unsigned)nn * (unsigned)nn * (unsigned)nn * 8u),
+                                              FS_BUF_WORD3)),
+          n(nn) {}
+};
+__device__ __forceinline__ float4 sample_rep_b(const RepTex& t, float qx, float qy, float qz) {
+    const int n = t.n;
+    const float tx = qx * (float)n - 0.5f, ty = qy * (float)n - 0.5f, tz = qz * (float)n - 0.5f;
+    const float flx = floorf(tx), fly = floorf(ty), flz = floorf(tz);
+    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+    const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
+    const int m = n - 1;
+    const int x0 = ix & m, y0 = iy & m, y1 = (iy + 1) & m, z0 = iz & m, z1 = (iz + 1) & m;
+    const int rows[4] = {(z0 * n + y0) * n, (z0 * n + y1) * n, (z1 * n + y0) * n, (z1 * n + y1) * n};
+    float4 a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const u32x4_t u = __builtin_amdgcn_raw_buffer_load_b128(t.r, (int)((unsigned)(rows[j] + x0) * 8u), 0, 0);
+        a[j] = Codec<StF16>::dec(u32x2_t{u.x, u.y});
+        b[j] = Codec<StF16>::dec(u32x2_t{u.z, u.w});
+    }
+    if (x0 == m) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[j] = Codec<StF16>::ldb(t.r, (int)((unsigned)rows[j] * 8u));
+    }
+    const float4 x00 = mix4(a[0], b[0], fx), x10 = mix4(a[1], b[1], fx);
+    const float4 x01 = mix4(a[2], b[2], fx), x11 = mix4(a[3], b[3], fx);
+    return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+}
 
 template <class S>
 __device__ __forceinline__ float3 backtrace(const typename S::raw* vel, const Dims& g, float cx, float cy, float cz,

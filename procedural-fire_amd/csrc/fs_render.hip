@@ -232,6 +232,7 @@ __global__ __launch_bounds__(256) void k_volume(RenderU R, Dims d, const typenam
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const FPair<SS> ST(sT, d);        // x-neighbour texel pairs through buffer resources for 8-byte texels
     const FPair<S2S> S2T(s2T, d);
+    const RepTex CT(in.curl, in.nn);
     const int ix = (tile % tilesX) * 16 + (wv & 1) * 8 + (lane & 7);
     const int iy = (tile / tilesX) * 16 + (wv >> 1) * 8 + (lane >> 3);
     if ((float)ix >= vw || (float)iy >= vh) return;
@@ -300,8 +301,8 @@ __global__ __launch_bounds__(256) void k_volume(RenderU R, Dims d, const typenam
                 const float q1x = (p.x - 0.0f) * fq * 0.25f + 0.71f * fl1;
                 const float q1y = (p.y - rise * f1 * 0.7f) * fq * 0.25f + 0.0f * fl1;
                 const float q1z = (p.z - 0.0f) * fq * 0.25f + 0.29f * fl1;
-                const float4 c0 = sample_rep(in.curl, in.nn, q0x, q0y, q0z);
-                const float4 c1 = sample_rep(in.curl, in.nn, q1x, q1y, q1z);
+                const float4 c0 = sample_rep_b(CT, q0x, q0y, q0z);
+                const float4 c1 = sample_rep_b(CT, q1x, q1y, q1z);
                 const float cwx = c0.x * w0 + c1.x * (1.0f - w0), cwy = c0.y * w0 + c1.y * (1.0f - w0);
                 const float cwz = c0.z * w0 + c1.z * (1.0f - w0), cww = c0.w * w0 + c1.w * (1.0f - w0);
                 const float k12 = detail * 0.12f;
