@@ -130,7 +130,10 @@ const char* fs_version(void);
  * (src/sim.js:49-101).  `logs` is packLogs() output: n_logs records of 8 floats
  * (a.xyz, r, b.xyz, burn) (src/scene.js:92-98); the noise and curl textures are
  * baked on the device from cfg->noise_seed (src/noise.js:6-90, src/main.js:36-37).
- * The state starts zeroed, as after reset(). */
+ * The state starts zeroed, as after reset().  FS_EINVAL (before any device call) for a
+ * resolution below 2 on an axis, or one whose 4-channel field reaches 4 GiB (fp32
+ * storage: 2^28 cells; fp16 / bf16: 2^29 cells; the kernels address a field with
+ * 32-bit byte offsets). */
 int  fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out);
 int  fs_destroy(fs_sim* h);
 
