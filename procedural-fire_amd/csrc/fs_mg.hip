@@ -386,8 +386,9 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
                 const int xx = 2 * hx + sgn;
                 const float bf = (float)r[sgn];
                 rhs_out[own] = bf;
-                p_out[own] = (red_first && ((xx + y + z) & 1) == 0) ? red_from_zero(bf, xx, y, z, nx, ny, nz, hred, omega)
-                                                                   : 0.f;
+                if (p_out)
+                    p_out[own] = (red_first && ((xx + y + z) & 1) == 0) ? red_from_zero(bf, xx, y, z, nx, ny, nz, hred, omega)
+                                                                       : 0.f;
             }
         }
     }
@@ -832,12 +833,13 @@ void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
 }
 
 // FP64 defect correction x <- x + S(b - A x) with the exact separable inner solve S (fs_spectral.hip)
-// in place of a V-cycle.  The residual kernel writes r as the right-hand side (and zeroes p); once the
-// stop flag is set, the correction and the residual of later iterations are skipped as for V-cycles.
+// in place of a V-cycle.  The residual kernel writes r as the right-hand side; p is not touched (the
+// solve overwrites all of it).  Once the stop flag is set, the correction and the residual of later
+// iterations are skipped as for V-cycles.
 void mg_spectral_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
-    k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
+    k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], nullptr,
                                                                   mg.partials, &mg.st->never);
     k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
 }
@@ -849,7 +851,7 @@ int mg_spectral_iter(GpuMG& mg, Spectral* sp, hipStream_t s, double tol, int max
     if (int rc = spectral_solve(sp, s, mg.b[0], mg.p[0])) return rc;
     const int n = (int)level_floats(mg.dims[0]);
     k_update<<<(n + RB - 1) / RB, RB, 0, s>>>(mg.x64, mg.p[0], n, done);
-    k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
+    k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], nullptr,
                                                                   mg.partials, done);
     k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
     return FS_OK;
