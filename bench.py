@@ -849,8 +849,9 @@ def main():
         "mixed_spectral_value": vspec,
         "mixed_spectral_ms_per_step": runs["mixed_spectral"]["t"] / args.steps * 1e3,
         "mixed_spectral_vs_fp32": vspec / v32,
-        "mixed_spectral_solve": "exact separable inner solve (DST-II/DCT-IV transforms as FP32 rocBLAS GEMMs) "
-                                "in FP64 defect correction, tol 1e-5 (FS_SOLVE_SPECTRAL64)",
+        "mixed_spectral_solve": "exact separable inner solve (DST-II/DCT-IV transforms on the matrix cores: x and y "
+                                "as FP32 rocBLAS GEMMs, z + eigenvalue division + inverse z as one fused MFMA "
+                                "kernel) in FP64 defect correction, tol 1e-5 (FS_SOLVE_SPECTRAL64)",
         "vcycles_per_step": {k: v["vcycles"] for k, v in runs.items()},
         "rel_residual": {k: v["rel"] for k, v in runs.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
