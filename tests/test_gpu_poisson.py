@@ -19,8 +19,10 @@ def _rhs(dims, seed=0):
     return b
 
 
+# 96^3 keeps three levels (96, 48, 24) outside the single-workgroup tail, so its level 2 takes the
+# prolongation fused into the first post-smoothing pass; the smaller grids reach the tail by level 2.
 @pytest.mark.parametrize("omega", [1.0, 1.25])
-@pytest.mark.parametrize("dims", [(32, 48, 32), (64, 96, 64), (20, 30, 22), (21, 30, 22), (12, 18, 10)])
+@pytest.mark.parametrize("dims", [(32, 48, 32), (64, 96, 64), (20, 30, 22), (21, 30, 22), (12, 18, 10), (96, 96, 96)])
 def test_vcycles_bit_exact_vs_oracle(dims, omega):
     print(f"poisson vcycles {dims} omega {omega}", flush=True)
     h = 1.0 / 128          # exactly representable: the device keeps the spacing as f32
