@@ -306,8 +306,29 @@ __global__ __launch_bounds__(256) void k_forces_adj(const DevParams* __restrict_
         return Lv.x * vx.d + Lv.y * vy.d + Lv.z * vz.d + Lv.w * phi.d + Ls.x * s[0].d + Ls.y * s[1].d + Ls.z * s[2].d +
                Ls.w * s[3].d + Ls2.x * s2[0].d + Ls2.y * s2[1].d + Ls2.z * s2[2].d + Ls2.w * s2[3].d;
     };
+    // the 20 state inputs: four seeded per vector-mode pass (Dual4 gives each column the scalar pass's bits)
     float l[FORCES_IN];
-    for (int j = 0; j < FORCES_IN; j++) l[j] = pass(j, -1, false);
+    for (int j0 = 0; j0 < FORCES_IN; j0 += 4) {
+        Dual4 x[FORCES_IN];
+#pragma unroll
+        for (int j = 0; j < FORCES_IN; j++) {
+            x[j] = Dual4(in0[j]);
+            FS_D4(x[j].d[u] = j == j0 + u ? 1.f : 0.f);
+        }
+        Dual4 vx = x[0], vy = x[1], vz = x[2], phi;
+        Dual4 s[4] = {x[3], x[4], x[5], x[6]}, s2[4] = {x[7], x[8], x[9], x[10]};
+        const Dual4 w[3] = {x[11], x[12], x[13]};
+        const Dual4 gx = x[14] - x[15], gy = x[16] - x[17], gz = x[18] - x[19];
+        forces_cell<Dual4>(P->f, P->d, k, spots, P->n_spots, -1, nullptr, face, Dual4(fv), vx, vy, vz, s, s2, w, gx, gy,
+                           gz, phi);
+        float out[4];
+        FS_D4(out[u] = Lv.x * vx.d[u] + Lv.y * vy.d[u] + Lv.z * vz.d[u] + Lv.w * phi.d[u] + Ls.x * s[0].d[u] +
+                       Ls.y * s[1].d[u] + Ls.z * s[2].d[u] + Ls.w * s[3].d[u] + Ls2.x * s2[0].d[u] + Ls2.y * s2[1].d[u] +
+                       Ls2.z * s2[2].d[u] + Ls2.w * s2[3].d[u]);
+#pragma unroll
+        for (int j = 0; j < FORCES_IN; j++)
+            FS_D4(if (j == j0 + u) l[j] = out[u]);
+    }
     lvY[c.i] = make_float4(l[0], l[1], l[2], 0.f);
     lsY[c.i] = make_float4(l[3], l[4], l[5], l[6]);
     ls2Y[c.i] = make_float4(l[7], l[8], l[9], l[10]);

@@ -77,6 +77,75 @@ template <> __device__ __forceinline__ Dual flux_as<Dual>(float f, int q, int se
 __device__ __forceinline__ bool gisnan(float a) { return isnan(a); }
 __device__ __forceinline__ bool gisnan(Dual a) { return isnan(a.v); }
 
+// ---- four directional derivatives at once (vector forward mode) -----------------------
+// Each derivative component goes through exactly the operations of Dual's, in the same order, so a
+// Dual4 pass gives four Dual passes' derivatives bit for bit while evaluating the value path once.
+struct Dual4 {
+    float v, d[4];
+    Dual4() = default;
+    __device__ __forceinline__ explicit Dual4(float v_) : v(v_), d{0.f, 0.f, 0.f, 0.f} {}
+};
+#define FS_D4(expr) do { _Pragma("unroll") for (int u = 0; u < 4; u++) { expr; } } while (0)
+__device__ __forceinline__ Dual4 operator+(Dual4 a, Dual4 b) { Dual4 r(a.v + b.v); FS_D4(r.d[u] = a.d[u] + b.d[u]); return r; }
+__device__ __forceinline__ Dual4 operator+(Dual4 a, float b) { Dual4 r(a.v + b); FS_D4(r.d[u] = a.d[u]); return r; }
+__device__ __forceinline__ Dual4 operator+(float a, Dual4 b) { Dual4 r(a + b.v); FS_D4(r.d[u] = b.d[u]); return r; }
+__device__ __forceinline__ Dual4 operator-(Dual4 a, Dual4 b) { Dual4 r(a.v - b.v); FS_D4(r.d[u] = a.d[u] - b.d[u]); return r; }
+__device__ __forceinline__ Dual4 operator-(Dual4 a, float b) { Dual4 r(a.v - b); FS_D4(r.d[u] = a.d[u]); return r; }
+__device__ __forceinline__ Dual4 operator-(float a, Dual4 b) { Dual4 r(a - b.v); FS_D4(r.d[u] = -b.d[u]); return r; }
+__device__ __forceinline__ Dual4 operator-(Dual4 a) { Dual4 r(-a.v); FS_D4(r.d[u] = -a.d[u]); return r; }
+__device__ __forceinline__ Dual4 operator*(Dual4 a, Dual4 b) {
+    Dual4 r(a.v * b.v);
+    FS_D4(r.d[u] = a.d[u] * b.v + a.v * b.d[u]);
+    return r;
+}
+__device__ __forceinline__ Dual4 operator*(Dual4 a, float b) { Dual4 r(a.v * b); FS_D4(r.d[u] = a.d[u] * b); return r; }
+__device__ __forceinline__ Dual4 operator*(float a, Dual4 b) { Dual4 r(a * b.v); FS_D4(r.d[u] = a * b.d[u]); return r; }
+__device__ __forceinline__ Dual4 operator/(Dual4 a, Dual4 b) {
+    Dual4 r(a.v / b.v);
+    FS_D4(r.d[u] = (a.d[u] * b.v - a.v * b.d[u]) / (b.v * b.v));
+    return r;
+}
+__device__ __forceinline__ Dual4 operator/(Dual4 a, float b) { Dual4 r(a.v / b); FS_D4(r.d[u] = a.d[u] / b); return r; }
+__device__ __forceinline__ Dual4 operator/(float a, Dual4 b) {
+    Dual4 r(a / b.v);
+    FS_D4(r.d[u] = -a * b.d[u] / (b.v * b.v));
+    return r;
+}
+__device__ __forceinline__ float gval(Dual4 a) { return a.v; }
+__device__ __forceinline__ Dual4 gmin(Dual4 a, Dual4 b) { return a.v <= b.v ? a : b; }
+__device__ __forceinline__ Dual4 gmax(Dual4 a, Dual4 b) { return a.v >= b.v ? a : b; }
+__device__ __forceinline__ Dual4 gmin(Dual4 a, float b) { return a.v <= b ? a : Dual4(b); }
+__device__ __forceinline__ Dual4 gmax(Dual4 a, float b) { return a.v >= b ? a : Dual4(b); }
+__device__ __forceinline__ Dual4 gmin(float a, Dual4 b) { return gmin(b, a); }
+__device__ __forceinline__ Dual4 gmax(float a, Dual4 b) { return gmax(b, a); }
+__device__ __forceinline__ Dual4 gsstep(float e0, float e1, Dual4 x) {
+    const Dual4 t = gclamp((x - e0) / (e1 - e0), 0.f, 1.f);
+    return t * t * (3.f - 2.f * t);
+}
+__device__ __forceinline__ Dual4 gsstep_c(float e0, float e1, Dual4 x) { return gsstep(e0, e1, x); }
+__device__ __forceinline__ Dual4 gexp(Dual4 x) {
+    const float e = fs_exp(x.v);
+    Dual4 r(e);
+    FS_D4(r.d[u] = e * x.d[u]);
+    return r;
+}
+__device__ __forceinline__ Dual4 gpow(Dual4 a, float b) {   // d/da a^b = b a^(b-1)
+    Dual4 r(fs_pow(a.v, b));
+    if (a.v > 0.f) {
+        const float k = b * fs_pow(a.v, b - 1.f);
+        FS_D4(r.d[u] = k * a.d[u]);
+    }
+    return r;
+}
+__device__ __forceinline__ Dual4 gsqrt(Dual4 a) {
+    const float rt = sqrtf(a.v);
+    Dual4 r(rt);
+    if (rt > 0.f) FS_D4(r.d[u] = a.d[u] / (2.f * rt));
+    return r;
+}
+template <> __device__ __forceinline__ Dual4 flux_as<Dual4>(float f, int, int, const float*) { return Dual4(f); }
+__device__ __forceinline__ bool gisnan(Dual4 a) { return isnan(a.v); }
+
 // ---- the state-independent part of one cell's update -----------------------------
 struct ForcesCell {
     float dt, dx, tm, I;
