@@ -80,7 +80,13 @@ FORMS = {"fp32": (F.PREC_FP32, F.SOLVE_VCYCLES), "fp32_tol": (F.PREC_FP32, F.SOL
          "mixed_spectral": (F.PREC_FP16, F.SOLVE_SPECTRAL64), "mixed": (F.PREC_FP16, F.SOLVE_REFINE64)}        # red-black sweeps before / after the coarse correction in the tolerance solves (0: default)
 
 
-def make_sim(precision, solve, use_graph=True):
+# rocprofv3 exports its ROCPROF_* settings to the program it runs.  Under it every mode launches the
+# step's kernels eagerly: a kernel-traced run with graph replay once ended in a host SIGSEGV inside the
+# HIP runtime's graph launch path, and eager launches trace every dispatch with the same kernels.
+UNDER_PROFILER = any(k.startswith("ROCPROF_") for k in os.environ)
+
+
+def make_sim(precision, solve, use_graph=not UNDER_PROFILER):
     sw = SWEEPS if solve != F.SOLVE_VCYCLES else (0, 0)
     if SCENE == "campfire":
         logs, _ = F.build_campfire(7)
@@ -112,7 +118,10 @@ def profile_kernels(sim, steps=5):
     return {k: v / steps for k, v in acc.items()}
 
 
-def cpu_baseline(gpu_sim, steps=3):
+def cpu_baseline(gpu_sim, steps=5, steps_1core=3):
+    """The oracle in reference semantics (fp16 storage, f32 math, 2 x V(2,2)) stepping the GPU's
+    warmed state on the host: the median of `steps` steps on every core the process may use (after
+    one warm-up step), then the median of `steps_1core` steps on one core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import fsoracle as O
     if SCENE == "campfire":
@@ -128,16 +137,28 @@ def cpu_baseline(gpu_sim, steps=3):
         orc.write(f, gpu_sim.read(f, np.float64))
     orc.write(O.P, gpu_sim.read(F.P, np.float64))
     orc.set_clock(gpu_sim.time, gpu_sim.frame)
+
+    def median_step(n):
+        ts = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            orc.step(DT, 1)
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    threads = O.num_threads()
     orc.step(DT, 1)   # warm-up
-    ts = []
-    for _ in range(steps):
-        t0 = time.perf_counter()
-        orc.step(DT, 1)
-        ts.append(time.perf_counter() - t0)
-    med = float(np.median(ts))
-    return {"value": float(np.prod(RES)) / med, "unit": "voxel-steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"{steps} steps (median, after 1 warm-up) of the 128x192x128 step in reference semantics "
-                      f"(fp16 storage, f32 math, 2 V(2,2)) from the warmed GPU state; {med:.2f} s/step"}
+    med = median_step(steps)
+    O.set_threads(1)
+    med1 = median_step(steps_1core)
+    O.set_threads(threads)
+    ncell = float(np.prod(RES))
+    return {"value": ncell / med, "unit": "voxel-steps/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "value_1core": ncell / med1, "s_per_step": med, "s_per_step_1core": med1,
+            "sample": f"median of {steps} steps (after 1 warm-up) on {threads} threads, then median of "
+                      f"{steps_1core} steps on 1 thread, of the 128x192x128 step in reference semantics (fp16 "
+                      f"storage, f32 math, 2 V(2,2)) from the warmed GPU state; {med:.2f} s/step on {threads} "
+                      f"threads, {med1:.2f} s/step on 1 (nproc {os.cpu_count()})"}
 
 
 def errors_vs_fp64(warmup):
@@ -288,10 +309,10 @@ def flame_statistics(args):
         if name == "campfire":
             logs, _ = F.build_campfire(7)
             sim = F.FireSim(res=sc["res"], size=(1.2, 1.8, 1.2), logs=logs, vcycles=sc["vcycles"], precision=prec,
-                            solve=solve, div_tol=1e-5, max_vcycles=8, noise_n=128, use_graph=True)
+                            solve=solve, div_tol=1e-5, max_vcycles=8, noise_n=128, use_graph=not UNDER_PROFILER)
         else:
             sim = scenes.make_scene(name, seed=7, precision=prec, solve=solve, div_tol=1e-5, max_vcycles=8,
-                                    noise_n=128, use_graph=True)
+                                    noise_n=128, use_graph=not UNDER_PROFILER)
         sim.diag_enable(steps, t_flame=1000.0)
         t0 = time.perf_counter()
         done = 0
@@ -481,7 +502,7 @@ def sparks_bench(args):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import fsoracle as O
     sim = scenes.make_scene("spots8_128", precision=F.PREC_FP16, solve=F.SOLVE_REFINE64, div_tol=1e-5, max_vcycles=8,
-                            noise_n=128, use_graph=True)
+                            noise_n=128, use_graph=not UNDER_PROFILER)
     sim.step(DT, 100)
     sim.sync()
     out = {"workload": "sparks over the 8-spot 128x192x128 fire (fp16 fields, 100 steps from rest), dt 1 ms"}
@@ -536,7 +557,7 @@ def capture_bench(args):
     out = {"workload": "capture loop, 80x120x80 campfire (fp16, 2 V-cycles), dt 1/240 s, 8 steps per 30-fps frame"}
     for label, with_sparks in (("driver+sparks", True), ("driver", False)):
         sim = F.FireSim(res=(80, 120, 80), size=(1.2, 1.8, 1.2), logs=logs, vcycles=2, precision=F.PREC_FP16,
-                        noise_n=128, use_graph=True)
+                        noise_n=128, use_graph=not UNDER_PROFILER)
         sparks = F.Sparks(sim, 8192) if with_sparks else None
         cap = D.Capture(sim, sparks, logs, sim_dt=1 / 240)
         cap.begin(preroll=1.0)
@@ -553,7 +574,7 @@ def capture_bench(args):
         sim.close()
         print(f"{label}: {t * 1e3:.3f} ms per frame", flush=True)
     sim = F.FireSim(res=(80, 120, 80), size=(1.2, 1.8, 1.2), logs=logs, vcycles=2, precision=F.PREC_FP16,
-                    noise_n=128, use_graph=True)
+                    noise_n=128, use_graph=not UNDER_PROFILER)
     sim.step(1 / 240, 240)
     sim.sync()
     t0 = time.perf_counter()
@@ -762,7 +783,7 @@ def main():
         volume_bench(args)
         return
     ncell = float(np.prod(RES))
-    graph = not args.no_graph
+    graph = not args.no_graph and not UNDER_PROFILER
     if args.trace_form:
         prec, solve = FORMS[args.trace_form]
         sim = make_sim(prec, solve, graph)

@@ -1234,6 +1234,19 @@ void orc_bake_noise(int n, int seed, float* noise_out, float* curl_out) {
 void orc_hash3f(int x, int y, int z, float* out) { hash3f(x, y, z, out); }
 void orc_f16_round(const float* in, float* out, long n) { for (long i = 0; i < n; i++) out[i] = f16_to_f32(f32_to_f16(in[i])); }
 void orc_bf16_round(const float* in, float* out, long n) { for (long i = 0; i < n; i++) out[i] = bf16_to_f32(f32_to_bf16(in[i])); }
+// The f32 transcendental functions of the oracle (and of the kernels): which = 0 exp(a), 1 log(a),
+// 2 pow(a, b), 3 sin(a), 4 cos(a).
+void orc_fmath(int which, const float* a, const float* b, float* out, long n) {
+    for (long i = 0; i < n; i++) {
+        switch (which) {
+            case 0: out[i] = fexp(a[i]); break;
+            case 1: out[i] = flog(a[i]); break;
+            case 2: out[i] = fpow(a[i], b[i]); break;
+            case 3: out[i] = fsin(a[i]); break;
+            default: out[i] = fcos(a[i]); break;
+        }
+    }
+}
 
 // ---- stand-alone sim.js multigrid (isolated Poisson), FP32 or FP64 storage ----
 struct orc_mg { Multigrid<float>* f; Multigrid<double>* d; std::vector<double> x; };
@@ -1338,6 +1351,8 @@ double orc_mjs_study(int N, int face, int mode, double omega, int pre, int post,
 
 const char* orc_last_error(void) { return g_err.c_str(); }
 int orc_num_threads(void) { return omp_get_max_threads(); }
+// Thread count of the oracle's parallel loops (the CPU baseline runs once on every core, once on one).
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 
 // ---- spark particles: the host side of src/sparks.js:17-46, 241-271 ------------------

@@ -154,6 +154,12 @@ def num_threads():
     return lib().orc_num_threads()
 
 
+def set_threads(n):
+    """Thread count of the oracle's OpenMP loops for the rest of the process."""
+    lib().orc_set_threads.argtypes = [ctypes.c_int]
+    lib().orc_set_threads(int(n))
+
+
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
@@ -205,6 +211,21 @@ def bf16_round(a):
     return out
 
 
+_FMATH = {"exp": 0, "log": 1, "pow": 2, "sin": 3, "cos": 4}
+
+
+def fmath(name, a, b=None):
+    """The oracle's f32 exp / log / pow / sin / cos (the textbook constructions the kernels share)."""
+    L = lib()
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.orc_fmath.argtypes = [ctypes.c_int, fp, fp, fp, ctypes.c_long]
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    bb = np.ascontiguousarray(b if b is not None else np.zeros_like(a), np.float32).ravel()
+    out = np.empty_like(a)
+    L.orc_fmath(_FMATH[name], _fp(a), _fp(bb), _fp(out), a.size)
+    return out
+
+
 class OracleSim:
     """CPU FireSim restatement.  precision: FP32/FP16/BF16 (f32 math) or FP64."""
 
@@ -247,6 +268,11 @@ class OracleSim:
     def set_param_block(self, p):
         p = np.ascontiguousarray(p, np.float32)
         lib().orc_set_params(self.h, _fp(p))
+
+    def set_logs(self, logs):
+        """Rewrites the packed logs (8 floats per log) and invalidates the geometry (src/main.js:147-149)."""
+        a = np.ascontiguousarray(logs, np.float32)
+        lib().orc_set_logs(self.h, _fp(a), a.size // 8)
 
     def get_params(self):
         out = np.zeros(NUM_PARAMS, np.float32)
