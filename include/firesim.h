@@ -249,6 +249,36 @@ int  fs_diag_enable(fs_sim* h, int capacity, double t_flame);
 /* Copies the held records, oldest first (at most max_records, the newest kept). */
 int  fs_diag_read(fs_sim* h, double* out, int max_records, int* n_records);
 
+/* ---- FP64 mass and energy budget of the steps (no counterpart in the reference) ----
+ * With the budget on, every later step runs its source kernel (src/sim.js:214-396) in a tracing
+ * form that writes the same fields, bit for bit, and reduces on the device, in FP64, the fuel
+ * fraction and the temperature at the boundaries of the cell update's sub-models.  Each term is
+ * a difference of consecutive f32 values of one cell, so the terms telescope:
+ *   FUEL = FUEL0 + TRANSPORT + SOURCES + INJECTED - BURNED - DISSIPATED - REMOVED + ROUNDING,
+ *   HEAT = HEAT0 + TRANSPORT + SOURCES + RELEASED - RADIATED - ENTRAINED - REMOVED + ROUNDING
+ * up to the FP64 summation.  Masses are rho0 dx^3 sum(Y) (kg, rho0 = FS_RHO0); heat is the
+ * constant-density thermal energy rho0 cp dx^3 sum(T - 293 K) (J, cp = 1005 J/(kg K)).
+ * TRANSPORT is the change by advection and projection (inflow and outflow through the open
+ * faces and the semi-Lagrangian scheme's non-conservation); ROUNDING the f32 conversions and
+ * the storage rounding of the fields.  FUEL_EXPECTED accumulates sum(spot flux) dt + sum(floor
+ * flux) dx^2 dt, what INJECTED equals where no cell reaches a fuel fraction of 1.  EXPANSION and
+ * DIVERGENCE are the last step's sum(phi) dx^3 and sum(div - phi) dx^3 (m^3/s): the volume the
+ * flame's gas expansion asks for, and the net outflow the projection must realise.  Cells whose
+ * trace is not finite are counted in BAD_CELLS and left out of the sums. */
+enum {
+    FS_BUDGET_STEPS = 0, FS_BUDGET_FUEL0, FS_BUDGET_FUEL, FS_BUDGET_FUEL_TRANSPORT, FS_BUDGET_FUEL_SOURCES,
+    FS_BUDGET_FUEL_INJECTED, FS_BUDGET_FUEL_BURNED, FS_BUDGET_FUEL_DISSIPATED, FS_BUDGET_FUEL_REMOVED,
+    FS_BUDGET_FUEL_ROUNDING, FS_BUDGET_HEAT0, FS_BUDGET_HEAT, FS_BUDGET_HEAT_TRANSPORT, FS_BUDGET_HEAT_SOURCES,
+    FS_BUDGET_HEAT_RELEASED, FS_BUDGET_HEAT_RADIATED, FS_BUDGET_HEAT_ENTRAINED, FS_BUDGET_HEAT_REMOVED,
+    FS_BUDGET_HEAT_ROUNDING, FS_BUDGET_FUEL_EXPECTED, FS_BUDGET_EXPANSION, FS_BUDGET_DIVERGENCE,
+    FS_BUDGET_BAD_CELLS, FS_BUDGET_N = 24
+};
+/* on = 1: clears the record and starts it from the current state (FUEL0, HEAT0); on = 0 stops
+ * recording (the record is kept for fs_budget_read). */
+int  fs_budget_enable(fs_sim* h, int on);
+/* Copies the FS_BUDGET_N doubles of the record (waits for the handle's stream). */
+int  fs_budget_read(fs_sim* h, double out[FS_BUDGET_N]);
+
 /* Measurement utility with no counterpart in the reference: times `iters`
  * device-to-device copies of `bytes` with a plain streaming kernel (the box's
  * achievable HBM bandwidth, and the calibration of bandwidth counters). */

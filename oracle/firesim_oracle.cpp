@@ -24,6 +24,7 @@
 #include <vector>
 
 #include <omp.h>
+#include <array>
 
 #include "../include/firesim.h"
 #include "oracle_common.h"
@@ -400,6 +401,8 @@ struct OracleBase {
     virtual void set_flux64(const double* spot, int n, const double* face) = 0;
     virtual void spark_update(const struct SparkU& U, float* sp, int count) const = 0;
     virtual void box(double org[3], float size[3]) const = 0;
+    virtual void budget_enable(int on) = 0;
+    virtual void budget_read(double* out) const = 0;
 };
 
 template <class R, class S, class M>
@@ -426,6 +429,11 @@ struct Oracle : OracleBase {
     // oracle with steps below float resolution); empty = the float values above
     std::vector<double> spot64, face64;
     double last_rel = -1; int last_used = 0;
+    // FP64 mass and energy budget (the library's fs_budget_*): per-z partial sums of the per-cell
+    // terms of the source kernel, combined in z order, and the record
+    bool bud_on = false;
+    double bud[FS_BUDGET_N] = {};
+    std::vector<std::array<double, 18>> bud_z;
 
     Oracle(const fs_config* c, const float* lg, int nl) {
         nx = c->res[0]; ny = c->res[1]; nz = c->res[2];
@@ -637,6 +645,7 @@ struct Oracle : OracleBase {
     V4<R> noise4(V3<R> q) const { return sample_rep(noise, q) * R(2) - V4<R>(1, 1, 1, 1); }   // sim.js:228
     void k_forces(double dtd) {
         const R dt = uDt(dtd), DX = uDx(), tm = (R)(float)time_;
+        if (bud_on) bud_z.assign(nz, std::array<double, 18>{});
         const R I = u(P_intensity);
 #pragma omp parallel for
         for (int z = 0; z < nz; z++)
@@ -664,6 +673,7 @@ struct Oracle : OracleBase {
                         V3<R> qb = p * R(2.4) + V3<R>(R(0.17), -tm * R(0.12), tm * R(0.05));
                         bed *= smoothstep<R>(R(0), R(0.3), noise4(qb).y);
                     }
+                    const R y_in = s.y, th_in = s.x;
                     s.y = wmax(s.y, src * u(P_fuelSrc));
                     s.y = wmax(s.y, bed * u(P_bedFuel));
                     s.x = wmax(s.x, src * u(P_srcTemp));
@@ -687,6 +697,7 @@ struct Oracle : OracleBase {
                         s2.w *= R(1) - bw;
                         s2.y = wmax(s2.y, R(0.05) * bw * wclamp<R>(s.y + s2.x, R(0), R(1)));
                     }
+                    const R y_src = s.y;
                     // pyrolysis spots and floor fuel flux (build extension, mass-flux sources)
                     for (size_t q = 0; q < spots.size(); q++) {
                         const fs_spot& sp = spots[q];
@@ -707,6 +718,7 @@ struct Oracle : OracleBase {
                         const R fv = face64.empty() ? (R)face[fi] : (R)face64[fi];
                         s.y = wmin(s.y + fv * dt / ((R)FS_RHO0 * DX), R(1));
                     }
+                    const R y_flux = s.y, th_flux = s.x;
                     R ageF = s2.y / wmax(s.y + s2.x, R(1e-4));
                     // mixing-limited combustion (sim.js:300-309)
                     R T0 = R(293) + R(1000) * s.x;
@@ -717,6 +729,7 @@ struct Oracle : OracleBase {
                     s2.x = wmin(s2.x + (R(1) + u(P_rs)) * b, R(1));
                     R dT = wmin(u(P_heatOut) * b, wmax(R(2100) - T0, R(0)));
                     R T = T0 + dT;
+                    const R y_burn = s.y, tk_burn = T;
                     R q = dT / dt;
                     R phi = g4.z > R(0.5) ? R(0) : wmin(u(P_expansion) * q / T, R(60));
                     // soot (sim.js:312-318)
@@ -728,11 +741,14 @@ struct Oracle : OracleBase {
                     // cooling (sim.js:321-330)
                     R th = T * R(0.001);
                     T -= wmin(u(P_radCool) * R(970) * s.z * th * th * (th * th * th * th - R(0.00737)), R(6000)) * dt;
+                    const R tk_rad = T;
                     R notBurning = R(1) - smoothstep<R>(R(500), R(5000), q);
                     T = R(293) + wmax(T - R(293), R(0)) * wexp<R>(-u(P_coolRate) * notBurning * dt);
                     s.x = (T - R(293)) * R(0.001);
+                    const R tk_cool = T, th_cool = s.x;
                     s.w = q * R(1e-4);
                     s.y *= wexp<R>(-u(P_fuelDissip) * dt);
+                    const R y_diss = s.y;
                     s2.x *= wexp<R>(-u(P_vitDecay) * dt);
                     s2.z *= wexp<R>(-u(P_smokeDissip) * dt);
                     s2.y = ageF * wclamp<R>(s.y + s2.x, R(0), R(1));
@@ -791,8 +807,70 @@ struct Oracle : OracleBase {
                     st(velX, ci, V4<R>(v.x, v.y, v.z, phi));
                     st(Sf, ci, s);
                     st(S2, ci, s2);
+                    if (bud_on) {   // the budget terms: differences of consecutive values of this cell
+                        const double yst = St::dec(St::enc((double)s.y)), thst = St::dec(St::enc((double)s.x));
+                        const double d[17] = {
+                            (double)y_in, (double)y_src - y_in, (double)y_flux - y_src, (double)y_flux - y_burn,
+                            (double)y_burn - y_diss, (double)y_diss - s.y, yst - s.y, yst,
+                            1000.0 * th_in, 1000.0 * ((double)th_flux - th_in), (double)tk_burn - T0,
+                            (double)tk_burn - tk_rad, (double)tk_rad - tk_cool, 1000.0 * ((double)th_cool - s.x),
+                            ((double)T0 - 293.0 - 1000.0 * (double)th_flux) + (1000.0 * (double)th_cool - ((double)tk_cool - 293.0)) +
+                                1000.0 * (thst - s.x),
+                            1000.0 * thst, St::dec(St::enc((double)phi))};
+                        bool fin = true;
+                        for (double e : d) fin = fin && std::isfinite(e);
+                        auto& a = bud_z[z];
+                        if (fin) for (int q2 = 0; q2 < 17; q2++) a[q2] += d[q2];
+                        else a[17] += 1.0;
+                    }
                 }
     }
+    // sums of the per-z partials in z order, into the record (the library's k_budget_step)
+    void budget_step(double dtd) {
+        double t[18] = {};
+        for (int z = 0; z < nz; z++) for (int k = 0; k < 18; k++) t[k] += bud_z[z][k];
+        double bsum = 0.0;
+        for (size_t i = 0; i < ncell; i++) bsum += (double)mg.L[0].b[i];
+        const double dxf = (double)(float)dx, V = dxf * dxf * dxf, kg = FS_RHO0 * V, J = FS_RHO0 * 1005.0 * V;
+        double* r = bud;
+        r[FS_BUDGET_STEPS] += 1.0;
+        r[FS_BUDGET_FUEL_TRANSPORT] += kg * t[0] - r[FS_BUDGET_FUEL];
+        r[FS_BUDGET_FUEL_SOURCES] += kg * t[1];
+        r[FS_BUDGET_FUEL_INJECTED] += kg * t[2];
+        r[FS_BUDGET_FUEL_BURNED] += kg * t[3];
+        r[FS_BUDGET_FUEL_DISSIPATED] += kg * t[4];
+        r[FS_BUDGET_FUEL_REMOVED] += kg * t[5];
+        r[FS_BUDGET_FUEL_ROUNDING] += kg * t[6];
+        r[FS_BUDGET_FUEL] = kg * t[7];
+        r[FS_BUDGET_HEAT_TRANSPORT] += J * t[8] - r[FS_BUDGET_HEAT];
+        r[FS_BUDGET_HEAT_SOURCES] += J * t[9];
+        r[FS_BUDGET_HEAT_RELEASED] += J * t[10];
+        r[FS_BUDGET_HEAT_RADIATED] += J * t[11];
+        r[FS_BUDGET_HEAT_ENTRAINED] += J * t[12];
+        r[FS_BUDGET_HEAT_REMOVED] += J * t[13];
+        r[FS_BUDGET_HEAT_ROUNDING] += J * t[14];
+        r[FS_BUDGET_HEAT] = J * t[15];
+        const double dt = (double)(float)dtd;
+        double inj = 0.0;
+        for (const fs_spot& sp : spots) inj += (double)sp.flux * dt;
+        double fx = 0.0;
+        for (float f : face) fx += f;
+        r[FS_BUDGET_FUEL_EXPECTED] += inj + fx * dxf * dxf * dt;
+        r[FS_BUDGET_EXPANSION] = V * t[16];
+        r[FS_BUDGET_DIVERGENCE] = V * bsum;
+        r[FS_BUDGET_BAD_CELLS] += t[17];
+    }
+    void budget_enable(int on) override {
+        bud_on = on != 0;
+        if (!bud_on) return;
+        for (double& e : bud) e = 0.0;
+        double y = 0.0, th = 0.0;
+        for (size_t i = 0; i < ncell; i++) { y += St::dec(Sf[i * 4 + 1]); th += 1000.0 * St::dec(Sf[i * 4]); }
+        const double dxf = (double)(float)dx, V = dxf * dxf * dxf;
+        bud[FS_BUDGET_FUEL0] = bud[FS_BUDGET_FUEL] = FS_RHO0 * V * y;
+        bud[FS_BUDGET_HEAT0] = bud[FS_BUDGET_HEAT] = FS_RHO0 * 1005.0 * V * th;
+    }
+    void budget_read(double* out) const override { for (int i = 0; i < FS_BUDGET_N; i++) out[i] = bud[i]; }
     // ---- divergence minus expansion (sim.js:403-413) ----
     void k_div() {
         const R DX = uDx();
@@ -875,7 +953,9 @@ struct Oracle : OracleBase {
     }
     void step(double dt) override {   // sim.js:627-648
         if (!geoReady) { k_geo(); geoReady = true; }
-        k_adv1(dt); k_adv2(dt); k_curl(); k_forces(dt); k_div(); k_mg(); k_project();
+        k_adv1(dt); k_adv2(dt); k_curl(); k_forces(dt); k_div();
+        if (bud_on) budget_step(dt);
+        k_mg(); k_project();
         time_ += dt; frame_++;
     }
 
@@ -1222,6 +1302,8 @@ double orc_residual(orc_sim* s, int* used) { return s->o->residual(used); }
 int orc_build_campfire(int seed, float* out, int max_logs) { return build_campfire(seed, out, max_logs); }
 int orc_set_spots(orc_sim* s, const fs_spot* sp, int n, fs_spot* out) { return s->o->set_spots(sp, n, out); }
 void orc_set_face_flux(orc_sim* s, const float* f) { s->o->set_face(f); }
+void orc_budget_enable(orc_sim* s, int on) { s->o->budget_enable(on); }
+void orc_budget_read(orc_sim* s, double* out) { s->o->budget_read(out); }
 // Double-precision spot / floor fluxes replacing the float ones (NULL: back to the float values).
 void orc_set_flux64(orc_sim* s, const double* spot, int n, const double* face) { s->o->set_flux64(spot, n, face); }
 

@@ -309,6 +309,17 @@ class OracleSim:
         lib().orc_set_flux64(self.h, None if sp is None else _dp(sp), 0 if sp is None else sp.size,
                              None if fc is None else _dp(fc))
 
+    def budget_enable(self, on=True):
+        lib().orc_budget_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib().orc_budget_enable(self.h, 1 if on else 0)
+
+    def budget(self):
+        """The FS_BUDGET_* record (same layout as the library's fs_budget_read)."""
+        lib().orc_budget_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        out = np.zeros(24, np.float64)
+        lib().orc_budget_read(self.h, _dp(out))
+        return out
+
     def reset(self):
         lib().orc_reset(self.h)
 

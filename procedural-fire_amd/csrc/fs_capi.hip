@@ -104,6 +104,10 @@ struct fs_sim {
     int* ring_count_d = nullptr;
     int diag_cap = 0;
     double diag_tflame = 1000.0;
+    // FP64 budget: per-block partials and the device record
+    bool budget_on = false;
+    double* budget_part = nullptr;
+    double* budget_rec = nullptr;
     uint64_t gen = 0;                // bumped by every call that may change the fields
 
     FieldSet fields() const {
@@ -189,9 +193,12 @@ void enqueue_pre(fs_sim* h, bool prof, bool begin_solve) {
     if (prof) (void)hipEventRecord(h->ev[2], s);
     launch_curl(h->prec, h->g, s, F);
     if (prof) (void)hipEventRecord(h->ev[3], s);
-    launch_forces(h->prec, h->g, s, h->P_d, F);
+    if (h->budget_on) launch_forces_budget(h->prec, h->g, s, h->P_d, F, h->budget_part);
+    else launch_forces(h->prec, h->g, s, h->P_d, F);
     if (prof) (void)hipEventRecord(h->ev[4], s);
-    launch_div(h->prec, h->g, s, h->P_d, F, h->refine() ? h->mg.bkeep : h->mg.b[0], 1);
+    float* const b_out = h->refine() ? h->mg.bkeep : h->mg.b[0];
+    launch_div(h->prec, h->g, s, h->P_d, F, b_out, 1);
+    if (h->budget_on) launch_budget_step(h->g, s, h->P_d, F, b_out, h->budget_part, h->budget_rec);
     if (prof) (void)hipEventRecord(h->ev[5], s);
     if (begin_solve) h->solve_begin(s);
 }
@@ -292,6 +299,8 @@ void free_sim(fs_sim* h) {
     if (h->stats_d) (void)hipFree(h->stats_d);
     if (h->ring_d) (void)hipFree(h->ring_d);
     if (h->ring_count_d) (void)hipFree(h->ring_count_d);
+    if (h->budget_part) (void)hipFree(h->budget_part);
+    if (h->budget_rec) (void)hipFree(h->budget_rec);
     mg_free(h->mg);
     spectral_free(h->spec);
     for (auto& e : h->ev) if (e) (void)hipEventDestroy(e);
@@ -1023,6 +1032,32 @@ int fs_diag_read(fs_sim* h, double* out, int max_records, int* n_records) {
         std::copy(&ring[(size_t)idx * rec], &ring[(size_t)idx * rec] + rec, out + (size_t)w * rec);
     }
     if (n_records) *n_records = w;
+    return FS_OK;
+}
+
+int fs_budget_enable(fs_sim* h, int on) {
+    if (!h) return fail(FS_EINVAL, "null handle");
+    if (set_device(h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    drop_graphs(h);                  // the tracing source kernel joins (or leaves) the captured step
+    h->budget_on = false;
+    if (!on) return FS_OK;
+    if (!h->budget_part) FS_HIP_CHECK(hipMalloc(&h->budget_part, budget_partials(h->g) * sizeof(double)));
+    if (!h->budget_rec) FS_HIP_CHECK(hipMalloc(&h->budget_rec, FS_BUDGET_N * sizeof(double)));
+    sync_params(h, h->dev_dt < 0 ? 0.0 : h->dev_dt);
+    launch_budget_init(h->prec, h->g, h->stream, h->P_d, h->fields(), h->budget_part, h->budget_rec);
+    FS_HIP_CHECK(hipGetLastError());
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->budget_on = true;
+    return FS_OK;
+}
+
+int fs_budget_read(fs_sim* h, double* out) {
+    if (!h || !out) return fail(FS_EINVAL, "null argument");
+    if (!h->budget_rec) return fail(FS_ESTATE, "the budget was never enabled");
+    if (set_device(h)) return FS_EDEVICE;
+    FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    FS_HIP_CHECK(hipMemcpy(out, h->budget_rec, FS_BUDGET_N * sizeof(double), hipMemcpyDeviceToHost));
     return FS_OK;
 }
 

@@ -27,6 +27,7 @@ template <> struct Codec<StF32> {
         return make_float4(__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z), __uint_as_float(u.w));
     }
     static __device__ __forceinline__ void st(float4* p, int i, float4 v) { p[i] = v; }
+    static __device__ __forceinline__ float rt(float v) { return v; }          // the stored value of v
 };
 template <> struct Codec<StF16> {
     static __device__ __forceinline__ float4 ld(const half4_t* p, int i) {
@@ -45,6 +46,7 @@ template <> struct Codec<StF16> {
         h.x = (_Float16)v.x; h.y = (_Float16)v.y; h.z = (_Float16)v.z; h.w = (_Float16)v.w;
         p[i] = h;
     }
+    static __device__ __forceinline__ float rt(float v) { return (float)(_Float16)v; }
 };
 template <> struct Codec<StBF16> {
     static __device__ __forceinline__ float4 ld(const ushort4_t* p, int i) {
@@ -63,4 +65,5 @@ template <> struct Codec<StBF16> {
         u.x = f2bf(v.x); u.y = f2bf(v.y); u.z = f2bf(v.z); u.w = f2bf(v.w);
         p[i] = u;
     }
+    static __device__ __forceinline__ float rt(float v) { return bf2f(f2bf(v)); }
 };

@@ -6,6 +6,8 @@
 // downstream of the state is written once, here, in the reference's order of
 // operations, so the float instantiation is the step's arithmetic exactly.
 #pragma once
+#include <type_traits>
+
 #include "fs_math.h"
 
 // ---- a forward-mode dual number ------------------------------------------------
@@ -158,17 +160,36 @@ struct ForcesCell {
     float wSx, wSz, ks, kt;    // sponge targets and weights, top fade
 };
 
+// Fuel fraction and temperature at the boundaries of the sub-models of one cell update, for the
+// FP64 mass and energy budget (fs_budget.hip).  NoTrace (the step itself) records nothing and
+// compiles to nothing.
+struct NoTrace {};
+struct CellTrace {
+    float y_in, th_in;           // after advection (the kernel's inputs)
+    float y_src, th_src;         // after the log-band, coal-bed and burst sources
+    float y_flux, th_flux;       // after the pyrolysis spots and the floor flux
+    float t0;                    // 293 + 1000 theta (K), the burner's input temperature
+    float y_burn, tk_burn;       // after mixing-limited burning (fuel; T0 + dT)
+    float tk_rad, tk_cool;       // after radiative and after entrainment cooling (K)
+    float th_cool;               // theta = (T - 293) / 1000
+    float y_diss;                // after fuel dissipation
+    float y_out, th_out;         // after the solid cells and the NaN reset
+};
+#define FS_TRACE(...) do { if constexpr (!std::is_same<Tr, NoTrace>::value) { __VA_ARGS__; } } while (0)
+
 // One cell: v (3), s (4: T, fuel, soot, q), s2 (4: vitiated, age, smoke, blue) in place;
 // w = own vorticity (3), gx/gy/gz = differences of |w| of the neighbours; phi out.
 // seed_spot: the spot whose flux carries a unit derivative (Dual only; -1: none), or
 // spot_dir: the derivative of every spot's flux (Dual only; null: use seed_spot);
 // face: the floor flux of this cell as T.
-template <class T>
+template <class T, class Tr = NoTrace>
 __device__ __forceinline__ void forces_cell(const float* U, const float* D, const ForcesCell& k, const fs_spot* __restrict__ spots,
                                             int n_spots, int seed_spot, const float* spot_dir, bool has_face,
                                             T face, T& vx, T& vy,
-                                            T& vz, T* s, T* s2, const T* w, T gx, T gy, T gz, T& phi_out) {
+                                            T& vz, T* s, T* s2, const T* w, T gx, T gy, T gz, T& phi_out,
+                                            Tr* tr = nullptr) {
     const float dt = k.dt, I = k.I;
+    FS_TRACE(tr->y_in = s[1]; tr->th_in = s[0]);
     // ---- sources (sim.js:246-297)
     s[1] = gmax(s[1], k.src * U[P_fuelSrc]);
     s[1] = gmax(s[1], k.bed * U[P_bedFuel]);
@@ -192,6 +213,7 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
         s2[3] = s2[3] * (1.f - bw);
         s2[1] = gmax(s2[1], 0.05f * bw * gclamp(s[1] + s2[0], 0.f, 1.f));
     }
+    FS_TRACE(tr->y_src = s[1]; tr->th_src = s[0]);
     // ---- pyrolysis spots and floor fuel flux (build extension: mass-flux sources)
     for (int q = 0; q < n_spots; q++) {
         const fs_spot sp = spots[q];
@@ -207,6 +229,7 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
         }
     }
     if (has_face && k.cy == 0) s[1] = gmin(s[1] + face * dt / (FS_RHO0 * k.dx), 1.f);
+    FS_TRACE(tr->y_flux = s[1]; tr->th_flux = s[0]);
     T ageF = s2[1] / gmax(s[1] + s2[0], 1e-4f);
 
     // ---- mixing-limited combustion (sim.js:300-309)
@@ -218,6 +241,7 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
     s2[0] = gmin(s2[0] + (1.f + U[P_rs]) * b, 1.f);
     const T dT = gmin(U[P_heatOut] * b, gmax(2100.f - T0, 0.f));
     T Tk = T0 + dT;
+    FS_TRACE(tr->t0 = T0; tr->y_burn = s[1]; tr->tk_burn = Tk);
     const T q = dT / dt;
     const T phi = k.gg.z > 0.5f ? T(0.f) : gmin(U[P_expansion] * q / Tk, 60.f);
 
@@ -231,11 +255,14 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
     // ---- cooling (sim.js:321-330)
     const T th = Tk * 0.001f;
     Tk = Tk - gmin(U[P_radCool] * 970.f * s[2] * th * th * (th * th * th * th - 0.00737f), 6000.f) * dt;
+    FS_TRACE(tr->tk_rad = Tk);
     const T notBurning = 1.f - gsstep_c(500.f, 5000.f, q);
     Tk = 293.f + gmax(Tk - 293.f, 0.f) * gexp(-U[P_coolRate] * notBurning * dt);
     s[0] = (Tk - 293.f) * 0.001f;
+    FS_TRACE(tr->tk_cool = Tk; tr->th_cool = s[0]);
     s[3] = q * 1e-4f;
     s[1] = s[1] * D[D_eFuelDissip];                      // exp(-fuelDissip dt)
+    FS_TRACE(tr->y_diss = s[1]);
     s2[0] = s2[0] * D[D_eVitDecay];                      // exp(-vitDecay dt)
     s2[2] = s2[2] * D[D_eSmokeDissip];                   // exp(-smokeDissip dt)
     s2[1] = ageF * gclamp(s[1] + s2[0], 0.f, 1.f);
@@ -293,8 +320,10 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
         for (int c = 0; c < 4; c++) { s[c] = T(0.f); s2[c] = T(0.f); }
         vx = T(0.f); vy = T(0.f); vz = T(0.f);
     }
+    FS_TRACE(tr->y_out = s[1]; tr->th_out = s[0]);
     phi_out = phi;
 }
+#undef FS_TRACE
 
 // The per-cell constants of the source kernel: the emitter terms driven by the noise
 // textures (sim.js:246-268), the curl-noise sample (sim.js:364-365), the sponge weights

@@ -268,6 +268,16 @@ void launch_volume(int prec, bool sharpened, hipStream_t s, const RenderU& R, co
                    const void* S2_field, const float4* lut, const float4* lights, const float* depth, int dw, int dh,
                    const half4_t* curl, int nn, const unsigned char* flags, int bx, int by, int bz, float inv_dx,
                    half4_t* vol, float2* aux);
+// ---- FP64 mass and energy budget (fs_budget.hip) ----------------------------------
+size_t budget_partials(const Dims& g);   // doubles of the partials buffer
+// the source kernel in its tracing form (same fields) writing per-block budget partials
+void launch_forces_budget(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, double* part);
+// after the divergence: reduce the partials (and sum(b)) into the record
+void launch_budget_step(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, const float* b, double* part,
+                        double* rec);
+// start the record from the current state
+void launch_budget_init(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, double* part,
+                        double* rec);
 // ---- diagnostics (fs_diag.hip) -------------------------------------------------
 int  stats_blocks(size_t n);
 void launch_stats(int prec, hipStream_t s, const void* field, size_t n, double* part);

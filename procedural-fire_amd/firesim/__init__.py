@@ -27,6 +27,12 @@ DT_F32, DT_F64, DT_NATIVE = 0, 1, 2
 (ST_GEO, ST_ADV1, ST_ADV2, ST_CURL, ST_FORCES, ST_DIV, ST_MG, ST_PROJECT) = range(8)
 STAGE_LABELS = ("adv1", "adv2", "curl", "forces", "div", "mg", "project")
 NUM_PARAMS = 60
+# the FS_BUDGET_* record of include/firesim.h, in order
+BUDGET_KEYS = ("steps", "fuel0", "fuel", "fuel_transport", "fuel_sources", "fuel_injected", "fuel_burned",
+               "fuel_dissipated", "fuel_removed", "fuel_rounding", "heat0", "heat", "heat_transport", "heat_sources",
+               "heat_released", "heat_radiated", "heat_entrained", "heat_removed", "heat_rounding", "fuel_expected",
+               "expansion", "divergence", "bad_cells")
+BUDGET_N = 24
 
 
 class FsError(RuntimeError):
@@ -142,6 +148,8 @@ def lib():
         "fs_profile": (ip, [vp, dp, dp, ip, ctypes.POINTER(ctypes.c_int)]),
         "fs_diag_enable": (ip, [vp, ip, ctypes.c_double]),
         "fs_diag_read": (ip, [vp, dp, ip, ctypes.POINTER(ctypes.c_int)]),
+        "fs_budget_enable": (ip, [vp, ip]),
+        "fs_budget_read": (ip, [vp, dp]),
         "fs_poisson_info": (ip, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "fs_set_clock": (ip, [vp, ctypes.c_double, ctypes.c_int64]),
         "fs_spark_burst_default": (None, [ctypes.POINTER(FsSparkBurst)]),
@@ -370,6 +378,17 @@ class FireSim:
         a = buf[:n.value * (4 + ny)].reshape(n.value, 4 + ny)
         return {"time": a[:, 0].copy(), "Q": a[:, 1].copy(), "Tmax": a[:, 2].copy(), "tip": a[:, 3].copy(),
                 "axisT": a[:, 4:].copy()}
+
+    def budget_enable(self, on=True):
+        """FP64 mass and energy budget of the following steps, reduced on the device (fs_budget_*);
+        on=True restarts the record from the current state."""
+        _check(lib().fs_budget_enable(self._h, 1 if on else 0))
+
+    def budget(self):
+        """The budget record as a dict (BUDGET_KEYS; kg, J, m^3/s)."""
+        buf = np.zeros(BUDGET_N, np.float64)
+        _check(lib().fs_budget_read(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return dict(zip(BUDGET_KEYS, buf.tolist()))
 
     # ---- stepping
     def reset(self):
