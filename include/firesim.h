@@ -232,6 +232,16 @@ int  fs_build_campfire(int seed, float* out, int max_logs);
  * value).  Min, max and mean are over the finite values; the mean divides by the
  * cell count, as the reference does.  Waits for the handle's stream. */
 int  fs_stats(fs_sim* h, int field, double out[20]);
+/* fs_stats without any allocation: no counterpart in the reference; for callers that cannot
+ * allocate device or host memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  Same results as fs_stats, bit for bit: the per-block partials and
+ * the combine run on the device in a fixed scratch array of the library (one per GPU, about
+ * 330 KiB, held until the process ends), so the field may have at most
+ * FS_STATS_NOALLOC_MAX_CELLS cells (FS_EINVAL beyond).  Every handle on one GPU shares that
+ * scratch: calls that overlap on one GPU (from several threads or streams) race.  Waits for the
+ * handle's stream. */
+#define FS_STATS_NOALLOC_MAX_CELLS (1 << 24)
+int  fs_stats_noalloc(fs_sim* h, int field, double out[20]);
 /* FIRE.profile() (src/main.js:322-355), reduced on the device from S: *q_watts =
  * heat release sum(354800 q / T * dx^3) over cells with q = |S.w| 1e4 > 0, T = 293 +
  * 1000 |S.x|; then for y-planes j = 0, 4, 8, ... one row of 6 doubles: height

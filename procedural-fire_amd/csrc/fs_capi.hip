@@ -965,6 +965,14 @@ int fs_stats(fs_sim* h, int field, double out[20]) {
     return FS_OK;
 }
 
+int fs_stats_noalloc(fs_sim* h, int field, double out[20]) {
+    if (!h || !out) return fail(FS_EINVAL, "null argument");
+    const int slot = field_slot(field);
+    if (slot < 0) return fail(FS_EINVAL, "stats needs a 4-channel state field");
+    if (set_device(h)) return FS_EDEVICE;
+    return stats_noalloc(h->prec, h->stream, h->cfg.device, h->f[slot], h->ncell, out);
+}
+
 int fs_profile(fs_sim* h, double* q_watts, double* rows, int max_rows, int* n_rows) {
     if (!h || (max_rows > 0 && !rows)) return fail(FS_EINVAL, "null argument");
     if (set_device(h)) return FS_EDEVICE;

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "fs_codec.h"
 #include "fs_internal.h"
@@ -112,6 +114,27 @@ __global__ __launch_bounds__(DB) void k_planes(const typename S::raw* __restrict
     if (threadIdx.x < 8) out[(size_t)j * 8 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// Combines the stats partials of nb blocks into the 20 results of fs_stats, exactly as the host
+// combine does (blocks in index order: the earlier maximum wins ties; sums added in order).
+__global__ __launch_bounds__(64) void k_stats_combine(const double* __restrict__ part, int nb, double ncell,
+                                                      double* __restrict__ out) {
+    const int c = threadIdx.x;
+    if (c >= 4) return;
+    double mn = INFINITY, mx = -INFINITY, sum = 0.0, bad = 0.0, arg = -1.0;
+    for (int b = 0; b < nb; b++) {
+        const double* q = part + ((size_t)b * 4 + c) * 5;
+        mn = fmin(mn, q[0]);
+        if (q[4] >= 0 && (arg < 0 || q[1] > mx)) { mx = q[1]; arg = q[4]; }
+        sum += q[2];
+        bad += q[3];
+    }
+    out[c * 5 + 0] = mn;
+    out[c * 5 + 1] = mx;
+    out[c * 5 + 2] = sum / ncell;
+    out[c * 5 + 3] = bad;
+    out[c * 5 + 4] = arg;
+}
+
 // One record per step into a ring of `cap` records of (4 + ny) doubles:
 // time (s, after the step), Q (W), Tmax (K), flame tip (m: top of the highest plane
 // whose max T exceeds t_flame, 0 if none), then the centre-column max T per plane.
@@ -139,6 +162,58 @@ __global__ __launch_bounds__(DB) void k_record(const DevParams* __restrict__ P, 
 }
 
 }  // namespace
+
+// ---- fs_stats without allocation ----------------------------------------------------
+// The partials and the combined result live in this file-scope device array, one copy per GPU for
+// the life of the process.  Host code never uses the array's name as an address: the device
+// address is looked up once per device (hipGetSymbolAddress, with that device current) and kept.
+constexpr int STATS_NOALLOC_BLOCKS = (FS_STATS_NOALLOC_MAX_CELLS + STATS_CELLS - 1) / STATS_CELLS;
+constexpr size_t STATS_SCRATCH_DOUBLES = (size_t)STATS_NOALLOC_BLOCKS * 20 + 32;
+static __device__ __align__(256) double scratch[STATS_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device (looked up on its first use there).
+static int scratch_address(int device, double** out) {
+    if (device < 0 || device >= MAX_DEVICES) {
+        fs_set_error("device ordinal out of range for the allocation-free statistics");
+        return FS_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fs_set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return FS_EDEVICE;
+        }
+        g_scratch_ptr[device] = (double*)p;
+    }
+    *out = g_scratch_ptr[device];
+    return FS_OK;
+}
+
+int stats_noalloc(int prec, hipStream_t s, int device, const void* field, size_t n, double out[20]) {
+    if (n > (size_t)FS_STATS_NOALLOC_MAX_CELLS) {
+        fs_set_error("fs_stats_noalloc: the field has " + std::to_string(n) + " cells, more than the " +
+                     std::to_string(FS_STATS_NOALLOC_MAX_CELLS) + " its fixed scratch holds; use fs_stats");
+        return FS_EINVAL;
+    }
+    double* ptr = nullptr;
+    if (int rc = scratch_address(device, &ptr)) return rc;
+    const int nb = stats_blocks(n);
+    double* res = ptr + (size_t)STATS_NOALLOC_BLOCKS * 20;     // the 20 combined values
+    launch_stats(prec, s, field, n, ptr);
+    k_stats_combine<<<1, 64, 0, s>>>(ptr, nb, (double)n, res);
+    FS_HIP_CHECK(hipGetLastError());
+    FS_HIP_CHECK(hipMemcpyAsync(out, res, 20 * sizeof(double), hipMemcpyDeviceToHost, s));
+    FS_HIP_CHECK(hipStreamSynchronize(s));
+    return FS_OK;
+}
 
 #define DISPATCH_PREC(prec, BODY)                                  \
     do {                                                           \

@@ -281,6 +281,9 @@ void launch_budget_init(int prec, const Dims& g, hipStream_t s, const DevParams*
 // ---- diagnostics (fs_diag.hip) -------------------------------------------------
 int  stats_blocks(size_t n);
 void launch_stats(int prec, hipStream_t s, const void* field, size_t n, double* part);
+// fs_stats through the library's fixed device scratch (no allocation); FS_EINVAL above
+// FS_STATS_NOALLOC_MAX_CELLS cells
+int  stats_noalloc(int prec, hipStream_t s, int device, const void* field, size_t n, double out[20]);
 void launch_planes(int prec, hipStream_t s, const void* S_field, int nx, int ny, int nz, double V, double* out);
 void launch_record(hipStream_t s, const DevParams* P, const double* planes, int ny, double dx, double t_flame,
                    double* ring, int cap, int* count);

@@ -145,6 +145,7 @@ def lib():
         "fs_poisson_read_x": (ip, [vp, ip, vp]),
         "fs_poisson_set_omega": (ip, [vp, ctypes.c_float]),
         "fs_stats": (ip, [vp, ip, dp]),
+        "fs_stats_noalloc": (ip, [vp, ip, dp]),
         "fs_profile": (ip, [vp, dp, dp, ip, ctypes.POINTER(ctypes.c_int)]),
         "fs_diag_enable": (ip, [vp, ip, ctypes.c_double]),
         "fs_diag_read": (ip, [vp, dp, ip, ctypes.POINTER(ctypes.c_int)]),
