@@ -113,10 +113,16 @@ typedef struct fs_config {
                              plain Gauss-Seidel).  FS_SOLVE_VCYCLES, the reference's solve,
                              always uses 1                                                    */
     int    pre_sweeps;    /* FS_SOLVE_REFINE64 / TOL32: red-black sweeps before / after the     */
-    int    post_sweeps;   /* coarse correction on every level but the coarsest (0 = default:
+    int    post_sweeps;   /* coarse correction on every level but the coarsest (pre -1 = none; 0 = default:
                              V(1,2), which with omega 1.25 converges faster per unit of work
                              than the reference's V(2,2)); FS_SOLVE_VCYCLES always uses V(2,2) */
-    int    reserved[3];
+    int    direct_level;  /* FS_SOLVE_REFINE64 / TOL32: the level whose coarse-grid problem is solved
+                             exactly (separable DST-II / DCT-IV transforms of its operator) in place of
+                             the V-cycle's coarser levels; 0 = automatic (the first level below the
+                             finest with x a multiple of 16, y of 4, z of 8, each <= 128, and at most
+                             2^19 cells; the plain V-cycle if none), -1 = none (the V-cycle to
+                             the coarsest level), k >= 1 = level k.  FS_SOLVE_VCYCLES never uses it */
+    int    reserved[2];
 } fs_config;
 
 /* Fills a config with the reference defaults (res 80x120x80, size 1.2x1.8x1.2,

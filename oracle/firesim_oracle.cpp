@@ -304,8 +304,122 @@ struct Multigrid {
                     f.p[i] = f.p[i] + acc;
                 }
     }
+    // ---- exact solve of one coarse level (the library's direct coarse solve, fs_direct.hip) ----
+    // The level's operator is diagonalised by DST-II in x and z and DCT-IV in y; p = A^-1 b is
+    // W = Tx b Tz^T per y-plane, W <- Ty^T diag(1/lambda) Ty W per (kx, kz) column, p = Tx^T W Tz.
+    // Every value is one FMA chain over its contraction index in increasing order, from 0, and the
+    // division uses ((lx + ly) + lz) / h^2, as the library's kernels do.
+    int direct = 0;
+    std::vector<M> dTx, dTy, dTz, dlx, dly, dlz;
+    M dih2 = 1;
+    static void basis(int n, int kind, std::vector<M>& T, std::vector<M>& lam) {
+        const double pi = 3.14159265358979323846;
+        T.assign((size_t)n * n, M(0));
+        lam.assign(n, M(0));
+        for (int k = 0; k < n; k++) {
+            const double th = kind == 0 ? pi * (k + 1) / n : pi * (k + 0.5) / n;
+            lam[k] = (M)(-(2.0 - 2.0 * std::cos(th)));
+            const double nrm = (kind == 0 && k == n - 1) ? std::sqrt(1.0 / n) : std::sqrt(2.0 / n);
+            for (int i = 0; i < n; i++)
+                T[(size_t)k * n + i] = (M)(nrm * (kind == 0 ? std::sin(th * (i + 0.5)) : std::cos(th * (i + 0.5))));
+        }
+    }
+    // the shapes the library's kernels take (x in chunks of 16, y in fours, z in eights, each <= 128,
+    // at most 2^19 cells, the kernels' padded LDS rows within 160 KiB)
+    static bool direct_fits(int X, int Y, int Z) {
+        if (X < 16 || Y < 4 || Z < 8 || X % 16 || Y % 4 || Z % 8 || X > 128 || Z > 128 || Y > 128) return false;
+        if ((size_t)X * Y * Z > ((size_t)1 << 19)) return false;
+        if (X * Z > 8192 || X * X > 8192 || Z * Z > 8192 || Y * Y > 12288) return false;   // one register batch
+        const size_t lim = 160 * 1024, m = std::max(X, Z) + 4;
+        const size_t fwd = 4 * ((size_t)(X / 2) * (X + 4) + (size_t)Z * m + (size_t)(X / 2) * (Z + 4));
+        const size_t yl = 4 * ((size_t)Y * (Y + 4) + 2 * (size_t)16 * (Y + 4));
+        const size_t inv = 4 * ((size_t)X * m + (size_t)(Z / 2) * (Z + 4) + (size_t)(Z / 2) * (X + 4));
+        return fwd <= lim && yl <= lim && inv <= lim;
+    }
+    // want: 0 automatic (the first level below the finest that fits), -1 none, k >= 1 level k
+    void set_direct(int want) {
+        direct = 0;
+        if (want < 0) return;
+        int lv = 0;
+        if (want == 0) {
+            for (int l = 1; l < (int)L.size() && !lv; l++)
+                if (direct_fits(L[l].nx, L[l].ny, L[l].nz)) lv = l;
+        } else if (want < (int)L.size() && direct_fits(L[want].nx, L[want].ny, L[want].nz)) {
+            lv = want;
+        }
+        if (!lv) return;
+        setup_direct(lv);
+    }
+    void setup_direct(int lv) {
+        direct = lv;
+        basis(L[lv].nx, 0, dTx, dlx);
+        basis(L[lv].ny, 1, dTy, dly);
+        basis(L[lv].nz, 0, dTz, dlz);
+        dih2 = M(1) / ((M)L[lv].h * (M)L[lv].h);
+    }
+    void direct_solve() {
+        Level<M>& l = L[direct];
+        const int X = l.nx, Y = l.ny, Z = l.nz;
+        std::vector<M> W((size_t)X * Y * Z);
+        auto bi = [&](int x, int y, int z) { return ((size_t)z * Y + y) * X + x; };
+#pragma omp parallel for
+        for (int y = 0; y < Y; y++) {
+            std::vector<M> C((size_t)X * Z);
+            for (int kx = 0; kx < X; kx++)
+                for (int z = 0; z < Z; z++) {
+                    M acc = 0;
+                    for (int x = 0; x < X; x++) acc = std::fma(dTx[(size_t)kx * X + x], l.b[bi(x, y, z)], acc);
+                    C[(size_t)kx * Z + z] = acc;
+                }
+            for (int kx = 0; kx < X; kx++)
+                for (int kz = 0; kz < Z; kz++) {
+                    M acc = 0;
+                    for (int z = 0; z < Z; z++) acc = std::fma(C[(size_t)kx * Z + z], dTz[(size_t)kz * Z + z], acc);
+                    W[bi(kx, y, kz)] = acc;
+                }
+        }
+#pragma omp parallel for
+        for (int kz = 0; kz < Z; kz++) {
+            std::vector<M> G(Y);
+            for (int kx = 0; kx < X; kx++) {
+                for (int ky = 0; ky < Y; ky++) {
+                    M acc = 0;
+                    for (int y = 0; y < Y; y++) acc = std::fma(dTy[(size_t)ky * Y + y], W[bi(kx, y, kz)], acc);
+                    const M d = ((dlx[kx] + dly[ky]) + dlz[kz]) * dih2;
+                    G[ky] = acc / d;
+                }
+                for (int y = 0; y < Y; y++) {
+                    M acc = 0;
+                    for (int ky = 0; ky < Y; ky++) acc = std::fma(dTy[(size_t)ky * Y + y], G[ky], acc);
+                    W[bi(kx, y, kz)] = acc;
+                }
+            }
+        }
+#pragma omp parallel for
+        for (int y = 0; y < Y; y++) {
+            std::vector<M> Q((size_t)X * Z);
+            for (int kx = 0; kx < X; kx++)
+                for (int z = 0; z < Z; z++) {
+                    M acc = 0;
+                    for (int kz = 0; kz < Z; kz++) acc = std::fma(W[bi(kx, y, kz)], dTz[(size_t)kz * Z + z], acc);
+                    Q[(size_t)kx * Z + z] = acc;
+                }
+            for (int x = 0; x < X; x++)
+                for (int z = 0; z < Z; z++) {
+                    M acc = 0;
+                    for (int kx = 0; kx < X; kx++) acc = std::fma(dTx[(size_t)kx * X + x], Q[(size_t)kx * Z + z], acc);
+                    l.p[bi(x, y, z)] = acc;
+                }
+        }
+    }
     void vcycle() {   // sim.js:588-599
         const int n = (int)L.size();
+        if (direct > 0) {   // the levels below `direct` replaced by its exact solve
+            for (int i = 0; i < direct; i++) { smooth(i, pre); restrict_(i); }
+            direct_solve();
+            for (int i = direct - 1; i >= 0; i--) { prolong(i); smooth(i, post); }
+            return;
+        }
         for (int i = 0; i < n - 1; i++) { smooth(i, pre); restrict_(i); }
         smooth(n - 1, coarse_iters);
         for (int i = n - 2; i >= 0; i--) { prolong(i); smooth(i, post); }
@@ -448,9 +562,10 @@ struct Oracle : OracleBase {
         mg.build(nx, ny, nz, (double)(float)dx, c->max_levels);   // L.h is an f32 uniform (sim.js:96)
         if (solve_mode != FS_SOLVE_VCYCLES) {
             mg.omega = (M)(c->omega > 0.f ? c->omega : 1.25f);
-            mg.pre = c->pre_sweeps > 0 ? c->pre_sweeps : 1;
+            mg.pre = c->pre_sweeps > 0 ? c->pre_sweeps : c->pre_sweeps < 0 ? 0 : 1;
             mg.post = c->post_sweeps > 0 ? c->post_sweeps : 2;
         }
+        if (solve_mode == FS_SOLVE_REFINE64 || solve_mode == FS_SOLVE_TOL32) mg.set_direct(c->direct_level);
         x64.assign(ncell, 0.0);
         set_logs(lg, nl);
         bake_noise(noise, c->noise_n, c->noise_seed);
@@ -1303,6 +1418,22 @@ int orc_build_campfire(int seed, float* out, int max_logs) { return build_campfi
 int orc_set_spots(orc_sim* s, const fs_spot* sp, int n, fs_spot* out) { return s->o->set_spots(sp, n, out); }
 void orc_set_face_flux(orc_sim* s, const float* f) { s->o->set_face(f); }
 void orc_budget_enable(orc_sim* s, int on) { s->o->budget_enable(on); }
+// The exact level solve alone on an nx x ny x nz level of cell size h (natural x-fastest order):
+// p = A^-1 b in f32 (fp64 = 0) or f64.  Returns 0, or -1 if the library's kernels do not take the shape.
+int orc_direct_solve(int nx, int ny, int nz, double h, int fp64, const double* b, double* p) {
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        if (!Multigrid<T>::direct_fits(nx, ny, nz)) return -1;
+        Multigrid<T> mg;
+        mg.build(nx, ny, nz, h, 1);
+        mg.setup_direct(0);
+        for (size_t i = 0; i < mg.L[0].b.size(); i++) mg.L[0].b[i] = (T)b[i];
+        mg.direct_solve();
+        for (size_t i = 0; i < mg.L[0].p.size(); i++) p[i] = (double)mg.L[0].p[i];
+        return 0;
+    };
+    return fp64 ? run(0.0) : run(0.0f);
+}
 void orc_budget_read(orc_sim* s, double* out) { s->o->budget_read(out); }
 // Double-precision spot / floor fluxes replacing the float ones (NULL: back to the float values).
 void orc_set_flux64(orc_sim* s, const double* spot, int n, const double* face) { s->o->set_flux64(spot, n, face); }

@@ -41,13 +41,14 @@ class FsConfig(ctypes.Structure):
         ("omega", ctypes.c_float),
         ("pre_sweeps", ctypes.c_int),
         ("post_sweeps", ctypes.c_int),
-        ("reserved", ctypes.c_int * 3),
+        ("direct_level", ctypes.c_int),
+        ("reserved", ctypes.c_int * 2),
     ]
 
 
 def make_config(res=(80, 120, 80), size=(1.2, 1.8, 1.2), vcycles=2, max_levels=0, precision=FP16,
                 solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1, omega=0.0, pre_sweeps=0,
-                post_sweeps=0):
+                post_sweeps=0, direct_level=0):
     c = FsConfig()
     c.res[:] = list(res)
     c.size[:] = list(size)
@@ -62,6 +63,7 @@ def make_config(res=(80, 120, 80), size=(1.2, 1.8, 1.2), vcycles=2, max_levels=0
     c.omega = omega
     c.pre_sweeps = pre_sweeps
     c.post_sweeps = post_sweeps
+    c.direct_level = direct_level
     return c
 
 
@@ -231,7 +233,7 @@ class OracleSim:
 
     def __init__(self, res=(64, 96, 64), size=None, logs=None, n_logs=None, precision=FP16, vcycles=2,
                  max_levels=0, solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1, omega=0.0,
-                 pre_sweeps=0, post_sweeps=0):
+                 pre_sweeps=0, post_sweeps=0, direct_level=0):
         if size is None:
             size = (res[0] * 0.015, res[1] * 0.015, res[2] * 0.015)
         if logs is None:
@@ -242,7 +244,7 @@ class OracleSim:
             n_logs = logs.size // 8
         self.res = tuple(res)
         self.cfg = make_config(res, size, vcycles, max_levels, precision, solve, div_tol, max_vcycles, noise_n, noise_seed,
-                               omega, pre_sweeps, post_sweeps)
+                               omega, pre_sweeps, post_sweeps, direct_level)
         self.h = lib().orc_create(ctypes.byref(self.cfg), _fp(logs), n_logs)
         if not self.h:
             raise RuntimeError("oracle create failed")
@@ -486,6 +488,20 @@ class OracleMG:
 
     def residual(self):
         return lib().orc_mg_residual(self.h)
+
+
+def direct_solve(b, h, fp64=False):
+    """The exact level solve of the tolerance solves' coarse level (b: (nz, ny, nx)); None if the
+    library's kernels do not take the shape."""
+    L = lib()
+    L.orc_direct_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    b = np.ascontiguousarray(b, np.float64)
+    nz, ny, nx = b.shape
+    p = np.zeros_like(b)
+    if L.orc_direct_solve(nx, ny, nz, float(h), 1 if fp64 else 0, _dp(b), _dp(p)):
+        return None
+    return p
 
 
 def mjs_study(N, face=True, mode=2, omega=1.0, pre=2, post=2):

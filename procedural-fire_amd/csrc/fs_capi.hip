@@ -440,8 +440,8 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     if (cfg->solve < FS_SOLVE_VCYCLES || cfg->solve > FS_SOLVE_SPECTRAL64) return fail(FS_EINVAL, "bad solve mode");
     if (cfg->solve == FS_SOLVE_SPECTRAL64 && nx % 2 != 0) return fail(FS_EINVAL, "FS_SOLVE_SPECTRAL64 needs an even nx");
     if (!(cfg->omega == 0.f || (cfg->omega > 0.f && cfg->omega < 2.f))) return fail(FS_EINVAL, "omega must be 0 or in (0, 2)");
-    if (cfg->pre_sweeps < 0 || cfg->pre_sweeps > 8 || cfg->post_sweeps < 0 || cfg->post_sweeps > 8)
-        return fail(FS_EINVAL, "pre/post sweeps must be in 0..8");
+    if (cfg->pre_sweeps < -1 || cfg->pre_sweeps > 8 || cfg->post_sweeps < 0 || cfg->post_sweeps > 8)
+        return fail(FS_EINVAL, "pre sweeps must be in -1..8, post sweeps in 0..8");
     if (cfg->vcycles < 0 || cfg->max_vcycles < 0 || cfg->max_vcycles > MG_MAX_REFINE) return fail(FS_EINVAL, "bad V-cycle count");
     const int nn = cfg->noise_n;
     if (nn < 4 || (nn & (nn - 1)) != 0 || nn > 512) return fail(FS_EINVAL, "noise_n must be a power of two in [4, 512]");
@@ -483,8 +483,12 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     // over-relaxed smoothing in the tolerance solves only; the reference's solve is plain GS
     if (cfg->solve != FS_SOLVE_VCYCLES) {
         h->mg.omega = cfg->omega > 0.f ? cfg->omega : 1.25f;
-        h->mg.pre = cfg->pre_sweeps > 0 ? cfg->pre_sweeps : 1;
+        h->mg.pre = cfg->pre_sweeps > 0 ? cfg->pre_sweeps : cfg->pre_sweeps < 0 ? 0 : 1;
         h->mg.post = cfg->post_sweeps > 0 ? cfg->post_sweeps : 2;
+    }
+    if (cfg->solve == FS_SOLVE_REFINE64 || cfg->solve == FS_SOLVE_TOL32) {
+        rc = mg_set_direct(h->mg, cfg->direct_level, err);
+        if (rc) return bail(fail(rc, err));
     }
     if (cfg->solve == FS_SOLVE_SPECTRAL64) {
         rc = spectral_build(&h->spec, nx, ny, nz, (double)h->mg.h[0], h->stream, h->mg.b[0], h->mg.p[0], err);

@@ -91,6 +91,8 @@ struct GpuMG {
     bool fuse_prolong = true;    // prolongation folded into the first post-smoothing pass
     float omega = 1.f;           // over-relaxation of the smoother (all levels but the coarsest)
     int pre = 2, post = 2;       // sweeps before / after the coarse correction (all levels but the coarsest)
+    int direct = 0;              // > 0: level solved exactly by separable transforms (the cycle stops there)
+    struct DirectSolve* ds = nullptr;
 };
 
 // Multigrid arrays are parity-split: each x-row stores its even-x cells, then its odd-x
@@ -109,6 +111,16 @@ __device__ __forceinline__ int fs_opaque(int i) {
 }
 
 int  mg_build(GpuMG& mg, int nx, int ny, int nz, float h, int max_levels, std::string& err);
+// ---- exact coarse-level solve (fs_direct.hip) ------------------------------------
+// Whether a level of X x Y x Z cells can be solved exactly by the separable-transform kernels.
+bool direct_fits(int X, int Y, int Z);
+int  direct_build(struct DirectSolve** out, int X, int Y, int Z, float h, std::string& err);
+void direct_free(struct DirectSolve* d);
+// p = A^-1 b on one level (parity-split arrays); the kernels exit at once when *skip is set
+void direct_solve(const struct DirectSolve* d, hipStream_t s, const float* b, float* p, const int* skip);
+// Chooses and sets up the exact coarse level of a tolerance solve: `want` = 0 automatic (the first
+// level below the finest that direct_fits), -1 none, k >= 1 level k (FS_EINVAL if it does not fit).
+int  mg_set_direct(GpuMG& mg, int want, std::string& err);
 size_t mg_level_floats(const GpuMG& mg, int l);   // array length of level l (with row padding)
 void mg_free(GpuMG& mg);
 // FP32 V(2,2) cycle on level-0 p/b; `skip` (device int, may be null) = early exit flag.

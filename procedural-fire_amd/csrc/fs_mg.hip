@@ -702,7 +702,33 @@ void mg_free(GpuMG& mg) {
     if (mg.partials) (void)hipFree(mg.partials);
     if (mg.st) (void)hipFree(mg.st);
     mg.x64 = nullptr; mg.bkeep = nullptr; mg.partials = nullptr; mg.st = nullptr;
+    direct_free(mg.ds);
+    mg.ds = nullptr;
+    mg.direct = 0;
     mg.nlev = 0;
+}
+
+int mg_set_direct(GpuMG& mg, int want, std::string& err) {
+    direct_free(mg.ds);
+    mg.ds = nullptr;
+    mg.direct = 0;
+    if (want < 0) return FS_OK;
+    int lv = 0;
+    if (want == 0) {
+        for (int l = 1; l < mg.nlev && !lv; l++)
+            if (direct_fits(mg.dims[l][0], mg.dims[l][1], mg.dims[l][2])) lv = l;
+        if (!lv) return FS_OK;                         // no level fits: the plain V-cycle
+    } else {
+        if (want >= mg.nlev || !direct_fits(mg.dims[want][0], mg.dims[want][1], mg.dims[want][2])) {
+            err = "direct_level: that level cannot be solved exactly (x a multiple of 16, y of 4, z of 8, each "
+                  "<= 128, at most 2^19 cells)";
+            return FS_EINVAL;
+        }
+        lv = want;
+    }
+    if (int rc = direct_build(&mg.ds, mg.dims[lv][0], mg.dims[lv][1], mg.dims[lv][2], mg.h[lv], err)) return rc;
+    mg.direct = lv;
+    return FS_OK;
 }
 
 void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, double* xupd, bool first_red_done) {   // sim.js:576-586
@@ -758,25 +784,28 @@ static void launch_tail(GpuMG& mg, hipStream_t s, const int* skip, bool red_done
 // must be a pre-smoothed level that is not the coarsest (whose sweeps stay plain GS).
 static bool fuse_red_into(const GpuMG& mg, int lc, int T) {
     if (mg.pre < 1 || lc >= mg.nlev - 1) return false;
+    if (mg.direct > 0 && lc >= mg.direct) return false;   // the exact solve takes that level unsmoothed
     return lc <= T;                                    // a per-level pass, or the tail's top level
 }
 
 bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd, bool red0_done) {   // sim.js:588-599
     if (!skip) skip = &mg.st->never;
     const int n = mg.nlev;
-    const int T = mg.use_tail ? mg.tail_from : n;      // levels T.. run in the single-workgroup tail
+    const int D = mg.direct;                            // > 0: levels below D are replaced by its exact solve
+    const int T = D > 0 ? n : mg.use_tail ? mg.tail_from : n;   // levels T.. run in the single-workgroup tail
     if (T == 0) {                                       // the whole hierarchy fits one workgroup's LDS
         launch_tail(mg, s, skip, false);
         return false;
     }
-    const int last = std::min(T, n - 1);                // levels 0..last-1 smooth and restrict
+    const int last = D > 0 ? D : std::min(T, n - 1);    // levels 0..last-1 smooth and restrict
     bool red_done = red0_done;
     for (int l = 0; l < last; l++) {
         mg_smooth(mg, s, l, mg.pre, skip, nullptr, red_done);
         red_done = fuse_red_into(mg, l + 1, T);
         launch_restrict(mg, s, l, skip, red_done);
     }
-    if (T < n) launch_tail(mg, s, skip, red_done);
+    if (D > 0) direct_solve(mg.ds, s, mg.b[D], mg.p[D], skip);
+    else if (T < n) launch_tail(mg, s, skip, red_done);
     else mg_smooth(mg, s, n - 1, mg.coarse_iters, skip);
     for (int l = last - 1; l >= 0; l--) {
         // coarse levels: one launch less; fine levels: cheaper apart.  Over-relaxation reads the

@@ -31,12 +31,13 @@ def copy_state(gpu, orc):
 
 
 def make_pair(precision=F.PREC_FP16, res=TEST_RES, size=TEST_SIZE, warm_steps=30, dt=DT, solve=F.SOLVE_VCYCLES,
-              vcycles=2, div_tol=1e-5, max_vcycles=8, noise_n=32, params=None, use_graph=False, sweeps=(0, 0)):
+              vcycles=2, div_tol=1e-5, max_vcycles=8, noise_n=32, params=None, use_graph=False, sweeps=(0, 0),
+              direct_level=0):
     """A GPU sim stepped `warm_steps` from rest and an oracle holding the identical state."""
     logs, _ = F.build_campfire(7)
     gpu = F.FireSim(res=res, size=size, logs=logs, vcycles=vcycles, precision=precision, solve=solve,
                     div_tol=div_tol, max_vcycles=max_vcycles, noise_n=noise_n, use_graph=use_graph,
-                    pre_sweeps=sweeps[0], post_sweeps=sweeps[1])
+                    pre_sweeps=sweeps[0], post_sweeps=sweeps[1], direct_level=direct_level)
     if params:
         gpu.set_params(params)
     if warm_steps:
@@ -44,7 +45,7 @@ def make_pair(precision=F.PREC_FP16, res=TEST_RES, size=TEST_SIZE, warm_steps=30
     gpu.sync()
     orc = O.OracleSim(res=res, size=size, logs=logs, n_logs=16, precision=PREC_TO_ORACLE[precision], vcycles=vcycles,
                       solve=solve, div_tol=div_tol, max_vcycles=max_vcycles, noise_n=noise_n, pre_sweeps=sweeps[0],
-                      post_sweeps=sweeps[1])
+                      post_sweeps=sweeps[1], direct_level=direct_level)
     if params:
         orc.set_params(**params)
     copy_state(gpu, orc)

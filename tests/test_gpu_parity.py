@@ -155,11 +155,15 @@ def test_deterministic_rerun():
         assert np.array_equal(sims[0].read(f, "native"), sims[1].read(f, "native"))
 
 
+@pytest.mark.parametrize("direct", [0, -1], ids=["exact-coarse", "full-V"])
 @pytest.mark.parametrize("sweeps", [(0, 0), (1, 2)], ids=["V22", "V12"])
-def test_tol32_solve_parity(sweeps):
-    """FP32 V-cycles until the FP64 residual reaches the tolerance: same cycle counts and bits as the oracle."""
-    print(f"tol32 sweeps {sweeps}", flush=True)
-    gpu, orc = make_pair(F.PREC_FP32, warm_steps=20, solve=F.SOLVE_TOL32, div_tol=1e-4, max_vcycles=8, sweeps=sweeps)
+def test_tol32_solve_parity(sweeps, direct):
+    """FP32 cycles until the FP64 residual reaches the tolerance: same cycle counts and bits as the oracle,
+    with the coarse level solved exactly by separable transforms (the default) and with the V-cycle
+    down to the coarsest level."""
+    print(f"tol32 sweeps {sweeps} direct {direct}", flush=True)
+    gpu, orc = make_pair(F.PREC_FP32, warm_steps=20, solve=F.SOLVE_TOL32, div_tol=1e-4, max_vcycles=8, sweeps=sweeps,
+                         direct_level=direct)
     for _ in range(3):
         gpu.step(DT, 1)
         orc.step(DT, 1)
@@ -186,3 +190,19 @@ def test_adaptive_and_async_solves_agree(solve):
         a = sims[0].read(f, "native")
         for s in sims[1:]:
             assert np.array_equal(a, s.read(f, "native")), f
+
+
+def test_direct_level_choice_and_refusal():
+    """Automatic choice: level 1 of 64x96x64 (32x48x32); an explicit level that the kernels do not take
+    is refused at create; the plain V-cycle and the exact coarse level reach the same tolerance."""
+    print("direct level choice", flush=True)
+    with pytest.raises(F.FsError):
+        F.FireSim(res=TEST_RES, size=TEST_SIZE, noise_n=32, solve=F.SOLVE_REFINE64, direct_level=4)
+    sims = [F.FireSim(res=TEST_RES, size=TEST_SIZE, noise_n=32, precision=F.PREC_FP16, solve=F.SOLVE_REFINE64,
+                      div_tol=1e-6, max_vcycles=12, direct_level=d) for d in (0, 1, -1)]
+    for s in sims:
+        s.step(DT, 15)
+    r = [s.residual() for s in sims]
+    assert r[0] == r[1] and all(rel <= 1e-6 for rel, _ in r), r
+    p = [s.read(F.P, np.float64) for s in sims]
+    assert np.array_equal(p[0], p[1]) and rel_l2(p[2], p[0]) < 1e-3

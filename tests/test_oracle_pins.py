@@ -371,3 +371,29 @@ def test_fp64_oracle_close_to_fp32_oracle_after_one_step():
     for f in (O.VEL, O.S, O.S2):
         x, y = a.read(f), b.read(f)
         assert np.linalg.norm(x - y) <= 1e-3 * np.linalg.norm(y), f
+
+
+# ---------------------------------------------------------------- exact coarse-level solve
+@pytest.mark.parametrize("dims", [(16, 24, 16), (32, 48, 32), (64, 96, 64), (48, 72, 48)])
+def test_direct_level_solve_is_the_closed_form(dims):
+    """The tolerance solves' exact coarse solve (separable DST-II / DCT-IV transforms of the level's
+    operator, FMA chains) equals the scipy spectral solution: to 1e-12 in FP64, to f32 rounding
+    times the conditioning in FP32; its residual in the ghost-cell operator is small."""
+    rng = np.random.default_rng(9)
+    nx, ny, nz = dims
+    h = 1.0 / 64
+    b = rng.standard_normal((nz, ny, nx))
+    exact = _spectral_solve(b, h)
+    p64 = O.direct_solve(b, h, fp64=True)
+    assert np.max(np.abs(p64 - exact)) <= 1e-12 * np.max(np.abs(exact))
+    b32 = b.astype(np.float32).astype(np.float64)
+    p32 = O.direct_solve(b32, h, fp64=False)
+    err = np.linalg.norm(p32 - _spectral_solve(b32, h)) / np.linalg.norm(exact)
+    assert err < 2e-5, err
+    res = np.linalg.norm(_apply_A(p32, h) - b32) / np.linalg.norm(b32)
+    assert res < 1e-5, res
+
+
+def test_direct_level_solve_refuses_shapes_the_kernels_do_not_take():
+    assert O.direct_solve(np.zeros((8, 12, 10)), 0.1) is None       # x not a multiple of 16
+    assert O.direct_solve(np.zeros((8, 13, 16)), 0.1) is None       # odd y
