@@ -48,15 +48,25 @@ constexpr int YC = 16;            // columns (kx) per block of the y kernel
 // wave's float4 reads of 8 consecutive rows touch distinct 16-byte bank groups.
 __host__ __device__ constexpr int ldp(int n) { return n + 4; }
 
+// n / d for n < 2^16, d < 2^16 by one multiply-high with m = ceil(2^32 / d) (exact in that range)
+struct Div {
+    unsigned m;
+    int d;
+};
+Div make_div(int d) { return Div{(unsigned)((((unsigned long long)1 << 32) + (unsigned)d - 1) / (unsigned)d), d}; }
+__device__ __forceinline__ int dv(int n, Div f) { return (int)__umulhi((unsigned)n, f.m); }
+
 // out[i][j] = sum_k A[i lda + k] B[j ldb + k] over i < M, j < N, k < K in increasing k (one FMA chain per
 // output, from 0): both factors are read along k, four steps at a time as float4.  A thread owns
 // TI x TJ outputs (M a multiple of TI, N of TJ, K of 4); store(i0, j0, acc[TI][TJ]).
+// ti = M / TI as a Div (the tile index splits without a division).
 template <int TI, int TJ, int NT, class Store>
 __device__ __forceinline__ void lds_gemm_nt(const float* __restrict__ A, int lda, const float* __restrict__ B, int ldb,
-                                            int M, int N, int K, Store store) {
-    const int ti = M / TI, nt = ti * (N / TJ);
+                                            Div ti, int N, int K, Store store) {
+    const int nt = ti.d * (N / TJ);
     for (int t = threadIdx.x; t < nt; t += NT) {
-        const int i0 = (t % ti) * TI, j0 = (t / ti) * TJ;
+        const int tj = dv(t, ti);
+        const int i0 = (t - tj * ti.d) * TI, j0 = tj * TJ;
         float acc[TI][TJ];
 #pragma unroll
         for (int a = 0; a < TI; a++)
@@ -104,13 +114,14 @@ __device__ __forceinline__ void scatter(const float4 (&v)[NV], int total, Store 
         if (i < total) store(i, v[u]);
     }
 }
-// float4 i of rows r0 .. of a row-major n-column matrix, and its place in padded LDS rows
-__device__ __forceinline__ float4 row4(const float* __restrict__ src, int r0, int n, int i) {
-    const int q = n >> 2, r = i / q, c = (i - r * q) * 4;
+// float4 i of rows r0 .. of a row-major n-column matrix (q = n / 4 as a Div), and its place in padded
+// LDS rows
+__device__ __forceinline__ float4 row4(const float* __restrict__ src, int r0, int n, Div q, int i) {
+    const int r = dv(i, q), c = (i - r * q.d) * 4;
     return *(const float4*)&src[(r0 + r) * n + c];
 }
-__device__ __forceinline__ void put_row4(float* dst, int n, int i, float4 v) {
-    const int q = n >> 2, r = i / q, c = (i - r * q) * 4;
+__device__ __forceinline__ void put_row4(float* dst, int n, Div q, int i, float4 v) {
+    const int r = dv(i, q), c = (i - r * q.d) * 4;
     *(float4*)&dst[r * ldp(n) + c] = v;
 }
 // The loaded registers are consumed here, so every load is issued before the skip test waits on them.
@@ -124,24 +135,24 @@ __device__ __forceinline__ void pin(const float4 (&v)[NV]) {
 // LDS: sT[r][x] = Tx[r0 + r] (R rows), sB[z][x] = b (Z rows; later sB[kz][z] = Tz), sC[r][z].
 __global__ __launch_bounds__(DT) void k_direct_fwd(const float* __restrict__ b, const float* __restrict__ Tx,
                                                    const float* __restrict__ Tz, float* __restrict__ W, int X, int Y, int Z,
-                                                   int R, const int* __restrict__ skip) {
+                                                   int R, Div qx, Div qz, Div tr, const int* __restrict__ skip) {
     const int sk = skip[fs_opaque(0)];
     extern __shared__ float4 sh4[];
     float* sT = (float*)sh4;
     float* sB = sT + R * ldp(X);
     float* sC = sB + Z * ldp(std::max(X, Z));
-    const int y = blockIdx.x, r0 = blockIdx.y * R, nh = X >> 1, qx = X >> 2;
+    const int y = blockIdx.x, r0 = blockIdx.y * R, nh = X >> 1;
     float4 vt[NV], vb[NV], vz[NV];
-    gather<DT>(vt, R * qx, [&](int i) { return row4(Tx, r0, X, i); });
+    gather<DT>(vt, R * qx.d, [&](int i) { return row4(Tx, r0, X, qx, i); });
     // b is parity-split: row (z Y + y) holds the even x, then the odd x
-    gather<DT>(vb, Z * qx, [&](int i) { const int z = i / qx; return *(const float4*)&b[(z * Y + y) * X + (i - z * qx) * 4]; });
-    gather<DT>(vz, Z * (Z >> 2), [&](int i) { return row4(Tz, 0, Z, i); });
+    gather<DT>(vb, Z * qx.d, [&](int i) { const int z = dv(i, qx); return *(const float4*)&b[(z * Y + y) * X + (i - z * qx.d) * 4]; });
+    gather<DT>(vz, Z * qz.d, [&](int i) { return row4(Tz, 0, Z, qz, i); });
     pin(vt); pin(vb); pin(vz);
     asm volatile("" ::"v"(sk));
     if (sk) return;                                    // uniform over the grid
-    scatter<DT>(vt, R * qx, [&](int i, float4 v) { put_row4(sT, X, i, v); });
-    scatter<DT>(vb, Z * qx, [&](int i, float4 v) {
-        const int z = i / qx, g = (i - z * qx) * 4;
+    scatter<DT>(vt, R * qx.d, [&](int i, float4 v) { put_row4(sT, X, qx, i, v); });
+    scatter<DT>(vb, Z * qx.d, [&](int i, float4 v) {
+        const int z = dv(i, qx), g = (i - z * qx.d) * 4;
         const float e[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -150,16 +161,16 @@ __global__ __launch_bounds__(DT) void k_direct_fwd(const float* __restrict__ b, 
         }
     });
     __syncthreads();
-    lds_gemm_nt<2, 4, DT>(sT, ldp(X), sB, ldp(X), R, Z, X, [&](int i0, int j0, float (&acc)[2][4]) {
+    lds_gemm_nt<2, 4, DT>(sT, ldp(X), sB, ldp(X), tr, Z, X, [&](int i0, int j0, float (&acc)[2][4]) {
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
             for (int c = 0; c < 4; c++) sC[(i0 + a) * ldp(Z) + j0 + c] = acc[a][c];
     });
     __syncthreads();
-    scatter<DT>(vz, Z * (Z >> 2), [&](int i, float4 v) { put_row4(sB, Z, i, v); });   // sB now holds Tz
+    scatter<DT>(vz, Z * qz.d, [&](int i, float4 v) { put_row4(sB, Z, qz, i, v); });   // sB now holds Tz
     __syncthreads();
-    lds_gemm_nt<2, 4, DT>(sC, ldp(Z), sB, ldp(Z), R, Z, Z, [&](int i0, int j0, float (&acc)[2][4]) {
+    lds_gemm_nt<2, 4, DT>(sC, ldp(Z), sB, ldp(Z), tr, Z, Z, [&](int i0, int j0, float (&acc)[2][4]) {
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             float* o = W + ((size_t)(j0 + c) * Y + y) * X + r0 + i0;
@@ -174,26 +185,26 @@ __global__ __launch_bounds__(DT) void k_direct_fwd(const float* __restrict__ b, 
 // LDS: sT = Ty (later TyT: sT[y][ky]), sV[c][y], sG[c][ky].
 __global__ __launch_bounds__(DTY) void k_direct_y(float* __restrict__ W, const float* __restrict__ Ty,
                                                  const float* __restrict__ TyT, const float* __restrict__ lam, int X, int Y,
-                                                 int Z, float ih2, const int* __restrict__ skip) {
+                                                 int Z, float ih2, Div qy, Div ty, const int* __restrict__ skip) {
     const int sk = skip[fs_opaque(0)];
     extern __shared__ float4 sh4[];
     float* sT = (float*)sh4;
     float* sV = sT + Y * ldp(Y);
     float* sG = sV + YC * ldp(Y);
-    const int c0 = blockIdx.x * YC, kz = blockIdx.y, qy = Y >> 2;
+    const int c0 = blockIdx.x * YC, kz = blockIdx.y;
     float4 vt[NV], vtt[NV], vv[NV];
-    gather<DTY>(vt, Y * qy, [&](int i) { return row4(Ty, 0, Y, i); });
-    gather<DTY>(vtt, Y * qy, [&](int i) { return row4(TyT, 0, Y, i); });
+    gather<DTY>(vt, Y * qy.d, [&](int i) { return row4(Ty, 0, Y, qy, i); });
+    gather<DTY>(vtt, Y * qy.d, [&](int i) { return row4(TyT, 0, Y, qy, i); });
     gather<DTY>(vv, Y * (YC >> 2), [&](int i) {
-        const int yy = i / (YC >> 2);
-        return *(const float4*)&W[((size_t)kz * Y + yy) * X + c0 + (i - yy * (YC >> 2)) * 4];
+        const int yy = i >> 2;                           // YC / 4 = 4 float4 per row
+        return *(const float4*)&W[((size_t)kz * Y + yy) * X + c0 + (i & 3) * 4];
     });
     pin(vt); pin(vtt); pin(vv);
     asm volatile("" ::"v"(sk));
     if (sk) return;
-    scatter<DTY>(vt, Y * qy, [&](int i, float4 v) { put_row4(sT, Y, i, v); });
+    scatter<DTY>(vt, Y * qy.d, [&](int i, float4 v) { put_row4(sT, Y, qy, i, v); });
     scatter<DTY>(vv, Y * (YC >> 2), [&](int i, float4 v) {
-        const int yy = i / (YC >> 2), c = (i - yy * (YC >> 2)) * 4;
+        const int yy = i >> 2, c = (i & 3) * 4;
         sV[(c + 0) * ldp(Y) + yy] = v.x;
         sV[(c + 1) * ldp(Y) + yy] = v.y;
         sV[(c + 2) * ldp(Y) + yy] = v.z;
@@ -203,7 +214,7 @@ __global__ __launch_bounds__(DTY) void k_direct_y(float* __restrict__ W, const f
     const float* lx = lam;
     const float* ly = lam + X;
     const float lzv = lam[X + Y + kz];
-    lds_gemm_nt<2, 2, DTY>(sT, ldp(Y), sV, ldp(Y), Y, YC, Y, [&](int i0, int j0, float (&acc)[2][2]) {
+    lds_gemm_nt<2, 2, DTY>(sT, ldp(Y), sV, ldp(Y), ty, YC, Y, [&](int i0, int j0, float (&acc)[2][2]) {
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -213,9 +224,9 @@ __global__ __launch_bounds__(DTY) void k_direct_y(float* __restrict__ W, const f
             }
     });
     __syncthreads();
-    scatter<DTY>(vtt, Y * qy, [&](int i, float4 v) { put_row4(sT, Y, i, v); });   // sT now holds TyT
+    scatter<DTY>(vtt, Y * qy.d, [&](int i, float4 v) { put_row4(sT, Y, qy, i, v); });   // sT now holds TyT
     __syncthreads();
-    lds_gemm_nt<2, 2, DTY>(sT, ldp(Y), sG, ldp(Y), Y, YC, Y, [&](int i0, int j0, float (&acc)[2][2]) {
+    lds_gemm_nt<2, 2, DTY>(sT, ldp(Y), sG, ldp(Y), ty, YC, Y, [&](int i0, int j0, float (&acc)[2][2]) {
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -228,39 +239,39 @@ __global__ __launch_bounds__(DTY) void k_direct_y(float* __restrict__ W, const f
 // LDS: sW[kx][kz] (X rows; later sW[x][kx] = TxT), sM[r][kz] = TzT[z0 + r] (R rows), sQ[r][kx].
 __global__ __launch_bounds__(DT) void k_direct_inv(const float* __restrict__ W, const float* __restrict__ TxT,
                                                    const float* __restrict__ TzT, float* __restrict__ p, int X, int Y, int Z,
-                                                   int R, const int* __restrict__ skip) {
+                                                   int R, Div qx, Div qz, Div tx, const int* __restrict__ skip) {
     const int sk = skip[fs_opaque(0)];
     extern __shared__ float4 sh4[];
     float* sW = (float*)sh4;
     float* sM = sW + X * ldp(std::max(X, Z));
     float* sQ = sM + R * ldp(Z);
-    const int y = blockIdx.x, z0 = blockIdx.y * R, nh = X >> 1, qx = X >> 2;
+    const int y = blockIdx.x, z0 = blockIdx.y * R, nh = X >> 1;
     float4 vw[NV], vm[NV], vx[NV];
-    gather<DT>(vw, Z * qx, [&](int i) { const int kz = i / qx; return *(const float4*)&W[((size_t)kz * Y + y) * X + (i - kz * qx) * 4]; });
-    gather<DT>(vm, R * (Z >> 2), [&](int i) { return row4(TzT, z0, Z, i); });
-    gather<DT>(vx, X * qx, [&](int i) { return row4(TxT, 0, X, i); });
+    gather<DT>(vw, Z * qx.d, [&](int i) { const int kz = dv(i, qx); return *(const float4*)&W[((size_t)kz * Y + y) * X + (i - kz * qx.d) * 4]; });
+    gather<DT>(vm, R * qz.d, [&](int i) { return row4(TzT, z0, Z, qz, i); });
+    gather<DT>(vx, X * qx.d, [&](int i) { return row4(TxT, 0, X, qx, i); });
     pin(vw); pin(vm); pin(vx);
     asm volatile("" ::"v"(sk));
     if (sk) return;
-    scatter<DT>(vw, Z * qx, [&](int i, float4 v) {
-        const int kz = i / qx, kx = (i - kz * qx) * 4;
+    scatter<DT>(vw, Z * qx.d, [&](int i, float4 v) {
+        const int kz = dv(i, qx), kx = (i - kz * qx.d) * 4;
         sW[(kx + 0) * ldp(Z) + kz] = v.x;
         sW[(kx + 1) * ldp(Z) + kz] = v.y;
         sW[(kx + 2) * ldp(Z) + kz] = v.z;
         sW[(kx + 3) * ldp(Z) + kz] = v.w;
     });
-    scatter<DT>(vm, R * (Z >> 2), [&](int i, float4 v) { put_row4(sM, Z, i, v); });
+    scatter<DT>(vm, R * qz.d, [&](int i, float4 v) { put_row4(sM, Z, qz, i, v); });
     __syncthreads();
-    lds_gemm_nt<2, 4, DT>(sW, ldp(Z), sM, ldp(Z), X, R, Z, [&](int i0, int j0, float (&acc)[2][4]) {
+    lds_gemm_nt<2, 4, DT>(sW, ldp(Z), sM, ldp(Z), tx, R, Z, [&](int i0, int j0, float (&acc)[2][4]) {
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
             for (int c = 0; c < 4; c++) sQ[(j0 + c) * ldp(X) + i0 + a] = acc[a][c];
     });
     __syncthreads();
-    scatter<DT>(vx, X * qx, [&](int i, float4 v) { put_row4(sW, X, i, v); });   // sW now holds TxT
+    scatter<DT>(vx, X * qx.d, [&](int i, float4 v) { put_row4(sW, X, qx, i, v); });   // sW now holds TxT
     __syncthreads();
-    lds_gemm_nt<2, 4, DT>(sW, ldp(X), sQ, ldp(X), X, R, X, [&](int i0, int j0, float (&acc)[2][4]) {
+    lds_gemm_nt<2, 4, DT>(sW, ldp(X), sQ, ldp(X), tx, R, X, [&](int i0, int j0, float (&acc)[2][4]) {
 #pragma unroll
         for (int a = 0; a < 2; a++) {
             const int x = i0 + a;
@@ -354,7 +365,11 @@ void direct_free(DirectSolve* d) {
 void direct_solve(const DirectSolve* d, hipStream_t s, const float* b, float* p, const int* skip) {
     const int X = d->X, Y = d->Y, Z = d->Z;
     const int Rf = X / d->sx, Ri = Z / d->sz;
-    k_direct_fwd<<<dim3(Y, d->sx), DT, fwd_lds(*d, Rf), s>>>(b, d->Tx, d->Tz, d->W, X, Y, Z, Rf, skip);
-    k_direct_y<<<dim3(X / YC, Z), DTY, y_lds(*d), s>>>(d->W, d->Ty, d->TyT, d->lam, X, Y, Z, d->ih2, skip);
-    k_direct_inv<<<dim3(Y, d->sz), DT, inv_lds(*d, Ri), s>>>(d->W, d->TxT, d->TzT, p, X, Y, Z, Ri, skip);
+    const Div qx = make_div(X >> 2), qy = make_div(Y >> 2), qz = make_div(Z >> 2);
+    k_direct_fwd<<<dim3(Y, d->sx), DT, fwd_lds(*d, Rf), s>>>(b, d->Tx, d->Tz, d->W, X, Y, Z, Rf, qx, qz, make_div(Rf / 2),
+                                                             skip);
+    k_direct_y<<<dim3(X / YC, Z), DTY, y_lds(*d), s>>>(d->W, d->Ty, d->TyT, d->lam, X, Y, Z, d->ih2, qy, make_div(Y / 2),
+                                                       skip);
+    k_direct_inv<<<dim3(Y, d->sz), DT, inv_lds(*d, Ri), s>>>(d->W, d->TxT, d->TzT, p, X, Y, Z, Ri, qx, qz, make_div(X / 2),
+                                                             skip);
 }
