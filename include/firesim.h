@@ -299,6 +299,11 @@ int  fs_budget_read(fs_sim* h, double out[FS_BUDGET_N]);
  * device-to-device copies of `bytes` with a plain streaming kernel (the box's
  * achievable HBM bandwidth, and the calibration of bandwidth counters). */
 int  fs_bench_copy(int device, size_t bytes, int iters, float* ms_per_copy);
+/* The achievable-bandwidth probe, no counterpart in the reference: times `iters` passes over `bytes`
+ * of device memory with kind 0 = the plain copy of fs_bench_copy, 1 = a tuned copy (8 16-byte words a
+ * thread, all loads in flight before the stores, non-temporal loads and stores), 2 = reads only,
+ * 3 = writes only.  A copy pass moves 2 x bytes, the others bytes. */
+int  fs_bench_stream(int device, size_t bytes, int kind, int iters, float* ms_per_pass);
 
 /* ---- isolated pressure Poisson solve (cell-centred 7-point operator,
  *      Dirichlet p=0 on the x/z faces and the top, Neumann floor) ------------- */

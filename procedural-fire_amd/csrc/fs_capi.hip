@@ -903,6 +903,41 @@ int fs_bench_copy(int device, size_t bytes, int iters, float* ms_per_copy) {
     return rc;
 }
 
+int fs_bench_stream(int device, size_t bytes, int kind, int iters, float* ms_per_pass) {
+    if (!ms_per_pass || iters < 1 || bytes < 16 || kind < 0 || kind > 3) return fail(FS_EINVAL, "bad arguments");
+    bytes &= ~(size_t)15;
+    if (hipSetDevice(device) != hipSuccess) return fail(FS_EDEVICE, "hipSetDevice failed");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FS_OK;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) rc = fail(FS_ENOMEM, "hipMalloc failed");
+    if (!rc && (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                hipEventCreate(&e1) != hipSuccess))
+        rc = fail(FS_EDEVICE, "stream/event create failed");
+    if (!rc) {
+        launch_fill_zero(s, a, bytes);
+        launch_fill_zero(s, b, bytes);
+        auto pass = [&]() {
+            if (kind == 0) launch_copy_probe(s, a, b, bytes);
+            else launch_stream_probe(s, kind, a, b, bytes);
+        };
+        pass();   // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int i = 0; i < iters; i++) pass();
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = fail(FS_EDEVICE, "stream probe failed");
+        float ms = 0.f;
+        if (!rc) { (void)hipEventElapsedTime(&ms, e0, e1); *ms_per_pass = ms / iters; }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
 // buildCampfire(seed).logs + packLogs (src/scene.js:13-42, 92-98)
 int fs_build_campfire(int seed, float* out, int max_logs) {
     if (!out || max_logs < 0) return fail(FS_EINVAL, "bad arguments");

@@ -368,6 +368,44 @@ void launch_copy_probe(hipStream_t s, const void* src, void* dst, size_t bytes) 
     k_copy_probe<<<2048, 256, 0, s>>>((const float4*)src, (float4*)dst, n);
 }
 
+// Tuned streams for the achievable-bandwidth probe: each thread moves U 16-byte words, all its loads
+// in flight before its stores, one block per 256 U words (no grid-stride loop), non-temporal loads
+// and stores (the data is touched once).  kind 1 copy, 2 read only, 3 write only.
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+template <int U>
+__global__ __launch_bounds__(256) void k_stream(const f4v_t* __restrict__ src, f4v_t* __restrict__ dst, size_t n, int kind) {
+    const size_t base = (size_t)blockIdx.x * 256 * U + threadIdx.x;
+    f4v_t v[U];
+    if (kind != 3) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const size_t i = base + (size_t)u * 256;
+            v[u] = i < n ? __builtin_nontemporal_load(&src[i]) : f4v_t{0.f, 0.f, 0.f, 0.f};
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = f4v_t{(float)u, 0.f, 0.f, 0.f};
+    }
+    if (kind == 2) {                     // keep the loads: fold them into a value that is stored only if it
+        f4v_t acc = v[0];                // matches a pattern no input produces (the buffers hold zeros)
+#pragma unroll
+        for (int u = 1; u < U; u++) acc += v[u];
+        if (acc.x == 1.5e30f && base < n) dst[base] = acc;
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const size_t i = base + (size_t)u * 256;
+        if (i < n) __builtin_nontemporal_store(v[u], &dst[i]);
+    }
+}
+void launch_stream_probe(hipStream_t s, int kind, const void* src, void* dst, size_t bytes) {
+    const size_t n = bytes / 16;
+    constexpr int U = 8;
+    const size_t blocks = (n + 256 * U - 1) / (256 * U);
+    k_stream<U><<<(unsigned)blocks, 256, 0, s>>>((const f4v_t*)src, (f4v_t*)dst, n, kind);
+}
+
 // ---- host launchers ---------------------------------------------------------------
 #define DISPATCH_PREC(prec, BODY)                                  \
     do {                                                           \

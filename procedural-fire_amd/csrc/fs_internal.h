@@ -194,6 +194,8 @@ void launch_set_params(hipStream_t s, DevParams* dst, const DevParams& src);
 void launch_fill_zero(hipStream_t s, void* ptr, size_t bytes);
 void launch_convert(hipStream_t s, int kind, const void* src, void* dst, size_t n);
 void launch_copy_probe(hipStream_t s, const void* src, void* dst, size_t bytes);
+// kind 1 tuned copy, 2 read only, 3 write only (fs_bench_stream)
+void launch_stream_probe(hipStream_t s, int kind, const void* src, void* dst, size_t bytes);
 // ---- adjoint of one step (fs_adjoint.hip), float4 storage ------------------------
 // Stage inputs are the forward's recorded fields; l* arguments are adjoint fields.
 // Gathers store (lvIn of project, lvY/lsY/ls2Y and the xyz of lvort in forces); every

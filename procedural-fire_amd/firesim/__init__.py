@@ -147,6 +147,7 @@ def lib():
         "fs_poisson_set_omega": (ip, [vp, ctypes.c_float]),
         "fs_stats": (ip, [vp, ip, dp]),
         "fs_stats_noalloc": (ip, [vp, ip, dp]),
+        "fs_bench_stream": (ip, [ip, ctypes.c_size_t, ip, ip, ctypes.POINTER(ctypes.c_float)]),
         "fs_profile": (ip, [vp, dp, dp, ip, ctypes.POINTER(ctypes.c_int)]),
         "fs_diag_enable": (ip, [vp, ip, ctypes.c_double]),
         "fs_diag_read": (ip, [vp, dp, ip, ctypes.POINTER(ctypes.c_int)]),
@@ -231,6 +232,19 @@ def bench_copy(nbytes=256 << 20, iters=10, device=0):
     ms = ctypes.c_float()
     _check(lib().fs_bench_copy(int(device), int(nbytes), int(iters), ctypes.byref(ms)))
     return ms.value
+
+
+STREAM_KINDS = {"copy_plain": 0, "copy": 1, "read": 2, "write": 3}
+
+
+def bench_stream(kind="copy", nbytes=1 << 30, iters=10, device=0):
+    """The achievable-bandwidth probe: GB/s of one pass of `kind` over `nbytes` (copies count the bytes
+    read and written)."""
+    ms = ctypes.c_float()
+    k = STREAM_KINDS[kind]
+    _check(lib().fs_bench_stream(int(device), int(nbytes), k, int(iters), ctypes.byref(ms)))
+    moved = nbytes * (2 if k in (0, 1) else 1)
+    return moved / (ms.value * 1e-3) / 1e9
 
 
 class FireSim:
