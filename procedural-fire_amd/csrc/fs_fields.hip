@@ -117,19 +117,25 @@ __global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, D
     float4 vmn = make_float4(1e9f, 1e9f, 1e9f, 0.f), vmx = make_float4(-1e9f, -1e9f, -1e9f, 0.f);
     float4 smn = make_float4(1e9f, 1e9f, 1e9f, 1e9f), smx = make_float4(-1e9f, -1e9f, -1e9f, -1e9f);
     float4 tmn = smn, tmx = smx;
-    float4 cv[8], cs[8], ct[8];
-    corners8_f<S>(V, g, b0x, b0y, b0z, cv);
-    corners8_f<S>(SF, g, b0x, b0y, b0z, cs);
-    corners8_f<S>(S2F, g, b0x, b0y, b0z, ct);
+    if constexpr (std::is_same<S, StF16>::value) {       // packed binary16 min / max over the raw texels
+        minmax8_h(V, g, b0x, b0y, b0z, vmn, vmx, vmn, vmx);
+        minmax8_h(SF, g, b0x, b0y, b0z, smn, smx, smn, smx);
+        minmax8_h(S2F, g, b0x, b0y, b0z, tmn, tmx, tmn, tmx);
+    } else {
+        float4 cv[8], cs[8], ct[8];
+        corners8_f<S>(V, g, b0x, b0y, b0z, cv);
+        corners8_f<S>(SF, g, b0x, b0y, b0z, cs);
+        corners8_f<S>(S2F, g, b0x, b0y, b0z, ct);
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const float4 vv = cv[k], ss = cs[k], tt = ct[k];
-        vmn.x = fminf(vmn.x, vv.x); vmn.y = fminf(vmn.y, vv.y); vmn.z = fminf(vmn.z, vv.z);
-        vmx.x = fmaxf(vmx.x, vv.x); vmx.y = fmaxf(vmx.y, vv.y); vmx.z = fmaxf(vmx.z, vv.z);
-        smn.x = fminf(smn.x, ss.x); smn.y = fminf(smn.y, ss.y); smn.z = fminf(smn.z, ss.z); smn.w = fminf(smn.w, ss.w);
-        smx.x = fmaxf(smx.x, ss.x); smx.y = fmaxf(smx.y, ss.y); smx.z = fmaxf(smx.z, ss.z); smx.w = fmaxf(smx.w, ss.w);
-        tmn.x = fminf(tmn.x, tt.x); tmn.y = fminf(tmn.y, tt.y); tmn.z = fminf(tmn.z, tt.z); tmn.w = fminf(tmn.w, tt.w);
-        tmx.x = fmaxf(tmx.x, tt.x); tmx.y = fmaxf(tmx.y, tt.y); tmx.z = fmaxf(tmx.z, tt.z); tmx.w = fmaxf(tmx.w, tt.w);
+        for (int k = 0; k < 8; k++) {
+            const float4 vv = cv[k], ss = cs[k], tt = ct[k];
+            vmn.x = fminf(vmn.x, vv.x); vmn.y = fminf(vmn.y, vv.y); vmn.z = fminf(vmn.z, vv.z);
+            vmx.x = fmaxf(vmx.x, vv.x); vmx.y = fmaxf(vmx.y, vv.y); vmx.z = fmaxf(vmx.z, vv.z);
+            smn.x = fminf(smn.x, ss.x); smn.y = fminf(smn.y, ss.y); smn.z = fminf(smn.z, ss.z); smn.w = fminf(smn.w, ss.w);
+            smx.x = fmaxf(smx.x, ss.x); smx.y = fmaxf(smx.y, ss.y); smx.z = fmaxf(smx.z, ss.z); smx.w = fmaxf(smx.w, ss.w);
+            tmn.x = fminf(tmn.x, tt.x); tmn.y = fminf(tmn.y, tt.y); tmn.z = fminf(tmn.z, tt.z); tmn.w = fminf(tmn.w, tt.w);
+            tmx.x = fmaxf(tmx.x, tt.x); tmx.y = fmaxf(tmx.y, tt.y); tmx.z = fmaxf(tmx.z, tt.z); tmx.w = fmaxf(tmx.w, tt.w);
+        }
     }
     if (outside_fade(g, bx + 0.5f, by + 0.5f, bz + 0.5f) < 1.f) {   // sim.js:190
         vNx = vH.x; vNy = vH.y; vNz = vH.z; sN = sHc; s2N = s2Hc;

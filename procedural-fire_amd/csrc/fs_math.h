@@ -145,6 +145,42 @@ template <class S> struct FPtr {
     __device__ __forceinline__ FPtr(const typename S::raw* q, const Dims&) : p(q) {}
     __device__ __forceinline__ float4 ld(int i) const { return Codec<S>::ld(p, i); }
 };
+// ---- binary16 operands straight from the raw words (gfx950 VOP3P) ----------------
+// v_fma_mix_f32 with an f16 first operand computes h * s + c in f32 with a single rounding; with c = -0
+// that is the rounded product of the (exactly) widened h and s, the same bits as converting h first and
+// multiplying (x + -0 = x for every x, -0 included).  So the first products of a blend absorb the
+// widening of their texel channels.
+__device__ __forceinline__ float hmul_lo(unsigned w, float s) {   // f32(low half of w) * s
+    float r;
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(w), "v"(s), "v"(__int_as_float(0x80000000)));
+    return r;
+}
+__device__ __forceinline__ float hmul_hi(unsigned w, float s) {   // f32(high half of w) * s
+    float r;
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(w), "v"(s), "v"(__int_as_float(0x80000000)));
+    return r;
+}
+// mix4(dec(a), dec(b), t) for two binary16 x 4 texels (mixf's a (1 - t) + b t per channel, same bits)
+__device__ __forceinline__ float4 mix4_h(u32x2_t a, u32x2_t b, float t) {
+    const float u = 1.f - t;
+    return make_float4(hmul_lo(a.x, u) + hmul_lo(b.x, t), hmul_hi(a.x, u) + hmul_hi(b.x, t),
+                       hmul_lo(a.y, u) + hmul_lo(b.y, t), hmul_hi(a.y, u) + hmul_hi(b.y, t));
+}
+// Per-half minimum / maximum of two binary16 pairs (IEEE minNum / maxNum, as fminf / fmaxf on the widened
+// values; widening is exact and monotonic, so min-then-widen equals widen-then-min).  The operands are
+// taken as they are: a signalling NaN would come out quiet here instead of being ignored, and the fields
+// never hold one (every binary16 store converts from f32, which only produces quiet NaNs).
+__device__ __forceinline__ unsigned pk_min_h(unsigned a, unsigned b) {
+    unsigned r;
+    asm("v_pk_min_f16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ unsigned pk_max_h(unsigned a, unsigned b) {
+    unsigned r;
+    asm("v_pk_max_f16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 template <class S> struct FBuf {
     __amdgpu_buffer_rsrc_t r;
     __device__ __forceinline__ FBuf(const typename S::raw* p, const Dims& g)
@@ -158,11 +194,17 @@ template <class S> struct FBuf {
     // The clamped texels x0 = clamp(ix, 0, n-1), x1 = clamp(ix+1, 0, n-1) of one row, from texels i and
     // i + 1 where i is the row start + clamp(ix, 0, n-2) (n >= 2): both are the first texel when ix < 0
     // (lo_both) and both the second when ix >= n - 1 (hi_both).  The selects act on the raw words.
-    __device__ __forceinline__ void ld_pair(int i, bool lo_both, bool hi_both, float4& a, float4& b) const {
+    __device__ __forceinline__ void ld_pair_raw(int i, bool lo_both, bool hi_both, u32x2_t& a, u32x2_t& b) const {
         const u32x4_t u = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((unsigned)i * 8u), 0, 0);
         const u32x2_t lo = {u.x, u.y}, hi = {u.z, u.w};
-        a = Codec<S>::dec(hi_both ? hi : lo);
-        b = Codec<S>::dec(lo_both ? lo : hi);
+        a = hi_both ? hi : lo;
+        b = lo_both ? lo : hi;
+    }
+    __device__ __forceinline__ void ld_pair(int i, bool lo_both, bool hi_both, float4& a, float4& b) const {
+        u32x2_t ra, rb;
+        ld_pair_raw(i, lo_both, hi_both, ra, rb);
+        a = Codec<S>::dec(ra);
+        b = Codec<S>::dec(rb);
     }
 };
 
@@ -191,7 +233,19 @@ __device__ __forceinline__ float4 sample_clamp_f(const F& f, const Dims& g, floa
     const int r00 = (z0 * g.ny + y0) * g.nx, r10 = (z0 * g.ny + y1) * g.nx;
     const int r01 = (z1 * g.ny + y0) * g.nx, r11 = (z1 * g.ny + y1) * g.nx;
     float4 c000, c100, c010, c110, c001, c101, c011, c111;
-    if constexpr (F::kPair) {
+    if constexpr (F::kPair && std::is_same<S, StF16>::value) {
+        // binary16 pairs: the x blends read the raw halves (mix4_h), the rest as below
+        const int xs = clampi(ix, 0, g.nx - 2);
+        const bool lo = ix < 0, hi = ix >= g.nx - 1;
+        u32x2_t a[4], b[4];
+        f.ld_pair_raw(r00 + xs, lo, hi, a[0], b[0]);
+        f.ld_pair_raw(r10 + xs, lo, hi, a[1], b[1]);
+        f.ld_pair_raw(r01 + xs, lo, hi, a[2], b[2]);
+        f.ld_pair_raw(r11 + xs, lo, hi, a[3], b[3]);
+        const float4 x00 = mix4_h(a[0], b[0], fx), x10 = mix4_h(a[1], b[1], fx);
+        const float4 x01 = mix4_h(a[2], b[2], fx), x11 = mix4_h(a[3], b[3], fx);
+        return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+    } else if constexpr (F::kPair) {
         const int xs = clampi(ix, 0, g.nx - 2);
         const bool lo = ix < 0, hi = ix >= g.nx - 1;
         f.ld_pair(r00 + xs, lo, hi, c000, c100);
@@ -226,6 +280,28 @@ __device__ __forceinline__ void corners8_f(const F& f, const Dims& g, int b0x, i
             c[2 * j + 1] = f.ld(row + clampi(b0x + 1, 0, g.nx - 1));
         }
     }
+}
+// The per-channel minimum and maximum over the same 8 texels of a binary16 field read in pairs, folded into
+// lo / hi starting values (fminf(lo, .) and fmaxf(hi, .) over the texels in any order give the same value:
+// minNum / maxNum of a set; all-NaN texels leave lo / hi).  Packed binary16 min / max, widened once.
+template <class F>
+__device__ __forceinline__ void minmax8_h(const F& f, const Dims& g, int b0x, int b0y, int b0z, float4 lo, float4 hi,
+                                          float4& mn, float4& mx) {
+    unsigned m0 = 0, m1 = 0, M0 = 0, M1 = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int y = clampi(b0y + (j & 1), 0, g.ny - 1), z = clampi(b0z + (j >> 1), 0, g.nz - 1);
+        const int row = (z * g.ny + y) * g.nx;
+        u32x2_t a, b;
+        f.ld_pair_raw(row + clampi(b0x, 0, g.nx - 2), b0x < 0, b0x >= g.nx - 1, a, b);
+        const unsigned n0 = pk_min_h(a.x, b.x), n1 = pk_min_h(a.y, b.y);
+        const unsigned x0 = pk_max_h(a.x, b.x), x1 = pk_max_h(a.y, b.y);
+        if (j == 0) { m0 = n0; m1 = n1; M0 = x0; M1 = x1; }
+        else { m0 = pk_min_h(m0, n0); m1 = pk_min_h(m1, n1); M0 = pk_max_h(M0, x0); M1 = pk_max_h(M1, x1); }
+    }
+    const float4 a = Codec<StF16>::dec(u32x2_t{m0, m1}), b = Codec<StF16>::dec(u32x2_t{M0, M1});
+    mn = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), fminf(lo.w, a.w));
+    mx = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), fmaxf(hi.w, b.w));
 }
 template <class S>
 __device__ __forceinline__ float4 sample_clamp(const typename S::raw* f, const Dims& g, float bx, float by, float bz) {
@@ -271,19 +347,19 @@ __device__ __forceinline__ float4 sample_rep_b(const RepTex& t, float qx, float 
     const int m = n - 1;
     const int x0 = ix & m, y0 = iy & m, y1 = (iy + 1) & m, z0 = iz & m, z1 = (iz + 1) & m;
     const int rows[4] = {(z0 * n + y0) * n, (z0 * n + y1) * n, (z1 * n + y0) * n, (z1 * n + y1) * n};
-    float4 a[4], b[4];
+    u32x2_t a[4], b[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const u32x4_t u = __builtin_amdgcn_raw_buffer_load_b128(t.r, (int)((unsigned)(rows[j] + x0) * 8u), 0, 0);
-        a[j] = Codec<StF16>::dec(u32x2_t{u.x, u.y});
-        b[j] = Codec<StF16>::dec(u32x2_t{u.z, u.w});
+        a[j] = u32x2_t{u.x, u.y};
+        b[j] = u32x2_t{u.z, u.w};
     }
     if (x0 == m) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) b[j] = Codec<StF16>::ldb(t.r, (int)((unsigned)rows[j] * 8u));
+        for (int j = 0; j < 4; j++) b[j] = __builtin_amdgcn_raw_buffer_load_b64(t.r, (int)((unsigned)rows[j] * 8u), 0, 0);
     }
-    const float4 x00 = mix4(a[0], b[0], fx), x10 = mix4(a[1], b[1], fx);
-    const float4 x01 = mix4(a[2], b[2], fx), x11 = mix4(a[3], b[3], fx);
+    const float4 x00 = mix4_h(a[0], b[0], fx), x10 = mix4_h(a[1], b[1], fx);
+    const float4 x01 = mix4_h(a[2], b[2], fx), x11 = mix4_h(a[3], b[3], fx);
     return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
 }
 
