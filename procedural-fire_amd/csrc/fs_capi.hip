@@ -77,6 +77,7 @@ struct fs_sim {
     std::vector<fs_spot> spots_h;
     float* face_d = nullptr;
     bool has_face = false;
+    unsigned* btpack_d = nullptr;    // the advection passes' per-cell limiter corner (null: grids above 1022)
     std::vector<float> logs_h;
     DevParams* P_d = nullptr;
     float prm[FS_NUM_PARAMS];
@@ -120,6 +121,7 @@ struct fs_sim {
         F.logs = logs_d;
         F.spots = spots_d;
         F.face_flux = face_d;
+        F.btpack = btpack_d;
         return F;
     }
     Spectral* spec = nullptr;        // FS_SOLVE_SPECTRAL64: the exact separable inner solve
@@ -294,6 +296,7 @@ void free_sim(fs_sim* h) {
     if (h->logs_d) (void)hipFree(h->logs_d);
     if (h->spots_d) (void)hipFree(h->spots_d);
     if (h->face_d) (void)hipFree(h->face_d);
+    if (h->btpack_d) (void)hipFree(h->btpack_d);
     if (h->P_d) (void)hipFree(h->P_d);
     if (h->planes_d) (void)hipFree(h->planes_d);
     if (h->stats_d) (void)hipFree(h->stats_d);
@@ -477,6 +480,8 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
         return bail(fail(FS_ENOMEM, "hipMalloc failed"));
     launch_fill_zero(h->stream, h->spots_d, FS_MAX_SPOTS * sizeof(fs_spot));
     launch_fill_zero(h->stream, h->face_d, (size_t)nx * nz * sizeof(float));
+    if (std::max(nx, std::max(ny, nz)) <= FS_BTPACK_MAX_DIM && hipMalloc(&h->btpack_d, h->ncell * sizeof(unsigned)) != hipSuccess)
+        return bail(fail(FS_ENOMEM, "hipMalloc failed"));
     std::string err;
     int rc = mg_build(h->mg, nx, ny, nz, (float)h->dx, cfg->max_levels, err);
     if (rc) return bail(fail(rc, err));
@@ -662,7 +667,8 @@ int fs_run_stage(fs_sim* h, int stage, double dt) {
     if (set_device(h)) return FS_EDEVICE;
     h->gen++;
     sync_params(h, dt);
-    const FieldSet F = h->fields();
+    FieldSet F = h->fields();
+    F.btpack = nullptr;   // a stage on its own: the second advection pass forms its own backtrace
     hipStream_t s = h->stream;
     switch (stage) {
         case FS_STAGE_GEO: launch_geo(h->prec, h->g, s, h->P_d, F); h->geo_ready = true; break;

@@ -60,17 +60,46 @@ __global__ __launch_bounds__(256) void k_geo(const DevParams* __restrict__ P, Di
 }
 
 // ---- advection (src/sim.js:123-197) -----------------------------------------
+// The limiter corner of a backtrace b and its outside test, as the second pass forms them (src/sim.js:180-190)
+struct LimCorner {
+    int x, y, z;
+    bool outside;
+};
+__device__ __forceinline__ LimCorner lim_corner(const Dims& g, float3 bt) {
+    const float bx = bt.x - 0.5f, by = bt.y - 0.5f, bz = bt.z - 0.5f;
+    LimCorner k;
+    k.x = (int)floorf(wclamp(bx, -2.f, (float)g.nx + 1.f));
+    k.y = (int)floorf(wclamp(by, -2.f, (float)g.ny + 1.f));
+    k.z = (int)floorf(wclamp(bz, -2.f, (float)g.nz + 1.f));
+    k.outside = outside_fade(g, bx + 0.5f, by + 0.5f, bz + 0.5f) < 1.f;
+    return k;
+}
+__device__ __forceinline__ unsigned pack_corner(const Dims& g, LimCorner k) {
+    return (unsigned)(clampi(k.x, -1, g.nx) + 1) | ((unsigned)(clampi(k.y, -1, g.ny) + 1) << 10) |
+           ((unsigned)(clampi(k.z, -1, g.nz) + 1) << 20) | (k.outside ? 1u << 30 : 0u);
+}
+__device__ __forceinline__ LimCorner unpack_corner(unsigned w) {
+    LimCorner k;
+    k.x = (int)(w & 1023u) - 1;
+    k.y = (int)((w >> 10) & 1023u) - 1;
+    k.z = (int)((w >> 20) & 1023u) - 1;
+    k.outside = (w >> 30) & 1u;
+    return k;
+}
+
 template <class S>
 __global__ __launch_bounds__(256) void k_adv1(const DevParams* __restrict__ P, Dims g,
                                               const typename S::raw* __restrict__ vel, const typename S::raw* __restrict__ sF,
                                               const typename S::raw* __restrict__ s2F, typename S::raw* __restrict__ velX,
-                                              typename S::raw* __restrict__ sX, typename S::raw* __restrict__ s2X) {
+                                              typename S::raw* __restrict__ sX, typename S::raw* __restrict__ s2X,
+                                              unsigned* __restrict__ btpack) {
     Cell c;
     if (!cell_of(g, c)) return;
     const float dt = (float)P->dt, dx = P->f[P_dx];
     const float cx = (float)c.x + 0.5f, cy = (float)c.y + 0.5f, cz = (float)c.z + 0.5f;
     const FBuf<S> V(vel, g), SF(sF, g), S2F(s2F, g);
     const float3 b = backtrace_c<S>(V, g, cx, cy, cz, dt, dx, V.ld(c.i));
+    if (btpack) btpack[c.i] = pack_corner(g, lim_corner(g, b));
     const float f = outside_fade(g, b.x, b.y, b.z);
     const float ws = sstep_c(0.03f, 0.4f, b.y * dx);
     const float wx = P->f[P_windX] * ws, wy = 0.f * ws, wz = P->f[P_windZ] * ws;
@@ -81,13 +110,14 @@ __global__ __launch_bounds__(256) void k_adv1(const DevParams* __restrict__ P, D
 }
 
 
-template <class S>
-__global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, Dims g,
+// PASS: the limiter corner and outside test come from k_adv1's btpack word instead of a second backtrace
+template <class S, bool PASS>
+__device__ __forceinline__ void adv2_cell(const DevParams* __restrict__ P, Dims g,
                                               const typename S::raw* __restrict__ vel, const typename S::raw* __restrict__ sF,
                                               const typename S::raw* __restrict__ s2F, const typename S::raw* __restrict__ velH,
                                               const typename S::raw* __restrict__ sH, const typename S::raw* __restrict__ s2H,
                                               typename S::raw* __restrict__ velY, typename S::raw* __restrict__ sY,
-                                              typename S::raw* __restrict__ s2Y) {
+                                              typename S::raw* __restrict__ s2Y, const unsigned* __restrict__ btpack) {
     Cell c;
     if (!cell_of(g, c)) return;
     const float dt = (float)P->dt, dx = P->f[P_dx];
@@ -109,11 +139,10 @@ __global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, D
     float4 s2N = make_float4(s2Hc.x + (s20.x - s2F4.x) * 0.5f, s2Hc.y + (s20.y - s2F4.y) * 0.5f,
                              s2Hc.z + (s20.z - s2F4.z) * 0.5f, s2Hc.w + (s20.w - s2F4.w) * 0.5f);
     // Selle et al. 2008 limiter over the 8 texels at floor(backtrace - 0.5) (sim.js:180-193)
-    const float3 bt = backtrace_c<S>(V, g, cx, cy, cz, dt, dx, v);
-    const float bx = bt.x - 0.5f, by = bt.y - 0.5f, bz = bt.z - 0.5f;
-    const int b0x = (int)floorf(wclamp(bx, -2.f, (float)g.nx + 1.f));
-    const int b0y = (int)floorf(wclamp(by, -2.f, (float)g.ny + 1.f));
-    const int b0z = (int)floorf(wclamp(bz, -2.f, (float)g.nz + 1.f));
+    LimCorner lc;
+    if constexpr (PASS) lc = unpack_corner(btpack[c.i]);
+    else lc = lim_corner(g, backtrace_c<S>(V, g, cx, cy, cz, dt, dx, v));
+    const int b0x = lc.x, b0y = lc.y, b0z = lc.z;
     float4 vmn = make_float4(1e9f, 1e9f, 1e9f, 0.f), vmx = make_float4(-1e9f, -1e9f, -1e9f, 0.f);
     float4 smn = make_float4(1e9f, 1e9f, 1e9f, 1e9f), smx = make_float4(-1e9f, -1e9f, -1e9f, -1e9f);
     float4 tmn = smn, tmx = smx;
@@ -137,7 +166,7 @@ __global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, D
             tmx.x = fmaxf(tmx.x, tt.x); tmx.y = fmaxf(tmx.y, tt.y); tmx.z = fmaxf(tmx.z, tt.z); tmx.w = fmaxf(tmx.w, tt.w);
         }
     }
-    if (outside_fade(g, bx + 0.5f, by + 0.5f, bz + 0.5f) < 1.f) {   // sim.js:190
+    if (lc.outside) {   // sim.js:190
         vNx = vH.x; vNy = vH.y; vNz = vH.z; sN = sHc; s2N = s2Hc;
     }
     vNx = clamp_lim(vNx, vmn.x, vmx.x, vH.x); vNy = clamp_lim(vNy, vmn.y, vmx.y, vH.y); vNz = clamp_lim(vNz, vmn.z, vmx.z, vH.z);
@@ -148,6 +177,26 @@ __global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, D
     Codec<S>::st(velY, c.i, make_float4(vNx, vNy, vNz, 0.f));
     Codec<S>::st(sY, c.i, make_float4(fmaxf(sN.x, 0.f), fmaxf(sN.y, 0.f), fmaxf(sN.z, 0.f), fmaxf(sN.w, 0.f)));
     Codec<S>::st(s2Y, c.i, make_float4(fmaxf(s2N.x, 0.f), fmaxf(s2N.y, 0.f), fmaxf(s2N.z, 0.f), fmaxf(s2N.w, 0.f)));
+}
+
+template <class S, bool PASS>
+__global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, Dims g, const typename S::raw* __restrict__ vel,
+                                              const typename S::raw* __restrict__ sF, const typename S::raw* __restrict__ s2F,
+                                              const typename S::raw* __restrict__ velH, const typename S::raw* __restrict__ sH,
+                                              const typename S::raw* __restrict__ s2H, typename S::raw* __restrict__ velY,
+                                              typename S::raw* __restrict__ sY, typename S::raw* __restrict__ s2Y,
+                                              const unsigned* __restrict__ btpack) {
+    adv2_cell<S, PASS>(P, g, vel, sF, s2F, velH, sH, s2H, velY, sY, s2Y, btpack);
+}
+// binary16 storage: held to 7 waves per SIMD (at most 72 VGPRs), where the gathers of this issue-bound pass
+// overlap best
+template <bool PASS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_adv2_h(
+    const DevParams* __restrict__ P, Dims g, const StF16::raw* __restrict__ vel, const StF16::raw* __restrict__ sF,
+    const StF16::raw* __restrict__ s2F, const StF16::raw* __restrict__ velH, const StF16::raw* __restrict__ sH,
+    const StF16::raw* __restrict__ s2H, StF16::raw* __restrict__ velY, StF16::raw* __restrict__ sY,
+    StF16::raw* __restrict__ s2Y, const unsigned* __restrict__ btpack) {
+    adv2_cell<StF16, PASS>(P, g, vel, sF, s2F, velH, sH, s2H, velY, sY, s2Y, btpack);
 }
 
 // ---- curl: w = curl(v) * dx, |w| (src/sim.js:203-211) ---------------------------
@@ -429,14 +478,28 @@ void launch_geo(int prec, const Dims& g, hipStream_t s, const DevParams* P, cons
     DISPATCH_PREC(prec, geo_t<S>(g, s, P, F));
 }
 template <class S> static void adv1_t(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F) {
-    k_adv1<S><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.vel), CRAW(F.S), CRAW(F.S2), RAW(F.velX), RAW(F.SX), RAW(F.S2X));
+    k_adv1<S><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.vel), CRAW(F.S), CRAW(F.S2), RAW(F.velX), RAW(F.SX), RAW(F.S2X),
+                                        F.btpack);
 }
 void launch_adv1(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F) {
     DISPATCH_PREC(prec, adv1_t<S>(g, s, P, F));
 }
 template <class S> static void adv2_t(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F) {
-    k_adv2<S><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.vel), CRAW(F.S), CRAW(F.S2), CRAW(F.velX), CRAW(F.SX), CRAW(F.S2X),
-                                        RAW(F.velY), RAW(F.SY), RAW(F.S2Y));
+    if constexpr (std::is_same<S, StF16>::value) {
+        if (F.btpack)
+            k_adv2_h<true><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.vel), CRAW(F.S), CRAW(F.S2), CRAW(F.velX), CRAW(F.SX),
+                                                     CRAW(F.S2X), RAW(F.velY), RAW(F.SY), RAW(F.S2Y), F.btpack);
+        else
+            k_adv2_h<false><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.vel), CRAW(F.S), CRAW(F.S2), CRAW(F.velX), CRAW(F.SX),
+                                                      CRAW(F.S2X), RAW(F.velY), RAW(F.SY), RAW(F.S2Y), nullptr);
+    } else {
+        if (F.btpack)
+            k_adv2<S, true><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.vel), CRAW(F.S), CRAW(F.S2), CRAW(F.velX), CRAW(F.SX),
+                                                      CRAW(F.S2X), RAW(F.velY), RAW(F.SY), RAW(F.S2Y), F.btpack);
+        else
+            k_adv2<S, false><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.vel), CRAW(F.S), CRAW(F.S2), CRAW(F.velX), CRAW(F.SX),
+                                                       CRAW(F.S2X), RAW(F.velY), RAW(F.SY), RAW(F.S2Y), nullptr);
+    }
 }
 void launch_adv2(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F) {
     DISPATCH_PREC(prec, adv2_t<S>(g, s, P, F));

@@ -180,7 +180,14 @@ struct FieldSet {
     const float* logs;
     const fs_spot* spots;        // FS_MAX_SPOTS entries
     const float* face_flux;      // nx*nz
+    // One word per cell from the first advection pass to the second: the limiter's corner cell and the
+    // outside test of the backtrace (k_adv1 writes it, k_adv2 reads it; null: k_adv2 recomputes both)
+    unsigned* btpack = nullptr;
 };
+// The backtrace's limiter corner floor(b - 1/2) per axis, clamped to [-1, n] (the texels it selects are
+// the same as for the unclamped value), plus one, in 10 bits each; bit 30: the backtrace leaves the
+// grid (outside_fade < 1).  Needs every dimension <= 1022.
+constexpr int FS_BTPACK_MAX_DIM = 1022;
 void launch_geo(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F);
 void launch_adv1(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F);
 void launch_adv2(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F);
