@@ -336,13 +336,13 @@ int  fs_poisson_info(fs_poisson* ps, int* levels, int dims[][3], int* coarse_ite
  * and fs_adjoint_backward sweeps the steps in reverse: each checkpoint interval is
  * recomputed from its checkpoint, keeping every step's intermediates, and the steps'
  * hand-written adjoint kernels run last to first (projection, the pressure solve,
- * divergence, sources, curl, MacCormack correction with the limiter's active set
- * frozen, semi-Lagrangian pass).  Device memory: (ceil(max_steps/k) + k + 1) states of
- * 56 B per cell, k step records of 136 B per cell (k = checkpoint_every) and 11 adjoint
- * fields of 16 B per cell.
+ * divergence, sources with the hot-gas viscosity's velocity Laplacian, curl, MacCormack
+ * correction with the limiter's active set frozen, semi-Lagrangian pass).  Device memory:
+ * (ceil(max_steps/k) + k + 1) states of 56 B per cell, k step records of 136 B per cell
+ * (k = checkpoint_every) and 12 adjoint fields of 16 B per cell.
  * The derivative is that of the step with a converged pressure solve (the solve is
  * x = A^-1 b for the symmetric 7-point operator A, so its adjoint is the same solve);
- * requirements: FS_PREC_FP32 storage, FS_SOLVE_REFINE64, `visc` = 0.  Float atomics
+ * requirements: FS_PREC_FP32 storage, FS_SOLVE_REFINE64.  Float atomics
  * accumulate scattered adjoints, so results vary in the last bits run to run.
  * Diagnostics records are not written by these steps. */
 typedef struct fs_adjoint fs_adjoint;

@@ -266,7 +266,10 @@ __global__ __launch_bounds__(256) void k_curl_adj(Dims g, const float4* __restri
 }
 
 // ---- source kernel^T (src/sim.js:214-396) through forces_cell<Dual> ------------------------
-constexpr int FORCES_IN = 20;   // v(3) s(4) s2(4) w(3) |w| of the 6 neighbours
+// v(3) s(4) s2(4) w(3) |w| of the 6 neighbours, and with hot-gas viscosity on the velocity Laplacian (3),
+// whose adjoint goes to llap for k_visc_adj to spread over the stencil
+constexpr int FORCES_IN = 23;
+constexpr int FORCES_IN_INVISCID = 20;
 __global__ __launch_bounds__(256) void k_forces_adj(const DevParams* __restrict__ P, Dims g, const float4* __restrict__ velY,
                                                     const float4* __restrict__ sY, const float4* __restrict__ s2Y,
                                                     const float4* __restrict__ vort, const float4* __restrict__ geo,
@@ -274,10 +277,12 @@ __global__ __launch_bounds__(256) void k_forces_adj(const DevParams* __restrict_
                                                     const fs_spot* __restrict__ spots, const float* __restrict__ faceFlux,
                                                     const float4* __restrict__ lvOut, const float4* __restrict__ lsOut,
                                                     const float4* __restrict__ ls2Out, float4* lvY, float4* lsY, float4* ls2Y,
-                                                    float4* lvort, double* gspot, double* gface) {
+                                                    float4* lvort, float4* __restrict__ llap, double* gspot, double* gface) {
     Cell c;
     if (!cell_of(g, c)) return;
     const ForcesCell k = forces_setup<S>(P, g, c, velY, geo, noise, curlT, nn);
+    const bool visc = P->f[P_visc] > 0.f;
+    const int nin = visc ? FORCES_IN : FORCES_IN_INVISCID;
     const float4 v4 = velY[c.i], s4 = sY[c.i], t4 = s2Y[c.i], w4 = vort[c.i];
     int nb[6];
     nb[0] = (c.z * g.ny + c.y) * g.nx + clampi(c.x + 1, 0, g.nx - 1);
@@ -289,7 +294,7 @@ __global__ __launch_bounds__(256) void k_forces_adj(const DevParams* __restrict_
     float wn[6];
     for (int j = 0; j < 6; j++) wn[j] = vort[nb[j]].w;
     const float in0[FORCES_IN] = {v4.x, v4.y, v4.z, s4.x, s4.y, s4.z, s4.w, t4.x, t4.y, t4.z, t4.w,
-                                  w4.x, w4.y, w4.z, wn[0], wn[1], wn[2], wn[3], wn[4], wn[5]};
+                                  w4.x, w4.y, w4.z, wn[0], wn[1], wn[2], wn[3], wn[4], wn[5], k.lapx, k.lapy, k.lapz};
     const float4 Lv = lvOut[c.i], Ls = lsOut[c.i], Ls2 = ls2Out[c.i];
     const bool face = P->has_face_flux != 0;
     const float fv = (face && c.y == 0) ? faceFlux[c.z * g.nx + c.x] : 0.f;
@@ -301,14 +306,15 @@ __global__ __launch_bounds__(256) void k_forces_adj(const DevParams* __restrict_
         Dual s[4] = {x[3], x[4], x[5], x[6]}, s2[4] = {x[7], x[8], x[9], x[10]};
         const Dual w[3] = {x[11], x[12], x[13]};
         const Dual gx = x[14] - x[15], gy = x[16] - x[17], gz = x[18] - x[19];
+        const Dual lap[3] = {x[20], x[21], x[22]};
         forces_cell<Dual>(P->f, P->d, k, spots, P->n_spots, sp, nullptr, face, Dual(fv, seed_face ? 1.f : 0.f), vx, vy, vz,
-                          s, s2, w, gx, gy, gz, phi);
+                          s, s2, w, gx, gy, gz, lap, phi);
         return Lv.x * vx.d + Lv.y * vy.d + Lv.z * vz.d + Lv.w * phi.d + Ls.x * s[0].d + Ls.y * s[1].d + Ls.z * s[2].d +
                Ls.w * s[3].d + Ls2.x * s2[0].d + Ls2.y * s2[1].d + Ls2.z * s2[2].d + Ls2.w * s2[3].d;
     };
     // the 20 state inputs: four seeded per vector-mode pass (Dual4 gives each column the scalar pass's bits)
-    float l[FORCES_IN];
-    for (int j0 = 0; j0 < FORCES_IN; j0 += 4) {
+    float l[FORCES_IN] = {};
+    for (int j0 = 0; j0 < nin; j0 += 4) {
         Dual4 x[FORCES_IN];
 #pragma unroll
         for (int j = 0; j < FORCES_IN; j++) {
@@ -319,8 +325,9 @@ __global__ __launch_bounds__(256) void k_forces_adj(const DevParams* __restrict_
         Dual4 s[4] = {x[3], x[4], x[5], x[6]}, s2[4] = {x[7], x[8], x[9], x[10]};
         const Dual4 w[3] = {x[11], x[12], x[13]};
         const Dual4 gx = x[14] - x[15], gy = x[16] - x[17], gz = x[18] - x[19];
+        const Dual4 lap[3] = {x[20], x[21], x[22]};
         forces_cell<Dual4>(P->f, P->d, k, spots, P->n_spots, -1, nullptr, face, Dual4(fv), vx, vy, vz, s, s2, w, gx, gy,
-                           gz, phi);
+                           gz, lap, phi);
         float out[4];
         FS_D4(out[u] = Lv.x * vx.d[u] + Lv.y * vy.d[u] + Lv.z * vz.d[u] + Lv.w * phi.d[u] + Ls.x * s[0].d[u] +
                        Ls.y * s[1].d[u] + Ls.z * s[2].d[u] + Ls.w * s[3].d[u] + Ls2.x * s2[0].d[u] + Ls2.y * s2[1].d[u] +
@@ -338,6 +345,7 @@ __global__ __launch_bounds__(256) void k_forces_adj(const DevParams* __restrict_
     aadd(comp(lvort, nb[0], 3), l[14]); aadd(comp(lvort, nb[1], 3), l[15]);
     aadd(comp(lvort, nb[2], 3), l[16]); aadd(comp(lvort, nb[3], 3), l[17]);
     aadd(comp(lvort, nb[4], 3), l[18]); aadd(comp(lvort, nb[5], 3), l[19]);
+    if (visc) llap[c.i] = make_float4(l[20], l[21], l[22], 0.f);
     // parameters: the spots whose sphere holds this cell, the floor flux of a floor cell
     for (int q = 0; q < P->n_spots; q++) {
         const fs_spot sp = spots[q];
@@ -348,6 +356,37 @@ __global__ __launch_bounds__(256) void k_forces_adj(const DevParams* __restrict_
         }
     }
     if (face && c.y == 0) gface[c.z * g.nx + c.x] += (double)pass(-1, -1, true);
+}
+
+// ---- the velocity Laplacian of the viscous term, transposed (src/sim.js:339-346) -------------
+// lap(c) = sum over the 6 directions d of v(clamp(c + d)) - 6 v(c) (clamped texel loads), so cell m
+// receives -6 llap(m), llap(m - d) for every direction d with m - d inside the grid, and llap(m) once
+// more for every d with m + d outside it (there the clamped neighbour of m is m itself).  A gather:
+// each cell adds to its own adjoint only.
+__global__ __launch_bounds__(256) void k_visc_adj(Dims g, const float4* __restrict__ llap, float4* __restrict__ lvY) {
+    Cell c;
+    if (!cell_of(g, c)) return;
+    const float4 me = llap[c.i];
+    float ax = -6.f * me.x, ay = -6.f * me.y, az = -6.f * me.z;
+    const int sx = 1, sy = g.nx, sz = g.nx * g.ny;
+    const int pos[3] = {c.x, c.y, c.z}, dim[3] = {g.nx, g.ny, g.nz}, st[3] = {sx, sy, sz};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+#pragma unroll
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+            // direction d = sgn e_a: the cell m - d uses m as its neighbour along d
+            const int q = pos[a] - sgn;
+            if (q >= 0 && q < dim[a]) {
+                const float4 o = llap[c.i - sgn * st[a]];
+                ax += o.x; ay += o.y; az += o.z;
+            }
+            const int e = pos[a] + sgn;
+            if (e < 0 || e >= dim[a]) { ax += me.x; ay += me.y; az += me.z; }
+        }
+    }
+    float4 v = lvY[c.i];
+    v.x += ax; v.y += ay; v.z += az;
+    lvY[c.i] = v;
 }
 
 // ---- div^T (src/sim.js:399-413) -----------------------------------------------------------
@@ -611,10 +650,18 @@ __global__ __launch_bounds__(256) void k_forces_tan(const DevParams* __restrict_
     Dual s2[4] = {Dual(t4.x, dt2.x), Dual(t4.y, dt2.y), Dual(t4.z, dt2.z), Dual(t4.w, dt2.w)};
     const Dual w[3] = {Dual(w4.x, dw.x), Dual(w4.y, dw.y), Dual(w4.z, dw.z)};
     const Dual gx = wn[0] - wn[1], gy = wn[2] - wn[3], gz = wn[4] - wn[5];
+    // the Laplacian's tangent, in forces_setup's order of terms (+x, -x, +y, -y, +z, -z, then -6 v)
+    Dual lap[3] = {Dual(k.lapx), Dual(k.lapy), Dual(k.lapz)};
+    if (P->f[P_visc] > 0.f) {
+        const float4 a = tvY[nb[0]], bb = tvY[nb[1]], cc = tvY[nb[2]], dd = tvY[nb[3]], ee = tvY[nb[4]], ff = tvY[nb[5]];
+        lap[0].d = a.x + bb.x + cc.x + dd.x + ee.x + ff.x - dv.x * 6.f;
+        lap[1].d = a.y + bb.y + cc.y + dd.y + ee.y + ff.y - dv.y * 6.f;
+        lap[2].d = a.z + bb.z + cc.z + dd.z + ee.z + ff.z - dv.z * 6.f;
+    }
     const bool face = P->has_face_flux != 0;
     const bool floor_face = face && c.y == 0;
     const Dual fv(floor_face ? faceFlux[c.z * g.nx + c.x] : 0.f, floor_face ? face_dir[c.z * g.nx + c.x] : 0.f);
-    forces_cell<Dual>(P->f, P->d, k, spots, P->n_spots, -1, spot_dir, face, fv, vx, vy, vz, s, s2, w, gx, gy, gz, phi);
+    forces_cell<Dual>(P->f, P->d, k, spots, P->n_spots, -1, spot_dir, face, fv, vx, vy, vz, s, s2, w, gx, gy, gz, lap, phi);
     tvF[c.i] = make_float4(vx.d, vy.d, vz.d, phi.d);
     tS[c.i] = make_float4(s[0].d, s[1].d, s[2].d, s[3].d);
     tS2[c.i] = make_float4(s2[0].d, s2[1].d, s2[2].d, s2[3].d);
@@ -689,10 +736,12 @@ void launch_curl_adj(const Dims& g, hipStream_t s, const float4* v, const float4
 }
 void launch_forces_adj(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, const float4* velY,
                        const float4* sY, const float4* s2Y, const float4* vort, const float4* lvOut, const float4* lsOut,
-                       const float4* ls2Out, float4* lvY, float4* lsY, float4* ls2Y, float4* lvort, double* gspot,
-                       double* gface) {
+                       const float4* ls2Out, float4* lvY, float4* lsY, float4* ls2Y, float4* lvort, float4* llap,
+                       bool visc, double* gspot, double* gface) {
     k_forces_adj<<<g.nblocks, 256, 0, s>>>(P, g, velY, sY, s2Y, vort, (const float4*)F.geo, F.noise, F.curl, F.noise_n,
-                                           F.spots, F.face_flux, lvOut, lsOut, ls2Out, lvY, lsY, ls2Y, lvort, gspot, gface);
+                                           F.spots, F.face_flux, lvOut, lsOut, ls2Out, lvY, lsY, ls2Y, lvort, llap, gspot,
+                                           gface);
+    if (visc) k_visc_adj<<<g.nblocks, 256, 0, s>>>(g, llap, lvY);
 }
 void launch_div_adj(const Dims& g, hipStream_t s, const DevParams* P, const double* lb, float4* lvF) {
     k_div_adj<double><<<g.nblocks, 256, 0, s>>>(P, g, lb, lvF);

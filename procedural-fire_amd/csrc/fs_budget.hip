@@ -83,8 +83,9 @@ __global__ __launch_bounds__(BB) void k_forces_budget(const DevParams* __restric
         const bool face = P->has_face_flux != 0;
         const float fv = (face && c.y == 0) ? faceFlux[c.z * g.nx + c.x] : 0.f;
         CellTrace t;
+        const float lap[3] = {k.lapx, k.lapy, k.lapz};
         forces_cell<float, CellTrace>(P->f, P->d, k, spots, P->n_spots, -1, nullptr, face, fv, vx, vy, vz, s, s2, wv, gx, gy,
-                                      gz, phi, &t);
+                                      gz, lap, phi, &t);
         Codec<S>::st(velOut, c.i, make_float4(vx, vy, vz, phi));
         Codec<S>::st(sOut, c.i, make_float4(s[0], s[1], s[2], s[3]));
         Codec<S>::st(s2Out, c.i, make_float4(s2[0], s2[1], s2[2], s2[3]));

@@ -1296,7 +1296,7 @@ struct fs_adjoint {
     int predict_step = 1, predict_lin = 1;   // V-cycles the last step / adjoint-or-tangent solve needed
     // adjoint (or tangent) fields: the state (l_vel, l_S, l_S2 = seed, then the result) and
     // the intermediates of one step
-    enum { L_VEL, L_S, L_S2, L_VF, L_VY, L_SY, L_S2Y, L_VORT, L_VH, L_SH, L_S2H, L_COUNT };
+    enum { L_VEL, L_S, L_S2, L_VF, L_VY, L_SY, L_S2Y, L_VORT, L_VH, L_SH, L_S2H, L_LAP, L_COUNT };
     float4* l[L_COUNT] = {};
     double* gspot = nullptr;         // FS_MAX_SPOTS
     double* gface = nullptr;         // nx * nz
@@ -1337,7 +1337,6 @@ int adj_rec_alloc(fs_adjoint* a, StepRec& r) {
 int adj_check(const fs_sim* h) {
     if (h->prec != FS_PREC_FP32) return fail(FS_ESTATE, "the adjoint needs FS_PREC_FP32 storage");
     if (h->cfg.solve != FS_SOLVE_REFINE64) return fail(FS_ESTATE, "the adjoint needs the converged solve FS_SOLVE_REFINE64");
-    if (h->prm[P_visc] != 0.f) return fail(FS_ESTATE, "the adjoint needs visc = 0");
     return FS_OK;
 }
 // FP64-refined solve of A x = bkeep from the current x64, run from the host like the
@@ -1433,7 +1432,7 @@ void reverse_step(fs_adjoint* a, int k, double tol, int maxc) {
     launch_fill_zero(s, l[fs_adjoint::L_VORT], fb);
     launch_forces_adj(g, s, h->P_d, F, r.vY, r.sY, r.s2Y, r.vort, l[fs_adjoint::L_VF], l[fs_adjoint::L_S],
                       l[fs_adjoint::L_S2], l[fs_adjoint::L_VY], l[fs_adjoint::L_SY], l[fs_adjoint::L_S2Y],
-                      l[fs_adjoint::L_VORT], a->gspot, a->gface);
+                      l[fs_adjoint::L_VORT], l[fs_adjoint::L_LAP], h->prm[P_visc] > 0.f, a->gspot, a->gface);
     // curl (sim.js:200-211)
     launch_curl_adj(g, s, r.vY, l[fs_adjoint::L_VORT], l[fs_adjoint::L_VY]);
     // MacCormack correction and semi-Lagrangian pass (sim.js:142-197)

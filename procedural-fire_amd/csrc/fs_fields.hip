@@ -237,7 +237,8 @@ __global__ __launch_bounds__(256) void k_forces(const DevParams* __restrict__ P,
     const float wv[3] = {w.x, w.y, w.z};
     const bool face = P->has_face_flux != 0;
     const float fv = (face && c.y == 0) ? faceFlux[c.z * g.nx + c.x] : 0.f;
-    forces_cell<float>(P->f, P->d, k, spots, P->n_spots, -1, nullptr, face, fv, vx, vy, vz, s, s2, wv, gx, gy, gz, phi);
+    const float lap[3] = {k.lapx, k.lapy, k.lapz};
+    forces_cell<float>(P->f, P->d, k, spots, P->n_spots, -1, nullptr, face, fv, vx, vy, vz, s, s2, wv, gx, gy, gz, lap, phi);
     Codec<S>::st(velOut, c.i, make_float4(vx, vy, vz, phi));
     Codec<S>::st(sOut, c.i, make_float4(s[0], s[1], s[2], s[3]));
     Codec<S>::st(s2Out, c.i, make_float4(s2[0], s2[1], s2[2], s2[3]));

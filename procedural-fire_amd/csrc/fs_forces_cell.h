@@ -156,7 +156,7 @@ struct ForcesCell {
     float4 gg;                 // geometry: band, bed, solid, sdf
     float src, srcW, bed, bedHot, anchor, wsrc;
     float cnx, cny, cnz;       // curl-noise sample
-    float lapx, lapy, lapz;    // Laplacian of the velocity (hot-gas viscosity; state-independent in the adjoint)
+    float lapx, lapy, lapz;    // 7-point Laplacian of the velocity (hot-gas viscosity on; else 0)
     float wSx, wSz, ks, kt;    // sponge targets and weights, top fade
 };
 
@@ -178,7 +178,8 @@ struct CellTrace {
 #define FS_TRACE(...) do { if constexpr (!std::is_same<Tr, NoTrace>::value) { __VA_ARGS__; } } while (0)
 
 // One cell: v (3), s (4: T, fuel, soot, q), s2 (4: vitiated, age, smoke, blue) in place;
-// w = own vorticity (3), gx/gy/gz = differences of |w| of the neighbours; phi out.
+// w = own vorticity (3), gx/gy/gz = differences of |w| of the neighbours, lap = the
+// 7-point Laplacian of the velocity (3; read only with hot-gas viscosity on); phi out.
 // seed_spot: the spot whose flux carries a unit derivative (Dual only; -1: none), or
 // spot_dir: the derivative of every spot's flux (Dual only; null: use seed_spot);
 // face: the floor flux of this cell as T.
@@ -186,8 +187,8 @@ template <class T, class Tr = NoTrace>
 __device__ __forceinline__ void forces_cell(const float* U, const float* D, const ForcesCell& k, const fs_spot* __restrict__ spots,
                                             int n_spots, int seed_spot, const float* spot_dir, bool has_face,
                                             T face, T& vx, T& vy,
-                                            T& vz, T* s, T* s2, const T* w, T gx, T gy, T gz, T& phi_out,
-                                            Tr* tr = nullptr) {
+                                            T& vz, T* s, T* s2, const T* w, T gx, T gy, T gz, const T* lap,
+                                            T& phi_out, Tr* tr = nullptr) {
     const float dt = k.dt, I = k.I;
     FS_TRACE(tr->y_in = s[1]; tr->th_in = s[0]);
     // ---- sources (sim.js:246-297)
@@ -274,7 +275,7 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
     if (U[P_visc] > 0.f) {
         const T nu = U[P_visc] * 1.5e-5f * gpow(gmax(Tk, 293.f) / 293.f, 1.7f);
         const T kv = gmin(nu * dt / (k.dx * k.dx), 0.15f);
-        vx = vx + k.lapx * kv; vy = vy + k.lapy * kv; vz = vz + k.lapz * kv;
+        vx = vx + lap[0] * kv; vy = vy + lap[1] * kv; vz = vz + lap[2] * kv;
     }
     // ---- laminar base gate (sim.js:350)
     const float lamAge = U[P_lamAge];

@@ -217,8 +217,8 @@ void launch_adv2_adj(const Dims& g, hipStream_t s, const DevParams* P, const flo
 void launch_curl_adj(const Dims& g, hipStream_t s, const float4* v, const float4* lvort, float4* lv);
 void launch_forces_adj(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, const float4* velY,
                        const float4* sY, const float4* s2Y, const float4* vort, const float4* lvOut, const float4* lsOut,
-                       const float4* ls2Out, float4* lvY, float4* lsY, float4* ls2Y, float4* lvort, double* gspot,
-                       double* gface);
+                       const float4* ls2Out, float4* lvY, float4* lsY, float4* ls2Y, float4* lvort, float4* llap,
+                       bool visc, double* gspot, double* gface);
 void launch_div_adj(const Dims& g, hipStream_t s, const DevParams* P, const double* lb, float4* lvF);
 void launch_project_adj(const Dims& g, hipStream_t s, const DevParams* P, const float4* vIn, const double* p,
                         const float4* geo, const float4* lvOut, float4* lvIn, float* lp);

@@ -527,7 +527,7 @@ class Adjoint:
     forward(dt, n) runs the steps (the sim ends in the final state) keeping the state every
     `checkpoint_every` steps; seed the adjoint of the final state (seed / seed_mismatch), then
     backward() gives the adjoint of the initial state (read) and the gradients of the spot
-    fluxes and the floor flux (grads).  Needs FP32 storage, the FP64-refined solve and visc 0.
+    fluxes and the floor flux (grads).  Needs FP32 storage and the FP64-refined solve.
     """
 
     def __init__(self, sim, max_steps, checkpoint_every=20):
