@@ -39,9 +39,11 @@ PARAMS = {"vortEps": 0.3}
 HBM_PEAK_GBS = 8000.0
 
 # algorithmic bytes per cell of each kernel: every input field read once, every
-# output written once; E = bytes per 4-channel texel (8 fp16/bf16, 16 fp32)
-def kernel_bytes(E, p_bytes=4):
-    return {"adv1": 6 * E, "adv2": 9 * E, "curl": 2 * E, "forces": 8 * E, "div": E + 4, "project": 3 * E + p_bytes}
+# output written once; E = bytes per 4-channel texel (8 fp16/bf16, 16 fp32); the
+# advection passes also hand over one 4-byte word per cell (grids up to 1022 a side)
+def kernel_bytes(E, p_bytes=4, pack=4):
+    return {"adv1": 6 * E + pack, "adv2": 9 * E + pack, "curl": 2 * E, "forces": 8 * E, "div": E + 4,
+            "project": 3 * E + p_bytes}
 
 
 def mg_bytes_per_cycle(levels, coarse_iters, sweeps=4):

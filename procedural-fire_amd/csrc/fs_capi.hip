@@ -139,12 +139,16 @@ struct fs_sim {
         else mg_tol32_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
         return FS_OK;
     }
-    int solve(hipStream_t s) {   // the pressure solve of one step (sim.js:642)
+    // the pressure solve of one step (sim.js:642); begun: its starting residual is already formed
+    int solve(hipStream_t s, bool begun = false) {
         if (cfg.solve == FS_SOLVE_SPECTRAL64) {
-            mg_spectral_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
+            if (!begun) mg_spectral_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
             for (int k = 0; k < std::min(cfg.max_vcycles, MG_MAX_REFINE); k++)
                 if (int rc = mg_spectral_iter(mg, spec, s, cfg.div_tol, cfg.max_vcycles)) return rc;
-        } else if (cfg.solve == FS_SOLVE_REFINE64) mg_refine(mg, s, cfg.div_tol, cfg.max_vcycles);
+        } else if (cfg.solve == FS_SOLVE_REFINE64) {
+            if (!begun) mg_refine_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
+            for (int k = 0; k < std::min(cfg.max_vcycles, MG_MAX_REFINE); k++) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
+        }
         else if (cfg.solve == FS_SOLVE_TOL32) mg_solve_tol32(mg, s, cfg.div_tol, cfg.max_vcycles);
         else for (int i = 0; i < cfg.vcycles; i++) mg_vcycle(mg, s, nullptr);
         return FS_OK;

@@ -23,6 +23,7 @@
 #include <cmath>
 
 #include "fs_internal.h"
+#include "fs_resid.h"
 
 namespace {
 
@@ -37,27 +38,6 @@ constexpr int RB = 256;   // threads per block of every multigrid kernel (64 x 4
 
 // grid over (half-row index or x, y, z) with 64 x 4 threads per block
 inline dim3 grid3(int w, int ny, int nz) { return dim3((unsigned)((w + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz); }
-
-// The colour-0 Gauss-Seidel update of cell (x, y, z) on a level whose iterate is all
-// zero: every neighbour contributes +0, so the update needs only the cell's own b.  The
-// kernels that zero a level (restriction, the refinement's residual) write it directly;
-// the operations are k_smooth's, in its order, so the values are identical.
-__device__ __forceinline__ float red_from_zero(float bv, int x, int y, int z, int nx, int ny, int nz, float h, float omega) {
-    float diag = 6.f;
-    if (x == 0) diag += 1.f;
-    if (x == nx - 1) diag += 1.f;
-    if (z == 0) diag += 1.f;
-    if (z == nz - 1) diag += 1.f;
-    if (y == 0) diag -= 1.f;
-    if (y == ny - 1) diag += 1.f;
-    const float sum = 0.f;
-    float pn = (sum - h * h * bv) / diag;
-    if (omega != 1.f) {
-        const float po = 0.f;
-        pn = po + omega * (pn - po);
-    }
-    return pn;
-}
 
 // one colour pass of red-black GS (sim.js:422-437).  Thread (hx, y, z) owns the cell
 // x = 2hx + s of this colour, s = (y + z + color) & 1.  With `xupd` set (the last pass
@@ -290,33 +270,36 @@ __global__ __launch_bounds__(RB) void k_smooth_prolong(float* __restrict__ p, co
     p[own] = pn;
 }
 
+constexpr int FIN = 1024;   // threads of the one block that sums the residual's partials
+
 // Sums the per-block partials in a fixed order.  `init`: the starting residual of a
 // solve (phase 0); otherwise the residual after one more V-cycle (the phase count
 // lives in the device state, so one captured iteration can be replayed any number
 // of times).  Sets st->done once rel <= tol or the phase reaches max_cycles.
 __device__ void finalize_sums(RefineState* st, const double* partials, int npart, double tol, int max_cycles, int init) {
-    __shared__ double sr[RB], sb[RB];
+    __shared__ double sr[FIN / 64], sb[FIN / 64];
     double rr = 0.0, bb = 0.0;
     const int done = init ? 0 : st->done;              // in flight with the partials' loads
-    // each thread sums partials j, j + RB, j + 2 RB, ... in that order, loading 8 at a time
+    // each thread sums partials j, j + FIN, j + 2 FIN, ... in that order, loading 8 at a time
     int j = threadIdx.x;
-    for (; j + 7 * RB < npart; j += 8 * RB) {
+    for (; j + 7 * FIN < npart; j += 8 * FIN) {
         double a[8], c[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) { a[u] = partials[2 * (j + u * RB)]; c[u] = partials[2 * (j + u * RB) + 1]; }
+        for (int u = 0; u < 8; u++) { a[u] = partials[2 * (j + u * FIN)]; c[u] = partials[2 * (j + u * FIN) + 1]; }
 #pragma unroll
         for (int u = 0; u < 8; u++) { rr += a[u]; bb += c[u]; }
     }
-    for (; j < npart; j += RB) { rr += partials[2 * j]; bb += partials[2 * j + 1]; }
+    for (; j < npart; j += FIN) { rr += partials[2 * j]; bb += partials[2 * j + 1]; }
     asm volatile("" ::"v"(rr), "v"(bb), "s"(done));
     if (done) return;                                  // uniform over the block
-    sr[threadIdx.x] = rr; sb[threadIdx.x] = bb;
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    if ((threadIdx.x & 63) == 0) { sr[threadIdx.x >> 6] = rr; sb[threadIdx.x >> 6] = bb; }
     __syncthreads();
-    for (int s = RB / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) { sr[threadIdx.x] += sr[threadIdx.x + s]; sb[threadIdx.x] += sb[threadIdx.x + s]; }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
+        double r2 = 0.0, b2 = 0.0;
+        for (int w = 0; w < FIN / 64; w++) { r2 += sr[w]; b2 += sb[w]; }
+        sr[0] = r2; sb[0] = b2;
         const double rel = sb[0] > 0.0 ? sqrt(sr[0] / sb[0]) : 0.0;
         const int phase = init ? 0 : st->used + 1;   // V-cycles applied so far in this solve
         st->rel[phase] = rel;
@@ -324,30 +307,6 @@ __device__ void finalize_sums(RefineState* st, const double* partials, int npart
         st->bnorm2 = sb[0];
         st->done = (rel <= tol || phase >= max_cycles) ? 1 : 0;
     }
-}
-
-// FP64 residual r = b - A x of the level-0 operator at one cell of parity s
-template <class X>
-__device__ __forceinline__ double resid_at(const X* __restrict__ x, const float* __restrict__ b, int xx, int y, int z, int sgn,
-                                           int hx, int nx, int ny, int nz, int nh, int rs, double ih2, double* bv) {
-    const int row = (z * ny + y) * rs;
-    const int own = row + sgn * nh + hx, oth = row + (1 - sgn) * nh;
-    const double me = (double)x[own];
-    const double vw = (double)x[fs_opaque(xx > 0 ? oth + hx - 1 + sgn : own)];
-    const double ve = (double)x[fs_opaque(xx < nx - 1 ? oth + hx + sgn : own)];
-    const double vk = (double)x[fs_opaque(z > 0 ? own - ny * rs : own)];
-    const double vf = (double)x[fs_opaque(z < nz - 1 ? own + ny * rs : own)];
-    const double vs = (double)x[fs_opaque(y > 0 ? own - rs : own)];
-    const double vn = (double)x[fs_opaque(y < ny - 1 ? own + rs : own)];
-    *bv = (double)b[own];
-    double sum = 0.0, diag = 6.0;
-    if (xx > 0) sum += vw; else diag += 1.0;
-    if (xx < nx - 1) sum += ve; else diag += 1.0;
-    if (z > 0) sum += vk; else diag += 1.0;
-    if (z < nz - 1) sum += vf; else diag += 1.0;
-    if (y > 0) sum += vs; else diag -= 1.0;
-    if (y < ny - 1) sum += vn; else diag += 1.0;
-    return *bv - (sum - diag * me) * ih2;
 }
 
 // FP64 residual over the level-0 grid.  Each block writes its (r.r, b.b) partial sums
@@ -358,7 +317,7 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
                                                 double h, float* __restrict__ rhs_out, float* __restrict__ p_out,
                                                 double* __restrict__ partials, const int* __restrict__ skip,
                                                 float hred = 0.f, float omega = 1.f, int red_first = 0) {
-    __shared__ double sr[RB], sb[RB];
+    __shared__ double sr[RB / 64], sb[RB / 64];
     const int nh = (nx + 1) >> 1, rs = 2 * nh;
     const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -370,7 +329,10 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
     if (inside) {
         // both cells of the pair (the odd one of a padded row slot is computed and dropped)
 #pragma unroll
-        for (int sgn = 0; sgn < 2; sgn++) r[sgn] = resid_at(x, b, 2 * hx + sgn, y, z, sgn, hx, nx, ny, nz, nh, rs, ih2, &bv[sgn]);
+        for (int sgn = 0; sgn < 2; sgn++) {
+            bv[sgn] = (double)b[(z * ny + y) * rs + sgn * nh + hx];
+            r[sgn] = resid_at(x, bv[sgn], 2 * hx + sgn, y, z, sgn, hx, nx, ny, nz, nh, rs, ih2);
+        }
     }
     const int sk = skip[fs_opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     asm volatile("" ::"v"(r[0]), "v"(r[1]), "v"(bv[0]), "v"(bv[1]), "v"(sk));
@@ -392,21 +354,19 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
             }
         }
     }
-    sr[threadIdx.x] = rr; sb[threadIdx.x] = bb;
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    if ((threadIdx.x & 63) == 0) { sr[threadIdx.x >> 6] = rr; sb[threadIdx.x >> 6] = bb; }
     __syncthreads();
-    for (int s = RB / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) { sr[threadIdx.x] += sr[threadIdx.x + s]; sb[threadIdx.x] += sb[threadIdx.x + s]; }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
         const int blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        partials[2 * blk] = sr[0];
-        partials[2 * blk + 1] = sb[0];
+        partials[2 * blk] = ((sr[0] + sr[1]) + sr[2]) + sr[3];
+        partials[2 * blk + 1] = ((sb[0] + sb[1]) + sb[2]) + sb[3];
     }
 }
 
 // one block: the fixed-order reduction of the partials and the stopping test
-__global__ __launch_bounds__(RB) void k_finalize(RefineState* __restrict__ st, const double* __restrict__ partials, int npart,
+__global__ __launch_bounds__(FIN) void k_finalize(RefineState* __restrict__ st, const double* __restrict__ partials, int npart,
                                                  double tol, int max_cycles, int init) {
     finalize_sums(st, partials, npart, tol, max_cycles, init);
 }
@@ -838,7 +798,7 @@ void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     const bool red = fuse_red0(mg);
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
                                                                   mg.partials, &mg.st->never, mg.h[0], mg.omega, red ? 1 : 0);
-    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
 }
 
 void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
@@ -852,7 +812,7 @@ void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     }
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
                                                                   mg.partials, done, mg.h[0], mg.omega, red ? 1 : 0);
-    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
 }
 
 void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
@@ -870,7 +830,7 @@ void mg_spectral_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], nullptr,
                                                                   mg.partials, &mg.st->never);
-    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
 }
 
 int mg_spectral_iter(GpuMG& mg, Spectral* sp, hipStream_t s, double tol, int max_cycles) {
@@ -882,7 +842,7 @@ int mg_spectral_iter(GpuMG& mg, Spectral* sp, hipStream_t s, double tol, int max
     k_update<<<(n + RB - 1) / RB, RB, 0, s>>>(mg.x64, mg.p[0], n, done);
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], nullptr,
                                                                   mg.partials, done);
-    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
     return FS_OK;
 }
 
@@ -891,7 +851,7 @@ void mg_tol32_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     k_resid64<float><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr,
                                                                  mg.partials, &mg.st->never);
-    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
 }
 
 void mg_tol32_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
@@ -901,7 +861,7 @@ void mg_tol32_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     mg_vcycle(mg, s, done);
     k_resid64<float><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr,
                                                                  mg.partials, done);
-    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
 }
 
 void mg_solve_tol32(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
@@ -914,5 +874,5 @@ void mg_residual_of_p(GpuMG& mg, hipStream_t s) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     k_resid64<float><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr,
                                                                  mg.partials, &mg.st->never);
-    k_finalize<<<1, RB, 0, s>>>(mg.st, mg.partials, mg.npart, 0.0, 0, 1);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, 0.0, 0, 1);
 }
