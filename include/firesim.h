@@ -44,9 +44,9 @@ extern "C" {
 #define FS_SOLVE_SPECTRAL64 3  /* build extension: the exact separable inner solve (DST-II / DCT-IV
                                   transforms as FP32 matrix-core GEMMs, one division by the
                                   eigenvalues) inside the FP64 defect correction to div_tol, in place
-                                  of the V-cycles; nx even; max_vcycles caps the corrections.
-                                  FS_SPECTRAL_BLAS_Z set in the environment at create runs the z
-                                  transforms through rocBLAS instead of the fused kernel */
+                                  of the V-cycles; nx even; adaptive = 1; max_vcycles caps the
+                                  corrections.  The z pass is one fused kernel for nz = 32, 64, 96
+                                  or 128 (nx ny a multiple of 32), rocBLAS GEMMs otherwise */
 #define FS_SOLVE_TOL32      2  /* FP32 V-cycles on p until the FP64-evaluated ||b - A p||/||b||
                                   <= div_tol or max_vcycles (no defect correction)        */
 
@@ -107,7 +107,8 @@ typedef struct fs_config {
                              as the previous step needed and checks the device's stopping test
                              once per step (one stream sync per step, no wasted launches);
                              0 = fully asynchronous: max_vcycles iterations are enqueued and the
-                             ones after convergence exit at once                              */
+                             ones after convergence exit at once.  FS_SOLVE_SPECTRAL64 needs 1
+                             (its library transforms cannot exit on the device's flag)         */
     float  omega;         /* FS_SOLVE_REFINE64 / TOL32: over-relaxation factor of the red-black
                              smoother on every level but the coarsest (0 = default 1.25; 1 =
                              plain Gauss-Seidel).  FS_SOLVE_VCYCLES, the reference's solve,

@@ -141,10 +141,14 @@ struct fs_sim {
     }
     // the pressure solve of one step (sim.js:642); begun: its starting residual is already formed
     int solve(hipStream_t s, bool begun = false) {
-        if (cfg.solve == FS_SOLVE_SPECTRAL64) {
+        if (cfg.solve == FS_SOLVE_SPECTRAL64) {   // corrections enqueued while the device's stop flag is clear
             if (!begun) mg_spectral_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
-            for (int k = 0; k < std::min(cfg.max_vcycles, MG_MAX_REFINE); k++)
+            for (int k = 0; k < std::min(cfg.max_vcycles, MG_MAX_REFINE); k++) {
+                FS_HIP_CHECK(hipMemcpyAsync(host_state, &mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+                FS_HIP_CHECK(hipStreamSynchronize(s));
+                if (host_state[1]) break;
                 if (int rc = mg_spectral_iter(mg, spec, s, cfg.div_tol, cfg.max_vcycles)) return rc;
+            }
         } else if (cfg.solve == FS_SOLVE_REFINE64) {
             if (!begun) mg_refine_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
             for (int k = 0; k < std::min(cfg.max_vcycles, MG_MAX_REFINE); k++) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
@@ -446,6 +450,10 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
         return fail(FS_EINVAL, "a field must stay below 4 GiB (fp32 storage: fewer than 2^28 cells)");
     if (cfg->solve < FS_SOLVE_VCYCLES || cfg->solve > FS_SOLVE_SPECTRAL64) return fail(FS_EINVAL, "bad solve mode");
     if (cfg->solve == FS_SOLVE_SPECTRAL64 && nx % 2 != 0) return fail(FS_EINVAL, "FS_SOLVE_SPECTRAL64 needs an even nx");
+    // its transforms are library GEMMs that cannot skip on the device's stop flag: iterations are enqueued as the
+    // host's check of that flag asks for them
+    if (cfg->solve == FS_SOLVE_SPECTRAL64 && !cfg->adaptive)
+        return fail(FS_EINVAL, "FS_SOLVE_SPECTRAL64 needs adaptive = 1 (the host enqueues each correction)");
     if (!(cfg->omega == 0.f || (cfg->omega > 0.f && cfg->omega < 2.f))) return fail(FS_EINVAL, "omega must be 0 or in (0, 2)");
     if (cfg->pre_sweeps < -1 || cfg->pre_sweeps > 8 || cfg->post_sweeps < 0 || cfg->post_sweeps > 8)
         return fail(FS_EINVAL, "pre sweeps must be in -1..8, post sweeps in 0..8");

@@ -291,7 +291,7 @@ int spectral_build(Spectral** out, int nx, int ny, int nz, double h, hipStream_t
     std::vector<float> tz(Tz.begin(), Tz.end());
     const int nrb = nz / 16;
     int dev = 0, ncu = 0;
-    sp->fused_z = nz % 32 == 0 && nrb >= 2 && nrb <= 8 && (nx * ny) % ZC == 0 && !getenv("FS_SPECTRAL_BLAS_Z") &&
+    sp->fused_z = nz % 32 == 0 && nrb >= 2 && nrb <= 8 && (nx * ny) % ZC == 0 &&
                   hipGetDevice(&dev) == hipSuccess &&
                   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0 &&
                   spec_z_prepare(nrb);
