@@ -224,7 +224,9 @@ __global__ __launch_bounds__(256) void k_forces(const DevParams* __restrict__ P,
                                                 const float* __restrict__ faceFlux, typename S::raw* __restrict__ velOut,
                                                 typename S::raw* __restrict__ sOut, typename S::raw* __restrict__ s2Out) {
     Cell c;
-    if (!cell_of(g, c)) return;
+    const bool in = cell_of(g, c);
+    const unsigned long long reach = spot_reach(P->f, g, c.x, c.y, c.z, spots, P->n_spots);   // every lane, then exit
+    if (!in) return;
     const ForcesCell k = forces_setup<S>(P, g, c, velY, geo, noise, curlT, nn);
     const float4 v4 = Codec<S>::ld(velY, c.i);
     const float4 s4 = Codec<S>::ld(sY, c.i), t4 = Codec<S>::ld(s2Y, c.i);
@@ -238,7 +240,8 @@ __global__ __launch_bounds__(256) void k_forces(const DevParams* __restrict__ P,
     const bool face = P->has_face_flux != 0;
     const float fv = (face && c.y == 0) ? faceFlux[c.z * g.nx + c.x] : 0.f;
     const float lap[3] = {k.lapx, k.lapy, k.lapz};
-    forces_cell<float>(P->f, P->d, k, spots, P->n_spots, -1, nullptr, face, fv, vx, vy, vz, s, s2, wv, gx, gy, gz, lap, phi);
+    forces_cell<float>(P->f, P->d, k, spots, P->n_spots, -1, nullptr, face, fv, vx, vy, vz, s, s2, wv, gx, gy, gz, lap, phi,
+                       (NoTrace*)nullptr, reach);
     Codec<S>::st(velOut, c.i, make_float4(vx, vy, vz, phi));
     Codec<S>::st(sOut, c.i, make_float4(s[0], s[1], s[2], s[3]));
     Codec<S>::st(s2Out, c.i, make_float4(s2[0], s2[1], s2[2], s2[3]));

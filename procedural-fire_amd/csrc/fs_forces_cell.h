@@ -182,13 +182,14 @@ struct CellTrace {
 // 7-point Laplacian of the velocity (3; read only with hot-gas viscosity on); phi out.
 // seed_spot: the spot whose flux carries a unit derivative (Dual only; -1: none), or
 // spot_dir: the derivative of every spot's flux (Dual only; null: use seed_spot);
-// face: the floor flux of this cell as T.
+// face: the floor flux of this cell as T; spot_mask: bit q clear only where spot q provably
+// misses the cell (spot_reach), so the spots loop visits the set bits in increasing q.
 template <class T, class Tr = NoTrace>
 __device__ __forceinline__ void forces_cell(const float* U, const float* D, const ForcesCell& k, const fs_spot* __restrict__ spots,
                                             int n_spots, int seed_spot, const float* spot_dir, bool has_face,
                                             T face, T& vx, T& vy,
                                             T& vz, T* s, T* s2, const T* w, T gx, T gy, T gz, const T* lap,
-                                            T& phi_out, Tr* tr = nullptr) {
+                                            T& phi_out, Tr* tr = nullptr, unsigned long long spot_mask = ~0ull) {
     const float dt = k.dt, I = k.I;
     FS_TRACE(tr->y_in = s[1]; tr->th_in = s[0]);
     // ---- sources (sim.js:246-297)
@@ -216,7 +217,8 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
     }
     FS_TRACE(tr->y_src = s[1]; tr->th_src = s[0]);
     // ---- pyrolysis spots and floor fuel flux (build extension: mass-flux sources)
-    for (int q = 0; q < n_spots; q++) {
+    for (unsigned long long m = spot_mask & (n_spots >= 64 ? ~0ull : (1ull << n_spots) - 1); m != 0; m &= m - 1) {
+        const int q = __builtin_ctzll(m);
         const fs_spot sp = spots[q];
         const float ddx = k.px - sp.pos[0], ddy = k.py - sp.pos[1], ddz = k.pz - sp.pos[2];
         const float d = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
@@ -325,6 +327,30 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
     phi_out = phi;
 }
 #undef FS_TRACE
+
+// The spots that can reach a cell of this wave's 64-cell x-row (y and z fixed), as a wave-uniform mask:
+// lane q tests spot q against the row's bounding box in float64.  A spot is left out only if the box lies
+// farther from its centre than radius / (1 - 1e-5); every cell centre of the row is at least that far, and
+// the cell's f32 distance (three rounded differences, squares and sums, a rounded sqrt: within a few units of
+// 2^-24 relative) is then still >= radius, so the cell's own test d < radius is false for it.  The mask
+// therefore only skips loop trips that change nothing.  All 64 lanes must call it (before any early exit).
+__device__ __forceinline__ unsigned long long spot_reach(const float* U, const Dims& g, int cx, int cy, int cz,
+                                                         const fs_spot* __restrict__ spots, int n_spots) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int x0 = cx - lane, x1 = min(x0 + 63, g.nx - 1);
+    const float dx = U[P_dx];
+    const double lo = (double)(U[P_ox] + ((float)x0 + 0.5f) * dx), hi = (double)(U[P_ox] + ((float)x1 + 0.5f) * dx);
+    const double py = (double)(U[P_oy] + ((float)cy + 0.5f) * dx), pz = (double)(U[P_oz] + ((float)cz + 0.5f) * dx);
+    bool near = false;
+    if (lane < n_spots) {
+        const fs_spot sp = spots[lane];
+        const double sx = sp.pos[0], ex = fmax(0.0, fmax(lo - sx, sx - hi));
+        const double ey = py - (double)sp.pos[1], ez = pz - (double)sp.pos[2];
+        const double r = (double)sp.radius / (1.0 - 1e-5);
+        near = ex * ex + ey * ey + ez * ez < r * r;
+    }
+    return __ballot(near);
+}
 
 // The per-cell constants of the source kernel: the emitter terms driven by the noise
 // textures (sim.js:246-268), the curl-noise sample (sim.js:364-365), the sponge weights
