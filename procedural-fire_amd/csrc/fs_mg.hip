@@ -39,6 +39,29 @@ constexpr int RB = 256;   // threads per block of every multigrid kernel (64 x 4
 // grid over (half-row index or x, y, z) with 64 x 4 threads per block
 inline dim3 grid3(int w, int ny, int nz) { return dim3((unsigned)((w + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz); }
 
+// p_fine += mirrored trilinear interpolation of the coarse p (sim.js:473-495);
+// thread (hx, y, z) updates the fine cells x = 2hx and 2hx+1
+__device__ __forceinline__ float prolong_at(const float* __restrict__ pc, int x, int y, int z, int cx, int cy, int cz) {
+    const int nhc = (cx + 1) >> 1, rsc = 2 * nhc;
+    const float gx = ((float)x + 0.5f) * 0.5f - 0.5f, gy = ((float)y + 0.5f) * 0.5f - 0.5f, gz = ((float)z + 0.5f) * 0.5f - 0.5f;
+    const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
+    const int g0x = (int)flx, g0y = (int)fly, g0z = (int)flz;
+    const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+        const int qx = g0x + ox, qy = g0y + oy, qz = g0z + oz;
+        const int sx = min(max(qx, 0), cx - 1), sy = min(max(qy, 0), cy - 1), sz = min(max(qz, 0), cz - 1);
+        float val = pc[(sz * cy + sy) * rsc + (sx & 1) * nhc + (sx >> 1)];
+        if (qx < 0 || qx >= cx) val = -val;
+        if (qz < 0 || qz >= cz) val = -val;
+        if (qy >= cy) val = -val;
+        const float wx = ox ? fx : 1.f - fx, wy = oy ? fy : 1.f - fy, wz = oz ? fz : 1.f - fz;
+        acc += val * wx * wy * wz;
+    }
+    return acc;
+}
 // one colour pass of red-black GS (sim.js:422-437).  Thread (hx, y, z) owns the cell
 // x = 2hx + s of this colour, s = (y + z + color) & 1.  With `xupd` set (the last pass
 // of a correction V-cycle) it also adds the final p of both cells x = 2hx, 2hx+1 to
@@ -46,7 +69,7 @@ inline dim3 grid3(int w, int ny, int nz) { return dim3((unsigned)((w + 63) / 64)
 // tolerance-driven solves; the reference's solve is omega = 1 exactly).
 __global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
                                                float h, int color, const int* __restrict__ skip, double* __restrict__ xupd,
-                                               float omega) {
+                                               float omega, const float* __restrict__ pc_own) {
     const int sk = skip[fs_opaque(0)];   // in flight with the data loads (a vector load: its wait does not hold up the
                                          // scalar kernel arguments); tested before any store
     const int nh = (nx + 1) >> 1, rs = 2 * nh;
@@ -68,7 +91,9 @@ __global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const floa
     const float vs = p[fs_opaque(y > 0 ? own - rs : own)];
     const float vn = p[fs_opaque(y < ny - 1 ? own + rs : own)];
     const float bv = b[own];
-    const float po = omega != 1.f ? p[own] : 0.f;
+    // pc_own: this colour's cells have not been prolonged yet (k_prolong_red left them); their own value is
+    // p + P(pc), formed as k_prolong forms it
+    const float po = omega != 1.f ? (pc_own ? p[own] + prolong_at(pc_own, x, y, z, nx >> 1, ny >> 1, nz >> 1) : p[own]) : 0.f;
     const bool pair = xupd && 2 * hx + 1 - sgn < nx;  // the pair's other cell, final already
     const float pother = pair ? p[oth + hx] : 0.f;
     const double xo = xupd ? xupd[own] : 0.0;
@@ -147,29 +172,6 @@ __global__ __launch_bounds__(RB) void k_restrict(const float* __restrict__ p, co
     pc[ci] = (red_first && ((xc + yc + zc) & 1) == 0) ? red_from_zero(bnew, xc, yc, zc, cx, cy, nz >> 1, hc, omega) : 0.f;
 }
 
-// p_fine += mirrored trilinear interpolation of the coarse p (sim.js:473-495);
-// thread (hx, y, z) updates the fine cells x = 2hx and 2hx+1
-__device__ __forceinline__ float prolong_at(const float* __restrict__ pc, int x, int y, int z, int cx, int cy, int cz) {
-    const int nhc = (cx + 1) >> 1, rsc = 2 * nhc;
-    const float gx = ((float)x + 0.5f) * 0.5f - 0.5f, gy = ((float)y + 0.5f) * 0.5f - 0.5f, gz = ((float)z + 0.5f) * 0.5f - 0.5f;
-    const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
-    const int g0x = (int)flx, g0y = (int)fly, g0z = (int)flz;
-    const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
-    float acc = 0.f;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
-        const int qx = g0x + ox, qy = g0y + oy, qz = g0z + oz;
-        const int sx = min(max(qx, 0), cx - 1), sy = min(max(qy, 0), cy - 1), sz = min(max(qz, 0), cz - 1);
-        float val = pc[(sz * cy + sy) * rsc + (sx & 1) * nhc + (sx >> 1)];
-        if (qx < 0 || qx >= cx) val = -val;
-        if (qz < 0 || qz >= cz) val = -val;
-        if (qy >= cy) val = -val;
-        const float wx = ox ? fx : 1.f - fx, wy = oy ? fy : 1.f - fy, wz = oz ? fz : 1.f - fz;
-        acc += val * wx * wy * wz;
-    }
-    return acc;
-}
 // One thread per coarse cell: its 3x3x3 coarse neighbourhood (mirror rule applied)
 // gives the corrections of its 8 children, each summed in prolong_at's order (the
 // child's two coarse cells per axis sit at offsets {-1, 0} or {0, +1}, weights 1/4
@@ -224,6 +226,107 @@ __global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const flo
     for (int k = 0; k < 8; k++) {
         const int a = k & 1, bb = (k >> 1) & 1, cc = (k >> 2) & 1;
         p[((2 * zc + cc) * ny + 2 * yc + bb) * rs + a * nh + xc] = fine[k];
+    }
+}
+
+// The prolongation correction of the child with offsets (a, bb, cc) of the coarse cell at offset (sx, sy, sz)
+// (each -1, 0 or +1) from the thread's own, from the thread's 3x3x3 mirrored neighbourhood c: k_prolong's sum
+// for that child, term for term (the parent's neighbourhood is c shifted by the offset).
+__device__ __forceinline__ float child_corr(const float (&c)[3][3][3], int sx, int sy, int sz, int a, int bb, int cc) {
+    const float fx = a ? 0.25f : 0.75f, fy = bb ? 0.25f : 0.75f, fz = cc ? 0.25f : 0.75f;
+    float acc = 0.f;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int ox = j & 1, oy = (j >> 1) & 1, oz = (j >> 2) & 1;
+        const float wx = ox ? fx : 1.f - fx, wy = oy ? fy : 1.f - fy, wz = oz ? fz : 1.f - fz;
+        acc += c[cc + oz + sz][bb + oy + sy][a + ox + sx] * wx * wy * wz;
+    }
+    return acc;
+}
+
+// Prolongation fused with the first post-smoothing pass (colour 0), at any omega.  One thread per
+// coarse cell forms its 8 children's prolonged values p + P(pc) as k_prolong does, then updates its
+// 4 colour-0 children as k_smooth does after the prolongation: a colour-0 child's 6 neighbours are
+// colour 1, 3 of them siblings (in registers) and 3 children of the adjacent coarse cells, whose
+// prolonged values the thread forms from its own 3x3x3 neighbourhood in k_prolong's order (the
+// coarse cells they interpolate lie within it).  Only colour-0 cells are stored: colour-1 cells are
+// read by neighbouring threads and keep their unprolonged values, which the colour-1 pass that
+// follows prolongs for itself (k_smooth's pc_own).  The values equal those of k_prolong followed by
+// the two colour passes, bit for bit, with one pass over p and one launch less.
+__global__ __launch_bounds__(RB) void k_prolong_red(float* __restrict__ p, const float* __restrict__ b,
+                                                    const float* __restrict__ pcoarse, int nx, int ny, int nz, float h,
+                                                    float omega, const int* __restrict__ skip) {
+    const int sk = skip[fs_opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;    // levels that prolong have even dims
+    const int xc = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int yc = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int zc = blockIdx.z;
+    if (xc >= cx || yc >= cy) return;
+    const int nhc = (cx + 1) >> 1, rsc = 2 * nhc;
+    float c[3][3][3];
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++) {
+        const int qz = zc + dz - 1, sz = min(max(qz, 0), cz - 1);
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++) {
+            const int qy = yc + dy - 1, sy = min(max(qy, 0), cy - 1);
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+                const int qx = xc + dx - 1, sx = min(max(qx, 0), cx - 1);
+                float v = pcoarse[(sz * cy + sy) * rsc + (sx & 1) * nhc + (sx >> 1)];
+                if (qx < 0 || qx >= cx) v = -v;
+                if (qz < 0 || qz >= cz) v = -v;
+                if (qy >= cy) v = -v;
+                c[dz][dy][dx] = v;
+            }
+        }
+    }
+    const int nh = nx >> 1, rs = nx;
+    // fine cell (x, y, z) in the parity-split layout
+    auto fidx = [&](int x, int y, int z) { return (z * ny + y) * rs + (x & 1) * nh + (x >> 1); };
+    float fine[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int a = k & 1, bb = (k >> 1) & 1, cc = (k >> 2) & 1;   // child offsets in x, y, z
+        fine[k] = p[fidx(2 * xc + a, 2 * yc + bb, 2 * zc + cc)] + child_corr(c, 0, 0, 0, a, bb, cc);
+    }
+    // the colour-0 children (a + bb + cc even) and their outer colour-1 neighbours (loaded from an in-bounds
+    // cell, the child itself, where the neighbour is outside; unused then)
+    constexpr int RED[4] = {0, 3, 5, 6};
+    float pn[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int k = RED[r];
+        const int a = k & 1, bb = (k >> 1) & 1, cc = (k >> 2) & 1;
+        const int x = 2 * xc + a, y = 2 * yc + bb, z = 2 * zc + cc;
+        const int own = fidx(x, y, z);
+        // x: the sibling on the inner side, the adjacent coarse cell's child on the outer side
+        const float vw = a ? fine[k - 1] : p[fs_opaque(x > 0 ? fidx(x - 1, y, z) : own)] + child_corr(c, -1, 0, 0, 1, bb, cc);
+        const float ve = a ? p[fs_opaque(x < nx - 1 ? fidx(x + 1, y, z) : own)] + child_corr(c, 1, 0, 0, 0, bb, cc) : fine[k + 1];
+        const float vk = cc ? fine[k - 4] : p[fs_opaque(z > 0 ? fidx(x, y, z - 1) : own)] + child_corr(c, 0, 0, -1, a, bb, 1);
+        const float vf = cc ? p[fs_opaque(z < nz - 1 ? fidx(x, y, z + 1) : own)] + child_corr(c, 0, 0, 1, a, bb, 0) : fine[k + 4];
+        const float vs = bb ? fine[k - 2] : p[fs_opaque(y > 0 ? fidx(x, y - 1, z) : own)] + child_corr(c, 0, -1, 0, a, 1, cc);
+        const float vn = bb ? p[fs_opaque(y < ny - 1 ? fidx(x, y + 1, z) : own)] + child_corr(c, 0, 1, 0, a, 0, cc) : fine[k + 2];
+        const float bv = b[own];
+        const float po = fine[k];
+        float sum = 0.f, diag = 6.f;
+        if (x > 0) sum += vw; else diag += 1.f;
+        if (x < nx - 1) sum += ve; else diag += 1.f;
+        if (z > 0) sum += vk; else diag += 1.f;
+        if (z < nz - 1) sum += vf; else diag += 1.f;
+        if (y > 0) sum += vs; else diag -= 1.f;
+        if (y < ny - 1) sum += vn; else diag += 1.f;
+        float q = (sum - h * h * bv) / diag;
+        if (omega != 1.f) q = po + omega * (q - po);
+        pn[r] = q;
+    }
+    asm volatile("" ::"v"(pn[0]), "v"(pn[1]), "v"(pn[2]), "v"(pn[3]), "v"(sk));
+    if (sk) return;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int k = RED[r];
+        const int a = k & 1, bb = (k >> 1) & 1, cc = (k >> 2) & 1;
+        p[fidx(2 * xc + a, 2 * yc + bb, 2 * zc + cc)] = pn[r];
     }
 }
 
@@ -700,7 +803,7 @@ void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, doub
         for (int color = 0; color < 2; color++) {
             if (k == 0 && color == 0 && first_red_done) continue;
             double* xu = (xupd && k == iters - 1 && color == 1) ? xupd : nullptr;
-            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, skip, xu, om);
+            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, skip, xu, om, nullptr);
         }
 }
 
@@ -775,8 +878,18 @@ bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd, bool red
             const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
             const dim3 g = grid3(nx / 2, ny, nz);
             k_smooth_prolong<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], mg.p[l + 1], nx, ny, nz, mg.h[l], skip);
-            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], 1, skip, nullptr, 1.f);
+            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], 1, skip, nullptr, 1.f, nullptr);
             mg_smooth(mg, s, l, 1, skip, l == 0 ? xupd : nullptr);
+        } else if (mg.fuse_prolong && mg.post >= 1) {
+            // prolongation + the first colour-0 pass from one thread per coarse cell; the first colour-1 pass
+            // prolongs its own cells; then the remaining sweeps (the last adds into the FP64 iterate)
+            const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
+            k_prolong_red<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(mg.p[l], mg.b[l], mg.p[l + 1], nx, ny, nz, mg.h[l],
+                                                                      mg.omega, skip);
+            double* const xu = l == 0 ? xupd : nullptr;
+            k_smooth<<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], 1, skip,
+                                                                mg.post == 1 ? xu : nullptr, mg.omega, mg.p[l + 1]);
+            if (mg.post > 1) mg_smooth(mg, s, l, mg.post - 1, skip, xu);
         } else {
             launch_prolong(mg, s, l, skip);
             mg_smooth(mg, s, l, mg.post, skip, l == 0 ? xupd : nullptr);
