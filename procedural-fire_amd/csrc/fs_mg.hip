@@ -373,7 +373,7 @@ __global__ __launch_bounds__(RB) void k_smooth_prolong(float* __restrict__ p, co
     p[own] = pn;
 }
 
-constexpr int FIN = 1024;   // threads of the one block that sums the residual's partials
+constexpr int FIN = 256;    // threads of the one block that sums the residual's partials
 
 // Sums the per-block partials in a fixed order.  `init`: the starting residual of a
 // solve (phase 0); otherwise the residual after one more V-cycle (the phase count
