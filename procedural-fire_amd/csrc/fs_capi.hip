@@ -152,6 +152,7 @@ struct fs_sim {
         } else if (cfg.solve == FS_SOLVE_REFINE64) {
             if (!begun) mg_refine_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
             for (int k = 0; k < std::min(cfg.max_vcycles, MG_MAX_REFINE); k++) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
+            mg_state_ready(mg, s, cfg.div_tol, cfg.max_vcycles);
         }
         else if (cfg.solve == FS_SOLVE_TOL32) mg_solve_tol32(mg, s, cfg.div_tol, cfg.max_vcycles);
         else for (int i = 0; i < cfg.vcycles; i++) mg_vcycle(mg, s, nullptr);
@@ -285,6 +286,8 @@ int adaptive_step(fs_sim* h, bool prof) {
     int k = 0;
     for (; k < std::min(h->predict, maxc); k++) if ((rc = run_part(h, fs_sim::G_ITER, prof))) return rc;
     while (true) {
+        // the last residual's reduction (the refinement leaves it to the next iteration's first kernel)
+        if (h->cfg.solve == FS_SOLVE_REFINE64) mg_state_ready(h->mg, h->stream, h->cfg.div_tol, maxc);
         FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
         FS_HIP_CHECK(hipStreamSynchronize(h->stream));
         if (h->host_state[1] || k >= maxc) break;
@@ -868,6 +871,7 @@ int fs_residual(fs_sim* h, double* rel, int* used) {
     if (set_device(h)) return FS_EDEVICE;
     const bool tracked = h->cfg.solve != FS_SOLVE_VCYCLES;   // the solve itself recorded its residuals
     if (!tracked) mg_residual_of_p(h->mg, h->stream);
+    else mg_state_ready(h->mg, h->stream, h->cfg.div_tol, h->cfg.max_vcycles);
     FS_HIP_CHECK(hipStreamSynchronize(h->stream));
     RefineState st;
     FS_HIP_CHECK(hipMemcpy(&st, h->mg.st, sizeof(st), hipMemcpyDeviceToHost));
@@ -1211,6 +1215,7 @@ int fs_poisson_solve(fs_poisson* ps, double tol, int max_cycles, double* hist, i
     mg_refine_begin(ps->mg, ps->stream, tol, max_cycles);
     for (int k = 0; k <= max_cycles; k++) {
         int used_done[2];
+        mg_state_ready(ps->mg, ps->stream, tol, max_cycles);
         FS_HIP_CHECK(hipMemcpyAsync(used_done, &ps->mg.st->used, sizeof(used_done), hipMemcpyDeviceToHost, ps->stream));
         FS_HIP_CHECK(hipStreamSynchronize(ps->stream));
         if (used_done[1] || k == max_cycles) break;
@@ -1362,6 +1367,7 @@ int refine_solve(fs_sim* h, double tol, int maxc, int* predict) {
     int k = 0;
     for (; k < std::min(*predict, maxc); k++) mg_refine_iter(h->mg, s, tol, maxc);
     while (true) {
+        mg_state_ready(h->mg, s, tol, maxc);
         FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         FS_HIP_CHECK(hipStreamSynchronize(s));
         if (h->host_state[1] || k >= maxc) break;

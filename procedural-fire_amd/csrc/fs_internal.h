@@ -69,8 +69,17 @@ struct RefineState {            // device-resident FP64 refinement bookkeeping
     double bnorm2;
     int used;                   // V-cycles applied in the current solve
     int done;                   // 1 once rel <= tol or used == max
-    unsigned counter;           // blocks of the running residual kernel that have finished
+    int pending;                // partials not yet reduced: 0 none, 1 a residual after a V-cycle, 2 a starting residual
     int never;                  // always 0: the skip flag of launches that run unconditionally
+};
+
+// A pending residual reduction handed to the first smoothing pass of the next V-cycle, whose
+// block (0, 0, 0) runs it before its own cells (the pass itself then cannot wait on the stop flag)
+struct FinArgs {
+    RefineState* st;
+    const double* partials;
+    int npart, max_cycles;
+    double tol;
 };
 
 struct GpuMG {
@@ -128,7 +137,9 @@ void mg_free(GpuMG& mg);
 // possible; returns true if it was.
 // With `red0_done`, level 0's first colour-0 pass has already been written (level 0 started
 // from p = 0 and the kernel that zeroed it did that pass).
-bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd = nullptr, bool red0_done = false);
+// With `fin`, the cycle's first kernel also runs that reduction.
+bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd = nullptr, bool red0_done = false,
+               const FinArgs* fin = nullptr);
 // FP64 refinement: r = bkeep - A x; sets level-0 b = r, p = 0; reduces norms;
 // then up to max_cycles of {V-cycle, x += p, residual}; all device-side.
 void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
@@ -136,8 +147,13 @@ void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 void mg_solve_tol32(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 // the same two solves split into their starting residual and one (skippable) iteration,
 // for callers that decide on the host how many iterations to enqueue
+// mg_refine_iter leaves its residual's reduction to the first kernel of the next iteration (or to
+// mg_state_ready): call mg_state_ready before the host reads the state
 void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
+// reduces a residual whose partials are still pending, with the solve's stopping test (a one-block
+// kernel that exits at once when none is)
+void mg_state_ready(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 void mg_tol32_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 void mg_tol32_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 // relative FP64 residual of the FP32 level-0 p against level-0 b -> st->rel[0]
@@ -156,7 +172,7 @@ void mg_spectral_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 int  mg_spectral_iter(GpuMG& mg, Spectral* sp, hipStream_t s, double tol, int max_cycles);
 // single-operator launches (tests)
 void mg_smooth(GpuMG& mg, hipStream_t s, int level, int iters, const int* skip, double* xupd = nullptr,
-               bool first_red_done = false);
+               bool first_red_done = false, const FinArgs* fin = nullptr);
 
 // ---- error handling ----------------------------------------------------------
 void fs_set_error(const std::string& msg);

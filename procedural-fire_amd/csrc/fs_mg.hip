@@ -62,14 +62,62 @@ __device__ __forceinline__ float prolong_at(const float* __restrict__ pc, int x,
     }
     return acc;
 }
+constexpr int FIN = RB;     // threads of the one block that sums the residual's partials
+
+// Sums the per-block partials of the last residual kernel in a fixed order, once.  st->pending
+// says whether a residual awaits its reduction and which: a solve's starting residual (phase 0),
+// or the residual after one more V-cycle (the phase count lives in the device state, so one
+// captured iteration can be replayed any number of times).  Sets st->done once rel <= tol or the
+// phase reaches max_cycles.  Every thread reads st->pending before the barrier that precedes its
+// only write, so the block takes one branch.
+__device__ void finalize_sums(RefineState* st, const double* partials, int npart, double tol, int max_cycles) {
+    __shared__ double sr[FIN / 64], sb[FIN / 64];
+    double rr = 0.0, bb = 0.0;
+    const int pend = st->pending, done = st->done;     // in flight with the partials' loads
+    // each thread sums partials j, j + FIN, j + 2 FIN, ... in that order, loading 8 at a time
+    int j = threadIdx.x;
+    for (; j + 7 * FIN < npart; j += 8 * FIN) {
+        double a[8], c[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) { a[u] = partials[2 * (j + u * FIN)]; c[u] = partials[2 * (j + u * FIN) + 1]; }
+#pragma unroll
+        for (int u = 0; u < 8; u++) { rr += a[u]; bb += c[u]; }
+    }
+    for (; j < npart; j += FIN) { rr += partials[2 * j]; bb += partials[2 * j + 1]; }
+    asm volatile("" ::"v"(rr), "v"(bb), "s"(pend), "s"(done));
+    if (pend == 0) return;                             // uniform over the block
+    const int init = pend == 2;
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    if ((threadIdx.x & 63) == 0) { sr[threadIdx.x >> 6] = rr; sb[threadIdx.x >> 6] = bb; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->pending = 0;
+        if (!init && done) return;                     // (a residual is not formed once the solve has stopped)
+        double r2 = 0.0, b2 = 0.0;
+        for (int w = 0; w < FIN / 64; w++) { r2 += sr[w]; b2 += sb[w]; }
+        const double rel = b2 > 0.0 ? sqrt(r2 / b2) : 0.0;
+        const int phase = init ? 0 : st->used + 1;     // V-cycles applied so far in this solve
+        st->rel[phase] = rel;
+        st->used = phase;
+        st->bnorm2 = b2;
+        st->done = (rel <= tol || phase >= max_cycles) ? 1 : 0;
+    }
+}
+
 // one colour pass of red-black GS (sim.js:422-437).  Thread (hx, y, z) owns the cell
 // x = 2hx + s of this colour, s = (y + z + color) & 1.  With `xupd` set (the last pass
 // of a correction V-cycle) it also adds the final p of both cells x = 2hx, 2hx+1 to
 // the FP64 iterate.  omega != 1 over-relaxes: p <- p + omega (p_GS - p) (the
-// tolerance-driven solves; the reference's solve is omega = 1 exactly).
+// tolerance-driven solves; the reference's solve is omega = 1 exactly).  WithFin: block
+// (0, 0, 0) first runs the pending reduction of the previous residual kernel (the pass is
+// then launched unconditionally: the stop flag it would test is the one being formed).
+template <bool WithFin>
 __global__ __launch_bounds__(RB) void k_smooth(float* __restrict__ p, const float* __restrict__ b, int nx, int ny, int nz,
                                                float h, int color, const int* __restrict__ skip, double* __restrict__ xupd,
-                                               float omega, const float* __restrict__ pc_own) {
+                                               float omega, const float* __restrict__ pc_own, FinArgs fin) {
+    if (WithFin && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0)
+        finalize_sums(fin.st, fin.partials, fin.npart, fin.tol, fin.max_cycles);
     const int sk = skip[fs_opaque(0)];   // in flight with the data loads (a vector load: its wait does not hold up the
                                          // scalar kernel arguments); tested before any store
     const int nh = (nx + 1) >> 1, rs = 2 * nh;
@@ -373,53 +421,16 @@ __global__ __launch_bounds__(RB) void k_smooth_prolong(float* __restrict__ p, co
     p[own] = pn;
 }
 
-constexpr int FIN = 256;    // threads of the one block that sums the residual's partials
-
-// Sums the per-block partials in a fixed order.  `init`: the starting residual of a
-// solve (phase 0); otherwise the residual after one more V-cycle (the phase count
-// lives in the device state, so one captured iteration can be replayed any number
-// of times).  Sets st->done once rel <= tol or the phase reaches max_cycles.
-__device__ void finalize_sums(RefineState* st, const double* partials, int npart, double tol, int max_cycles, int init) {
-    __shared__ double sr[FIN / 64], sb[FIN / 64];
-    double rr = 0.0, bb = 0.0;
-    const int done = init ? 0 : st->done;              // in flight with the partials' loads
-    // each thread sums partials j, j + FIN, j + 2 FIN, ... in that order, loading 8 at a time
-    int j = threadIdx.x;
-    for (; j + 7 * FIN < npart; j += 8 * FIN) {
-        double a[8], c[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) { a[u] = partials[2 * (j + u * FIN)]; c[u] = partials[2 * (j + u * FIN) + 1]; }
-#pragma unroll
-        for (int u = 0; u < 8; u++) { rr += a[u]; bb += c[u]; }
-    }
-    for (; j < npart; j += FIN) { rr += partials[2 * j]; bb += partials[2 * j + 1]; }
-    asm volatile("" ::"v"(rr), "v"(bb), "s"(done));
-    if (done) return;                                  // uniform over the block
-    rr = wave_sum(rr);
-    bb = wave_sum(bb);
-    if ((threadIdx.x & 63) == 0) { sr[threadIdx.x >> 6] = rr; sb[threadIdx.x >> 6] = bb; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r2 = 0.0, b2 = 0.0;
-        for (int w = 0; w < FIN / 64; w++) { r2 += sr[w]; b2 += sb[w]; }
-        sr[0] = r2; sb[0] = b2;
-        const double rel = sb[0] > 0.0 ? sqrt(sr[0] / sb[0]) : 0.0;
-        const int phase = init ? 0 : st->used + 1;   // V-cycles applied so far in this solve
-        st->rel[phase] = rel;
-        st->used = phase;
-        st->bnorm2 = sb[0];
-        st->done = (rel <= tol || phase >= max_cycles) ? 1 : 0;
-    }
-}
-
 // FP64 residual over the level-0 grid.  Each block writes its (r.r, b.b) partial sums
-// (k_finalize reduces them in index order).  With `rhs_out` set it also stores r (f32)
-// as the V-cycle right-hand side and zeroes the V-cycle iterate.
+// (finalize_sums reduces them in index order, later); block 0 marks them pending (`init`:
+// a solve's starting residual).  With `rhs_out` set it also stores r (f32) as the V-cycle
+// right-hand side and zeroes the V-cycle iterate.
 template <class X>
 __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const float* __restrict__ b, int nx, int ny, int nz,
                                                 double h, float* __restrict__ rhs_out, float* __restrict__ p_out,
-                                                double* __restrict__ partials, const int* __restrict__ skip,
-                                                float hred = 0.f, float omega = 1.f, int red_first = 0) {
+                                                double* __restrict__ partials, RefineState* __restrict__ st, int init,
+                                                const int* __restrict__ skip, float hred = 0.f, float omega = 1.f,
+                                                int red_first = 0) {
     __shared__ double sr[RB / 64], sb[RB / 64];
     const int nh = (nx + 1) >> 1, rs = 2 * nh;
     const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -465,13 +476,14 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
         const int blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         partials[2 * blk] = ((sr[0] + sr[1]) + sr[2]) + sr[3];
         partials[2 * blk + 1] = ((sb[0] + sb[1]) + sb[2]) + sb[3];
+        if (blk == 0) st->pending = init ? 2 : 1;
     }
 }
 
 // one block: the fixed-order reduction of the partials and the stopping test
 __global__ __launch_bounds__(FIN) void k_finalize(RefineState* __restrict__ st, const double* __restrict__ partials, int npart,
-                                                 double tol, int max_cycles, int init) {
-    finalize_sums(st, partials, npart, tol, max_cycles, init);
+                                                 double tol, int max_cycles) {
+    finalize_sums(st, partials, npart, tol, max_cycles);
 }
 
 // x += e (the V-cycle's correction), FP64, over the whole parity-split array
@@ -794,7 +806,9 @@ int mg_set_direct(GpuMG& mg, int want, std::string& err) {
     return FS_OK;
 }
 
-void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, double* xupd, bool first_red_done) {   // sim.js:576-586
+// `fin`: the first pass also runs that pending reduction (and is not skippable)
+void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, double* xupd, bool first_red_done,
+               const FinArgs* fin) {   // sim.js:576-586
     if (!skip) skip = &mg.st->never;
     const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
     const dim3 g = grid3((nx + 1) / 2, ny, nz);
@@ -803,8 +817,16 @@ void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, doub
         for (int color = 0; color < 2; color++) {
             if (k == 0 && color == 0 && first_red_done) continue;
             double* xu = (xupd && k == iters - 1 && color == 1) ? xupd : nullptr;
-            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, skip, xu, om, nullptr);
+            if (fin) {
+                k_smooth<true><<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, &mg.st->never, xu, om, nullptr,
+                                                *fin);
+                fin = nullptr;
+            } else {
+                k_smooth<false><<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, skip, xu, om, nullptr,
+                                                 FinArgs{});
+            }
         }
+    if (fin) k_finalize<<<1, FIN, 0, s>>>(fin->st, fin->partials, fin->npart, fin->tol, fin->max_cycles);
 }
 
 static void launch_restrict(GpuMG& mg, hipStream_t s, int l, const int* skip, bool red_first) {
@@ -851,19 +873,24 @@ static bool fuse_red_into(const GpuMG& mg, int lc, int T) {
     return lc <= T;                                    // a per-level pass, or the tail's top level
 }
 
-bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd, bool red0_done) {   // sim.js:588-599
+bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd, bool red0_done,
+               const FinArgs* fin) {   // sim.js:588-599
     if (!skip) skip = &mg.st->never;
     const int n = mg.nlev;
     const int D = mg.direct;                            // > 0: levels below D are replaced by its exact solve
     const int T = D > 0 ? n : mg.use_tail ? mg.tail_from : n;   // levels T.. run in the single-workgroup tail
+    const int last = D > 0 ? D : std::min(T, n - 1);    // levels 0..last-1 smooth and restrict
+    if (fin && (T == 0 || last == 0 || mg.pre < 1)) {   // the cycle does not open with a level-0 pass: a kernel of its own
+        k_finalize<<<1, FIN, 0, s>>>(fin->st, fin->partials, fin->npart, fin->tol, fin->max_cycles);
+        fin = nullptr;
+    }
     if (T == 0) {                                       // the whole hierarchy fits one workgroup's LDS
         launch_tail(mg, s, skip, false);
         return false;
     }
-    const int last = D > 0 ? D : std::min(T, n - 1);    // levels 0..last-1 smooth and restrict
     bool red_done = red0_done;
     for (int l = 0; l < last; l++) {
-        mg_smooth(mg, s, l, mg.pre, skip, nullptr, red_done);
+        mg_smooth(mg, s, l, mg.pre, skip, nullptr, red_done, l == 0 ? fin : nullptr);
         red_done = fuse_red_into(mg, l + 1, T);
         launch_restrict(mg, s, l, skip, red_done);
     }
@@ -878,7 +905,7 @@ bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd, bool red
             const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
             const dim3 g = grid3(nx / 2, ny, nz);
             k_smooth_prolong<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], mg.p[l + 1], nx, ny, nz, mg.h[l], skip);
-            k_smooth<<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], 1, skip, nullptr, 1.f, nullptr);
+            k_smooth<false><<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], 1, skip, nullptr, 1.f, nullptr, FinArgs{});
             mg_smooth(mg, s, l, 1, skip, l == 0 ? xupd : nullptr);
         } else if (mg.fuse_prolong && mg.post >= 1) {
             // prolongation + the first colour-0 pass from one thread per coarse cell; the first colour-1 pass
@@ -887,8 +914,9 @@ bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd, bool red
             k_prolong_red<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(mg.p[l], mg.b[l], mg.p[l + 1], nx, ny, nz, mg.h[l],
                                                                       mg.omega, skip);
             double* const xu = l == 0 ? xupd : nullptr;
-            k_smooth<<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], 1, skip,
-                                                                mg.post == 1 ? xu : nullptr, mg.omega, mg.p[l + 1]);
+            k_smooth<false><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], 1, skip,
+                                                                       mg.post == 1 ? xu : nullptr, mg.omega, mg.p[l + 1],
+                                                                       FinArgs{});
             if (mg.post > 1) mg_smooth(mg, s, l, mg.post - 1, skip, xu);
         } else {
             launch_prolong(mg, s, l, skip);
@@ -905,13 +933,16 @@ static bool fuse_red0(const GpuMG& mg) {
     return mg.pre >= 1 && mg.nlev > 1 && T >= 1;
 }
 
+// The reduction of each residual runs in block 0 of the next V-cycle's first kernel, which is
+// therefore launched unconditionally: once the solve has stopped it still smooths level 0's
+// V-cycle iterate p (a scratch array the refinement rewrites before its next use) and never
+// touches the FP64 iterate; every later kernel of the iteration tests the stop flag as before.
 void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     const bool red = fuse_red0(mg);
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
-                                                                  mg.partials, &mg.st->never, mg.h[0], mg.omega, red ? 1 : 0);
-    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
+                                                                  mg.partials, mg.st, 1, &mg.st->never, mg.h[0], mg.omega, red ? 1 : 0);
 }
 
 void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
@@ -919,19 +950,25 @@ void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     int* done = &mg.st->done;
     const bool red = fuse_red0(mg);
-    if (!mg_vcycle(mg, s, done, mg.x64, red)) {
+    const FinArgs fin{mg.st, mg.partials, mg.npart, max_cycles, tol};
+    if (!mg_vcycle(mg, s, done, mg.x64, red, &fin)) {
         const int n = (int)level_floats(mg.dims[0]);
         k_update<<<(n + RB - 1) / RB, RB, 0, s>>>(mg.x64, mg.p[0], n, done);
     }
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
-                                                                  mg.partials, done, mg.h[0], mg.omega, red ? 1 : 0);
-    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
+                                                                  mg.partials, mg.st, 0, done, mg.h[0], mg.omega, red ? 1 : 0);
+}
+
+void mg_state_ready(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
+    if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles);
 }
 
 void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     mg_refine_begin(mg, s, tol, max_cycles);
     for (int k = 1; k <= max_cycles; k++) mg_refine_iter(mg, s, tol, max_cycles);
+    mg_state_ready(mg, s, tol, max_cycles);
 }
 
 // FP64 defect correction x <- x + S(b - A x) with the exact separable inner solve S (fs_spectral.hip)
@@ -942,8 +979,8 @@ void mg_spectral_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], nullptr,
-                                                                  mg.partials, &mg.st->never);
-    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
+                                                                  mg.partials, mg.st, 1, &mg.st->never);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles);
 }
 
 int mg_spectral_iter(GpuMG& mg, Spectral* sp, hipStream_t s, double tol, int max_cycles) {
@@ -954,8 +991,8 @@ int mg_spectral_iter(GpuMG& mg, Spectral* sp, hipStream_t s, double tol, int max
     const int n = (int)level_floats(mg.dims[0]);
     k_update<<<(n + RB - 1) / RB, RB, 0, s>>>(mg.x64, mg.p[0], n, done);
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], nullptr,
-                                                                  mg.partials, done);
-    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
+                                                                  mg.partials, mg.st, 0, done);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles);
     return FS_OK;
 }
 
@@ -963,8 +1000,8 @@ void mg_tol32_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     k_resid64<float><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr,
-                                                                 mg.partials, &mg.st->never);
-    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 1);
+                                                                 mg.partials, mg.st, 1, &mg.st->never);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles);
 }
 
 void mg_tol32_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
@@ -973,8 +1010,8 @@ void mg_tol32_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     int* done = &mg.st->done;
     mg_vcycle(mg, s, done);
     k_resid64<float><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr,
-                                                                 mg.partials, done);
-    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, 0);
+                                                                 mg.partials, mg.st, 0, done);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles);
 }
 
 void mg_solve_tol32(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
@@ -986,6 +1023,6 @@ void mg_solve_tol32(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
 void mg_residual_of_p(GpuMG& mg, hipStream_t s) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     k_resid64<float><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[0], mg.b[0], nx, ny, nz, (double)mg.h[0], nullptr, nullptr,
-                                                                 mg.partials, &mg.st->never);
-    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, 0.0, 0, 1);
+                                                                 mg.partials, mg.st, 1, &mg.st->never);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, 0.0, 0);
 }
