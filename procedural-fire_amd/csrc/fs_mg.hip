@@ -220,6 +220,37 @@ __global__ __launch_bounds__(RB) void k_restrict(const float* __restrict__ p, co
     pc[ci] = (red_first && ((xc + yc + zc) & 1) == 0) ? red_from_zero(bnew, xc, yc, zc, cx, cy, nz >> 1, hc, omega) : 0.f;
 }
 
+// The mirrored trilinear weights are 0.25 or 0.75, so a term ((v wx) wy) wz rounds only at its factors
+// 0.75 (a factor 0.25 scales exactly, and exact scaling commutes with rounding): it equals m_n(v) 4^(n-3),
+// n the number of factors 0.75, m_0 = v, m_k = m_(k-1) 0.75 rounded; and acc + m_n(v) 4^(n-3) is one FMA
+// with the same rounding as the sum.  With m_1..m_3 formed once per coarse value (m[k] below), each term
+// costs one FMA instead of three products and a sum, bit for bit (for |v| >= 2^-120, where the scaling
+// by 4^-3 is exact; the multigrid's iterates are far from that).
+__device__ __forceinline__ void chain75(const float (&c)[3][3][3], float (&m)[4][3][3][3]) {
+#pragma unroll
+    for (int i = 0; i < 27; i++) {
+        const float v = (&c[0][0][0])[i];
+        const float m1 = v * 0.75f, m2 = m1 * 0.75f;
+        (&m[0][0][0][0])[i] = v;
+        (&m[1][0][0][0])[i] = m1;
+        (&m[2][0][0][0])[i] = m2;
+        (&m[3][0][0][0])[i] = m2 * 0.75f;
+    }
+}
+// the prolongation correction of child (a, bb, cc) of the coarse cell offset (sx, sy, sz) from the
+// window's centre (k_prolong's sum, term for term)
+__device__ __forceinline__ float child_corr(const float (&m)[4][3][3][3], int sx, int sy, int sz, int a, int bb, int cc) {
+    float acc = 0.f;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int ox = j & 1, oy = (j >> 1) & 1, oz = (j >> 2) & 1;
+        const int n = (ox != a) + (oy != bb) + (oz != cc);   // factors 0.75: the nearer coarse cell on an axis
+        const float sc = n == 3 ? 1.f : n == 2 ? 0.25f : n == 1 ? 0.0625f : 0.015625f;
+        acc = __builtin_fmaf(m[n][cc + oz + sz][bb + oy + sy][a + ox + sx], sc, acc);
+    }
+    return acc;
+}
+
 // One thread per coarse cell: its 3x3x3 coarse neighbourhood (mirror rule applied)
 // gives the corrections of its 8 children, each summed in prolong_at's order (the
 // child's two coarse cells per axis sit at offsets {-1, 0} or {0, +1}, weights 1/4
@@ -253,19 +284,14 @@ __global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const flo
         }
     }
     const int nh = nx >> 1, rs = nx;
+    float m[4][3][3][3];
+    chain75(c, m);
     float fine[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int a = k & 1, bb = (k >> 1) & 1, cc = (k >> 2) & 1;   // child offsets in x, y, z
-        const float fx = a ? 0.25f : 0.75f, fy = bb ? 0.25f : 0.75f, fz = cc ? 0.25f : 0.75f;
-        float acc = 0.f;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int ox = j & 1, oy = (j >> 1) & 1, oz = (j >> 2) & 1;
-            const float wx = ox ? fx : 1.f - fx, wy = oy ? fy : 1.f - fy, wz = oz ? fz : 1.f - fz;
-            acc += c[cc + oz][bb + oy][a + ox] * wx * wy * wz;
-        }
-        fine[k] = p[((2 * zc + cc) * ny + 2 * yc + bb) * rs + a * nh + xc] + acc;
+        // sum_j c * wx * wy * wz over the 8 coarse cells, wx = 0.75 on the nearer side (0.25 on the other)
+        fine[k] = p[((2 * zc + cc) * ny + 2 * yc + bb) * rs + a * nh + xc] + child_corr(m, 0, 0, 0, a, bb, cc);
     }
     asm volatile("" ::"v"(fine[0]), "v"(fine[1]), "v"(fine[2]), "v"(fine[3]), "v"(fine[4]), "v"(fine[5]), "v"(fine[6]),
                  "v"(fine[7]), "v"(sk));
@@ -280,18 +306,6 @@ __global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const flo
 // The prolongation correction of the child with offsets (a, bb, cc) of the coarse cell at offset (sx, sy, sz)
 // (each -1, 0 or +1) from the thread's own, from the thread's 3x3x3 mirrored neighbourhood c: k_prolong's sum
 // for that child, term for term (the parent's neighbourhood is c shifted by the offset).
-__device__ __forceinline__ float child_corr(const float (&c)[3][3][3], int sx, int sy, int sz, int a, int bb, int cc) {
-    const float fx = a ? 0.25f : 0.75f, fy = bb ? 0.25f : 0.75f, fz = cc ? 0.25f : 0.75f;
-    float acc = 0.f;
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const int ox = j & 1, oy = (j >> 1) & 1, oz = (j >> 2) & 1;
-        const float wx = ox ? fx : 1.f - fx, wy = oy ? fy : 1.f - fy, wz = oz ? fz : 1.f - fz;
-        acc += c[cc + oz + sz][bb + oy + sy][a + ox + sx] * wx * wy * wz;
-    }
-    return acc;
-}
-
 // Prolongation fused with the first post-smoothing pass (colour 0), at any omega.  One thread per
 // coarse cell forms its 8 children's prolonged values p + P(pc) as k_prolong does, then updates its
 // 4 colour-0 children as k_smooth does after the prolongation: a colour-0 child's 6 neighbours are
@@ -332,11 +346,13 @@ __global__ __launch_bounds__(RB) void k_prolong_red(float* __restrict__ p, const
     const int nh = nx >> 1, rs = nx;
     // fine cell (x, y, z) in the parity-split layout
     auto fidx = [&](int x, int y, int z) { return (z * ny + y) * rs + (x & 1) * nh + (x >> 1); };
+    float m[4][3][3][3];
+    chain75(c, m);
     float fine[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int a = k & 1, bb = (k >> 1) & 1, cc = (k >> 2) & 1;   // child offsets in x, y, z
-        fine[k] = p[fidx(2 * xc + a, 2 * yc + bb, 2 * zc + cc)] + child_corr(c, 0, 0, 0, a, bb, cc);
+        fine[k] = p[fidx(2 * xc + a, 2 * yc + bb, 2 * zc + cc)] + child_corr(m, 0, 0, 0, a, bb, cc);
     }
     // the colour-0 children (a + bb + cc even) and their outer colour-1 neighbours (loaded from an in-bounds
     // cell, the child itself, where the neighbour is outside; unused then)
@@ -349,12 +365,12 @@ __global__ __launch_bounds__(RB) void k_prolong_red(float* __restrict__ p, const
         const int x = 2 * xc + a, y = 2 * yc + bb, z = 2 * zc + cc;
         const int own = fidx(x, y, z);
         // x: the sibling on the inner side, the adjacent coarse cell's child on the outer side
-        const float vw = a ? fine[k - 1] : p[fs_opaque(x > 0 ? fidx(x - 1, y, z) : own)] + child_corr(c, -1, 0, 0, 1, bb, cc);
-        const float ve = a ? p[fs_opaque(x < nx - 1 ? fidx(x + 1, y, z) : own)] + child_corr(c, 1, 0, 0, 0, bb, cc) : fine[k + 1];
-        const float vk = cc ? fine[k - 4] : p[fs_opaque(z > 0 ? fidx(x, y, z - 1) : own)] + child_corr(c, 0, 0, -1, a, bb, 1);
-        const float vf = cc ? p[fs_opaque(z < nz - 1 ? fidx(x, y, z + 1) : own)] + child_corr(c, 0, 0, 1, a, bb, 0) : fine[k + 4];
-        const float vs = bb ? fine[k - 2] : p[fs_opaque(y > 0 ? fidx(x, y - 1, z) : own)] + child_corr(c, 0, -1, 0, a, 1, cc);
-        const float vn = bb ? p[fs_opaque(y < ny - 1 ? fidx(x, y + 1, z) : own)] + child_corr(c, 0, 1, 0, a, 0, cc) : fine[k + 2];
+        const float vw = a ? fine[k - 1] : p[fs_opaque(x > 0 ? fidx(x - 1, y, z) : own)] + child_corr(m, -1, 0, 0, 1, bb, cc);
+        const float ve = a ? p[fs_opaque(x < nx - 1 ? fidx(x + 1, y, z) : own)] + child_corr(m, 1, 0, 0, 0, bb, cc) : fine[k + 1];
+        const float vk = cc ? fine[k - 4] : p[fs_opaque(z > 0 ? fidx(x, y, z - 1) : own)] + child_corr(m, 0, 0, -1, a, bb, 1);
+        const float vf = cc ? p[fs_opaque(z < nz - 1 ? fidx(x, y, z + 1) : own)] + child_corr(m, 0, 0, 1, a, bb, 0) : fine[k + 4];
+        const float vs = bb ? fine[k - 2] : p[fs_opaque(y > 0 ? fidx(x, y - 1, z) : own)] + child_corr(m, 0, -1, 0, a, 1, cc);
+        const float vn = bb ? p[fs_opaque(y < ny - 1 ? fidx(x, y + 1, z) : own)] + child_corr(m, 0, 1, 0, a, 0, cc) : fine[k + 2];
         const float bv = b[own];
         const float po = fine[k];
         float sum = 0.f, diag = 6.f;
