@@ -192,6 +192,27 @@ def test_adaptive_and_async_solves_agree(solve):
             assert np.array_equal(a, s.read(f, "native")), f
 
 
+@pytest.mark.parametrize("res", [(64, 72, 80), (96, 88, 48)], ids=["32x36x40", "48x44x24"])
+def test_tol32_exact_coarse_partial_tiles(res):
+    """The exact coarse solve on levels whose y and z are not multiples of 16 (32x36x40, 48x44x24): the
+    matrix-core products then run padded 16 x 16 tiles and 8-column inverse blocks.  Same cycle counts and
+    bits as the oracle's FMA chains.  direct_level=1 is refused unless the kernels take the level."""
+    print(f"tol32 exact coarse level on {res}", flush=True)
+    dx = TEST_SIZE[0] / TEST_RES[0]
+    size = tuple(n * dx for n in res)
+    gpu, orc = make_pair(F.PREC_FP32, res=res, size=size, warm_steps=10, solve=F.SOLVE_TOL32, div_tol=1e-4,
+                         max_vcycles=8, sweeps=(1, 2), direct_level=1)
+    for _ in range(2):
+        gpu.step(DT, 1)
+        orc.step(DT, 1)
+        rel, used = gpu.residual()
+        orel, oused = orc.residual()
+        print(f"  gpu rel {rel:.3e} used {used}; oracle rel {orel:.3e} used {oused}", flush=True)
+        assert used == oused and abs(rel - orel) <= 1e-9 * max(orel, 1e-30) + 1e-15
+    for f in (F.VEL, F.S, F.S2, F.P):
+        assert rel_l2(gpu.read(f, np.float64), orc.read(f)) == 0.0, f
+
+
 def test_direct_level_choice_and_refusal():
     """Automatic choice: level 1 of 64x96x64 (32x48x32); an explicit level that the kernels do not take
     is refused at create; the plain V-cycle and the exact coarse level reach the same tolerance."""
