@@ -612,10 +612,11 @@ struct Oracle : OracleBase {
         R flx = std::floor(tx), fly = std::floor(ty), flz = std::floor(tz);
         int ix = (int)flx, iy = (int)fly, iz = (int)flz;
         R fx = tx - flx, fy = ty - fly, fz = tz - flz;
-        V4<R> c000 = ld(f, ix, iy, iz), c100 = ld(f, ix + 1, iy, iz), c010 = ld(f, ix, iy + 1, iz), c110 = ld(f, ix + 1, iy + 1, iz);
-        V4<R> c001 = ld(f, ix, iy, iz + 1), c101 = ld(f, ix + 1, iy, iz + 1), c011 = ld(f, ix, iy + 1, iz + 1), c111 = ld(f, ix + 1, iy + 1, iz + 1);
-        V4<R> x00 = mix4(c000, c100, fx), x10 = mix4(c010, c110, fx), x01 = mix4(c001, c101, fx), x11 = mix4(c011, c111, fx);
-        return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+        V4<R> c[8];
+        for (int k = 0; k < 8; k++) c[k] = ld(f, ix + (k & 1), iy + ((k >> 1) & 1), iz + (k >> 2));
+        R w[8];
+        tri_weights<R>(fx, fy, fz, w);
+        return tri_blend<R>(c, w);
     }
     // repeat-addressed sample of a noise texture at normalised coordinate q
     V4<R> sample_rep(const NoiseTex& t, V3<R> q) const {
@@ -629,9 +630,11 @@ struct Oracle : OracleBase {
             const float* s = &t.v[(((size_t)z * n + y) * n + x) * 4];
             return V4<R>((R)s[0], (R)s[1], (R)s[2], (R)s[3]);
         };
-        V4<R> x00 = mix4(L(ix, iy, iz), L(ix + 1, iy, iz), fx), x10 = mix4(L(ix, iy + 1, iz), L(ix + 1, iy + 1, iz), fx);
-        V4<R> x01 = mix4(L(ix, iy, iz + 1), L(ix + 1, iy, iz + 1), fx), x11 = mix4(L(ix, iy + 1, iz + 1), L(ix + 1, iy + 1, iz + 1), fx);
-        return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+        V4<R> c[8];
+        for (int k = 0; k < 8; k++) c[k] = L(ix + (k & 1), iy + ((k >> 1) & 1), iz + (k >> 2));
+        R w[8];
+        tri_weights<R>(fx, fy, fz, w);
+        return tri_blend<R>(c, w);
     }
 
     // ---- geo (sim.js:498-523) ----

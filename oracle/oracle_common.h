@@ -201,6 +201,28 @@ template <class R> struct V4 {
 template <class R> static inline V4<R> mix4(const V4<R>& a, const V4<R>& b, R t) {
     return V4<R>(mix(a.x, b.x, t), mix(a.y, b.y, t), mix(a.z, b.z, t), mix(a.w, b.w, t));
 }
+// Trilinear filter of the 2x2x2 texels c[k] at fractions (fx, fy, fz), corner k = (k & 1, (k >> 1) & 1, k >> 2):
+// weight w_k = wx_a * (wy_b * wz_c) (wx_0 = 1 - fx, wx_1 = fx, ...), value = c_0 w_0, then fma(c_k, w_k, .) for
+// k = 1..7.  The arithmetic of the texture sampler is the implementation's (src/sim.js:129, 154-156, 176-178,
+// 228, 365 call textureSampleLevel); this fixes one, and the HIP kernels round identically.
+template <class R> static inline void tri_weights(R fx, R fy, R fz, R* w) {
+    const R ux = R(1) - fx, uy = R(1) - fy, uz = R(1) - fz;
+    const R yz[4] = {uy * uz, fy * uz, uy * fz, fy * fz};
+    for (int j = 0; j < 4; j++) {
+        w[2 * j] = ux * yz[j];
+        w[2 * j + 1] = fx * yz[j];
+    }
+}
+template <class R> static inline V4<R> tri_blend(const V4<R>* c, const R* w) {
+    V4<R> r(c[0].x * w[0], c[0].y * w[0], c[0].z * w[0], c[0].w * w[0]);
+    for (int k = 1; k < 8; k++) {
+        r.x = std::fma(c[k].x, w[k], r.x);
+        r.y = std::fma(c[k].y, w[k], r.y);
+        r.z = std::fma(c[k].z, w[k], r.z);
+        r.w = std::fma(c[k].w, w[k], r.w);
+    }
+    return r;
+}
 
 // ---- hashing: pcg (src/shaders/lib.js:11-17) --------------------------------
 static inline uint32_t pcg1(uint32_t n) {

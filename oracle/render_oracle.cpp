@@ -53,10 +53,11 @@ struct Grid {
         const float flx = std::floor(tx), fly = std::floor(ty), flz = std::floor(tz);
         const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
         const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
-        const F4 x00 = mix4(ld(ix, iy, iz), ld(ix + 1, iy, iz), fx), x10 = mix4(ld(ix, iy + 1, iz), ld(ix + 1, iy + 1, iz), fx);
-        const F4 x01 = mix4(ld(ix, iy, iz + 1), ld(ix + 1, iy, iz + 1), fx);
-        const F4 x11 = mix4(ld(ix, iy + 1, iz + 1), ld(ix + 1, iy + 1, iz + 1), fx);
-        return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+        F4 c[8];
+        for (int k = 0; k < 8; k++) c[k] = ld(ix + (k & 1), iy + ((k >> 1) & 1), iz + (k >> 2));
+        float w[8];
+        tri_weights<float>(fx, fy, fz, w);
+        return tri_blend<float>(c, w);
     }
     // cubic B-spline from 8 trilinear fetches (Sigg & Hadwiger 2005), src/renderer.js:215-233
     F4 cubic(F3 g) const {
@@ -95,9 +96,11 @@ F4 sample_rep(const float* t, int n, float qx, float qy, float qz) {
         const float* s = t + (((size_t)z * n + y) * n + x) * 4;
         return F4(s[0], s[1], s[2], s[3]);
     };
-    const F4 x00 = mix4(L(ix, iy, iz), L(ix + 1, iy, iz), fx), x10 = mix4(L(ix, iy + 1, iz), L(ix + 1, iy + 1, iz), fx);
-    const F4 x01 = mix4(L(ix, iy, iz + 1), L(ix + 1, iy, iz + 1), fx), x11 = mix4(L(ix, iy + 1, iz + 1), L(ix + 1, iy + 1, iz + 1), fx);
-    return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+    F4 c[8];
+    for (int k = 0; k < 8; k++) c[k] = L(ix + (k & 1), iy + ((k >> 1) & 1), iz + (k >> 2));
+    float w[8];
+    tri_weights<float>(fx, fy, fz, w);
+    return tri_blend<float>(c, w);
 }
 
 // bbRow (src/renderer.js:74-80)

@@ -50,15 +50,15 @@ __device__ __forceinline__ Tri tri_at(const Dims& g, float bx, float by, float b
     for (int k = 0; k < 8; k++) {
         const int a = k & 1, b = (k >> 1) & 1, c = (k >> 2) & 1;
         t.idx[k] = (zs[c] * g.ny + ys[b]) * g.nx + xs[a];
-        t.w[k] = (a ? t.fx : 1.f - t.fx) * (b ? t.fy : 1.f - t.fy) * (c ? t.fz : 1.f - t.fz);
     }
+    tri_weights(t.fx, t.fy, t.fz, t.w);   // the forward sampler's weights
     return t;
 }
-// value and gradient (d/dx, d/dy, d/dz of the position) of channel values v[8]
+// value (the forward sampler's FMA chain) and gradient (d/dx, d/dy, d/dz of the position) of channel values v[8]
 __device__ __forceinline__ void tri_eval(const Tri& t, const float* v, float& val, float& dx, float& dy, float& dz) {
-    val = 0.f;
+    val = v[0] * t.w[0];
 #pragma unroll
-    for (int k = 0; k < 8; k++) val += t.w[k] * v[k];
+    for (int k = 1; k < 8; k++) val = __builtin_fmaf(v[k], t.w[k], val);
     const float fx = t.fx, fy = t.fy, fz = t.fz;
     dx = t.gx * ((1.f - fy) * (1.f - fz) * (v[1] - v[0]) + fy * (1.f - fz) * (v[3] - v[2]) +
                  (1.f - fy) * fz * (v[5] - v[4]) + fy * fz * (v[7] - v[6]));

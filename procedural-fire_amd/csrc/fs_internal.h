@@ -92,6 +92,7 @@ struct GpuMG {
     // FP64 defect correction
     double* x64 = nullptr;       // level-0 FP64 iterate
     float* bkeep = nullptr;      // level-0 right-hand side (the divergence)
+    float* p0post = nullptr;     // level-0 V-cycle correction after the prolongation (refinement)
     double* partials = nullptr;  // per-block (r.r, b.b) pairs
     int npart = 0;
     RefineState* st = nullptr;
@@ -172,7 +173,7 @@ void mg_spectral_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 int  mg_spectral_iter(GpuMG& mg, Spectral* sp, hipStream_t s, double tol, int max_cycles);
 // single-operator launches (tests)
 void mg_smooth(GpuMG& mg, hipStream_t s, int level, int iters, const int* skip, double* xupd = nullptr,
-               bool first_red_done = false, const FinArgs* fin = nullptr);
+               bool first_red_done = false, const FinArgs* fin = nullptr, float* pl = nullptr);
 
 // ---- error handling ----------------------------------------------------------
 void fs_set_error(const std::string& msg);

@@ -160,11 +160,58 @@ __device__ __forceinline__ float hmul_hi(unsigned w, float s) {   // f32(high ha
     asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(w), "v"(s), "v"(__int_as_float(0x80000000)));
     return r;
 }
-// mix4(dec(a), dec(b), t) for two binary16 x 4 texels (mixf's a (1 - t) + b t per channel, same bits)
-__device__ __forceinline__ float4 mix4_h(u32x2_t a, u32x2_t b, float t) {
-    const float u = 1.f - t;
-    return make_float4(hmul_lo(a.x, u) + hmul_lo(b.x, t), hmul_hi(a.x, u) + hmul_hi(b.x, t),
-                       hmul_lo(a.y, u) + hmul_lo(b.y, t), hmul_hi(a.y, u) + hmul_hi(b.y, t));
+__device__ __forceinline__ float hfma_lo(unsigned w, float s, float c) {   // fma(f32(low half of w), s, c)
+    float r;
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(w), "v"(s), "v"(c));
+    return r;
+}
+__device__ __forceinline__ float hfma_hi(unsigned w, float s, float c) {   // fma(f32(high half of w), s, c)
+    float r;
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(w), "v"(s), "v"(c));
+    return r;
+}
+
+// ---- trilinear filtering ------------------------------------------------------------
+// Corner k = (a, b, c) = (k & 1, (k >> 1) & 1, k >> 2) of the 2x2x2 texel block has the weight
+// w_k = wx_a * (wy_b * wz_c), with wx_0 = 1 - fx, wx_1 = fx (and so on), and each channel is the
+// chain v = t_0 w_0, then v = fma(t_k, w_k, v) for k = 1..7.  The weights are formed once per sample
+// position and serve every field read there; a channel costs 8 VALU instead of the 21 of three
+// levels of two-product blends.  The oracle's samplers round the same way (the WebGPU sampler's own
+// fixed-point filter is unpinned either way).  At a texel centre the weights are exactly 1 and 0.
+__device__ __forceinline__ void tri_weights(float fx, float fy, float fz, float* w) {
+    const float ux = 1.f - fx, uy = 1.f - fy, uz = 1.f - fz;
+    const float yz[4] = {uy * uz, fy * uz, uy * fz, fy * fz};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        w[2 * j] = ux * yz[j];
+        w[2 * j + 1] = fx * yz[j];
+    }
+}
+__device__ __forceinline__ float4 tri_blend(const float4* t, const float* w) {
+    float4 r = make_float4(t[0].x * w[0], t[0].y * w[0], t[0].z * w[0], t[0].w * w[0]);
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        r.x = __builtin_fmaf(t[k].x, w[k], r.x);
+        r.y = __builtin_fmaf(t[k].y, w[k], r.y);
+        r.z = __builtin_fmaf(t[k].z, w[k], r.z);
+        r.w = __builtin_fmaf(t[k].w, w[k], r.w);
+    }
+    return r;
+}
+// the same for binary16 texels as raw words: a[j] is corner 2j, b[j] corner 2j + 1; the widening of each
+// half rides in v_fma_mix_f32 (exact, so the bits are tri_blend's of the decoded texels)
+__device__ __forceinline__ float4 tri_blend_h(const u32x2_t* a, const u32x2_t* b, const float* w) {
+    float4 r = make_float4(hmul_lo(a[0].x, w[0]), hmul_hi(a[0].x, w[0]), hmul_lo(a[0].y, w[0]), hmul_hi(a[0].y, w[0]));
+    r.x = hfma_lo(b[0].x, w[1], r.x); r.y = hfma_hi(b[0].x, w[1], r.y);
+    r.z = hfma_lo(b[0].y, w[1], r.z); r.w = hfma_hi(b[0].y, w[1], r.w);
+#pragma unroll
+    for (int j = 1; j < 4; j++) {
+        r.x = hfma_lo(a[j].x, w[2 * j], r.x); r.y = hfma_hi(a[j].x, w[2 * j], r.y);
+        r.z = hfma_lo(a[j].y, w[2 * j], r.z); r.w = hfma_hi(a[j].y, w[2 * j], r.w);
+        r.x = hfma_lo(b[j].x, w[2 * j + 1], r.x); r.y = hfma_hi(b[j].x, w[2 * j + 1], r.y);
+        r.z = hfma_lo(b[j].y, w[2 * j + 1], r.z); r.w = hfma_hi(b[j].y, w[2 * j + 1], r.w);
+    }
+    return r;
 }
 // Per-half minimum / maximum of two binary16 pairs (IEEE minNum / maxNum, as fminf / fmaxf on the widened
 // values; widening is exact and monotonic, so min-then-widen equals widen-then-min).  The operands are
@@ -230,38 +277,34 @@ __device__ __forceinline__ float4 sample_clamp_f(const F& f, const Dims& g, floa
     const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
     const int y0 = clampi(iy, 0, g.ny - 1), y1 = clampi(iy + 1, 0, g.ny - 1);
     const int z0 = clampi(iz, 0, g.nz - 1), z1 = clampi(iz + 1, 0, g.nz - 1);
-    const int r00 = (z0 * g.ny + y0) * g.nx, r10 = (z0 * g.ny + y1) * g.nx;
-    const int r01 = (z1 * g.ny + y0) * g.nx, r11 = (z1 * g.ny + y1) * g.nx;
-    float4 c000, c100, c010, c110, c001, c101, c011, c111;
+    const int rows[4] = {(z0 * g.ny + y0) * g.nx, (z0 * g.ny + y1) * g.nx, (z1 * g.ny + y0) * g.nx, (z1 * g.ny + y1) * g.nx};
+    float w[8];
+    tri_weights(fx, fy, fz, w);
     if constexpr (F::kPair && std::is_same<S, StF16>::value) {
-        // binary16 pairs: the x blends read the raw halves (mix4_h), the rest as below
+        // binary16 pairs: the blend reads the raw halves (tri_blend_h)
         const int xs = clampi(ix, 0, g.nx - 2);
         const bool lo = ix < 0, hi = ix >= g.nx - 1;
         u32x2_t a[4], b[4];
-        f.ld_pair_raw(r00 + xs, lo, hi, a[0], b[0]);
-        f.ld_pair_raw(r10 + xs, lo, hi, a[1], b[1]);
-        f.ld_pair_raw(r01 + xs, lo, hi, a[2], b[2]);
-        f.ld_pair_raw(r11 + xs, lo, hi, a[3], b[3]);
-        const float4 x00 = mix4_h(a[0], b[0], fx), x10 = mix4_h(a[1], b[1], fx);
-        const float4 x01 = mix4_h(a[2], b[2], fx), x11 = mix4_h(a[3], b[3], fx);
-        return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
-    } else if constexpr (F::kPair) {
-        const int xs = clampi(ix, 0, g.nx - 2);
-        const bool lo = ix < 0, hi = ix >= g.nx - 1;
-        f.ld_pair(r00 + xs, lo, hi, c000, c100);
-        f.ld_pair(r10 + xs, lo, hi, c010, c110);
-        f.ld_pair(r01 + xs, lo, hi, c001, c101);
-        f.ld_pair(r11 + xs, lo, hi, c011, c111);
+#pragma unroll
+        for (int j = 0; j < 4; j++) f.ld_pair_raw(rows[j] + xs, lo, hi, a[j], b[j]);
+        return tri_blend_h(a, b, w);
     } else {
-        const int x0 = clampi(ix, 0, g.nx - 1), x1 = clampi(ix + 1, 0, g.nx - 1);
-        c000 = f.ld(r00 + x0); c100 = f.ld(r00 + x1);
-        c010 = f.ld(r10 + x0); c110 = f.ld(r10 + x1);
-        c001 = f.ld(r01 + x0); c101 = f.ld(r01 + x1);
-        c011 = f.ld(r11 + x0); c111 = f.ld(r11 + x1);
+        float4 c[8];
+        if constexpr (F::kPair) {
+            const int xs = clampi(ix, 0, g.nx - 2);
+            const bool lo = ix < 0, hi = ix >= g.nx - 1;
+#pragma unroll
+            for (int j = 0; j < 4; j++) f.ld_pair(rows[j] + xs, lo, hi, c[2 * j], c[2 * j + 1]);
+        } else {
+            const int x0 = clampi(ix, 0, g.nx - 1), x1 = clampi(ix + 1, 0, g.nx - 1);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                c[2 * j] = f.ld(rows[j] + x0);
+                c[2 * j + 1] = f.ld(rows[j] + x1);
+            }
+        }
+        return tri_blend(c, w);
     }
-    const float4 x00 = mix4(c000, c100, fx), x10 = mix4(c010, c110, fx);
-    const float4 x01 = mix4(c001, c101, fx), x11 = mix4(c011, c111, fx);
-    return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
 }
 // FBuf where it pays in every kernel (8-byte texels, paired); 16-byte texels keep plain pointer reads
 template <class S> using FPair = typename std::conditional<sizeof(typename S::raw) == 8, FBuf<S>, FPtr<S>>::type;
@@ -319,12 +362,16 @@ __device__ __forceinline__ float4 sample_rep(const half4_t* t, int n, float qx, 
     const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
     const int m = n - 1;
     const int x0 = ix & m, x1 = (ix + 1) & m, y0 = iy & m, y1 = (iy + 1) & m, z0 = iz & m, z1 = (iz + 1) & m;
-    const int r00 = (z0 * n + y0) * n, r10 = (z0 * n + y1) * n, r01 = (z1 * n + y0) * n, r11 = (z1 * n + y1) * n;
-    const float4 x00 = mix4(ld_h4(t, r00 + x0), ld_h4(t, r00 + x1), fx);
-    const float4 x10 = mix4(ld_h4(t, r10 + x0), ld_h4(t, r10 + x1), fx);
-    const float4 x01 = mix4(ld_h4(t, r01 + x0), ld_h4(t, r01 + x1), fx);
-    const float4 x11 = mix4(ld_h4(t, r11 + x0), ld_h4(t, r11 + x1), fx);
-    return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+    const int rows[4] = {(z0 * n + y0) * n, (z0 * n + y1) * n, (z1 * n + y0) * n, (z1 * n + y1) * n};
+    float4 c[8];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        c[2 * j] = ld_h4(t, rows[j] + x0);
+        c[2 * j + 1] = ld_h4(t, rows[j] + x1);
+    }
+    float w[8];
+    tri_weights(fx, fy, fz, w);
+    return tri_blend(c, w);
 }
 // The same sample from an n^3 binary16 texture read through a buffer resource: each row's texels x0, x0 + 1
 // come in one 16-byte load.  Where x0 = n - 1 wraps to x1 = 0, the second half of that load (the next row's
@@ -358,9 +405,9 @@ __device__ __forceinline__ float4 sample_rep_b(const RepTex& t, float qx, float 
 #pragma unroll
         for (int j = 0; j < 4; j++) b[j] = __builtin_amdgcn_raw_buffer_load_b64(t.r, (int)((unsigned)rows[j] * 8u), 0, 0);
     }
-    const float4 x00 = mix4_h(a[0], b[0], fx), x10 = mix4_h(a[1], b[1], fx);
-    const float4 x01 = mix4_h(a[2], b[2], fx), x11 = mix4_h(a[3], b[3], fx);
-    return mix4(mix4(x00, x10, fy), mix4(x01, x11, fy), fz);
+    float w[8];
+    tri_weights(fx, fy, fz, w);
+    return tri_blend_h(a, b, w);
 }
 
 template <class S>

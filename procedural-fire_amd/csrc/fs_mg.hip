@@ -315,9 +315,12 @@ __global__ __launch_bounds__(RB) void k_prolong(float* __restrict__ p, const flo
 // read by neighbouring threads and keep their unprolonged values, which the colour-1 pass that
 // follows prolongs for itself (k_smooth's pc_own).  The values equal those of k_prolong followed by
 // the two colour passes, bit for bit, with one pass over p and one launch less.
+// With `pout` set, the thread stores all 8 children there instead (the colour-0 ones updated, the colour-1
+// ones prolonged), so the colour-1 pass that follows runs on pout and reads its cells' own values instead of
+// forming their prolongation again; p is then only read.
 __global__ __launch_bounds__(RB) void k_prolong_red(float* __restrict__ p, const float* __restrict__ b,
                                                     const float* __restrict__ pcoarse, int nx, int ny, int nz, float h,
-                                                    float omega, const int* __restrict__ skip) {
+                                                    float omega, const int* __restrict__ skip, float* __restrict__ pout) {
     const int sk = skip[fs_opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;    // levels that prolong have even dims
     const int xc = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -386,6 +389,15 @@ __global__ __launch_bounds__(RB) void k_prolong_red(float* __restrict__ p, const
     }
     asm volatile("" ::"v"(pn[0]), "v"(pn[1]), "v"(pn[2]), "v"(pn[3]), "v"(sk));
     if (sk) return;
+    if (pout) {
+        fine[0] = pn[0]; fine[3] = pn[1]; fine[5] = pn[2]; fine[6] = pn[3];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int a = k & 1, bb = (k >> 1) & 1, cc = (k >> 2) & 1;
+            pout[fidx(2 * xc + a, 2 * yc + bb, 2 * zc + cc)] = fine[k];
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int k = RED[r];
@@ -772,6 +784,7 @@ int mg_build(GpuMG& mg, int nx, int ny, int nz, float h, int max_levels, std::st
     const dim3 g = grid3((nx + 1) / 2, ny, nz);
     mg.npart = (int)(g.x * g.y * g.z);
     if (hipMalloc(&mg.x64, n0 * sizeof(double)) != hipSuccess || hipMalloc(&mg.bkeep, n0 * sizeof(float)) != hipSuccess ||
+        hipMalloc(&mg.p0post, n0 * sizeof(float)) != hipSuccess ||
         hipMalloc(&mg.partials, (size_t)mg.npart * 2 * sizeof(double)) != hipSuccess ||
         hipMalloc(&mg.st, sizeof(RefineState)) != hipSuccess) {
         err = "hipMalloc failed for refinement buffers"; return FS_ENOMEM;
@@ -790,9 +803,10 @@ void mg_free(GpuMG& mg) {
     }
     if (mg.x64) (void)hipFree(mg.x64);
     if (mg.bkeep) (void)hipFree(mg.bkeep);
+    if (mg.p0post) (void)hipFree(mg.p0post);
     if (mg.partials) (void)hipFree(mg.partials);
     if (mg.st) (void)hipFree(mg.st);
-    mg.x64 = nullptr; mg.bkeep = nullptr; mg.partials = nullptr; mg.st = nullptr;
+    mg.x64 = nullptr; mg.bkeep = nullptr; mg.p0post = nullptr; mg.partials = nullptr; mg.st = nullptr;
     direct_free(mg.ds);
     mg.ds = nullptr;
     mg.direct = 0;
@@ -824,8 +838,9 @@ int mg_set_direct(GpuMG& mg, int want, std::string& err) {
 
 // `fin`: the first pass also runs that pending reduction (and is not skippable)
 void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, double* xupd, bool first_red_done,
-               const FinArgs* fin) {   // sim.js:576-586
+               const FinArgs* fin, float* pl) {   // sim.js:576-586
     if (!skip) skip = &mg.st->never;
+    if (!pl) pl = mg.p[l];
     const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
     const dim3 g = grid3((nx + 1) / 2, ny, nz);
     const float om = l == mg.nlev - 1 ? 1.f : mg.omega;   // the coarsest level's sweeps stay plain GS
@@ -834,11 +849,11 @@ void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, doub
             if (k == 0 && color == 0 && first_red_done) continue;
             double* xu = (xupd && k == iters - 1 && color == 1) ? xupd : nullptr;
             if (fin) {
-                k_smooth<true><<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, &mg.st->never, xu, om, nullptr,
+                k_smooth<true><<<g, RB, 0, s>>>(pl, mg.b[l], nx, ny, nz, mg.h[l], color, &mg.st->never, xu, om, nullptr,
                                                 *fin);
                 fin = nullptr;
             } else {
-                k_smooth<false><<<g, RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], color, skip, xu, om, nullptr,
+                k_smooth<false><<<g, RB, 0, s>>>(pl, mg.b[l], nx, ny, nz, mg.h[l], color, skip, xu, om, nullptr,
                                                  FinArgs{});
             }
         }
@@ -927,13 +942,17 @@ bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd, bool red
             // prolongation + the first colour-0 pass from one thread per coarse cell; the first colour-1 pass
             // prolongs its own cells; then the remaining sweeps (the last adds into the FP64 iterate)
             const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
+            // a refinement's level-0 correction is scratch: the post-smoothing runs on a second array, which
+            // the prolongation fills with every child (the colour-1 pass then reads its own prolonged values)
+            float* const post = (l == 0 && xupd) ? mg.p0post : nullptr;
             k_prolong_red<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(mg.p[l], mg.b[l], mg.p[l + 1], nx, ny, nz, mg.h[l],
-                                                                      mg.omega, skip);
+                                                                      mg.omega, skip, post);
             double* const xu = l == 0 ? xupd : nullptr;
-            k_smooth<false><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], 1, skip,
-                                                                       mg.post == 1 ? xu : nullptr, mg.omega, mg.p[l + 1],
-                                                                       FinArgs{});
-            if (mg.post > 1) mg_smooth(mg, s, l, mg.post - 1, skip, xu);
+            float* const pw = post ? post : mg.p[l];
+            k_smooth<false><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(pw, mg.b[l], nx, ny, nz, mg.h[l], 1, skip,
+                                                                       mg.post == 1 ? xu : nullptr, mg.omega,
+                                                                       post ? nullptr : mg.p[l + 1], FinArgs{});
+            if (mg.post > 1) mg_smooth(mg, s, l, mg.post - 1, skip, xu, false, nullptr, pw);
         } else {
             launch_prolong(mg, s, l, skip);
             mg_smooth(mg, s, l, mg.post, skip, l == 0 ? xupd : nullptr);
