@@ -86,17 +86,21 @@ struct fs_sim {
     GpuMG mg;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    // captured graphs: the whole step (fixed solves), or its three parts (adaptive solves)
-    enum { G_FULL, G_PRE, G_ITER, G_POST, G_COUNT };
+    // captured graphs: the whole step (fixed solves); the whole step of an adaptive solve with the number of
+    // iterations the last step needed (step_k) and the readback of the solve's state; its three parts
+    enum { G_FULL, G_STEP, G_PRE, G_ITER, G_POST, G_COUNT };
+    int step_k = 0;
     hipGraphExec_t gexec[G_COUNT] = {};
     hipGraph_t graph[G_COUNT] = {};
     int predict = 1;                 // V-cycles the last adaptive solve needed
     int* host_state = nullptr;       // pinned: {used, done} of the last solve
+    int* host_state_dev = nullptr;   // its device address
     bool geo_ready = false;
     double time = 0.0;
     int64_t frame = 0;
     bool profile = false;
     hipEvent_t ev[8] = {};
+    hipEvent_t ev_state = nullptr;   // the stop-flag readback of an adaptive step has landed
     bool timings_valid = false;
     // diagnostics: per-plane partials, stats partials, the per-step record ring
     double* planes_d = nullptr;      // ny * 8
@@ -232,12 +236,26 @@ int enqueue_step(fs_sim* h, bool prof) {
     return FS_OK;
 }
 
+// an adaptive step with step_k iterations, the stop-flag readback and the projection (speculative: the
+// host checks the flag afterwards)
+int enqueue_adaptive_step(fs_sim* h) {
+    enqueue_pre(h, false, true);
+    const int maxc = std::min(h->cfg.max_vcycles, MG_MAX_REFINE);
+    for (int k = 0; k < h->step_k; k++)
+        if (int rc = h->solve_iter(h->stream)) return rc;
+    // the stopping state goes to the host from the refinement's last reduction kernel, or by a copy
+    if (h->cfg.solve == FS_SOLVE_REFINE64) mg_state_ready(h->mg, h->stream, h->cfg.div_tol, maxc, h->host_state_dev);
+    else FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    return FS_OK;
+}
+
 int capture(fs_sim* h, int which) {
     if (h->gexec[which]) return FS_OK;
     FS_HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int rc = FS_OK;
     switch (which) {
         case fs_sim::G_FULL: rc = enqueue_step(h, false); break;
+        case fs_sim::G_STEP: rc = enqueue_adaptive_step(h); break;
         case fs_sim::G_PRE: enqueue_pre(h, false, true); break;
         case fs_sim::G_ITER: rc = h->solve_iter(h->stream); break;
         case fs_sim::G_POST: enqueue_post(h, false); break;
@@ -262,6 +280,7 @@ int run_part(fs_sim* h, int which, bool prof) {
     }
     switch (which) {
         case fs_sim::G_FULL: return enqueue_step(h, prof);
+        case fs_sim::G_STEP: return enqueue_adaptive_step(h);
         case fs_sim::G_PRE: enqueue_pre(h, prof, true); break;
         case fs_sim::G_ITER: return h->solve_iter(h->stream);
         case fs_sim::G_POST: enqueue_post(h, prof); break;
@@ -277,25 +296,59 @@ void drop_graphs(fs_sim* h) {
     }
 }
 
-// one step with a tolerance-driven solve: enqueue as many iterations as the last
-// step needed, then read the device's stopping state and continue if needed
+// One step with a tolerance-driven solve: enqueue as many iterations as the last step needed, the readback
+// of the device's stopping state and the projection at once, before the host has read that state; with
+// graphs all of it is one captured graph (per iteration count).  If the solve had not stopped yet, the
+// host enqueues the further iterations and the projection again: the projection only reads the advected
+// velocity (which nothing between the two runs writes) and the pressure, so its second run gives the state
+// of the unspeculated order, bit for bit.  Not speculated: profiled steps and steps that append a per-step
+// diagnostics record (the projection part would record twice).
 int adaptive_step(fs_sim* h, bool prof) {
-    int rc = run_part(h, fs_sim::G_PRE, prof);
-    if (rc) return rc;
     const int maxc = std::min(h->cfg.max_vcycles, MG_MAX_REFINE);
-    int k = 0;
-    for (; k < std::min(h->predict, maxc); k++) if ((rc = run_part(h, fs_sim::G_ITER, prof))) return rc;
+    const bool spec = !prof && h->diag_cap == 0;
+    bool projected = false;
+    int k = 0, rc = FS_OK;
+    if (spec && h->cfg.use_graph) {
+        const int kk = std::min(h->predict, maxc);
+        if (h->step_k != kk && h->gexec[fs_sim::G_STEP]) {
+            (void)hipGraphExecDestroy(h->gexec[fs_sim::G_STEP]);
+            (void)hipGraphDestroy(h->graph[fs_sim::G_STEP]);
+            h->gexec[fs_sim::G_STEP] = nullptr;
+            h->graph[fs_sim::G_STEP] = nullptr;
+        }
+        h->step_k = kk;
+        if ((rc = run_part(h, fs_sim::G_STEP, prof))) return rc;
+        FS_HIP_CHECK(hipEventRecord(h->ev_state, h->stream));
+        if ((rc = run_part(h, fs_sim::G_POST, prof))) return rc;
+        FS_HIP_CHECK(hipEventSynchronize(h->ev_state));
+        k = kk;
+        if (h->host_state[1] || k >= maxc) {
+            h->predict = std::max(1, h->host_state[0]);
+            return FS_OK;
+        }
+        if ((rc = run_part(h, fs_sim::G_ITER, prof))) return rc;
+        k++;
+    } else {
+        if ((rc = run_part(h, fs_sim::G_PRE, prof))) return rc;
+        for (; k < std::min(h->predict, maxc); k++) if ((rc = run_part(h, fs_sim::G_ITER, prof))) return rc;
+    }
     while (true) {
         // the last residual's reduction (the refinement leaves it to the next iteration's first kernel)
         if (h->cfg.solve == FS_SOLVE_REFINE64) mg_state_ready(h->mg, h->stream, h->cfg.div_tol, maxc);
         FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
-        FS_HIP_CHECK(hipStreamSynchronize(h->stream));
+        FS_HIP_CHECK(hipEventRecord(h->ev_state, h->stream));
+        if (spec) {
+            if ((rc = run_part(h, fs_sim::G_POST, prof))) return rc;
+            projected = true;
+        }
+        FS_HIP_CHECK(hipEventSynchronize(h->ev_state));
         if (h->host_state[1] || k >= maxc) break;
+        projected = false;
         if ((rc = run_part(h, fs_sim::G_ITER, prof))) return rc;
         k++;
     }
     h->predict = std::max(1, h->host_state[0]);
-    return run_part(h, fs_sim::G_POST, prof);
+    return projected ? FS_OK : run_part(h, fs_sim::G_POST, prof);
 }
 
 void free_sim(fs_sim* h) {
@@ -318,6 +371,7 @@ void free_sim(fs_sim* h) {
     mg_free(h->mg);
     spectral_free(h->spec);
     for (auto& e : h->ev) if (e) (void)hipEventDestroy(e);
+    if (h->ev_state) (void)hipEventDestroy(h->ev_state);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -515,8 +569,12 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
         if (rc) return bail(fail(rc, err));
     }
     for (auto& e : h->ev) if (hipEventCreate(&e) != hipSuccess) return bail(fail(FS_EDEVICE, "event create failed"));
+    if (hipEventCreateWithFlags(&h->ev_state, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(FS_EDEVICE, "event create failed"));
     if (hipHostMalloc((void**)&h->host_state, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return bail(fail(FS_ENOMEM, "pinned host allocation failed"));
+    if (hipHostGetDevicePointer((void**)&h->host_state_dev, h->host_state, 0) != hipSuccess)
+        return bail(fail(FS_EDEVICE, "pinned host memory is not device-accessible"));
     launch_noise_bake(h->stream, h->noise, h->curl, nn, cfg->noise_seed);   // src/main.js:36-37
     std::vector<float> none;
     rc = upload_logs(h, n_logs > 0 ? logs : none.data(), n_logs);

@@ -154,7 +154,8 @@ void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 // reduces a residual whose partials are still pending, with the solve's stopping test (a one-block
 // kernel that exits at once when none is)
-void mg_state_ready(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
+// With `host` (pinned host memory, device-accessible) the kernel also stores {used, done} there.
+void mg_state_ready(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int* host = nullptr);
 void mg_tol32_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 void mg_tol32_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 // relative FP64 residual of the FP32 level-0 p against level-0 b -> st->rel[0]

@@ -508,10 +508,15 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
     }
 }
 
-// one block: the fixed-order reduction of the partials and the stopping test
+// one block: the fixed-order reduction of the partials and the stopping test; with `host` set (pinned host
+// memory) it also stores {used, done} there for the host to read after the kernel
 __global__ __launch_bounds__(FIN) void k_finalize(RefineState* __restrict__ st, const double* __restrict__ partials, int npart,
-                                                 double tol, int max_cycles) {
+                                                 double tol, int max_cycles, int* __restrict__ host = nullptr) {
     finalize_sums(st, partials, npart, tol, max_cycles);
+    if (host && threadIdx.x == 0) {   // thread 0 made the block's only state writes
+        host[0] = st->used;
+        host[1] = st->done;
+    }
 }
 
 // x += e (the V-cycle's correction), FP64, over the whole parity-split array
@@ -994,9 +999,9 @@ void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
                                                                   mg.partials, mg.st, 0, done, mg.h[0], mg.omega, red ? 1 : 0);
 }
 
-void mg_state_ready(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
+void mg_state_ready(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int* host) {
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
-    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, host);
 }
 
 void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
