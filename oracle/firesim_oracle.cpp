@@ -609,9 +609,8 @@ struct Oracle : OracleBase {
     V4<R> sample(const std::vector<S>& f, V3<R> b) const {
         R tx = b.x - R(0.5), ty = b.y - R(0.5), tz = b.z - R(0.5);
         tx = wclamp<R>(tx, R(-1), (R)nx); ty = wclamp<R>(ty, R(-1), (R)ny); tz = wclamp<R>(tz, R(-1), (R)nz);
-        R flx = std::floor(tx), fly = std::floor(ty), flz = std::floor(tz);
-        int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-        R fx = tx - flx, fy = ty - fly, fz = tz - flz;
+        R fx, fy, fz;
+        const int ix = tri_axis<R>(tx, nx, fx), iy = tri_axis<R>(ty, ny, fy), iz = tri_axis<R>(tz, nz, fz);
         V4<R> c[8];
         for (int k = 0; k < 8; k++) c[k] = ld(f, ix + (k & 1), iy + ((k >> 1) & 1), iz + (k >> 2));
         R w[8];

@@ -205,6 +205,15 @@ template <class R> static inline V4<R> mix4(const V4<R>& a, const V4<R>& b, R t)
 // weight w_k = wx_a * (wy_b * wz_c) (wx_0 = 1 - fx, wx_1 = fx, ...), value = c_0 w_0, then fma(c_k, w_k, .) for
 // k = 1..7.  The arithmetic of the texture sampler is the implementation's (src/sim.js:129, 154-156, 176-178,
 // 228, 365 call textureSampleLevel); this fixes one, and the HIP kernels round identically.
+// One axis of the clamp-to-edge footprint at texel-space coordinate t (n >= 2): texels i0, i0 + 1 with
+// i0 = clamp(floor(t), 0, n - 2) and the fraction f of the second.  Where floor(t) is outside [0, n - 2]
+// both clamped texels are the same edge texel, and the fraction 0 (below) or 1 (above) takes it exactly.
+template <class R> static inline int tri_axis(R t, int n, R& f) {
+    const R fl = std::floor(t);
+    const int i = (int)fl;
+    f = i < 0 ? R(0) : i > n - 2 ? R(1) : t - fl;
+    return std::min(std::max(i, 0), n - 2);
+}
 template <class R> static inline void tri_weights(R fx, R fy, R fz, R* w) {
     const R ux = R(1) - fx, uy = R(1) - fy, uz = R(1) - fz;
     const R yz[4] = {uy * uz, fy * uz, uy * fz, fy * fz};

@@ -50,9 +50,8 @@ struct Grid {
     F4 sample(float bx, float by, float bz) const {
         const float tx = wclamp<float>(bx - 0.5f, -1.f, (float)nx), ty = wclamp<float>(by - 0.5f, -1.f, (float)ny),
                     tz = wclamp<float>(bz - 0.5f, -1.f, (float)nz);
-        const float flx = std::floor(tx), fly = std::floor(ty), flz = std::floor(tz);
-        const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-        const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
+        float fx, fy, fz;
+        const int ix = tri_axis<float>(tx, nx, fx), iy = tri_axis<float>(ty, ny, fy), iz = tri_axis<float>(tz, nz, fz);
         F4 c[8];
         for (int k = 0; k < 8; k++) c[k] = ld(ix + (k & 1), iy + ((k >> 1) & 1), iz + (k >> 2));
         float w[8];

@@ -35,22 +35,17 @@ struct Tri {
 };
 __device__ __forceinline__ Tri tri_at(const Dims& g, float bx, float by, float bz) {
     Tri t;
-    const float ux = bx - 0.5f, uy = by - 0.5f, uz = bz - 0.5f;
-    const float tx = wclamp(ux, -1.f, (float)g.nx), ty = wclamp(uy, -1.f, (float)g.ny), tz = wclamp(uz, -1.f, (float)g.nz);
-    t.gx = (ux > -1.f && ux < (float)g.nx) ? 1.f : 0.f;
-    t.gy = (uy > -1.f && uy < (float)g.ny) ? 1.f : 0.f;
-    t.gz = (uz > -1.f && uz < (float)g.nz) ? 1.f : 0.f;
-    const float flx = floorf(tx), fly = floorf(ty), flz = floorf(tz);
-    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-    t.fx = tx - flx; t.fy = ty - fly; t.fz = tz - flz;
-    const int xs[2] = {clampi(ix, 0, g.nx - 1), clampi(ix + 1, 0, g.nx - 1)};
-    const int ys[2] = {clampi(iy, 0, g.ny - 1), clampi(iy + 1, 0, g.ny - 1)};
-    const int zs[2] = {clampi(iz, 0, g.nz - 1), clampi(iz + 1, 0, g.nz - 1)};
+    const float tx = wclamp(bx - 0.5f, -1.f, (float)g.nx), ty = wclamp(by - 0.5f, -1.f, (float)g.ny),
+                tz = wclamp(bz - 0.5f, -1.f, (float)g.nz);
+    // the forward sampler's footprint and fractions; a fraction moves with the position only where the
+    // footprint is not clamped (floor in [0, n - 2])
+    const int xs = tri_axis(tx, g.nx, t.fx), ys = tri_axis(ty, g.ny, t.fy), zs = tri_axis(tz, g.nz, t.fz);
+    const int ix = (int)floorf(tx), iy = (int)floorf(ty), iz = (int)floorf(tz);
+    t.gx = (ix >= 0 && ix <= g.nx - 2) ? 1.f : 0.f;
+    t.gy = (iy >= 0 && iy <= g.ny - 2) ? 1.f : 0.f;
+    t.gz = (iz >= 0 && iz <= g.nz - 2) ? 1.f : 0.f;
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int a = k & 1, b = (k >> 1) & 1, c = (k >> 2) & 1;
-        t.idx[k] = (zs[c] * g.ny + ys[b]) * g.nx + xs[a];
-    }
+    for (int k = 0; k < 8; k++) t.idx[k] = ((zs + (k >> 2)) * g.ny + ys + ((k >> 1) & 1)) * g.nx + xs + (k & 1);
     tri_weights(t.fx, t.fy, t.fz, t.w);   // the forward sampler's weights
     return t;
 }

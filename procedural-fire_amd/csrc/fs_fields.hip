@@ -188,10 +188,10 @@ __global__ __launch_bounds__(256) void k_adv2(const DevParams* __restrict__ P, D
                                               const unsigned* __restrict__ btpack) {
     adv2_cell<S, PASS>(P, g, vel, sF, s2F, velH, sH, s2H, velY, sY, s2Y, btpack);
 }
-// binary16 storage: held to 7 waves per SIMD (at most 72 VGPRs), where the gathers of this issue-bound pass
-// overlap best
+// binary16 storage: the register budget of 6 waves per SIMD (at most 80 VGPRs) lets the compiler schedule
+// this issue-bound pass without spills in 70 VGPRs, so it runs at 7 waves (a budget of 7, 72 VGPRs, spilled)
 template <bool PASS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_adv2_h(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_adv2_h(
     const DevParams* __restrict__ P, Dims g, const StF16::raw* __restrict__ vel, const StF16::raw* __restrict__ sF,
     const StF16::raw* __restrict__ s2F, const StF16::raw* __restrict__ velH, const StF16::raw* __restrict__ sH,
     const StF16::raw* __restrict__ s2H, StF16::raw* __restrict__ velY, StF16::raw* __restrict__ sY,

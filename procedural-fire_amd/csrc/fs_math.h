@@ -247,6 +247,18 @@ template <class S> struct FBuf {
         a = hi_both ? hi : lo;
         b = lo_both ? lo : hi;
     }
+    // texels i and i + 1 as they are (raw words)
+    __device__ __forceinline__ void ld_pair_raw(int i, u32x2_t& a, u32x2_t& b) const {
+        const u32x4_t u = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((unsigned)i * 8u), 0, 0);
+        a = u32x2_t{u.x, u.y};
+        b = u32x2_t{u.z, u.w};
+    }
+    __device__ __forceinline__ void ld_pair(int i, float4& a, float4& b) const {
+        u32x2_t ra, rb;
+        ld_pair_raw(i, ra, rb);
+        a = Codec<S>::dec(ra);
+        b = Codec<S>::dec(rb);
+    }
     __device__ __forceinline__ void ld_pair(int i, bool lo_both, bool hi_both, float4& a, float4& b) const {
         u32x2_t ra, rb;
         ld_pair_raw(i, lo_both, hi_both, ra, rb);
@@ -265,42 +277,45 @@ template <class S>
 __device__ __forceinline__ float4 ldc(const typename S::raw* f, const Dims& g, int x, int y, int z) {
     return ldc_f<S>(FPtr<S>(f, g), g, x, y, z);
 }
+// One axis of the clamp-to-edge footprint at texel-space coordinate t (n >= 2): the texels i0 and i0 + 1 with
+// i0 = clamp(floor(t), 0, n - 2), and the weight fraction f of the second.  Where floor(t) lies outside
+// [0, n - 2] both clamped texels of the footprint are the same edge texel; the fraction 0 (below) or 1
+// (above) then takes that texel exactly, so the footprint never needs two copies of one texel.
+__device__ __forceinline__ int tri_axis(float t, int n, float& f) {
+    const float fl = floorf(t);
+    const int i = (int)fl;
+    f = i < 0 ? 0.f : i > n - 2 ? 1.f : t - fl;
+    return clampi(i, 0, n - 2);
+}
 // trilinear, clamp-to-edge, at cell-space position b (texel centres at i + 0.5).
 // Clamping the coordinate to [-1, n] first leaves the clamped-texel result unchanged.
 template <class S, class F>
 __device__ __forceinline__ float4 sample_clamp_f(const F& f, const Dims& g, float bx, float by, float bz) {
-    float tx = wclamp(bx - 0.5f, -1.f, (float)g.nx);
-    float ty = wclamp(by - 0.5f, -1.f, (float)g.ny);
-    float tz = wclamp(bz - 0.5f, -1.f, (float)g.nz);
-    const float flx = floorf(tx), fly = floorf(ty), flz = floorf(tz);
-    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-    const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
-    const int y0 = clampi(iy, 0, g.ny - 1), y1 = clampi(iy + 1, 0, g.ny - 1);
-    const int z0 = clampi(iz, 0, g.nz - 1), z1 = clampi(iz + 1, 0, g.nz - 1);
-    const int rows[4] = {(z0 * g.ny + y0) * g.nx, (z0 * g.ny + y1) * g.nx, (z1 * g.ny + y0) * g.nx, (z1 * g.ny + y1) * g.nx};
+    const float tx = wclamp(bx - 0.5f, -1.f, (float)g.nx);
+    const float ty = wclamp(by - 0.5f, -1.f, (float)g.ny);
+    const float tz = wclamp(bz - 0.5f, -1.f, (float)g.nz);
+    float fx, fy, fz;
+    const int xs = tri_axis(tx, g.nx, fx), ys = tri_axis(ty, g.ny, fy), zs = tri_axis(tz, g.nz, fz);
+    const int r0 = (zs * g.ny + ys) * g.nx + xs;
+    const int rows[4] = {r0, r0 + g.nx, r0 + g.ny * g.nx, r0 + (g.ny + 1) * g.nx};
     float w[8];
     tri_weights(fx, fy, fz, w);
     if constexpr (F::kPair && std::is_same<S, StF16>::value) {
         // binary16 pairs: the blend reads the raw halves (tri_blend_h)
-        const int xs = clampi(ix, 0, g.nx - 2);
-        const bool lo = ix < 0, hi = ix >= g.nx - 1;
         u32x2_t a[4], b[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) f.ld_pair_raw(rows[j] + xs, lo, hi, a[j], b[j]);
+        for (int j = 0; j < 4; j++) f.ld_pair_raw(rows[j], a[j], b[j]);
         return tri_blend_h(a, b, w);
     } else {
         float4 c[8];
         if constexpr (F::kPair) {
-            const int xs = clampi(ix, 0, g.nx - 2);
-            const bool lo = ix < 0, hi = ix >= g.nx - 1;
 #pragma unroll
-            for (int j = 0; j < 4; j++) f.ld_pair(rows[j] + xs, lo, hi, c[2 * j], c[2 * j + 1]);
+            for (int j = 0; j < 4; j++) f.ld_pair(rows[j], c[2 * j], c[2 * j + 1]);
         } else {
-            const int x0 = clampi(ix, 0, g.nx - 1), x1 = clampi(ix + 1, 0, g.nx - 1);
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                c[2 * j] = f.ld(rows[j] + x0);
-                c[2 * j + 1] = f.ld(rows[j] + x1);
+                c[2 * j] = f.ld(rows[j]);
+                c[2 * j + 1] = f.ld(rows[j] + 1);
             }
         }
         return tri_blend(c, w);
