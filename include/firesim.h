@@ -123,7 +123,10 @@ typedef struct fs_config {
                              finest with x a multiple of 16, y of 4, z of 8, each <= 128, and at most
                              2^19 cells; the plain V-cycle if none), -1 = none (the V-cycle to
                              the coarsest level), k >= 1 = level k.  FS_SOLVE_VCYCLES never uses it */
-    int    reserved[2];
+    int    first_cycles;  /* FS_SOLVE_REFINE64: V-cycles of a step's first FP64 correction (the FP32 V-cycles
+                             iterate on the correction, its residual reduced in FP32 inside each restriction);
+                             later corrections take one each.  0 = default 3; 1 = one V-cycle per FP64 residual */
+    int    reserved;
 } fs_config;
 
 /* Fills a config with the reference defaults (res 80x120x80, size 1.2x1.8x1.2,

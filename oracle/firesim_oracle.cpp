@@ -457,12 +457,13 @@ struct Multigrid {
     }
 };
 
-// FP32 V-cycles inside FP64 defect correction: x <- x + V(b - A x) until
-// ||b - A x|| / ||b|| <= tol (build's mixed-precision solve).  b_keep holds the
-// level-0 right-hand side; x is the FP64 iterate.  Returns V-cycles used.
+// FP32 V-cycles inside FP64 defect correction: x <- x + e, e from V-cycles on A e = b - A x (from e = 0), until
+// ||b - A x|| / ||b|| <= tol (build's mixed-precision solve).  The first correction takes `first` V-cycles
+// (capped at max_cycles), each later one a single V-cycle.  b_keep holds the level-0 right-hand side; x is the
+// FP64 iterate.  Returns V-cycles used; hist[k] is the residual after k + 1 V-cycles (set after each correction).
 template <class M>
 int refine64(Multigrid<M>& mg, std::vector<double>& x, const std::vector<M>& b_keep, double tol,
-             int max_cycles, double* rel_out, double* hist) {
+             int max_cycles, double* rel_out, double* hist, int first = 1) {
     Level<M>& l0 = mg.L[0];
     std::vector<double> r(x.size());
     int used = 0;
@@ -471,14 +472,15 @@ int refine64(Multigrid<M>& mg, std::vector<double>& x, const std::vector<M>& b_k
     mg.residual64(x.data(), &r, &rr, &bb);
     double rel = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     while (rel > tol && used < max_cycles) {
+        const int k = used == 0 ? std::max(1, std::min(first, max_cycles)) : 1;
         for (size_t i = 0; i < x.size(); i++) { l0.b[i] = (M)r[i]; l0.p[i] = 0; }
-        mg.vcycle();
+        for (int c = 0; c < k; c++) mg.vcycle();
         for (size_t i = 0; i < x.size(); i++) x[i] += (double)l0.p[i];
         l0.b = b_keep;
         mg.residual64(x.data(), &r, &rr, &bb);
         rel = bb > 0 ? std::sqrt(rr / bb) : 0.0;
-        if (hist) hist[used] = rel;
-        used++;
+        used += k;
+        if (hist) hist[used - 1] = rel;
     }
     l0.b = b_keep;
     if (rel_out) *rel_out = rel;
@@ -527,6 +529,7 @@ struct Oracle : OracleBase {
     double dx, ox, oy, oz;
     float size_[3];
     int vcycles, solve_mode, max_vcycles;
+    int first_cycles = 1;      // V-cycles of a step's first FP64 correction (FS_SOLVE_REFINE64)
     double div_tol;
     float prm[FS_NUM_PARAMS];
     std::vector<float> logs;
@@ -564,6 +567,7 @@ struct Oracle : OracleBase {
             mg.omega = (M)(c->omega > 0.f ? c->omega : 1.25f);
             mg.pre = c->pre_sweeps > 0 ? c->pre_sweeps : c->pre_sweeps < 0 ? 0 : 1;
             mg.post = c->post_sweeps > 0 ? c->post_sweeps : 2;
+            if (solve_mode == FS_SOLVE_REFINE64) first_cycles = c->first_cycles > 0 ? c->first_cycles : 3;
         }
         if (solve_mode == FS_SOLVE_REFINE64 || solve_mode == FS_SOLVE_TOL32) mg.set_direct(c->direct_level);
         x64.assign(ncell, 0.0);
@@ -1009,7 +1013,7 @@ struct Oracle : OracleBase {
     void k_mg() {
         if (solve_mode == FS_SOLVE_REFINE64) {
             std::vector<M> bk = mg.L[0].b;
-            last_used = refine64(mg, x64, bk, div_tol, max_vcycles, &last_rel, nullptr);
+            last_used = refine64(mg, x64, bk, div_tol, max_vcycles, &last_rel, nullptr, first_cycles);
             for (size_t i = 0; i < ncell; i++) mg.L[0].p[i] = (M)x64[i];
         } else if (solve_mode == FS_SOLVE_TOL32) {
             double rr, bb;

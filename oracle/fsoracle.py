@@ -42,13 +42,14 @@ class FsConfig(ctypes.Structure):
         ("pre_sweeps", ctypes.c_int),
         ("post_sweeps", ctypes.c_int),
         ("direct_level", ctypes.c_int),
-        ("reserved", ctypes.c_int * 2),
+        ("first_cycles", ctypes.c_int),
+        ("reserved", ctypes.c_int),
     ]
 
 
 def make_config(res=(80, 120, 80), size=(1.2, 1.8, 1.2), vcycles=2, max_levels=0, precision=FP16,
                 solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1, omega=0.0, pre_sweeps=0,
-                post_sweeps=0, direct_level=0):
+                post_sweeps=0, direct_level=0, first_cycles=0):
     c = FsConfig()
     c.res[:] = list(res)
     c.size[:] = list(size)
@@ -64,6 +65,7 @@ def make_config(res=(80, 120, 80), size=(1.2, 1.8, 1.2), vcycles=2, max_levels=0
     c.pre_sweeps = pre_sweeps
     c.post_sweeps = post_sweeps
     c.direct_level = direct_level
+    c.first_cycles = first_cycles
     return c
 
 
@@ -233,7 +235,7 @@ class OracleSim:
 
     def __init__(self, res=(64, 96, 64), size=None, logs=None, n_logs=None, precision=FP16, vcycles=2,
                  max_levels=0, solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128, noise_seed=1, omega=0.0,
-                 pre_sweeps=0, post_sweeps=0, direct_level=0):
+                 pre_sweeps=0, post_sweeps=0, direct_level=0, first_cycles=0):
         if size is None:
             size = (res[0] * 0.015, res[1] * 0.015, res[2] * 0.015)
         if logs is None:
@@ -244,7 +246,7 @@ class OracleSim:
             n_logs = logs.size // 8
         self.res = tuple(res)
         self.cfg = make_config(res, size, vcycles, max_levels, precision, solve, div_tol, max_vcycles, noise_n, noise_seed,
-                               omega, pre_sweeps, post_sweeps, direct_level)
+                               omega, pre_sweeps, post_sweeps, direct_level, first_cycles)
         self.h = lib().orc_create(ctypes.byref(self.cfg), _fp(logs), n_logs)
         if not self.h:
             raise RuntimeError("oracle create failed")

@@ -88,7 +88,7 @@ struct fs_sim {
     bool own_stream = false;
     // captured graphs: the whole step (fixed solves); the whole step of an adaptive solve with the number of
     // iterations the last step needed (step_k) and the readback of the solve's state; its three parts
-    enum { G_FULL, G_STEP, G_PRE, G_ITER, G_POST, G_COUNT };
+    enum { G_FULL, G_STEP, G_PRE, G_FIRST, G_ITER, G_POST, G_COUNT };
     int step_k = 0;
     hipGraphExec_t gexec[G_COUNT] = {};
     hipGraph_t graph[G_COUNT] = {};
@@ -137,9 +137,14 @@ struct fs_sim {
         else if (cfg.solve == FS_SOLVE_REFINE64) mg_refine_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
         else mg_tol32_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
     }
-    int solve_iter(hipStream_t s) {
+    // V-cycles of a step's first correction (FS_SOLVE_REFINE64; the other solves take one per iteration)
+    int first_cycles() const {
+        return cfg.solve == FS_SOLVE_REFINE64 ? std::max(1, std::min(mg.first, std::min(cfg.max_vcycles, MG_MAX_REFINE))) : 1;
+    }
+    // one iteration of the solve; a refinement's correction takes `cycles` V-cycles
+    int solve_iter(hipStream_t s, int cycles = 1) {
         if (cfg.solve == FS_SOLVE_SPECTRAL64) return mg_spectral_iter(mg, spec, s, cfg.div_tol, cfg.max_vcycles);
-        if (cfg.solve == FS_SOLVE_REFINE64) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
+        if (cfg.solve == FS_SOLVE_REFINE64) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles, cycles);
         else mg_tol32_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
         return FS_OK;
     }
@@ -153,9 +158,11 @@ struct fs_sim {
                 if (host_state[1]) break;
                 if (int rc = mg_spectral_iter(mg, spec, s, cfg.div_tol, cfg.max_vcycles)) return rc;
             }
-        } else if (cfg.solve == FS_SOLVE_REFINE64) {
+        } else if (cfg.solve == FS_SOLVE_REFINE64) {   // the first correction, then one V-cycle per correction
             if (!begun) mg_refine_begin(mg, s, cfg.div_tol, cfg.max_vcycles);
-            for (int k = 0; k < std::min(cfg.max_vcycles, MG_MAX_REFINE); k++) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
+            const int f = first_cycles();
+            mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles, f);
+            for (int k = f; k < std::min(cfg.max_vcycles, MG_MAX_REFINE); k++) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
             mg_state_ready(mg, s, cfg.div_tol, cfg.max_vcycles);
         }
         else if (cfg.solve == FS_SOLVE_TOL32) mg_solve_tol32(mg, s, cfg.div_tol, cfg.max_vcycles);
@@ -236,12 +243,14 @@ int enqueue_step(fs_sim* h, bool prof) {
     return FS_OK;
 }
 
-// an adaptive step with step_k iterations, the stop-flag readback and the projection (speculative: the
-// host checks the flag afterwards)
+// an adaptive step with step_k V-cycles (the first correction's, then one per correction) and the stop-flag
+// readback (the projection follows as its own part; the host checks the flag afterwards)
 int enqueue_adaptive_step(fs_sim* h) {
     enqueue_pre(h, false, true);
     const int maxc = std::min(h->cfg.max_vcycles, MG_MAX_REFINE);
-    for (int k = 0; k < h->step_k; k++)
+    const int f = h->first_cycles();
+    if (int rc = h->solve_iter(h->stream, f)) return rc;
+    for (int k = f; k < h->step_k; k++)
         if (int rc = h->solve_iter(h->stream)) return rc;
     // the stopping state goes to the host from the refinement's last reduction kernel, or by a copy
     if (h->cfg.solve == FS_SOLVE_REFINE64) mg_state_ready(h->mg, h->stream, h->cfg.div_tol, maxc, h->host_state_dev);
@@ -257,6 +266,7 @@ int capture(fs_sim* h, int which) {
         case fs_sim::G_FULL: rc = enqueue_step(h, false); break;
         case fs_sim::G_STEP: rc = enqueue_adaptive_step(h); break;
         case fs_sim::G_PRE: enqueue_pre(h, false, true); break;
+        case fs_sim::G_FIRST: rc = h->solve_iter(h->stream, h->first_cycles()); break;
         case fs_sim::G_ITER: rc = h->solve_iter(h->stream); break;
         case fs_sim::G_POST: enqueue_post(h, false); break;
     }
@@ -282,6 +292,7 @@ int run_part(fs_sim* h, int which, bool prof) {
         case fs_sim::G_FULL: return enqueue_step(h, prof);
         case fs_sim::G_STEP: return enqueue_adaptive_step(h);
         case fs_sim::G_PRE: enqueue_pre(h, prof, true); break;
+        case fs_sim::G_FIRST: return h->solve_iter(h->stream, h->first_cycles());
         case fs_sim::G_ITER: return h->solve_iter(h->stream);
         case fs_sim::G_POST: enqueue_post(h, prof); break;
     }
@@ -308,8 +319,9 @@ int adaptive_step(fs_sim* h, bool prof) {
     const bool spec = !prof && h->diag_cap == 0;
     bool projected = false;
     int k = 0, rc = FS_OK;
+    const int f = h->first_cycles();
     if (spec && h->cfg.use_graph) {
-        const int kk = std::min(h->predict, maxc);
+        const int kk = std::min(std::max(h->predict, f), maxc);
         if (h->step_k != kk && h->gexec[fs_sim::G_STEP]) {
             (void)hipGraphExecDestroy(h->gexec[fs_sim::G_STEP]);
             (void)hipGraphDestroy(h->graph[fs_sim::G_STEP]);
@@ -330,7 +342,8 @@ int adaptive_step(fs_sim* h, bool prof) {
         k++;
     } else {
         if ((rc = run_part(h, fs_sim::G_PRE, prof))) return rc;
-        for (; k < std::min(h->predict, maxc); k++) if ((rc = run_part(h, fs_sim::G_ITER, prof))) return rc;
+        if ((rc = run_part(h, fs_sim::G_FIRST, prof))) return rc;
+        for (k = f; k < std::min(h->predict, maxc); k++) if ((rc = run_part(h, fs_sim::G_ITER, prof))) return rc;
     }
     while (true) {
         // the last residual's reduction (the refinement leaves it to the next iteration's first kernel)
@@ -559,6 +572,7 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
         h->mg.omega = cfg->omega > 0.f ? cfg->omega : 1.25f;
         h->mg.pre = cfg->pre_sweeps > 0 ? cfg->pre_sweeps : cfg->pre_sweeps < 0 ? 0 : 1;
         h->mg.post = cfg->post_sweeps > 0 ? cfg->post_sweeps : 2;
+        if (cfg->solve == FS_SOLVE_REFINE64) h->mg.first = cfg->first_cycles > 0 ? cfg->first_cycles : 3;
     }
     if (cfg->solve == FS_SOLVE_REFINE64 || cfg->solve == FS_SOLVE_TOL32) {
         rc = mg_set_direct(h->mg, cfg->direct_level, err);

@@ -71,6 +71,7 @@ struct RefineState {            // device-resident FP64 refinement bookkeeping
     int done;                   // 1 once rel <= tol or used == max
     int pending;                // partials not yet reduced: 0 none, 1 a residual after a V-cycle, 2 a starting residual
     int never;                  // always 0: the skip flag of launches that run unconditionally
+    int inc;                    // V-cycles applied since the pending residual's predecessor
 };
 
 // A pending residual reduction handed to the first smoothing pass of the next V-cycle, whose
@@ -101,6 +102,7 @@ struct GpuMG {
     bool fuse_prolong = true;    // prolongation folded into the first post-smoothing pass
     float omega = 1.f;           // over-relaxation of the smoother (all levels but the coarsest)
     int pre = 2, post = 2;       // sweeps before / after the coarse correction (all levels but the coarsest)
+    int first = 1;               // V-cycles of a step's first FP64 correction (FS_SOLVE_REFINE64)
     int direct = 0;              // > 0: level solved exactly by separable transforms (the cycle stops there)
     struct DirectSolve* ds = nullptr;
 };
@@ -139,8 +141,11 @@ void mg_free(GpuMG& mg);
 // With `red0_done`, level 0's first colour-0 pass has already been written (level 0 started
 // from p = 0 and the kernel that zeroed it did that pass).
 // With `fin`, the cycle's first kernel also runs that reduction.
+// With p0 / p0alt, level 0's iterate is in p0 (default mg.p[0]) and the post-smoothing runs on p0alt, which the
+// prolongation fills (a level-0 iterate that the cycle's caller treats as scratch); *p0out receives the array that
+// holds level 0's iterate after the cycle.
 bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd = nullptr, bool red0_done = false,
-               const FinArgs* fin = nullptr);
+               const FinArgs* fin = nullptr, float* p0 = nullptr, float* p0alt = nullptr, float** p0out = nullptr);
 // FP64 refinement: r = bkeep - A x; sets level-0 b = r, p = 0; reduces norms;
 // then up to max_cycles of {V-cycle, x += p, residual}; all device-side.
 void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
@@ -151,7 +156,8 @@ void mg_solve_tol32(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 // mg_refine_iter leaves its residual's reduction to the first kernel of the next iteration (or to
 // mg_state_ready): call mg_state_ready before the host reads the state
 void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
-void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
+// one FP64 correction x <- x + e, e from `cycles` FP32 V-cycles on A e = r (the first from e = 0), then the residual
+void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cycles = 1);
 // reduces a residual whose partials are still pending, with the solve's stopping test (a one-block
 // kernel that exits at once when none is)
 // With `host` (pinned host memory, device-accessible) the kernel also stores {used, done} there.

@@ -97,7 +97,7 @@ __device__ void finalize_sums(RefineState* st, const double* partials, int npart
         double r2 = 0.0, b2 = 0.0;
         for (int w = 0; w < FIN / 64; w++) { r2 += sr[w]; b2 += sb[w]; }
         const double rel = b2 > 0.0 ? sqrt(r2 / b2) : 0.0;
-        const int phase = init ? 0 : st->used + 1;     // V-cycles applied so far in this solve
+        const int phase = init ? 0 : st->used + max(st->inc, 1);   // V-cycles applied so far in this solve
         st->rel[phase] = rel;
         st->used = phase;
         st->bnorm2 = b2;
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
                                                 double h, float* __restrict__ rhs_out, float* __restrict__ p_out,
                                                 double* __restrict__ partials, RefineState* __restrict__ st, int init,
                                                 const int* __restrict__ skip, float hred = 0.f, float omega = 1.f,
-                                                int red_first = 0) {
+                                                int red_first = 0, int cycles = 1) {
     __shared__ double sr[RB / 64], sb[RB / 64];
     const int nh = (nx + 1) >> 1, rs = 2 * nh;
     const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -504,7 +504,10 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
         const int blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         partials[2 * blk] = ((sr[0] + sr[1]) + sr[2]) + sr[3];
         partials[2 * blk + 1] = ((sb[0] + sb[1]) + sb[2]) + sb[3];
-        if (blk == 0) st->pending = init ? 2 : 1;
+        if (blk == 0) {
+            st->pending = init ? 2 : 1;
+            st->inc = cycles;                          // the V-cycles this residual follows
+        }
     }
 }
 
@@ -865,14 +868,14 @@ void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, doub
     if (fin) k_finalize<<<1, FIN, 0, s>>>(fin->st, fin->partials, fin->npart, fin->tol, fin->max_cycles);
 }
 
-static void launch_restrict(GpuMG& mg, hipStream_t s, int l, const int* skip, bool red_first) {
+static void launch_restrict(GpuMG& mg, hipStream_t s, int l, const int* skip, bool red_first, const float* pl) {
     const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
-    k_restrict<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(mg.p[l], mg.b[l], nx, ny, nz, mg.h[l], mg.b[l + 1], mg.p[l + 1], skip,
+    k_restrict<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(pl, mg.b[l], nx, ny, nz, mg.h[l], mg.b[l + 1], mg.p[l + 1], skip,
                                                             mg.h[l + 1], mg.omega, red_first ? 1 : 0);
 }
-static void launch_prolong(GpuMG& mg, hipStream_t s, int l, const int* skip) {
+static void launch_prolong(GpuMG& mg, hipStream_t s, int l, const int* skip, float* pl) {
     const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
-    k_prolong<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(mg.p[l], mg.p[l + 1], nx, ny, nz, skip);
+    k_prolong<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(pl, mg.p[l + 1], nx, ny, nz, skip);
 }
 
 static void launch_tail(GpuMG& mg, hipStream_t s, const int* skip, bool red_done) {
@@ -910,8 +913,11 @@ static bool fuse_red_into(const GpuMG& mg, int lc, int T) {
 }
 
 bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd, bool red0_done,
-               const FinArgs* fin) {   // sim.js:588-599
+               const FinArgs* fin, float* p0, float* p0alt, float** p0out) {   // sim.js:588-599
     if (!skip) skip = &mg.st->never;
+    if (!p0) p0 = mg.p[0];
+    if (p0out) *p0out = p0;
+    auto lp = [&](int l) { return l == 0 ? p0 : mg.p[l]; };   // level l's iterate
     const int n = mg.nlev;
     const int D = mg.direct;                            // > 0: levels below D are replaced by its exact solve
     const int T = D > 0 ? n : mg.use_tail ? mg.tail_from : n;   // levels T.. run in the single-workgroup tail
@@ -920,15 +926,15 @@ bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd, bool red
         k_finalize<<<1, FIN, 0, s>>>(fin->st, fin->partials, fin->npart, fin->tol, fin->max_cycles);
         fin = nullptr;
     }
-    if (T == 0) {                                       // the whole hierarchy fits one workgroup's LDS
+    if (T == 0) {                                       // the whole hierarchy fits one workgroup's LDS (level 0 in mg.p[0])
         launch_tail(mg, s, skip, false);
         return false;
     }
     bool red_done = red0_done;
     for (int l = 0; l < last; l++) {
-        mg_smooth(mg, s, l, mg.pre, skip, nullptr, red_done, l == 0 ? fin : nullptr);
+        mg_smooth(mg, s, l, mg.pre, skip, nullptr, red_done, l == 0 ? fin : nullptr, lp(l));
         red_done = fuse_red_into(mg, l + 1, T);
-        launch_restrict(mg, s, l, skip, red_done);
+        launch_restrict(mg, s, l, skip, red_done, lp(l));
     }
     if (D > 0) direct_solve(mg.ds, s, mg.b[D], mg.p[D], skip);
     else if (T < n) launch_tail(mg, s, skip, red_done);
@@ -949,18 +955,19 @@ bool mg_vcycle(GpuMG& mg, hipStream_t s, const int* skip, double* xupd, bool red
             const int nx = mg.dims[l][0], ny = mg.dims[l][1], nz = mg.dims[l][2];
             // a refinement's level-0 correction is scratch: the post-smoothing runs on a second array, which
             // the prolongation fills with every child (the colour-1 pass then reads its own prolonged values)
-            float* const post = (l == 0 && xupd) ? mg.p0post : nullptr;
-            k_prolong_red<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(mg.p[l], mg.b[l], mg.p[l + 1], nx, ny, nz, mg.h[l],
+            float* const post = l == 0 ? p0alt : nullptr;
+            k_prolong_red<<<grid3(nx / 2, ny / 2, nz / 2), RB, 0, s>>>(lp(l), mg.b[l], mg.p[l + 1], nx, ny, nz, mg.h[l],
                                                                       mg.omega, skip, post);
             double* const xu = l == 0 ? xupd : nullptr;
-            float* const pw = post ? post : mg.p[l];
+            float* const pw = post ? post : lp(l);
+            if (post && p0out) *p0out = post;
             k_smooth<false><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(pw, mg.b[l], nx, ny, nz, mg.h[l], 1, skip,
                                                                        mg.post == 1 ? xu : nullptr, mg.omega,
                                                                        post ? nullptr : mg.p[l + 1], FinArgs{});
             if (mg.post > 1) mg_smooth(mg, s, l, mg.post - 1, skip, xu, false, nullptr, pw);
         } else {
-            launch_prolong(mg, s, l, skip);
-            mg_smooth(mg, s, l, mg.post, skip, l == 0 ? xupd : nullptr);
+            launch_prolong(mg, s, l, skip, lp(l));
+            mg_smooth(mg, s, l, mg.post, skip, l == 0 ? xupd : nullptr, false, nullptr, lp(l));
         }
     }
     return xupd != nullptr && last > 0;
@@ -985,18 +992,32 @@ void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
                                                                   mg.partials, mg.st, 1, &mg.st->never, mg.h[0], mg.omega, red ? 1 : 0);
 }
 
-void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
+// The correction's V-cycles iterate on e (the first from e = 0, whose first colour-0 pass the residual kernel
+// wrote); level 0's iterate moves between mg.p[0] and mg.p0post (each cycle's prolongation writes the other);
+// the last cycle's last pass adds e into the FP64 iterate.
+void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cycles) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
+    if (cycles < 1) cycles = 1;
     int* done = &mg.st->done;
     const bool red = fuse_red0(mg);
     const FinArgs fin{mg.st, mg.partials, mg.npart, max_cycles, tol};
-    if (!mg_vcycle(mg, s, done, mg.x64, red, &fin)) {
+    float* cur = mg.p[0];
+    bool fused = false;
+    for (int c = 0; c < cycles; c++) {
+        float* const alt = cur == mg.p[0] ? mg.p0post : mg.p[0];
+        float* out = cur;
+        fused = mg_vcycle(mg, s, done, c == cycles - 1 ? mg.x64 : nullptr, c == 0 && red, c == 0 ? &fin : nullptr, cur, alt,
+                          &out);
+        cur = out;
+    }
+    if (!fused) {
         const int n = (int)level_floats(mg.dims[0]);
-        k_update<<<(n + RB - 1) / RB, RB, 0, s>>>(mg.x64, mg.p[0], n, done);
+        k_update<<<(n + RB - 1) / RB, RB, 0, s>>>(mg.x64, cur, n, done);
     }
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
-                                                                  mg.partials, mg.st, 0, done, mg.h[0], mg.omega, red ? 1 : 0);
+                                                                  mg.partials, mg.st, 0, done, mg.h[0], mg.omega, red ? 1 : 0,
+                                                                  cycles);
 }
 
 void mg_state_ready(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int* host) {

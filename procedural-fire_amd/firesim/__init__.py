@@ -59,7 +59,8 @@ class FsConfig(ctypes.Structure):
         ("pre_sweeps", ctypes.c_int),
         ("post_sweeps", ctypes.c_int),
         ("direct_level", ctypes.c_int),
-        ("reserved", ctypes.c_int * 2),
+        ("first_cycles", ctypes.c_int),
+        ("reserved", ctypes.c_int),
     ]
 
 
@@ -253,7 +254,7 @@ class FireSim:
     def __init__(self, res=(80, 120, 80), size=(1.2, 1.8, 1.2), logs=None, vcycles=2, max_levels=0,
                  precision=PREC_FP16, solve=SOLVE_VCYCLES, div_tol=1e-5, max_vcycles=8, noise_n=128,
                  noise_seed=1, device=0, stream=None, use_graph=False, adaptive=True, omega=0.0, pre_sweeps=0,
-                 post_sweeps=0, direct_level=0):
+                 post_sweeps=0, direct_level=0, first_cycles=0):
         L = lib()
         cfg = FsConfig()
         L.fs_config_default(ctypes.byref(cfg))
@@ -275,6 +276,7 @@ class FireSim:
         cfg.pre_sweeps = int(pre_sweeps)
         cfg.post_sweeps = int(post_sweeps)
         cfg.direct_level = int(direct_level)
+        cfg.first_cycles = int(first_cycles)
         if logs is None:
             logs, _ = build_campfire(7)   # src/main.js:34-35: buildCampfire(7) packed to 16 slots
         logs = np.ascontiguousarray(logs, np.float32)

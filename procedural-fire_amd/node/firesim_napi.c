@@ -124,6 +124,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
             if (get_prop_num(env, argv[0], "preSweeps", &d)) cfg.pre_sweeps = (int)d;
             if (get_prop_num(env, argv[0], "postSweeps", &d)) cfg.post_sweeps = (int)d;
             if (get_prop_num(env, argv[0], "directLevel", &d)) cfg.direct_level = (int)d;
+            if (get_prop_num(env, argv[0], "firstCycles", &d)) cfg.first_cycles = (int)d;
             bool has = false;
             napi_has_named_property(env, argv[0], "logs", &has);
             if (has) {
