@@ -88,7 +88,7 @@ struct fs_sim {
     bool own_stream = false;
     // captured graphs: the whole step (fixed solves); the whole step of an adaptive solve with the number of
     // iterations the last step needed (step_k) and the readback of the solve's state; its three parts
-    enum { G_FULL, G_STEP, G_PRE, G_FIRST, G_ITER, G_POST, G_COUNT };
+    enum { G_FULL, G_STEP, G_PRE, G_FIRST, G_ITER, G_ITER_RHS, G_POST, G_COUNT };
     int step_k = 0;
     hipGraphExec_t gexec[G_COUNT] = {};
     hipGraph_t graph[G_COUNT] = {};
@@ -141,10 +141,11 @@ struct fs_sim {
     int first_cycles() const {
         return cfg.solve == FS_SOLVE_REFINE64 ? std::max(1, std::min(mg.first, std::min(cfg.max_vcycles, MG_MAX_REFINE))) : 1;
     }
-    // one iteration of the solve; a refinement's correction takes `cycles` V-cycles
-    int solve_iter(hipStream_t s, int cycles = 1) {
+    // one iteration of the solve; a refinement's correction takes `cycles` V-cycles (form_rhs / keep_rhs: see
+    // mg_refine_iter)
+    int solve_iter(hipStream_t s, int cycles = 1, bool form_rhs = false, bool keep_rhs = true) {
         if (cfg.solve == FS_SOLVE_SPECTRAL64) return mg_spectral_iter(mg, spec, s, cfg.div_tol, cfg.max_vcycles);
-        if (cfg.solve == FS_SOLVE_REFINE64) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles, cycles);
+        if (cfg.solve == FS_SOLVE_REFINE64) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles, cycles, form_rhs, keep_rhs);
         else mg_tol32_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
         return FS_OK;
     }
@@ -248,10 +249,12 @@ int enqueue_step(fs_sim* h, bool prof) {
 int enqueue_adaptive_step(fs_sim* h) {
     enqueue_pre(h, false, true);
     const int maxc = std::min(h->cfg.max_vcycles, MG_MAX_REFINE);
+    // the last correction's closing residual reduces only the norms: a further correction forms its own
+    // right-hand side (G_ITER_RHS), and the usual step stops there
     const int f = h->first_cycles();
-    if (int rc = h->solve_iter(h->stream, f)) return rc;
+    if (int rc = h->solve_iter(h->stream, f, false, h->step_k > f)) return rc;
     for (int k = f; k < h->step_k; k++)
-        if (int rc = h->solve_iter(h->stream)) return rc;
+        if (int rc = h->solve_iter(h->stream, 1, false, k + 1 < h->step_k)) return rc;
     // the stopping state goes to the host from the refinement's last reduction kernel, or by a copy
     if (h->cfg.solve == FS_SOLVE_REFINE64) mg_state_ready(h->mg, h->stream, h->cfg.div_tol, maxc, h->host_state_dev);
     else FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -268,6 +271,7 @@ int capture(fs_sim* h, int which) {
         case fs_sim::G_PRE: enqueue_pre(h, false, true); break;
         case fs_sim::G_FIRST: rc = h->solve_iter(h->stream, h->first_cycles()); break;
         case fs_sim::G_ITER: rc = h->solve_iter(h->stream); break;
+        case fs_sim::G_ITER_RHS: rc = h->solve_iter(h->stream, 1, true, false); break;
         case fs_sim::G_POST: enqueue_post(h, false); break;
     }
     hipGraph_t gr = nullptr;
@@ -294,6 +298,7 @@ int run_part(fs_sim* h, int which, bool prof) {
         case fs_sim::G_PRE: enqueue_pre(h, prof, true); break;
         case fs_sim::G_FIRST: return h->solve_iter(h->stream, h->first_cycles());
         case fs_sim::G_ITER: return h->solve_iter(h->stream);
+        case fs_sim::G_ITER_RHS: return h->solve_iter(h->stream, 1, true, false);
         case fs_sim::G_POST: enqueue_post(h, prof); break;
     }
     return FS_OK;
@@ -330,16 +335,20 @@ int adaptive_step(fs_sim* h, bool prof) {
         }
         h->step_k = kk;
         if ((rc = run_part(h, fs_sim::G_STEP, prof))) return rc;
-        FS_HIP_CHECK(hipEventRecord(h->ev_state, h->stream));
-        if ((rc = run_part(h, fs_sim::G_POST, prof))) return rc;
-        FS_HIP_CHECK(hipEventSynchronize(h->ev_state));
         k = kk;
-        if (h->host_state[1] || k >= maxc) {
-            h->predict = std::max(1, h->host_state[0]);
-            return FS_OK;
+        while (true) {   // the state after the step's corrections, the projection enqueued behind it
+            FS_HIP_CHECK(hipEventRecord(h->ev_state, h->stream));
+            if ((rc = run_part(h, fs_sim::G_POST, prof))) return rc;
+            FS_HIP_CHECK(hipEventSynchronize(h->ev_state));
+            if (h->host_state[1] || k >= maxc) break;
+            // one more correction (forming its right-hand side), its state, and the projection again
+            if ((rc = run_part(h, fs_sim::G_ITER_RHS, prof))) return rc;
+            k++;
+            if (h->cfg.solve == FS_SOLVE_REFINE64) mg_state_ready(h->mg, h->stream, h->cfg.div_tol, maxc, h->host_state_dev);
+            else FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
         }
-        if ((rc = run_part(h, fs_sim::G_ITER, prof))) return rc;
-        k++;
+        h->predict = std::max(1, h->host_state[0]);
+        return FS_OK;
     } else {
         if ((rc = run_part(h, fs_sim::G_PRE, prof))) return rc;
         if ((rc = run_part(h, fs_sim::G_FIRST, prof))) return rc;

@@ -156,8 +156,11 @@ void mg_solve_tol32(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 // mg_refine_iter leaves its residual's reduction to the first kernel of the next iteration (or to
 // mg_state_ready): call mg_state_ready before the host reads the state
 void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
-// one FP64 correction x <- x + e, e from `cycles` FP32 V-cycles on A e = r (the first from e = 0), then the residual
-void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cycles = 1);
+// one FP64 correction x <- x + e, e from `cycles` FP32 V-cycles on A e = r (the first from e = 0), then the residual;
+// form_rhs: r is formed first (the previous residual kept only its norms); keep_rhs: the closing residual writes
+// the next correction's right-hand side
+void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cycles = 1, bool form_rhs = false,
+                    bool keep_rhs = true);
 // reduces a residual whose partials are still pending, with the solve's stopping test (a one-block
 // kernel that exits at once when none is)
 // With `host` (pinned host memory, device-accessible) the kernel also stores {used, done} there.

@@ -496,6 +496,7 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
             }
         }
     }
+    if (!partials) return;                             // (only the right-hand side was asked for; uniform)
     rr = wave_sum(rr);
     bb = wave_sum(bb);
     if ((threadIdx.x & 63) == 0) { sr[threadIdx.x >> 6] = rr; sb[threadIdx.x >> 6] = bb; }
@@ -994,14 +995,20 @@ void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
 
 // The correction's V-cycles iterate on e (the first from e = 0, whose first colour-0 pass the residual kernel
 // wrote); level 0's iterate moves between mg.p[0] and mg.p0post (each cycle's prolongation writes the other);
-// the last cycle's last pass adds e into the FP64 iterate.
-void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cycles) {
+// the last cycle's last pass adds e into the FP64 iterate.  form_rhs: the correction's right-hand side is formed
+// first (the previous residual kernel only reduced the norms); keep_rhs: the closing residual kernel also writes
+// the next correction's right-hand side (otherwise only its norms, for a solve that usually stops there).
+void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cycles, bool form_rhs, bool keep_rhs) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     if (cycles < 1) cycles = 1;
     int* done = &mg.st->done;
     const bool red = fuse_red0(mg);
     const FinArgs fin{mg.st, mg.partials, mg.npart, max_cycles, tol};
+    if (form_rhs)   // r = b - A x as the V-cycles' right-hand side, e = 0 (first colour-0 pass written); no norms
+        k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0],
+                                                                      mg.p[0], nullptr, mg.st, 0, done, mg.h[0], mg.omega,
+                                                                      red ? 1 : 0);
     float* cur = mg.p[0];
     bool fused = false;
     for (int c = 0; c < cycles; c++) {
@@ -1015,7 +1022,8 @@ void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cy
         const int n = (int)level_floats(mg.dims[0]);
         k_update<<<(n + RB - 1) / RB, RB, 0, s>>>(mg.x64, cur, n, done);
     }
-    k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
+    k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0],
+                                                                  keep_rhs ? mg.b[0] : nullptr, keep_rhs ? mg.p[0] : nullptr,
                                                                   mg.partials, mg.st, 0, done, mg.h[0], mg.omega, red ? 1 : 0,
                                                                   cycles);
 }
