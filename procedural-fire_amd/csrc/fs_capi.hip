@@ -88,7 +88,10 @@ struct fs_sim {
     bool own_stream = false;
     // captured graphs: the whole step (fixed solves); the whole step of an adaptive solve with the number of
     // iterations the last step needed (step_k) and the readback of the solve's state; its three parts
-    enum { G_FULL, G_STEP, G_PRE, G_FIRST, G_ITER, G_ITER_RHS, G_POST, G_COUNT };
+    // The refinement's adaptive step closes its last correction with the projection itself (G_POSTR after a
+    // first correction, G_POSTR1 after a one-cycle one: the fused projection + residual norms and the stop state
+    // for the host); an extra correction forms its own right-hand side and leaves its residual to them (G_ITER_NC).
+    enum { G_FULL, G_STEP, G_PRE, G_FIRST, G_ITER, G_POST, G_POSTR, G_POSTR1, G_ITER_NC, G_COUNT };
     int step_k = 0;
     hipGraphExec_t gexec[G_COUNT] = {};
     hipGraph_t graph[G_COUNT] = {};
@@ -143,9 +146,10 @@ struct fs_sim {
     }
     // one iteration of the solve; a refinement's correction takes `cycles` V-cycles (form_rhs / keep_rhs: see
     // mg_refine_iter)
-    int solve_iter(hipStream_t s, int cycles = 1, bool form_rhs = false, bool keep_rhs = true) {
+    int solve_iter(hipStream_t s, int cycles = 1, bool form_rhs = false, bool keep_rhs = true, bool close = true) {
         if (cfg.solve == FS_SOLVE_SPECTRAL64) return mg_spectral_iter(mg, spec, s, cfg.div_tol, cfg.max_vcycles);
-        if (cfg.solve == FS_SOLVE_REFINE64) mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles, cycles, form_rhs, keep_rhs);
+        if (cfg.solve == FS_SOLVE_REFINE64)
+            mg_refine_iter(mg, s, cfg.div_tol, cfg.max_vcycles, cycles, form_rhs, keep_rhs, close);
         else mg_tol32_iter(mg, s, cfg.div_tol, cfg.max_vcycles);
         return FS_OK;
     }
@@ -248,17 +252,27 @@ int enqueue_step(fs_sim* h, bool prof) {
 // readback (the projection follows as its own part; the host checks the flag afterwards)
 int enqueue_adaptive_step(fs_sim* h) {
     enqueue_pre(h, false, true);
-    const int maxc = std::min(h->cfg.max_vcycles, MG_MAX_REFINE);
-    // the last correction's closing residual reduces only the norms: a further correction forms its own
-    // right-hand side (G_ITER_RHS), and the usual step stops there
     const int f = h->first_cycles();
-    if (int rc = h->solve_iter(h->stream, f, false, h->step_k > f)) return rc;
+    if (h->cfg.solve == FS_SOLVE_REFINE64) {
+        // the last correction leaves its closing residual to the projection (G_POSTR / G_POSTR1); a further
+        // correction forms its own right-hand side (G_ITER_NC), and the usual step stops there
+        if (int rc = h->solve_iter(h->stream, f, false, h->step_k > f, h->step_k > f)) return rc;
+        for (int k = f; k < h->step_k; k++)
+            if (int rc = h->solve_iter(h->stream, 1, false, k + 1 < h->step_k, k + 1 < h->step_k)) return rc;
+        return FS_OK;
+    }
+    if (int rc = h->solve_iter(h->stream, f)) return rc;
     for (int k = f; k < h->step_k; k++)
-        if (int rc = h->solve_iter(h->stream, 1, false, k + 1 < h->step_k)) return rc;
-    // the stopping state goes to the host from the refinement's last reduction kernel, or by a copy
-    if (h->cfg.solve == FS_SOLVE_REFINE64) mg_state_ready(h->mg, h->stream, h->cfg.div_tol, maxc, h->host_state_dev);
-    else FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        if (int rc = h->solve_iter(h->stream)) return rc;
+    // the stopping state goes to the host by a copy
+    FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     return FS_OK;
+}
+// the projection fused with the closing residual of the refinement's last correction (of `cycles` V-cycles),
+// then the stop state for the host
+void enqueue_post_resid(fs_sim* h, int cycles) {
+    launch_project_resid(h->prec, h->g, h->stream, h->P_d, h->fields(), h->mg, cycles);
+    mg_state_ready(h->mg, h->stream, h->cfg.div_tol, std::min(h->cfg.max_vcycles, MG_MAX_REFINE), h->host_state_dev);
 }
 
 int capture(fs_sim* h, int which) {
@@ -271,8 +285,10 @@ int capture(fs_sim* h, int which) {
         case fs_sim::G_PRE: enqueue_pre(h, false, true); break;
         case fs_sim::G_FIRST: rc = h->solve_iter(h->stream, h->first_cycles()); break;
         case fs_sim::G_ITER: rc = h->solve_iter(h->stream); break;
-        case fs_sim::G_ITER_RHS: rc = h->solve_iter(h->stream, 1, true, false); break;
         case fs_sim::G_POST: enqueue_post(h, false); break;
+        case fs_sim::G_POSTR: enqueue_post_resid(h, h->first_cycles()); break;
+        case fs_sim::G_POSTR1: enqueue_post_resid(h, 1); break;
+        case fs_sim::G_ITER_NC: rc = h->solve_iter(h->stream, 1, true, false, false); break;
     }
     hipGraph_t gr = nullptr;
     if (rc) {   // end the capture before reporting the failure
@@ -298,8 +314,10 @@ int run_part(fs_sim* h, int which, bool prof) {
         case fs_sim::G_PRE: enqueue_pre(h, prof, true); break;
         case fs_sim::G_FIRST: return h->solve_iter(h->stream, h->first_cycles());
         case fs_sim::G_ITER: return h->solve_iter(h->stream);
-        case fs_sim::G_ITER_RHS: return h->solve_iter(h->stream, 1, true, false);
         case fs_sim::G_POST: enqueue_post(h, prof); break;
+        case fs_sim::G_POSTR: enqueue_post_resid(h, h->first_cycles()); break;
+        case fs_sim::G_POSTR1: enqueue_post_resid(h, 1); break;
+        case fs_sim::G_ITER_NC: return h->solve_iter(h->stream, 1, true, false, false);
     }
     return FS_OK;
 }
@@ -317,8 +335,9 @@ void drop_graphs(fs_sim* h) {
 // graphs all of it is one captured graph (per iteration count).  If the solve had not stopped yet, the
 // host enqueues the further iterations and the projection again: the projection only reads the advected
 // velocity (which nothing between the two runs writes) and the pressure, so its second run gives the state
-// of the unspeculated order, bit for bit.  Not speculated: profiled steps and steps that append a per-step
-// diagnostics record (the projection part would record twice).
+// of the unspeculated order, bit for bit.  The FP64 refinement goes further: the projection itself forms the
+// closing residual's norms (it reads the same iterate values) and the stop state follows it.  Not speculated:
+// profiled steps and steps that append a per-step diagnostics record (the projection part would record twice).
 int adaptive_step(fs_sim* h, bool prof) {
     const int maxc = std::min(h->cfg.max_vcycles, MG_MAX_REFINE);
     const bool spec = !prof && h->diag_cap == 0;
@@ -336,16 +355,31 @@ int adaptive_step(fs_sim* h, bool prof) {
         h->step_k = kk;
         if ((rc = run_part(h, fs_sim::G_STEP, prof))) return rc;
         k = kk;
+        if (h->cfg.solve == FS_SOLVE_REFINE64) {
+            // the projection closes each correction with its residual's norms and the state for the host; a
+            // correction more (forming its right-hand side) and the projection again while the state says so
+            int part = kk > f ? fs_sim::G_POSTR1 : fs_sim::G_POSTR;
+            while (true) {
+                if ((rc = run_part(h, part, prof))) return rc;
+                FS_HIP_CHECK(hipEventRecord(h->ev_state, h->stream));
+                FS_HIP_CHECK(hipEventSynchronize(h->ev_state));
+                if (h->host_state[1] || k >= maxc) break;
+                if ((rc = run_part(h, fs_sim::G_ITER_NC, prof))) return rc;
+                k++;
+                part = fs_sim::G_POSTR1;
+            }
+            h->predict = std::max(1, h->host_state[0]);
+            return FS_OK;
+        }
         while (true) {   // the state after the step's corrections, the projection enqueued behind it
             FS_HIP_CHECK(hipEventRecord(h->ev_state, h->stream));
             if ((rc = run_part(h, fs_sim::G_POST, prof))) return rc;
             FS_HIP_CHECK(hipEventSynchronize(h->ev_state));
             if (h->host_state[1] || k >= maxc) break;
-            // one more correction (forming its right-hand side), its state, and the projection again
-            if ((rc = run_part(h, fs_sim::G_ITER_RHS, prof))) return rc;
+            // one more iteration, its state, and the projection again
+            if ((rc = run_part(h, fs_sim::G_ITER, prof))) return rc;
             k++;
-            if (h->cfg.solve == FS_SOLVE_REFINE64) mg_state_ready(h->mg, h->stream, h->cfg.div_tol, maxc, h->host_state_dev);
-            else FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+            FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
         }
         h->predict = std::max(1, h->host_state[0]);
         return FS_OK;

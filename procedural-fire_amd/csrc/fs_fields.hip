@@ -13,6 +13,7 @@
 #include "fs_internal.h"
 #include "fs_math.h"
 #include "fs_forces_cell.h"
+#include "fs_resid.h"
 
 
 
@@ -270,6 +271,29 @@ __global__ __launch_bounds__(256) void k_div(DevParams* __restrict__ P, Dims g, 
 }
 
 // ---- projection (src/sim.js:532-550) ------------------------------------------
+// One cell's projected velocity from the pressure at the cell (me) and at its six neighbours (a neighbour
+// outside the grid is passed but unused): the ghost rules are -p outside x, z and above the top (Dirichlet),
+// +p below the floor (Neumann).
+__device__ __forceinline__ float4 project_cell(const Dims& g, int x, int y, int z, float dx, float4 vi, float solid, float me,
+                                              float vxp, float vxm, float vyp, float vym, float vzp, float vzm) {
+    const float pxp = x + 1 < g.nx ? vxp : -me;
+    const float pxm = x > 0 ? vxm : -me;
+    const float pyp = y + 1 < g.ny ? vyp : -me;
+    const float pym = y > 0 ? vym : me;
+    const float pzp = z + 1 < g.nz ? vzp : -me;
+    const float pzm = z > 0 ? vzm : -me;
+    const float gx = (pxp - pxm) / (2.f * dx), gy = (pyp - pym) / (2.f * dx), gz = (pzp - pzm) / (2.f * dx);
+    float vx = vi.x - gx, vy = vi.y - gy, vz = vi.z - gz;
+    if (solid > 0.5f) { vx = 0.f; vy = 0.f; vz = 0.f; }
+    if (y == 0) vy = fmaxf(vy, 0.f);
+    const float vl = sqrtf(vx * vx + vy * vy + vz * vz);
+    if (vl > 20.f || vl != vl) {
+        if (vl != vl) { vx = 0.f; vy = 0.f; vz = 0.f; }
+        else { vx = vx * 20.f / vl; vy = vy * 20.f / vl; vz = vz * 20.f / vl; }
+    }
+    return make_float4(vx, vy, vz, 0.f);
+}
+
 template <class S, class PT>
 __global__ __launch_bounds__(256) void k_project(const DevParams* __restrict__ P, Dims g, const typename S::raw* __restrict__ vIn,
                                                  const PT* __restrict__ p, const typename S::raw* __restrict__ geo,
@@ -289,23 +313,79 @@ __global__ __launch_bounds__(256) void k_project(const DevParams* __restrict__ P
     const float vzm = (float)p[fs_opaque(c.z > 0 ? rb_index(c.x, c.y, c.z - 1, g.nx, g.ny) : io)];
     const float4 vi = Codec<S>::ld(vIn, c.i);
     const float solid = Codec<S>::ld(geo, c.i).z;
-    // ghost rules: -p outside x, z and above the top (Dirichlet), +p below the floor (Neumann)
-    const float pxp = c.x + 1 < g.nx ? vxp : -me;
-    const float pxm = c.x > 0 ? vxm : -me;
-    const float pyp = c.y + 1 < g.ny ? vyp : -me;
-    const float pym = c.y > 0 ? vym : me;
-    const float pzp = c.z + 1 < g.nz ? vzp : -me;
-    const float pzm = c.z > 0 ? vzm : -me;
-    const float gx = (pxp - pxm) / (2.f * dx), gy = (pyp - pym) / (2.f * dx), gz = (pzp - pzm) / (2.f * dx);
-    float vx = vi.x - gx, vy = vi.y - gy, vz = vi.z - gz;
-    if (solid > 0.5f) { vx = 0.f; vy = 0.f; vz = 0.f; }
-    if (c.y == 0) vy = fmaxf(vy, 0.f);
-    const float vl = sqrtf(vx * vx + vy * vy + vz * vz);
-    if (vl > 20.f || vl != vl) {
-        if (vl != vl) { vx = 0.f; vy = 0.f; vz = 0.f; }
-        else { vx = vx * 20.f / vl; vy = vy * 20.f / vl; vz = vz * 20.f / vl; }
+    Codec<S>::st(vOut, c.i, project_cell(g, c.x, c.y, c.z, dx, vi, solid, me, vxp, vxm, vyp, vym, vzp, vzm));
+}
+
+// The projection with the FP64 iterate x as the pressure, fused with the refinement's closing residual
+// r = b - A x, which reads the same seven values of x around each cell.  The threads, blocks and operations
+// of the residual part are the multigrid residual kernel's (k_resid64: a thread takes the parity-split pair
+// x = 2 hx, 2 hx + 1 of row (y, z), blocks of 64 x 4 pairs, the same cell function and the same per-block
+// (r.r, b.b) partials with block 0 marking them pending), so the partial sums are the same bits; the
+// projection of each cell is k_project's.  With the solve's stop flag set (an earlier residual met the
+// tolerance) only the projection runs.
+template <class S>
+__global__ __launch_bounds__(256) void k_project_resid(const DevParams* __restrict__ P, Dims g,
+                                                       const typename S::raw* __restrict__ vIn, const double* __restrict__ x,
+                                                       const float* __restrict__ b, const typename S::raw* __restrict__ geo,
+                                                       typename S::raw* __restrict__ vOut, double* __restrict__ partials,
+                                                       RefineState* __restrict__ st, const int* __restrict__ skip, double h,
+                                                       int cycles) {
+    __shared__ double sr[4], sb[4];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int nh = (nx + 1) >> 1, rs = 2 * nh;
+    const int hx = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    const double ih2 = 1.0 / (h * h);
+    const float dx = P->f[P_dx];
+    double rr = 0.0, bb = 0.0;
+    if (hx < nh && y < ny) {
+        const int row = (z * ny + y) * rs;
+#pragma unroll
+        for (int sgn = 0; sgn < 2; sgn++) {
+            const int xx = 2 * hx + sgn;
+            if (xx >= nx) continue;                    // the odd slot of a padded row
+            const int own = row + sgn * nh + hx, oth = row + (1 - sgn) * nh;
+            const double me = x[own];
+            const double vw = x[fs_opaque(xx > 0 ? oth + hx - 1 + sgn : own)];
+            const double ve = x[fs_opaque(xx < nx - 1 ? oth + hx + sgn : own)];
+            const double vk = x[fs_opaque(z > 0 ? own - ny * rs : own)];
+            const double vf = x[fs_opaque(z < nz - 1 ? own + ny * rs : own)];
+            const double vs = x[fs_opaque(y > 0 ? own - rs : own)];
+            const double vn = x[fs_opaque(y < ny - 1 ? own + rs : own)];
+            const double bv = (double)b[own];
+            const int ci = (z * ny + y) * nx + xx;
+            const float4 vi = Codec<S>::ld(vIn, ci);
+            const float solid = Codec<S>::ld(geo, ci).z;
+            // the residual (resid_at's operations)
+            double sum = 0.0, diag = 6.0;
+            if (xx > 0) sum += vw; else diag += 1.0;
+            if (xx < nx - 1) sum += ve; else diag += 1.0;
+            if (z > 0) sum += vk; else diag += 1.0;
+            if (z < nz - 1) sum += vf; else diag += 1.0;
+            if (y > 0) sum += vs; else diag -= 1.0;
+            if (y < ny - 1) sum += vn; else diag += 1.0;
+            const double r = bv - (sum - diag * me) * ih2;
+            rr += r * r;
+            bb += bv * bv;
+            Codec<S>::st(vOut, ci, project_cell(g, xx, y, z, dx, vi, solid, (float)me, (float)ve, (float)vw, (float)vn, (float)vs,
+                                                (float)vf, (float)vk));
+        }
     }
-    Codec<S>::st(vOut, c.i, make_float4(vx, vy, vz, 0.f));
+    if (skip[0]) return;                               // uniform over the grid
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    if ((threadIdx.x & 63) == 0) { sr[threadIdx.x >> 6] = rr; sb[threadIdx.x >> 6] = bb; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[2 * blk] = ((sr[0] + sr[1]) + sr[2]) + sr[3];
+        partials[2 * blk + 1] = ((sb[0] + sb[1]) + sb[2]) + sb[3];
+        if (blk == 0) {
+            st->pending = 1;
+            st->inc = cycles;                          // the V-cycles this residual follows
+        }
+    }
 }
 
 // ---- init-time noise and curl textures (src/noise.js:8-54, 69-80) ---------------
@@ -533,6 +613,15 @@ template <class S> static void project_t(const Dims& g, hipStream_t s, const Dev
 void launch_project(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, const float* p32,
                     const double* p64) {
     DISPATCH_PREC(prec, project_t<S>(g, s, P, F, p32, p64));
+}
+template <class S> static void project_resid_t(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, GpuMG& mg,
+                                               int cycles) {
+    const dim3 grid((unsigned)(((g.nx + 1) / 2 + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nz);
+    k_project_resid<S><<<grid, 256, 0, s>>>(P, g, CRAW(F.velX), mg.x64, mg.bkeep, CRAW(F.geo), RAW(F.vel), mg.partials, mg.st,
+                                            &mg.st->done, (double)mg.h[0], cycles);
+}
+void launch_project_resid(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, GpuMG& mg, int cycles) {
+    DISPATCH_PREC(prec, project_resid_t<S>(g, s, P, F, mg, cycles));
 }
 void launch_noise_bake(hipStream_t s, half4_t* noise, half4_t* curl, int n, int seed) {
     const int cells = n * n * n;

@@ -158,9 +158,10 @@ void mg_solve_tol32(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 // one FP64 correction x <- x + e, e from `cycles` FP32 V-cycles on A e = r (the first from e = 0), then the residual;
 // form_rhs: r is formed first (the previous residual kept only its norms); keep_rhs: the closing residual writes
-// the next correction's right-hand side
+// the next correction's right-hand side; close = false: no closing residual (the caller forms its norms, e.g. by
+// launch_project_resid)
 void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cycles = 1, bool form_rhs = false,
-                    bool keep_rhs = true);
+                    bool keep_rhs = true, bool close = true);
 // reduces a residual whose partials are still pending, with the solve's stopping test (a one-block
 // kernel that exits at once when none is)
 // With `host` (pinned host memory, device-accessible) the kernel also stores {used, done} there.
@@ -223,6 +224,9 @@ void launch_forces(int prec, const Dims& g, hipStream_t s, const DevParams* P, c
 void launch_div(int prec, const Dims& g, hipStream_t s, DevParams* P, const FieldSet& F, float* b_out, int advance_clock);
 void launch_project(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F,
                     const float* p32, const double* p64);
+// the projection with the FP64 refinement's iterate, fused with the closing residual's norms (mg.partials,
+// pending for mg_state_ready; `cycles`: the V-cycles of the correction it closes)
+void launch_project_resid(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, GpuMG& mg, int cycles);
 void launch_noise_bake(hipStream_t s, half4_t* noise, half4_t* curl, int n, int seed);
 void launch_set_params(hipStream_t s, DevParams* dst, const DevParams& src);
 void launch_fill_zero(hipStream_t s, void* ptr, size_t bytes);

@@ -997,8 +997,10 @@ void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
 // wrote); level 0's iterate moves between mg.p[0] and mg.p0post (each cycle's prolongation writes the other);
 // the last cycle's last pass adds e into the FP64 iterate.  form_rhs: the correction's right-hand side is formed
 // first (the previous residual kernel only reduced the norms); keep_rhs: the closing residual kernel also writes
-// the next correction's right-hand side (otherwise only its norms, for a solve that usually stops there).
-void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cycles, bool form_rhs, bool keep_rhs) {
+// the next correction's right-hand side (otherwise only its norms, for a solve that usually stops there);
+// close = false: no closing residual here.
+void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cycles, bool form_rhs, bool keep_rhs,
+                    bool close) {
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     if (cycles < 1) cycles = 1;
@@ -1022,6 +1024,7 @@ void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cy
         const int n = (int)level_floats(mg.dims[0]);
         k_update<<<(n + RB - 1) / RB, RB, 0, s>>>(mg.x64, cur, n, done);
     }
+    if (!close) return;
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0],
                                                                   keep_rhs ? mg.b[0] : nullptr, keep_rhs ? mg.p[0] : nullptr,
                                                                   mg.partials, mg.st, 0, done, mg.h[0], mg.omega, red ? 1 : 0,
