@@ -88,10 +88,10 @@ struct fs_sim {
     bool own_stream = false;
     // captured graphs: the whole step (fixed solves); the whole step of an adaptive solve with the number of
     // iterations the last step needed (step_k) and the readback of the solve's state; its three parts
-    // The refinement's adaptive step closes its last correction with the projection itself (G_POSTR after a
-    // first correction, G_POSTR1 after a one-cycle one: the fused projection + residual norms and the stop state
-    // for the host); an extra correction forms its own right-hand side and leaves its residual to them (G_ITER_NC).
-    enum { G_FULL, G_STEP, G_PRE, G_FIRST, G_ITER, G_POST, G_POSTR, G_POSTR1, G_ITER_NC, G_COUNT };
+    // The refinement's adaptive step graph closes its last correction with the projection itself (the fused
+    // projection + residual norms, then the stop state for the host); an extra correction forms its own
+    // right-hand side (G_ITER_NC) and is closed the same way (G_POSTR1).
+    enum { G_FULL, G_STEP, G_PRE, G_FIRST, G_ITER, G_POST, G_POSTR1, G_ITER_NC, G_COUNT };
     int step_k = 0;
     hipGraphExec_t gexec[G_COUNT] = {};
     hipGraph_t graph[G_COUNT] = {};
@@ -248,17 +248,26 @@ int enqueue_step(fs_sim* h, bool prof) {
     return FS_OK;
 }
 
+// the projection fused with the closing residual of the refinement's last correction (of `cycles` V-cycles),
+// then the stop state for the host
+void enqueue_post_resid(fs_sim* h, int cycles) {
+    launch_project_resid(h->prec, h->g, h->stream, h->P_d, h->fields(), h->mg, cycles);
+    mg_state_ready(h->mg, h->stream, h->cfg.div_tol, std::min(h->cfg.max_vcycles, MG_MAX_REFINE), h->host_state_dev);
+}
 // an adaptive step with step_k V-cycles (the first correction's, then one per correction) and the stop-flag
-// readback (the projection follows as its own part; the host checks the flag afterwards)
+// readback (the refinement's step ends with its projection; the other solves' projection follows as its own
+// part; the host checks the flag afterwards)
 int enqueue_adaptive_step(fs_sim* h) {
     enqueue_pre(h, false, true);
     const int f = h->first_cycles();
     if (h->cfg.solve == FS_SOLVE_REFINE64) {
-        // the last correction leaves its closing residual to the projection (G_POSTR / G_POSTR1); a further
-        // correction forms its own right-hand side (G_ITER_NC), and the usual step stops there
+        // the last correction's closing residual is formed by the projection, which ends the graph with the
+        // stop state for the host; a further correction forms its own right-hand side (G_ITER_NC) and is
+        // projected again (G_POSTR1), and the usual step stops here
         if (int rc = h->solve_iter(h->stream, f, false, h->step_k > f, h->step_k > f)) return rc;
         for (int k = f; k < h->step_k; k++)
             if (int rc = h->solve_iter(h->stream, 1, false, k + 1 < h->step_k, k + 1 < h->step_k)) return rc;
+        enqueue_post_resid(h, h->step_k > f ? 1 : f);
         return FS_OK;
     }
     if (int rc = h->solve_iter(h->stream, f)) return rc;
@@ -267,12 +276,6 @@ int enqueue_adaptive_step(fs_sim* h) {
     // the stopping state goes to the host by a copy
     FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     return FS_OK;
-}
-// the projection fused with the closing residual of the refinement's last correction (of `cycles` V-cycles),
-// then the stop state for the host
-void enqueue_post_resid(fs_sim* h, int cycles) {
-    launch_project_resid(h->prec, h->g, h->stream, h->P_d, h->fields(), h->mg, cycles);
-    mg_state_ready(h->mg, h->stream, h->cfg.div_tol, std::min(h->cfg.max_vcycles, MG_MAX_REFINE), h->host_state_dev);
 }
 
 int capture(fs_sim* h, int which) {
@@ -286,7 +289,6 @@ int capture(fs_sim* h, int which) {
         case fs_sim::G_FIRST: rc = h->solve_iter(h->stream, h->first_cycles()); break;
         case fs_sim::G_ITER: rc = h->solve_iter(h->stream); break;
         case fs_sim::G_POST: enqueue_post(h, false); break;
-        case fs_sim::G_POSTR: enqueue_post_resid(h, h->first_cycles()); break;
         case fs_sim::G_POSTR1: enqueue_post_resid(h, 1); break;
         case fs_sim::G_ITER_NC: rc = h->solve_iter(h->stream, 1, true, false, false); break;
     }
@@ -315,7 +317,6 @@ int run_part(fs_sim* h, int which, bool prof) {
         case fs_sim::G_FIRST: return h->solve_iter(h->stream, h->first_cycles());
         case fs_sim::G_ITER: return h->solve_iter(h->stream);
         case fs_sim::G_POST: enqueue_post(h, prof); break;
-        case fs_sim::G_POSTR: enqueue_post_resid(h, h->first_cycles()); break;
         case fs_sim::G_POSTR1: enqueue_post_resid(h, 1); break;
         case fs_sim::G_ITER_NC: return h->solve_iter(h->stream, 1, true, false, false);
     }
@@ -356,17 +357,16 @@ int adaptive_step(fs_sim* h, bool prof) {
         if ((rc = run_part(h, fs_sim::G_STEP, prof))) return rc;
         k = kk;
         if (h->cfg.solve == FS_SOLVE_REFINE64) {
-            // the projection closes each correction with its residual's norms and the state for the host; a
-            // correction more (forming its right-hand side) and the projection again while the state says so
-            int part = kk > f ? fs_sim::G_POSTR1 : fs_sim::G_POSTR;
+            // the step graph ended with the projection, which closes the last correction with its residual's
+            // norms, and the state for the host; a correction more (forming its right-hand side) and the
+            // projection again while the state says so
             while (true) {
-                if ((rc = run_part(h, part, prof))) return rc;
                 FS_HIP_CHECK(hipEventRecord(h->ev_state, h->stream));
                 FS_HIP_CHECK(hipEventSynchronize(h->ev_state));
                 if (h->host_state[1] || k >= maxc) break;
                 if ((rc = run_part(h, fs_sim::G_ITER_NC, prof))) return rc;
+                if ((rc = run_part(h, fs_sim::G_POSTR1, prof))) return rc;
                 k++;
-                part = fs_sim::G_POSTR1;
             }
             h->predict = std::max(1, h->host_state[0]);
             return FS_OK;

@@ -183,19 +183,20 @@ def test_adaptive_and_async_solves_agree(solve, first, tol):
     (max_vcycles iterations, early exit on the device flag) give identical bits, graph or not.  The
     graph-replayed adaptive refinement closes its last correction inside the projection kernel; the
     tighter tolerances and one-cycle first corrections make steps take several corrections, so the
-    extra-correction parts (right-hand side formed first, residual left to the projection) run too."""
+    extra-correction parts (right-hand side formed first, residual left to the projection) run too.  Steps
+    go three to a call, so the next step's first advection pass runs ahead of the stopping test."""
     print(f"adaptive vs async solve {solve} first {first} tol {tol}", flush=True)
     sims = [F.FireSim(res=TEST_RES, size=TEST_SIZE, noise_n=32, precision=F.PREC_FP16, solve=solve, div_tol=tol,
                       max_vcycles=10, use_graph=g, adaptive=a, first_cycles=first)
             for a, g in ((True, True), (False, True), (True, False))]
     used = []
-    for _ in range(12):
+    for _ in range(5):
         for s in sims:
-            s.step(DT, 1)
+            s.step(DT, 3)
         r = [s.residual() for s in sims]
         assert r[0] == r[1] == r[2], r
         used.append(r[0][1])
-    print(f"  V-cycles per step {used}", flush=True)
+    print(f"  V-cycles of each call's last step {used}", flush=True)
     if first == 1:
         assert min(used) >= 2                    # several one-cycle corrections per step
     for f in (F.VEL, F.S, F.S2, F.P):
