@@ -46,19 +46,47 @@ def kernel_bytes(E, p_bytes=4, pack=4):
             "project": 3 * E + p_bytes}
 
 
-def mg_bytes_per_cycle(levels, coarse_iters, sweeps=4):
-    """Traffic of one V-cycle per fine cell: `sweeps` red-black sweeps x 12 B, restriction 9 B and
-    prolongation 8.5 B per cell of each non-coarsest level; 12 B per sweep on the coarsest
-    (src/sim.js:588-599).  V(2,2) has 4 sweeps per level, V(1,2) 3."""
+def vcycle_bytes(levels, pre, post, sor, zero_start0, direct=0, coarse_iters=0):
+    """Algorithmic bytes of one V(pre, post) cycle per fine cell, kernel by kernel as they run (src/sim.js:588-599;
+    float32 parity-split level arrays), per cell of each smoothed level:
+      a colour pass reads the other colour's p and its own b, writes its own p: 6 B (8 B with over-relaxation,
+        which also reads the own p); a level whose iterate starts at zero has its first colour-0 pass written by
+        the kernel that zeroes it (restriction or FP64 residual), so one pre pass less;
+      restriction: p and b at every fine cell read, the coarse b and p written: 9 B;
+      prolongation fused with the first colour-0 post pass: p, colour-0 b and the coarse p read, every child
+        written: 10.5 B;
+    the coarsest level `coarse_iters` plain sweeps of 12 B, or with direct = l the exact solve of level l: three
+    transform launches reading and writing one array each, 24 B per level-l cell."""
     n0 = float(np.prod(levels[0]))
+    P = 8.0 if sor else 6.0
+    last = direct if direct else len(levels) - 1
     tot = 0.0
-    for i, d in enumerate(levels):
-        w = np.prod(d) / n0
-        tot += w * (12.0 * coarse_iters if i == len(levels) - 1 else 12.0 * sweeps + 17.5)
+    for l in range(last):
+        w = np.prod(levels[l]) / n0
+        pre_passes = 2 * pre - (1 if (l > 0 or zero_start0) else 0)
+        tot += w * (pre_passes * P + 9.0 + 10.5 + (2 * post - 1) * P)
+    if direct:
+        tot += 24.0 * np.prod(levels[direct]) / n0
+    else:
+        tot += 12.0 * coarse_iters * np.prod(levels[last]) / n0
     return tot
 
 
-REFINE_BYTES = 40.0   # per fine cell per refinement iteration: residual (x8 b4 r4 p4) + update (x8 e4 x8)
+# FP64 refinement per fine cell: the residual (x 8 and b 4 read, right-hand side and first colour-0 pass written)
+# and the update in the last colour-1 pass (x read and written)
+RESID64_BYTES = 20.0
+UPDATE64_BYTES = 16.0
+
+
+def refine_solve_bytes(levels, nv, first, direct=1, pre=1, post=2):
+    """Algorithmic bytes per fine cell of the mixed step's pressure solve: FP64 refinement whose first correction
+    takes `first` FP32 V(pre, post) over-relaxed cycles with the exact level-`direct` solve (the later ones one each),
+    nv cycles in all, as the profiled (unspeculated) schedule runs it: the starting residual, the cycles (the 2nd
+    and later of a correction start from a nonzero iterate: one colour-0 pass more), and per correction the update
+    and the closing residual."""
+    cyc = vcycle_bytes(levels, pre, post, True, True, direct=direct)
+    ncorr = 1.0 + max(0.0, nv - first)
+    return RESID64_BYTES + nv * cyc + 8.0 * (min(nv, first) - 1) + ncorr * (UPDATE64_BYTES + RESID64_BYTES)
 
 
 def measured_traffic(kernel):
@@ -251,7 +279,8 @@ def poisson_bench(args):
                 ts.append(t)
         cyc = len(hist) - 1
         ms = float(np.median(ts)) * 1e3
-        alg = (mg_bytes_per_cycle(ldims, coarse) + REFINE_BYTES) * cyc * n + 20.0 * n
+        alg = ((vcycle_bytes(ldims, 2, 2, omega != 1.0, True, coarse_iters=coarse) + UPDATE64_BYTES + RESID64_BYTES) * cyc
+               + RESID64_BYTES) * n
         out[f"omega_{omega}"] = {"ms_per_solve": ms, "vcycles": cyc, "ms_per_vcycle": ms / cyc, "history": list(hist),
                                  "achieved_GBs": alg / (ms * 1e-3) / 1e9, "levels": levels, "coarse_iters": coarse}
         print(f"omega {omega}: {cyc} cycles, {ms:.3f} ms, final {hist[-1]:.2e}", flush=True)
@@ -840,11 +869,12 @@ def main():
     dom = max(kb, key=lambda k: profm[k])
     dom_ms = profm[dom]
     achieved = kb[dom] * ncell / (dom_ms * 1e-3) / 1e9
-    sw_tol = sum(SWEEPS) if all(SWEEPS) else 3                # the tolerance solves' sweeps per level (default V(1,2))
-    mgc = mg_bytes_per_cycle(levels, coarse, sw_tol)
-    mg_alg = (mgc + REFINE_BYTES) * nv * ncell + 20.0 * ncell
+    # the mixed solve: V(1,2) cycles (or --sweeps) with level 1 solved exactly (the library's choice for this
+    # grid), the first correction 3 cycles
+    mg_cell = refine_solve_bytes(levels, nv, 3, pre=SWEEPS[0] if all(SWEEPS) else 1, post=SWEEPS[1] if all(SWEEPS) else 2)
+    mg_alg = mg_cell * ncell
     mg_gbs = mg_alg / (profm["mg"] * 1e-3) / 1e9
-    mg32_gbs = mg_bytes_per_cycle(levels, coarse, 4) * 2 * ncell / (prof32["mg"] * 1e-3) / 1e9
+    mg32_gbs = 2 * vcycle_bytes(levels, 2, 2, False, False, coarse_iters=coarse) * ncell / (prof32["mg"] * 1e-3) / 1e9
     out = {
         "metric": "sim voxel-steps/s @128x192x128 (mixed vs FP32) + MG HBM GB/s % of peak",
         "value": vmix,
@@ -862,8 +892,10 @@ def main():
         "config": {"workload": "fire step 128x192x128 dx=5mm dt=1ms vortEps=0.3 "
                                + ("8 spots (seed 7)" if SCENE != "campfire" else "campfire logs (seed 7)"),
                    "grid": list(RES),
-                   "solve_mixed": "FP32 V(1,2) cycles (red-black SOR, omega 1.25) with an exact separable "
-                                  "solve of level 1 (DST-II/DCT-IV transforms) in FP64 defect correction, tol 1e-5",
+                   "solve_mixed": "FP64 defect correction to 1e-5: the first correction 3 FP32 V(1,2) cycles "
+                                  "(red-black SOR, omega 1.25; level 1 solved exactly by DST-II/DCT-IV transforms on "
+                                  "the matrix cores), later ones 1; the last correction's residual norms formed by the "
+                                  "projection kernel; one graph per step",
                    "solve_fp32": "2 x V(2,2) FP32", "solve_fp32_tol": "FP32 V(1,2) (omega 1.25) until 1e-5 (cap 8)",
                    "graph": graph},
         "fp32_value": v32,
@@ -883,7 +915,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "alg_bytes_per_launch": kb[dom] * ncell, "avg_launch_ms": dom_ms},
         "mg": {"achieved_GBs": mg_gbs, "frac": mg_gbs / HBM_PEAK_GBS, "ms_per_step": profm["mg"],
-               "alg_bytes_per_vcycle_per_cell": mgc, "fp32_achieved_GBs": mg32_gbs},
+               "alg_bytes_per_cell": mg_cell, "fp32_achieved_GBs": mg32_gbs},
         "kernel_ms_mixed": profm,
         "kernel_ms_fp32": prof32,
         "kernel_ms_fp32_tol": runs["fp32_tol"]["prof"],

@@ -345,7 +345,8 @@ int adaptive_step(fs_sim* h, bool prof) {
     bool projected = false;
     int k = 0, rc = FS_OK;
     const int f = h->first_cycles();
-    if (spec && h->cfg.use_graph) {
+    // the refinement's form (projection fused with the closing residual) also runs eagerly, one launch at a time
+    if (spec && (h->cfg.use_graph || h->cfg.solve == FS_SOLVE_REFINE64)) {
         const int kk = std::min(std::max(h->predict, f), maxc);
         if (h->step_k != kk && h->gexec[fs_sim::G_STEP]) {
             (void)hipGraphExecDestroy(h->gexec[fs_sim::G_STEP]);

@@ -41,6 +41,11 @@ template <> struct Codec<StF16> {
     static __device__ __forceinline__ float4 ldb(__amdgpu_buffer_rsrc_t r, int off) {
         return dec(__builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
     }
+    static __device__ __forceinline__ u32x2_t enc(float4 v) {   // the texel st writes, as two words
+        half4_t h;
+        h.x = (_Float16)v.x; h.y = (_Float16)v.y; h.z = (_Float16)v.z; h.w = (_Float16)v.w;
+        return __builtin_bit_cast(u32x2_t, h);
+    }
     static __device__ __forceinline__ void st(half4_t* p, int i, float4 v) {
         half4_t h;
         h.x = (_Float16)v.x; h.y = (_Float16)v.y; h.z = (_Float16)v.z; h.w = (_Float16)v.w;
@@ -59,6 +64,11 @@ template <> struct Codec<StBF16> {
     }
     static __device__ __forceinline__ float4 ldb(__amdgpu_buffer_rsrc_t r, int off) {
         return dec(__builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    }
+    static __device__ __forceinline__ u32x2_t enc(float4 v) {   // the texel st writes, as two words
+        ushort4_t u;
+        u.x = f2bf(v.x); u.y = f2bf(v.y); u.z = f2bf(v.z); u.w = f2bf(v.w);
+        return __builtin_bit_cast(u32x2_t, u);
     }
     static __device__ __forceinline__ void st(ushort4_t* p, int i, float4 v) {
         ushort4_t u;

@@ -341,6 +341,28 @@ __global__ __launch_bounds__(256) void k_project_resid(const DevParams* __restri
     double rr = 0.0, bb = 0.0;
     if (hx < nh && y < ny) {
         const int row = (z * ny + y) * rs;
+        const int ci0 = (z * ny + y) * nx + 2 * hx;    // the pair's first cell in the fields
+        // 8-byte texels of an even-width grid: the pair's two texels are one aligned 16-byte word in each field
+        constexpr bool pack = sizeof(typename S::raw) == 8;
+        const bool paired = pack && (nx & 1) == 0;
+        float4 vi[2], po[2];
+        float solid[2];
+        if constexpr (pack) {
+            if (paired) {
+                const u32x4_t a = *reinterpret_cast<const u32x4_t*>(vIn + ci0);
+                const u32x4_t q = *reinterpret_cast<const u32x4_t*>(geo + ci0);
+                vi[0] = Codec<S>::dec(a.xy); vi[1] = Codec<S>::dec(a.zw);
+                solid[0] = Codec<S>::dec(q.xy).z; solid[1] = Codec<S>::dec(q.zw).z;
+            }
+        }
+        if (!paired) {
+#pragma unroll
+            for (int sgn = 0; sgn < 2; sgn++) {
+                const int ci = 2 * hx + sgn < nx ? ci0 + sgn : ci0;   // (the odd slot of a padded row: unused)
+                vi[sgn] = Codec<S>::ld(vIn, ci);
+                solid[sgn] = Codec<S>::ld(geo, ci).z;
+            }
+        }
 #pragma unroll
         for (int sgn = 0; sgn < 2; sgn++) {
             const int xx = 2 * hx + sgn;
@@ -354,9 +376,6 @@ __global__ __launch_bounds__(256) void k_project_resid(const DevParams* __restri
             const double vs = x[fs_opaque(y > 0 ? own - rs : own)];
             const double vn = x[fs_opaque(y < ny - 1 ? own + rs : own)];
             const double bv = (double)b[own];
-            const int ci = (z * ny + y) * nx + xx;
-            const float4 vi = Codec<S>::ld(vIn, ci);
-            const float solid = Codec<S>::ld(geo, ci).z;
             // the residual (resid_at's operations)
             double sum = 0.0, diag = 6.0;
             if (xx > 0) sum += vw; else diag += 1.0;
@@ -368,8 +387,18 @@ __global__ __launch_bounds__(256) void k_project_resid(const DevParams* __restri
             const double r = bv - (sum - diag * me) * ih2;
             rr += r * r;
             bb += bv * bv;
-            Codec<S>::st(vOut, ci, project_cell(g, xx, y, z, dx, vi, solid, (float)me, (float)ve, (float)vw, (float)vn, (float)vs,
-                                                (float)vf, (float)vk));
+            po[sgn] = project_cell(g, xx, y, z, dx, vi[sgn], solid[sgn], (float)me, (float)ve, (float)vw, (float)vn,
+                                   (float)vs, (float)vf, (float)vk);
+        }
+        if constexpr (pack) {
+            if (paired) {
+                const u32x2_t lo = Codec<S>::enc(po[0]), hi = Codec<S>::enc(po[1]);
+                *reinterpret_cast<u32x4_t*>(vOut + ci0) = u32x4_t{lo.x, lo.y, hi.x, hi.y};
+            }
+        }
+        if (!paired) {
+            Codec<S>::st(vOut, ci0, po[0]);
+            if (2 * hx + 1 < nx) Codec<S>::st(vOut, ci0 + 1, po[1]);
         }
     }
     if (skip[0]) return;                               // uniform over the grid
