@@ -319,8 +319,8 @@ __global__ __launch_bounds__(256) void k_project(const DevParams* __restrict__ P
 // The projection with the FP64 iterate x as the pressure, fused with the refinement's closing residual
 // r = b - A x, which reads the same seven values of x around each cell.  The threads, blocks and operations
 // of the residual part are the multigrid residual kernel's (k_resid64: a thread takes the parity-split pair
-// x = 2 hx, 2 hx + 1 of row (y, z), blocks of 64 x 4 pairs, the same cell function and the same per-block
-// (r.r, b.b) partials with block 0 marking them pending), so the partial sums are the same bits; the
+// x = 2 hx, 2 hx + 1 of row (y, z), blocks of 64 x 4 pairs, the same pair stencil and cell function and the same
+// per-block (r.r, b.b) partials with block 0 marking them pending), so the partial sums are the same bits; the
 // projection of each cell is k_project's.  With the solve's stop flag set (an earlier residual met the
 // tolerance) only the projection runs.
 template <class S>
@@ -340,7 +340,6 @@ __global__ __launch_bounds__(256) void k_project_resid(const DevParams* __restri
     const float dx = P->f[P_dx];
     double rr = 0.0, bb = 0.0;
     if (hx < nh && y < ny) {
-        const int row = (z * ny + y) * rs;
         const int ci0 = (z * ny + y) * nx + 2 * hx;    // the pair's first cell in the fields
         // 8-byte texels of an even-width grid: the pair's two texels are one aligned 16-byte word in each field
         constexpr bool pack = sizeof(typename S::raw) == 8;
@@ -363,32 +362,18 @@ __global__ __launch_bounds__(256) void k_project_resid(const DevParams* __restri
                 solid[sgn] = Codec<S>::ld(geo, ci).z;
             }
         }
+        const PairStencil ps = pair_stencil(LevelBuf<double>(x, nz * ny * rs), hx, y, z, nx, ny, nz, nh, rs);
 #pragma unroll
         for (int sgn = 0; sgn < 2; sgn++) {
             const int xx = 2 * hx + sgn;
             if (xx >= nx) continue;                    // the odd slot of a padded row
-            const int own = row + sgn * nh + hx, oth = row + (1 - sgn) * nh;
-            const double me = x[own];
-            const double vw = x[fs_opaque(xx > 0 ? oth + hx - 1 + sgn : own)];
-            const double ve = x[fs_opaque(xx < nx - 1 ? oth + hx + sgn : own)];
-            const double vk = x[fs_opaque(z > 0 ? own - ny * rs : own)];
-            const double vf = x[fs_opaque(z < nz - 1 ? own + ny * rs : own)];
-            const double vs = x[fs_opaque(y > 0 ? own - rs : own)];
-            const double vn = x[fs_opaque(y < ny - 1 ? own + rs : own)];
-            const double bv = (double)b[own];
-            // the residual (resid_at's operations)
-            double sum = 0.0, diag = 6.0;
-            if (xx > 0) sum += vw; else diag += 1.0;
-            if (xx < nx - 1) sum += ve; else diag += 1.0;
-            if (z > 0) sum += vk; else diag += 1.0;
-            if (z < nz - 1) sum += vf; else diag += 1.0;
-            if (y > 0) sum += vs; else diag -= 1.0;
-            if (y < ny - 1) sum += vn; else diag += 1.0;
-            const double r = bv - (sum - diag * me) * ih2;
+            const double bv = (double)b[sgn ? ps.o1 : ps.o0];
+            const double r = pair_resid(ps, sgn, bv, hx, y, z, nx, ny, nz, ih2);
             rr += r * r;
             bb += bv * bv;
-            po[sgn] = project_cell(g, xx, y, z, dx, vi[sgn], solid[sgn], (float)me, (float)ve, (float)vw, (float)vn,
-                                   (float)vs, (float)vf, (float)vk);
+            const double vxp = sgn ? ps.e1 : ps.m[1], vxm = sgn ? ps.m[0] : ps.w0;
+            po[sgn] = project_cell(g, xx, y, z, dx, vi[sgn], solid[sgn], (float)ps.m[sgn], (float)vxp, (float)vxm,
+                                   (float)ps.n[sgn], (float)ps.s[sgn], (float)ps.f[sgn], (float)ps.k[sgn]);
         }
         if constexpr (pack) {
             if (paired) {

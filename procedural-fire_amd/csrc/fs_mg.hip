@@ -470,11 +470,11 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
     double r[2] = {0.0, 0.0}, bv[2] = {0.0, 0.0};
     if (inside) {
         // both cells of the pair (the odd one of a padded row slot is computed and dropped)
-#pragma unroll
-        for (int sgn = 0; sgn < 2; sgn++) {
-            bv[sgn] = (double)b[(z * ny + y) * rs + sgn * nh + hx];
-            r[sgn] = resid_at(x, bv[sgn], 2 * hx + sgn, y, z, sgn, hx, nx, ny, nz, nh, rs, ih2);
-        }
+        const PairStencil ps = pair_stencil(LevelBuf<X>(x, nz * ny * rs), hx, y, z, nx, ny, nz, nh, rs);
+        bv[0] = (double)b[ps.o0];
+        bv[1] = (double)b[ps.o1];
+        r[0] = pair_resid(ps, 0, bv[0], hx, y, z, nx, ny, nz, ih2);
+        r[1] = pair_resid(ps, 1, bv[1], hx, y, z, nx, ny, nz, ih2);
     }
     const int sk = skip[fs_opaque(0)];   // a vector load: its wait does not hold up the scalar kernel arguments
     asm volatile("" ::"v"(r[0]), "v"(r[1]), "v"(bv[0]), "v"(bv[1]), "v"(sk));

@@ -1,9 +1,10 @@
-// Cell functions of the multigrid's level-0 operator shared by the multigrid kernels and the divergence
-// kernel that also forms the tolerance solve's starting residual: the FP64 residual at one cell of the
-// parity-split iterate, and the first colour-0 update of a level whose iterate is zero.
+// Cell functions of the multigrid's level-0 operator shared by the multigrid kernels and the projection kernel
+// that also forms the refinement's closing residual: the FP64 residual at the two cells of a parity-split pair,
+// and the first colour-0 update of a level whose iterate is zero.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "fs_codec.h"
 #include "fs_internal.h"
 
 // The colour-0 Gauss-Seidel update of cell (x, y, z) on a level whose iterate is all
@@ -27,30 +28,69 @@ __device__ __forceinline__ float red_from_zero(float bv, int x, int y, int z, in
     return pn;
 }
 
-// FP64 residual bv - A x of the level-0 operator (src/sim.js:416-437 ghost rules: -p on the x/z faces and
-// the top, +p below the floor) at cell xx = 2 hx + sgn of row (y, z); x is parity-split with half-row nh and
-// row stride rs.  Every neighbour is loaded (an outside one from the cell itself, unused) so the loads issue
-// together.
-template <class X>
-__device__ __forceinline__ double resid_at(const X* __restrict__ x, double bv, int xx, int y, int z, int sgn, int hx, int nx,
-                                           int ny, int nz, int nh, int rs, double ih2) {
-    const int row = (z * ny + y) * rs;
-    const int own = row + sgn * nh + hx, oth = row + (1 - sgn) * nh;
-    const double me = (double)x[own];
-    const double vw = (double)x[fs_opaque(xx > 0 ? oth + hx - 1 + sgn : own)];
-    const double ve = (double)x[fs_opaque(xx < nx - 1 ? oth + hx + sgn : own)];
-    const double vk = (double)x[fs_opaque(z > 0 ? own - ny * rs : own)];
-    const double vf = (double)x[fs_opaque(z < nz - 1 ? own + ny * rs : own)];
-    const double vs = (double)x[fs_opaque(y > 0 ? own - rs : own)];
-    const double vn = (double)x[fs_opaque(y < ny - 1 ? own + rs : own)];
+// A level-0 array of the multigrid read through a buffer resource: 32-bit element offsets, no 64-bit address
+// arithmetic per load (the array stays below 2 GiB).
+template <class X> struct LevelBuf;
+template <> struct LevelBuf<double> {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ LevelBuf(const double* p, int n)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, n * 8, FS_BUF_WORD3)) {}
+    __device__ __forceinline__ double operator()(int i) const {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, 0, 0));
+    }
+};
+template <> struct LevelBuf<float> {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ LevelBuf(const float* p, int n)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, n * 4, FS_BUF_WORD3)) {}
+    __device__ __forceinline__ double operator()(int i) const {
+        return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0));
+    }
+};
+
+// The iterate around the parity-split pair of cells x = 2 hx (even half-row, slot o0) and 2 hx + 1 (odd half-row,
+// slot o1) of row (y, z): each cell's own value, its outer x neighbour (the pair's cells are each other's inner
+// one) and its y and z neighbours, twelve loads for the two seven-point stencils.  A neighbour outside the grid is
+// loaded from the cell itself (unused), so every load issues at once.
+struct PairStencil {
+    double m[2], w0, e1, k[2], f[2], s[2], n[2];
+    int o0, o1;
+};
+template <class L>
+__device__ __forceinline__ PairStencil pair_stencil(const L& x, int hx, int y, int z, int nx, int ny, int nz, int nh, int rs) {
+    PairStencil p;
+    const int row = (z * ny + y) * rs, dz = ny * rs;
+    p.o0 = row + hx;
+    p.o1 = row + nh + hx;
+    const int xa = 2 * hx;
+    const int o[2] = {p.o0, p.o1};
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        p.m[c] = x(o[c]);
+        p.k[c] = x(fs_opaque(z > 0 ? o[c] - dz : o[c]));
+        p.f[c] = x(fs_opaque(z < nz - 1 ? o[c] + dz : o[c]));
+        p.s[c] = x(fs_opaque(y > 0 ? o[c] - rs : o[c]));
+        p.n[c] = x(fs_opaque(y < ny - 1 ? o[c] + rs : o[c]));
+    }
+    p.w0 = x(fs_opaque(xa > 0 ? p.o1 - 1 : p.o0));          // x - 1 of cell 0: the odd cell hx - 1
+    p.e1 = x(fs_opaque(xa + 2 < nx ? p.o0 + 1 : p.o1));     // x + 1 of cell 1: the even cell hx + 1
+    return p;
+}
+// The FP64 residual bv - A x of the level-0 operator (src/sim.js:416-437 ghost rules: -p on the x/z faces and the
+// top, +p below the floor) at cell c of the pair, from its stencil (c = 1 of a padded row's odd slot: in-bounds
+// values, a result to be dropped).
+__device__ __forceinline__ double pair_resid(const PairStencil& p, int c, double bv, int hx, int y, int z, int nx, int ny,
+                                             int nz, double ih2) {
+    const int xx = 2 * hx + c;
+    const double vw = c == 0 ? p.w0 : p.m[0], ve = c == 0 ? p.m[1] : p.e1;
     double sum = 0.0, diag = 6.0;
     if (xx > 0) sum += vw; else diag += 1.0;
     if (xx < nx - 1) sum += ve; else diag += 1.0;
-    if (z > 0) sum += vk; else diag += 1.0;
-    if (z < nz - 1) sum += vf; else diag += 1.0;
-    if (y > 0) sum += vs; else diag -= 1.0;
-    if (y < ny - 1) sum += vn; else diag += 1.0;
-    return bv - (sum - diag * me) * ih2;
+    if (z > 0) sum += p.k[c]; else diag += 1.0;
+    if (z < nz - 1) sum += p.f[c]; else diag += 1.0;
+    if (y > 0) sum += p.s[c]; else diag -= 1.0;
+    if (y < ny - 1) sum += p.n[c]; else diag += 1.0;
+    return bv - (sum - diag * p.m[c]) * ih2;
 }
 
 // The sum of v over the 64 lanes of a wave, in a fixed butterfly order; lane 0's value is reproducible.
