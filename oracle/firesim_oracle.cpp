@@ -893,7 +893,7 @@ struct Oracle : OracleBase {
                     R gl = length(g);
                     if (gl > R(1e-6)) {
                         R gate = smoothstep<R>(R(450), R(900), T) * (R(1) - srcAll) * smoothstep<R>(R(0.02), R(0.08), p.y) * lam;
-                        v = v + cross(g / gl, w) * (u(P_vortEps) * gate * dt);
+                        v = v + cross(unit_dir<R>(g, gl), w) * (u(P_vortEps) * gate * dt);
                     }
                     // curl noise (sim.js:364-368)
                     V3<R> tq = (p - V3<R>(R(0), tm * u(P_turbRise), R(0))) * u(P_turbFreq) + V3<R>(R(0), R(0), tm * u(P_turbEvolve));

@@ -98,7 +98,8 @@ template <class R> static inline R mix(R a, R b, R t) { return a * (R(1) - t) + 
 //   log: frexp to m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
 //        log m = 2 s (1 + s^2/3 + s^4/5 + s^6/7 + s^8/9), plus e ln2.
 //   pow(a, b) = exp(b log a) for finite a > 0.
-// Evaluated in plain f32 without FMA contraction (mul and add round separately).
+// Evaluated in f32: the Horner steps and log's closing sum as fused multiply-adds (std::fma, one rounding each),
+// the other operations rounding separately (no contraction).
 static inline float fexp(float x) {
     if (x != x) return x;
     if (x > 88.72283f) return INFINITY;
@@ -106,13 +107,13 @@ static inline float fexp(float x) {
     const float k = std::rint(x * 1.44269504f);
     const float r = (x - k * 0.693145751953125f) - k * 1.42860682e-6f;
     float p = 1.0f / 5040.0f;
-    p = 1.0f / 720.0f + r * p;
-    p = 1.0f / 120.0f + r * p;
-    p = 1.0f / 24.0f + r * p;
-    p = 1.0f / 6.0f + r * p;
-    p = 0.5f + r * p;
-    p = 1.0f + r * p;
-    p = 1.0f + r * p;
+    p = std::fma(r, p, 1.0f / 720.0f);
+    p = std::fma(r, p, 1.0f / 120.0f);
+    p = std::fma(r, p, 1.0f / 24.0f);
+    p = std::fma(r, p, 1.0f / 6.0f);
+    p = std::fma(r, p, 0.5f);
+    p = std::fma(r, p, 1.0f);
+    p = std::fma(r, p, 1.0f);
     return std::ldexp(p, (int)k);
 }
 static inline float flog(float a) {
@@ -125,12 +126,12 @@ static inline float flog(float a) {
     const float s = (m - 1.0f) / (m + 1.0f);
     const float s2 = s * s;
     float q = 1.0f / 9.0f;
-    q = 1.0f / 7.0f + s2 * q;
-    q = 1.0f / 5.0f + s2 * q;
-    q = 1.0f / 3.0f + s2 * q;
+    q = std::fma(s2, q, 1.0f / 7.0f);
+    q = std::fma(s2, q, 1.0f / 5.0f);
+    q = std::fma(s2, q, 1.0f / 3.0f);
     const float t = 2.0f * s;
     const float fe = (float)e;
-    return fe * 0.693145751953125f + (fe * 1.42860682e-6f + (t + t * (s2 * q)));
+    return std::fma(fe, 0.693145751953125f, std::fma(fe, 1.42860682e-6f, std::fma(t, s2 * q, t)));
 }
 static inline float fpow(float a, float b) {
     if (a == 1.0f || b == 0.0f) return 1.0f;
@@ -184,6 +185,20 @@ template <class R> struct V3 {
     V3 operator*(R s) const { return V3(x * s, y * s, z * s); }
     V3 operator/(R s) const { return V3(x / s, y / s, z / s); }
 };
+// a / b through z = RN(1 / b) and one Markstein correction (q = RN(a z), r = fma(-q, b, a) exact, RN(q + r z)): the
+// correctly rounded quotient while nothing underflows (Markstein 1990), the kernels' form of the vorticity
+// confinement's three quotients by one length (WGSL's normalize divides with 2.5 ulp of accuracy)
+static inline float div_recip(float a, float b, float z) {
+    const float q = a * z;
+    const float r = std::fma(-q, b, a);
+    return std::fma(r, z, q);
+}
+template <class R> static inline V3<R> unit_dir(const V3<R>& g, R gl);   // g / gl
+template <> inline V3<float> unit_dir<float>(const V3<float>& g, float gl) {
+    const float z = 1.0f / gl;
+    return V3<float>(div_recip(g.x, gl, z), div_recip(g.y, gl, z), div_recip(g.z, gl, z));
+}
+template <> inline V3<double> unit_dir<double>(const V3<double>& g, double gl) { return g / gl; }
 template <class R> static inline R dot(const V3<R>& a, const V3<R>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <class R> static inline R length(const V3<R>& a) { return std::sqrt(dot(a, a)); }
 template <class R> static inline V3<R> cross(const V3<R>& a, const V3<R>& b) {

@@ -216,8 +216,9 @@ __global__ __launch_bounds__(256) void k_curl(Dims g, const typename S::raw* __r
 
 // ---- sources, combustion, soot, cooling, buoyancy, confinement, curl noise,
 //      wind, sponges and clamps: one fused cell update (src/sim.js:230-396) -------
+// the register budget of 6 waves per SIMD (at most 80 VGPRs): this issue-bound kernel needs the waves
 template <class S>
-__global__ __launch_bounds__(256) void k_forces(const DevParams* __restrict__ P, Dims g,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_forces(const DevParams* __restrict__ P, Dims g,
                                                 const typename S::raw* __restrict__ velY, const typename S::raw* __restrict__ sY,
                                                 const typename S::raw* __restrict__ s2Y, const typename S::raw* __restrict__ vort,
                                                 const typename S::raw* __restrict__ geo, const half4_t* __restrict__ noise,

@@ -148,6 +148,18 @@ __device__ __forceinline__ Dual4 gsqrt(Dual4 a) {
 template <> __device__ __forceinline__ Dual4 flux_as<Dual4>(float f, int, int, const float*) { return Dual4(f); }
 __device__ __forceinline__ bool gisnan(Dual4 a) { return isnan(a.v); }
 
+// a / b with z = 1 / b.v given: the value through the reciprocal and one Markstein correction (fs_div_recip: the
+// correctly rounded quotient while nothing underflows), the derivative that of the plain quotient
+__device__ __forceinline__ float gdivr(float a, float b, float z) { return fs_div_recip(a, b, z); }
+__device__ __forceinline__ Dual gdivr(Dual a, Dual b, float z) {
+    return Dual(fs_div_recip(a.v, b.v, z), (a.d * b.v - a.v * b.d) / (b.v * b.v));
+}
+__device__ __forceinline__ Dual4 gdivr(Dual4 a, Dual4 b, float z) {
+    Dual4 r(fs_div_recip(a.v, b.v, z));
+    FS_D4(r.d[u] = (a.d[u] * b.v - a.v * b.d[u]) / (b.v * b.v));
+    return r;
+}
+
 // ---- the state-independent part of one cell's update -----------------------------
 struct ForcesCell {
     float dt, dx, tm, I;
@@ -287,7 +299,8 @@ __device__ __forceinline__ void forces_cell(const float* U, const float* D, cons
     const T gl = gsqrt(gx * gx + gy * gy + gz * gz);
     if (gval(gl) > 1e-6f) {
         const T gate = gsstep_c(450.f, 900.f, Tk) * (1.f - srcAll) * sstep_c(0.02f, 0.08f, k.py) * lam;
-        const T nx_ = gx / gl, ny_ = gy / gl, nz_ = gz / gl;
+        const float zr = 1.f / gval(gl);                // the three quotients by gl through its reciprocal
+        const T nx_ = gdivr(gx, gl, zr), ny_ = gdivr(gy, gl, zr), nz_ = gdivr(gz, gl, zr);
         const T kc = U[P_vortEps] * gate * dt;
         vx = vx + (ny_ * w[2] - nz_ * w[1]) * kc;
         vy = vy + (nz_ * w[0] - nx_ * w[2]) * kc;

@@ -38,43 +38,44 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v,
 // construction (identical to the oracle's, so results agree bit for bit):
 //   exp: Cody-Waite reduction x = k ln2 + r (hi part of ln2 has 9 trailing zero
 //        bits, so k*hi is exact for |k| < 512), degree-7 Taylor polynomial on
-//        |r| <= ln2/2 in Horner form, then ldexp(., k): <= 1.2 ulp.
+//        |r| <= ln2/2 in Horner form with fused multiply-adds, then ldexp(., k).
 //   log: frexp to m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
-//        log m = 2s(1 + s^2/3 + s^4/5 + s^6/7 + s^8/9) + e ln2: <= 1.8 ulp.
+//        log m = 2s(1 + s^2/3 + s^4/5 + s^6/7 + s^8/9) + e ln2, the same way.
 //   pow(a, b) = exp(b log a) for finite a > 0.
+// exp and log take their special cases as selects after the polynomial (every lane runs the same instructions).
 __device__ __forceinline__ float fs_exp(float x) {
-    if (x != x) return x;
-    if (x > 88.72283f) return __int_as_float(0x7f800000);
-    if (x < -103.97208f) return 0.0f;
     const float k = rintf(x * 1.44269504f);
     const float r = (x - k * 0.693145751953125f) - k * 1.42860682e-6f;
     float p = 1.0f / 5040.0f;
-    p = 1.0f / 720.0f + r * p;
-    p = 1.0f / 120.0f + r * p;
-    p = 1.0f / 24.0f + r * p;
-    p = 1.0f / 6.0f + r * p;
-    p = 0.5f + r * p;
-    p = 1.0f + r * p;
-    p = 1.0f + r * p;
-    return ldexpf(p, (int)k);
+    p = fmaf(r, p, 1.0f / 720.0f);
+    p = fmaf(r, p, 1.0f / 120.0f);
+    p = fmaf(r, p, 1.0f / 24.0f);
+    p = fmaf(r, p, 1.0f / 6.0f);
+    p = fmaf(r, p, 0.5f);
+    p = fmaf(r, p, 1.0f);
+    p = fmaf(r, p, 1.0f);
+    const float e = ldexpf(p, (int)fminf(fmaxf(k, -256.0f), 256.0f));   // k is within +-151 wherever e is used
+    return x != x ? x : x > 88.72283f ? __int_as_float(0x7f800000) : x < -103.97208f ? 0.0f : e;
 }
 __device__ __forceinline__ float fs_log(float a) {
-    if (a != a || a < 0.0f) return __int_as_float(0x7fc00000);
-    if (a == 0.0f) return -__int_as_float(0x7f800000);
-    if (a == __int_as_float(0x7f800000)) return a;
     int e;
     float m = frexpf(a, &e);
     if (m < 0.70710678f) { m = m * 2.0f; e = e - 1; }
     const float s = (m - 1.0f) / (m + 1.0f);
     const float s2 = s * s;
     float q = 1.0f / 9.0f;
-    q = 1.0f / 7.0f + s2 * q;
-    q = 1.0f / 5.0f + s2 * q;
-    q = 1.0f / 3.0f + s2 * q;
+    q = fmaf(s2, q, 1.0f / 7.0f);
+    q = fmaf(s2, q, 1.0f / 5.0f);
+    q = fmaf(s2, q, 1.0f / 3.0f);
     const float t = 2.0f * s;
     const float fe = (float)e;
-    return fe * 0.693145751953125f + (fe * 1.42860682e-6f + (t + t * (s2 * q)));
+    const float l = fmaf(fe, 0.693145751953125f, fmaf(fe, 1.42860682e-6f, fmaf(t, s2 * q, t)));
+    return (a != a || a < 0.0f) ? __int_as_float(0x7fc00000)
+         : a == 0.0f ? -__int_as_float(0x7f800000)
+         : a == __int_as_float(0x7f800000) ? a : l;
 }
+// pow keeps its early exits: where a wave's every lane has a = 1 (the source kernel's emitter terms away from the
+// sources) it skips the log and the exp
 __device__ __forceinline__ float fs_pow(float a, float b) {
     if (a == 1.0f || b == 0.0f) return 1.0f;
     if (a == 0.0f) return b > 0.0f ? 0.0f : __int_as_float(0x7f800000);

@@ -83,13 +83,17 @@ __device__ __forceinline__ double pair_resid(const PairStencil& p, int c, double
                                              int nz, double ih2) {
     const int xx = 2 * hx + c;
     const double vw = c == 0 ? p.w0 : p.m[0], ve = c == 0 ? p.m[1] : p.e1;
-    double sum = 0.0, diag = 6.0;
-    if (xx > 0) sum += vw; else diag += 1.0;
-    if (xx < nx - 1) sum += ve; else diag += 1.0;
-    if (z > 0) sum += p.k[c]; else diag += 1.0;
-    if (z < nz - 1) sum += p.f[c]; else diag += 1.0;
-    if (y > 0) sum += p.s[c]; else diag -= 1.0;
-    if (y < ny - 1) sum += p.n[c]; else diag += 1.0;
+    // the operator's `if (inside) sum += v; else diag += 1` (-= 1 below the floor) in its order: a face outside
+    // adds -0.0, which leaves every sum as it is (+0 included), and diag is an exact small integer
+    const bool bw = xx > 0, be = xx < nx - 1, bk = z > 0, bf = z < nz - 1, bs = y > 0, bn = y < ny - 1;
+    double sum = 0.0;
+    sum += bw ? vw : -0.0;
+    sum += be ? ve : -0.0;
+    sum += bk ? p.k[c] : -0.0;
+    sum += bf ? p.f[c] : -0.0;
+    sum += bs ? p.s[c] : -0.0;
+    sum += bn ? p.n[c] : -0.0;
+    const double diag = (double)(6 + !bw + !be + !bk + !bf - !bs + !bn);
     return bv - (sum - diag * p.m[c]) * ih2;
 }
 
