@@ -88,16 +88,29 @@ struct fs_sim {
     bool own_stream = false;
     // captured graphs: the whole step (fixed solves); the whole step of an adaptive solve with the number of
     // iterations the last step needed (step_k) and the readback of the solve's state; its three parts
-    // The refinement's adaptive step graph closes its last correction with the projection itself (the fused
-    // projection + residual norms, then the stop state for the host); an extra correction forms its own
-    // right-hand side (G_ITER_NC) and is closed the same way (G_POSTR1).
-    enum { G_FULL, G_STEP, G_PRE, G_FIRST, G_ITER, G_POST, G_POSTR1, G_ITER_NC, G_COUNT };
+    // The refinement's adaptive step graph (rexec below) closes its last correction with the projection itself
+    // (the fused projection + residual norms, then the stop state for the host); an extra correction forms its
+    // own right-hand side (G_ITER_NC) and is closed the same way (pexec below).
+    enum { G_FULL, G_STEP, G_PRE, G_FIRST, G_ITER, G_POST, G_ITER_NC, G_COUNT };
     int step_k = 0;
     hipGraphExec_t gexec[G_COUNT] = {};
     hipGraph_t graph[G_COUNT] = {};
     int predict = 1;                 // V-cycles the last adaptive solve needed
-    int* host_state = nullptr;       // pinned: {used, done} of the last solve
+    int* host_state = nullptr;       // pinned: {used, done} of the last solve (the refinement: two slots, by step parity)
     int* host_state_dev = nullptr;   // its device address
+    // The refinement's steps of one call run one ahead: a step's graph (per V-cycle count and host-state slot) is
+    // enqueued behind the previous step before the host has read that step's stop state; its kernels exit at once
+    // while that state asks for another correction (st->hold), and the host then enqueues the step again.
+    hipGraphExec_t rexec[MG_MAX_REFINE + 1][2] = {};
+    hipGraph_t rgraph[MG_MAX_REFINE + 1][2] = {};
+    hipGraphExec_t pexec[2] = {};    // a further correction's projection and stop state, per slot
+    hipGraph_t pgraph[2] = {};
+    hipEvent_t ev_slot[2] = {};
+    int slot = 0;                    // host-state slot of the next step
+    int cur_slot = 0;                // the slot of the step being recorded
+    bool inflight = false;           // the next step is enqueued already
+    int inflight_k = 0;              // with this many V-cycles
+    bool gated = false;              // recording a step that may run ahead: its kernels test st->hold
     bool geo_ready = false;
     double time = 0.0;
     int64_t frame = 0;
@@ -124,6 +137,7 @@ struct fs_sim {
         F.S = f[F_S]; F.SX = f[F_SX]; F.SY = f[F_SY];
         F.S2 = f[F_S2]; F.S2X = f[F_S2X]; F.S2Y = f[F_S2Y];
         F.vort = f[F_VORT]; F.geo = f[F_GEO];
+        F.hold = gated ? &mg.st->hold : nullptr;
         F.noise = noise; F.curl = curl; F.noise_n = cfg.noise_n;
         F.logs = logs_d;
         F.spots = spots_d;
@@ -252,30 +266,69 @@ int enqueue_step(fs_sim* h, bool prof) {
 // then the stop state for the host
 void enqueue_post_resid(fs_sim* h, int cycles) {
     launch_project_resid(h->prec, h->g, h->stream, h->P_d, h->fields(), h->mg, cycles);
-    mg_state_ready(h->mg, h->stream, h->cfg.div_tol, std::min(h->cfg.max_vcycles, MG_MAX_REFINE), h->host_state_dev);
+    mg_state_ready(h->mg, h->stream, h->cfg.div_tol, std::min(h->cfg.max_vcycles, MG_MAX_REFINE),
+                   h->host_state_dev + 2 * h->cur_slot, h->gated ? &h->mg.st->hold : nullptr, true);
 }
 // an adaptive step with step_k V-cycles (the first correction's, then one per correction) and the stop-flag
 // readback (the refinement's step ends with its projection; the other solves' projection follows as its own
 // part; the host checks the flag afterwards)
 int enqueue_adaptive_step(fs_sim* h) {
-    enqueue_pre(h, false, true);
     const int f = h->first_cycles();
-    if (h->cfg.solve == FS_SOLVE_REFINE64) {
-        // the last correction's closing residual is formed by the projection, which ends the graph with the
-        // stop state for the host; a further correction forms its own right-hand side (G_ITER_NC) and is
-        // projected again (G_POSTR1), and the usual step stops here
-        if (int rc = h->solve_iter(h->stream, f, false, h->step_k > f, h->step_k > f)) return rc;
-        for (int k = f; k < h->step_k; k++)
-            if (int rc = h->solve_iter(h->stream, 1, false, k + 1 < h->step_k, k + 1 < h->step_k)) return rc;
-        enqueue_post_resid(h, h->step_k > f ? 1 : f);
-        return FS_OK;
+    if (h->cfg.solve == FS_SOLVE_REFINE64) {   // recorded held on st->hold (it may run ahead of its predecessor's test)
+        h->gated = true;
+        h->mg.gate = &h->mg.st->hold;
+        enqueue_pre(h, false, true);
+        int rc = h->solve_iter(h->stream, f, false, h->step_k > f, h->step_k > f);
+        for (int k = f; k < h->step_k && !rc; k++) rc = h->solve_iter(h->stream, 1, false, k + 1 < h->step_k, k + 1 < h->step_k);
+        if (!rc) enqueue_post_resid(h, h->step_k > f ? 1 : f);
+        h->gated = false;
+        h->mg.gate = nullptr;
+        return rc;
     }
+    enqueue_pre(h, false, true);
     if (int rc = h->solve_iter(h->stream, f)) return rc;
     for (int k = f; k < h->step_k; k++)
         if (int rc = h->solve_iter(h->stream)) return rc;
     // the stopping state goes to the host by a copy
     FS_HIP_CHECK(hipMemcpyAsync(h->host_state, &h->mg.st->used, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     return FS_OK;
+}
+
+// Records `enqueue` into (gr, ex) on first use and launches it; without graphs, enqueues it directly.
+template <class Enq>
+int run_graph(fs_sim* h, hipGraph_t& gr, hipGraphExec_t& ex, Enq enqueue) {
+    if (!h->cfg.use_graph) return enqueue();
+    if (!ex) {
+        FS_HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue();
+        hipGraph_t g = nullptr;
+        if (rc) {   // end the capture before reporting the failure
+            (void)hipStreamEndCapture(h->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        FS_HIP_CHECK(hipStreamEndCapture(h->stream, &g));
+        gr = g;
+        FS_HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    }
+    FS_HIP_CHECK(hipGraphLaunch(ex, h->stream));
+    return FS_OK;
+}
+// the refinement's step with k V-cycles, its stop state into host slot `slot`, held while st->hold
+int run_refine_step(fs_sim* h, int k, int slot) {
+    return run_graph(h, h->rgraph[k][slot], h->rexec[k][slot], [&] {
+        h->step_k = k;
+        h->cur_slot = slot;
+        return enqueue_adaptive_step(h);
+    });
+}
+// a further correction's projection with the closing residual and the stop state into slot `slot` (not held)
+int run_refine_post(fs_sim* h, int slot) {
+    return run_graph(h, h->pgraph[slot], h->pexec[slot], [&] {
+        h->cur_slot = slot;
+        enqueue_post_resid(h, 1);
+        return FS_OK;
+    });
 }
 
 int capture(fs_sim* h, int which) {
@@ -289,7 +342,6 @@ int capture(fs_sim* h, int which) {
         case fs_sim::G_FIRST: rc = h->solve_iter(h->stream, h->first_cycles()); break;
         case fs_sim::G_ITER: rc = h->solve_iter(h->stream); break;
         case fs_sim::G_POST: enqueue_post(h, false); break;
-        case fs_sim::G_POSTR1: enqueue_post_resid(h, 1); break;
         case fs_sim::G_ITER_NC: rc = h->solve_iter(h->stream, 1, true, false, false); break;
     }
     hipGraph_t gr = nullptr;
@@ -317,7 +369,6 @@ int run_part(fs_sim* h, int which, bool prof) {
         case fs_sim::G_FIRST: return h->solve_iter(h->stream, h->first_cycles());
         case fs_sim::G_ITER: return h->solve_iter(h->stream);
         case fs_sim::G_POST: enqueue_post(h, prof); break;
-        case fs_sim::G_POSTR1: enqueue_post_resid(h, 1); break;
         case fs_sim::G_ITER_NC: return h->solve_iter(h->stream, 1, true, false, false);
     }
     return FS_OK;
@@ -329,6 +380,18 @@ void drop_graphs(fs_sim* h) {
         h->gexec[i] = nullptr;
         h->graph[i] = nullptr;
     }
+    for (int s = 0; s < 2; s++) {
+        for (int k = 0; k <= MG_MAX_REFINE; k++) {
+            if (h->rexec[k][s]) (void)hipGraphExecDestroy(h->rexec[k][s]);
+            if (h->rgraph[k][s]) (void)hipGraphDestroy(h->rgraph[k][s]);
+            h->rexec[k][s] = nullptr;
+            h->rgraph[k][s] = nullptr;
+        }
+        if (h->pexec[s]) (void)hipGraphExecDestroy(h->pexec[s]);
+        if (h->pgraph[s]) (void)hipGraphDestroy(h->pgraph[s]);
+        h->pexec[s] = nullptr;
+        h->pgraph[s] = nullptr;
+    }
 }
 
 // One step with a tolerance-driven solve: enqueue as many iterations as the last step needed, the readback
@@ -337,16 +400,55 @@ void drop_graphs(fs_sim* h) {
 // host enqueues the further iterations and the projection again: the projection only reads the advected
 // velocity (which nothing between the two runs writes) and the pressure, so its second run gives the state
 // of the unspeculated order, bit for bit.  The FP64 refinement goes further: the projection itself forms the
-// closing residual's norms (it reads the same iterate values) and the stop state follows it.  Not speculated:
-// profiled steps and steps that append a per-step diagnostics record (the projection part would record twice).
-int adaptive_step(fs_sim* h, bool prof) {
+// closing residual's norms (it reads the same iterate values) and the stop state follows it, and the next step
+// of the same call is enqueued behind the step before the host reads that state: its kernels exit at once while
+// the state asks for another correction (st->hold), in which case the host enqueues the correction and then the
+// next step again.  Not speculated: profiled steps and steps that append a per-step diagnostics record (the
+// projection part would record twice).  more: another step of the same call follows.
+int adaptive_step(fs_sim* h, bool prof, bool more) {
     const int maxc = std::min(h->cfg.max_vcycles, MG_MAX_REFINE);
     const bool spec = !prof && h->diag_cap == 0;
     bool projected = false;
     int k = 0, rc = FS_OK;
     const int f = h->first_cycles();
-    // the refinement's form (projection fused with the closing residual) also runs eagerly, one launch at a time
-    if (spec && (h->cfg.use_graph || h->cfg.solve == FS_SOLVE_REFINE64)) {
+    if (spec && h->cfg.solve == FS_SOLVE_REFINE64) {   // with graphs or one launch at a time
+        const int kk = std::min(std::max(h->predict, f), maxc);
+        const int s0 = h->slot;
+        if (h->budget_on) more = false;                    // (the budget's kernels are not held)
+        if (!h->inflight) {
+            if ((rc = run_refine_step(h, kk, s0))) return rc;
+            h->inflight_k = kk;
+        }
+        h->inflight = false;
+        k = h->inflight_k;                                 // the V-cycles this step was enqueued with
+        FS_HIP_CHECK(hipEventRecord(h->ev_slot[s0], h->stream));
+        if (more) {                                        // the next step, held on the device until this one is done
+            if ((rc = run_refine_step(h, kk, 1 - s0))) return rc;
+            h->inflight = true;
+            h->inflight_k = kk;
+        }
+        const int* hs = h->host_state + 2 * s0;
+        FS_HIP_CHECK(hipEventSynchronize(h->ev_slot[s0]));
+        bool corrected = false;
+        while (!hs[1] && k < maxc) {
+            // a correction more (forming its right-hand side), the projection and the stop state again, not held
+            if ((rc = run_part(h, fs_sim::G_ITER_NC, prof))) return rc;
+            if ((rc = run_refine_post(h, s0))) return rc;
+            FS_HIP_CHECK(hipEventRecord(h->ev_slot[s0], h->stream));
+            FS_HIP_CHECK(hipEventSynchronize(h->ev_slot[s0]));
+            k++;
+            corrected = true;
+        }
+        if (corrected) h->inflight = false;                // the next step ran held: it is enqueued again
+        if (!hs[1]) {   // (not reached: a solve at its V-cycle cap is done) release the hold all the same
+            launch_fill_zero(h->stream, &h->mg.st->hold, 2 * sizeof(int));
+            h->inflight = false;
+        }
+        h->predict = std::max(1, hs[0]);
+        h->slot = 1 - s0;
+        return FS_OK;
+    }
+    if (spec && h->cfg.use_graph) {
         const int kk = std::min(std::max(h->predict, f), maxc);
         if (h->step_k != kk && h->gexec[fs_sim::G_STEP]) {
             (void)hipGraphExecDestroy(h->gexec[fs_sim::G_STEP]);
@@ -357,21 +459,6 @@ int adaptive_step(fs_sim* h, bool prof) {
         h->step_k = kk;
         if ((rc = run_part(h, fs_sim::G_STEP, prof))) return rc;
         k = kk;
-        if (h->cfg.solve == FS_SOLVE_REFINE64) {
-            // the step graph ended with the projection, which closes the last correction with its residual's
-            // norms, and the state for the host; a correction more (forming its right-hand side) and the
-            // projection again while the state says so
-            while (true) {
-                FS_HIP_CHECK(hipEventRecord(h->ev_state, h->stream));
-                FS_HIP_CHECK(hipEventSynchronize(h->ev_state));
-                if (h->host_state[1] || k >= maxc) break;
-                if ((rc = run_part(h, fs_sim::G_ITER_NC, prof))) return rc;
-                if ((rc = run_part(h, fs_sim::G_POSTR1, prof))) return rc;
-                k++;
-            }
-            h->predict = std::max(1, h->host_state[0]);
-            return FS_OK;
-        }
         while (true) {   // the state after the step's corrections, the projection enqueued behind it
             FS_HIP_CHECK(hipEventRecord(h->ev_state, h->stream));
             if ((rc = run_part(h, fs_sim::G_POST, prof))) return rc;
@@ -429,6 +516,7 @@ void free_sim(fs_sim* h) {
     spectral_free(h->spec);
     for (auto& e : h->ev) if (e) (void)hipEventDestroy(e);
     if (h->ev_state) (void)hipEventDestroy(h->ev_state);
+    for (auto& e : h->ev_slot) if (e) (void)hipEventDestroy(e);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -629,7 +717,9 @@ int fs_create(const fs_config* cfg, const float* logs, int n_logs, fs_sim** out)
     for (auto& e : h->ev) if (hipEventCreate(&e) != hipSuccess) return bail(fail(FS_EDEVICE, "event create failed"));
     if (hipEventCreateWithFlags(&h->ev_state, hipEventDisableTiming) != hipSuccess)
         return bail(fail(FS_EDEVICE, "event create failed"));
-    if (hipHostMalloc((void**)&h->host_state, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    for (auto& e : h->ev_slot)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return bail(fail(FS_EDEVICE, "event create failed"));
+    if (hipHostMalloc((void**)&h->host_state, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return bail(fail(FS_ENOMEM, "pinned host allocation failed"));
     if (hipHostGetDevicePointer((void**)&h->host_state_dev, h->host_state, 0) != hipSuccess)
         return bail(fail(FS_EDEVICE, "pinned host memory is not device-accessible"));
@@ -769,13 +859,18 @@ int fs_step(fs_sim* h, double dt, int nsteps) {   // sim.js:627-648
     if (nsteps < 0 || !(dt > 0.0) || !std::isfinite(dt)) return fail(FS_EINVAL, "dt must be positive and finite, nsteps >= 0");
     if (set_device(h)) return FS_EDEVICE;
     h->gen++;
+    h->inflight = false;
+    h->slot = 0;
+    if (h->cfg.solve == FS_SOLVE_REFINE64 && nsteps > 0)   // no step is held from an earlier call
+        launch_fill_zero(h->stream, &h->mg.st->hold, 2 * sizeof(int));
     for (int k = 0; k < nsteps; k++) {
         sync_params(h, dt);
         if (!h->geo_ready) {   // sim.js:632
             launch_geo(h->prec, h->g, h->stream, h->P_d, h->fields());
             h->geo_ready = true;
         }
-        const int rc = h->adaptive() ? adaptive_step(h, h->profile) : run_part(h, fs_sim::G_FULL, h->profile);
+        const int rc = h->adaptive() ? adaptive_step(h, h->profile, k + 1 < nsteps)
+                                     : run_part(h, fs_sim::G_FULL, h->profile);
         if (rc) return rc;
         h->time += dt;   // sim.js:646-647 (the device clock advances inside the step)
         h->frame += 1;

@@ -93,7 +93,10 @@ __global__ __launch_bounds__(256) void k_adv1(const DevParams* __restrict__ P, D
                                               const typename S::raw* __restrict__ vel, const typename S::raw* __restrict__ sF,
                                               const typename S::raw* __restrict__ s2F, typename S::raw* __restrict__ velX,
                                               typename S::raw* __restrict__ sX, typename S::raw* __restrict__ s2X,
-                                              unsigned* __restrict__ btpack) {
+                                              unsigned* __restrict__ btpack, const int* __restrict__ hold) {
+    // a step enqueued ahead of its predecessor's stopping test: the flag's load goes out first, the test comes
+    // before the stores that matter (velX still holds the velocity the predecessor may project again)
+    const int held = hold ? hold[0] : 0;
     Cell c;
     if (!cell_of(g, c)) return;
     const float dt = (float)P->dt, dx = P->f[P_dx];
@@ -105,6 +108,7 @@ __global__ __launch_bounds__(256) void k_adv1(const DevParams* __restrict__ P, D
     const float ws = sstep_c(0.03f, 0.4f, b.y * dx);
     const float wx = P->f[P_windX] * ws, wy = 0.f * ws, wz = P->f[P_windZ] * ws;
     const float4 sv = sample_clamp_f<S>(V, g, b.x, b.y, b.z);
+    if (held) return;                                  // (uniform)
     Codec<S>::st(velX, c.i, make_float4(mixf(wx, sv.x, f), mixf(wy, sv.y, f), mixf(wz, sv.z, f), 0.f));
     Codec<S>::st(sX, c.i, scale4(sample_clamp_f<S>(SF, g, b.x, b.y, b.z), f));
     Codec<S>::st(s2X, c.i, scale4(sample_clamp_f<S>(S2F, g, b.x, b.y, b.z), f));
@@ -224,7 +228,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                                                 const typename S::raw* __restrict__ geo, const half4_t* __restrict__ noise,
                                                 const half4_t* __restrict__ curlT, int nn, const fs_spot* __restrict__ spots,
                                                 const float* __restrict__ faceFlux, typename S::raw* __restrict__ velOut,
-                                                typename S::raw* __restrict__ sOut, typename S::raw* __restrict__ s2Out) {
+                                                typename S::raw* __restrict__ sOut, typename S::raw* __restrict__ s2Out,
+                                                const int* __restrict__ hold) {
+    const int held = hold ? hold[0] : 0;              // (tested before the stores, as in k_adv1)
     Cell c;
     const bool in = cell_of(g, c);
     const unsigned long long reach = spot_reach(P->f, g, c.x, c.y, c.z, spots, P->n_spots);   // every lane, then exit
@@ -244,6 +250,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const float lap[3] = {k.lapx, k.lapy, k.lapz};
     forces_cell<float>(P->f, P->d, k, spots, P->n_spots, -1, nullptr, face, fv, vx, vy, vz, s, s2, wv, gx, gy, gz, lap, phi,
                        (NoTrace*)nullptr, reach);
+    if (held) return;
     Codec<S>::st(velOut, c.i, make_float4(vx, vy, vz, phi));
     Codec<S>::st(sOut, c.i, make_float4(s[0], s[1], s[2], s[3]));
     Codec<S>::st(s2Out, c.i, make_float4(s2[0], s2[1], s2[2], s2[3]));
@@ -252,10 +259,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 // ---- divergence minus the expansion target (src/sim.js:403-413) ----------------
 template <class S>
 __global__ __launch_bounds__(256) void k_div(DevParams* __restrict__ P, Dims g, const typename S::raw* __restrict__ v,
-                                             float* __restrict__ bOut, int advance_clock) {
+                                             float* __restrict__ bOut, int advance_clock, const int* __restrict__ hold) {
+    const int held = hold ? hold[0] : 0;              // (tested before the stores, as in k_adv1)
     Cell c;
     const bool in = cell_of(g, c);
-    if (advance_clock && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (advance_clock && !held && blockIdx.x == 0 && threadIdx.x == 0) {
         // time += dt; frame++ (sim.js:646-647).  Nothing after the forces kernel in a step reads the clock.
         P->time = P->time + P->dt;
         P->frame = P->frame + 1;
@@ -268,7 +276,9 @@ __global__ __launch_bounds__(256) void k_div(DevParams* __restrict__ P, Dims g, 
     const float T = ldc<S>(v, g, c.x, c.y + 1, c.z).y;
     const float K = ldc<S>(v, g, c.x, c.y, c.z - 1).z, F = ldc<S>(v, g, c.x, c.y, c.z + 1).z;
     if (c.y == 0) B = -me.y;
-    bOut[rb_index(c.x, c.y, c.z, g.nx, g.ny)] = ((R - L) + (T - B) + (F - K)) / (2.f * dx) - me.w;   // parity-split layout
+    const float bv = ((R - L) + (T - B) + (F - K)) / (2.f * dx) - me.w;
+    if (held) return;
+    bOut[rb_index(c.x, c.y, c.z, g.nx, g.ny)] = bv;   // parity-split layout
 }
 
 // ---- projection (src/sim.js:532-550) ------------------------------------------
@@ -330,7 +340,8 @@ __global__ __launch_bounds__(256) void k_project_resid(const DevParams* __restri
                                                        const float* __restrict__ b, const typename S::raw* __restrict__ geo,
                                                        typename S::raw* __restrict__ vOut, double* __restrict__ partials,
                                                        RefineState* __restrict__ st, const int* __restrict__ skip, double h,
-                                                       int cycles) {
+                                                       int cycles, const int* __restrict__ hold) {
+    const int held = hold ? hold[0] : 0;              // (tested before the stores, as in k_adv1)
     __shared__ double sr[4], sb[4];
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int nh = (nx + 1) >> 1, rs = 2 * nh;
@@ -377,17 +388,17 @@ __global__ __launch_bounds__(256) void k_project_resid(const DevParams* __restri
                                    (float)ps.n[sgn], (float)ps.s[sgn], (float)ps.f[sgn], (float)ps.k[sgn]);
         }
         if constexpr (pack) {
-            if (paired) {
+            if (paired && !held) {
                 const u32x2_t lo = Codec<S>::enc(po[0]), hi = Codec<S>::enc(po[1]);
                 *reinterpret_cast<u32x4_t*>(vOut + ci0) = u32x4_t{lo.x, lo.y, hi.x, hi.y};
             }
         }
-        if (!paired) {
+        if (!paired && !held) {
             Codec<S>::st(vOut, ci0, po[0]);
             if (2 * hx + 1 < nx) Codec<S>::st(vOut, ci0 + 1, po[1]);
         }
     }
-    if (skip[0]) return;                               // uniform over the grid
+    if (held || skip[0]) return;                       // uniform over the grid
     rr = wave_sum(rr);
     bb = wave_sum(bb);
     if ((threadIdx.x & 63) == 0) { sr[threadIdx.x >> 6] = rr; sb[threadIdx.x >> 6] = bb; }
@@ -578,7 +589,7 @@ void launch_geo(int prec, const Dims& g, hipStream_t s, const DevParams* P, cons
 }
 template <class S> static void adv1_t(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F) {
     k_adv1<S><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.vel), CRAW(F.S), CRAW(F.S2), RAW(F.velX), RAW(F.SX), RAW(F.S2X),
-                                        F.btpack);
+                                        F.btpack, F.hold);
 }
 void launch_adv1(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F) {
     DISPATCH_PREC(prec, adv1_t<S>(g, s, P, F));
@@ -609,13 +620,13 @@ template <class S> static void curl_t(const Dims& g, hipStream_t s, const FieldS
 void launch_curl(int prec, const Dims& g, hipStream_t s, const FieldSet& F) { DISPATCH_PREC(prec, curl_t<S>(g, s, F)); }
 template <class S> static void forces_t(const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F) {
     k_forces<S><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.velY), CRAW(F.SY), CRAW(F.S2Y), CRAW(F.vort), CRAW(F.geo), F.noise,
-                                          F.curl, F.noise_n, F.spots, F.face_flux, RAW(F.velX), RAW(F.S), RAW(F.S2));
+                                          F.curl, F.noise_n, F.spots, F.face_flux, RAW(F.velX), RAW(F.S), RAW(F.S2), F.hold);
 }
 void launch_forces(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F) {
     DISPATCH_PREC(prec, forces_t<S>(g, s, P, F));
 }
 template <class S> static void divk_t(const Dims& g, hipStream_t s, DevParams* P, const FieldSet& F, float* b, int adv) {
-    k_div<S><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.velX), b, adv);
+    k_div<S><<<g.nblocks, 256, 0, s>>>(P, g, CRAW(F.velX), b, adv, F.hold);
 }
 void launch_div(int prec, const Dims& g, hipStream_t s, DevParams* P, const FieldSet& F, float* b_out, int advance_clock) {
     DISPATCH_PREC(prec, divk_t<S>(g, s, P, F, b_out, advance_clock));
@@ -633,7 +644,7 @@ template <class S> static void project_resid_t(const Dims& g, hipStream_t s, con
                                                int cycles) {
     const dim3 grid((unsigned)(((g.nx + 1) / 2 + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nz);
     k_project_resid<S><<<grid, 256, 0, s>>>(P, g, CRAW(F.velX), mg.x64, mg.bkeep, CRAW(F.geo), RAW(F.vel), mg.partials, mg.st,
-                                            &mg.st->done, (double)mg.h[0], cycles);
+                                            &mg.st->done, (double)mg.h[0], cycles, F.hold);
 }
 void launch_project_resid(int prec, const Dims& g, hipStream_t s, const DevParams* P, const FieldSet& F, GpuMG& mg, int cycles) {
     DISPATCH_PREC(prec, project_resid_t<S>(g, s, P, F, mg, cycles));

@@ -72,6 +72,11 @@ struct RefineState {            // device-resident FP64 refinement bookkeeping
     int pending;                // partials not yet reduced: 0 none, 1 a residual after a V-cycle, 2 a starting residual
     int never;                  // always 0: the skip flag of launches that run unconditionally
     int inc;                    // V-cycles applied since the pending residual's predecessor
+    // Steps enqueued ahead of their predecessor's stopping test: hold = 1 while the last finished step still
+    // needs a correction (every kernel of a step enqueued behind it then exits at once); skipv = done | hold,
+    // the stop flag of such a step's multigrid kernels
+    int hold;
+    int skipv;
 };
 
 // A pending residual reduction handed to the first smoothing pass of the next V-cycle, whose
@@ -104,6 +109,9 @@ struct GpuMG {
     int pre = 2, post = 2;       // sweeps before / after the coarse correction (all levels but the coarsest)
     int first = 1;               // V-cycles of a step's first FP64 correction (FS_SOLVE_REFINE64)
     int direct = 0;              // > 0: level solved exactly by separable transforms (the cycle stops there)
+    // set (to &st->hold) while a step enqueued ahead of its predecessor's stopping test is recorded: its starting
+    // residual and the first pass of each V-cycle skip on it, the other kernels on st->skipv
+    const int* gate = nullptr;
     struct DirectSolve* ds = nullptr;
 };
 
@@ -164,8 +172,10 @@ void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cy
                     bool keep_rhs = true, bool close = true);
 // reduces a residual whose partials are still pending, with the solve's stopping test (a one-block
 // kernel that exits at once when none is)
-// With `host` (pinned host memory, device-accessible) the kernel also stores {used, done} there.
-void mg_state_ready(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int* host = nullptr);
+// With `host` (pinned host memory, device-accessible) the kernel also stores {used, done} there.  gate: the
+// kernel exits at once while *gate; set_hold: it then sets st->hold = !done (a step's closing test).
+void mg_state_ready(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int* host = nullptr, const int* gate = nullptr,
+                    bool set_hold = false);
 void mg_tol32_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 void mg_tol32_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles);
 // relative FP64 residual of the FP32 level-0 p against level-0 b -> st->rel[0]
@@ -204,6 +214,7 @@ struct FieldSet {
     void* S; void* SX; void* SY;
     void* S2; void* S2X; void* S2Y;
     void* vort; void* geo;
+    const int* hold = nullptr;   // set: the kernels exit at once while *hold (a step enqueued ahead of its predecessor's test)
     const half4_t* noise; const half4_t* curl; int noise_n;
     const float* logs;
     const fs_spot* spots;        // FS_MAX_SPOTS entries

@@ -102,6 +102,7 @@ __device__ void finalize_sums(RefineState* st, const double* partials, int npart
         st->used = phase;
         st->bnorm2 = b2;
         st->done = (rel <= tol || phase >= max_cycles) ? 1 : 0;
+        st->skipv = st->done | st->hold;
     }
 }
 
@@ -515,11 +516,19 @@ __global__ __launch_bounds__(RB) void k_resid64(const X* __restrict__ x, const f
 // one block: the fixed-order reduction of the partials and the stopping test; with `host` set (pinned host
 // memory) it also stores {used, done} there for the host to read after the kernel
 __global__ __launch_bounds__(FIN) void k_finalize(RefineState* __restrict__ st, const double* __restrict__ partials, int npart,
-                                                 double tol, int max_cycles, int* __restrict__ host = nullptr) {
+                                                 double tol, int max_cycles, int* __restrict__ host = nullptr,
+                                                 const int* __restrict__ gate = nullptr, int set_hold = 0) {
+    if (gate && gate[0]) return;                      // (uniform; read before the block's only writes)
     finalize_sums(st, partials, npart, tol, max_cycles);
-    if (host && threadIdx.x == 0) {   // thread 0 made the block's only state writes
-        host[0] = st->used;
-        host[1] = st->done;
+    if (threadIdx.x == 0) {   // thread 0 made the block's only state writes
+        if (set_hold) {       // the step's closing test: a step enqueued behind it runs only once this one is done
+            st->hold = st->done ? 0 : 1;
+            st->skipv = 1;
+        }
+        if (host) {
+            host[0] = st->used;
+            host[1] = st->done;
+        }
     }
 }
 
@@ -858,7 +867,8 @@ void mg_smooth(GpuMG& mg, hipStream_t s, int l, int iters, const int* skip, doub
             if (k == 0 && color == 0 && first_red_done) continue;
             double* xu = (xupd && k == iters - 1 && color == 1) ? xupd : nullptr;
             if (fin) {
-                k_smooth<true><<<g, RB, 0, s>>>(pl, mg.b[l], nx, ny, nz, mg.h[l], color, &mg.st->never, xu, om, nullptr,
+                k_smooth<true><<<g, RB, 0, s>>>(pl, mg.b[l], nx, ny, nz, mg.h[l], color, mg.gate ? mg.gate : &mg.st->never, xu,
+                                                om, nullptr,
                                                 *fin);
                 fin = nullptr;
             } else {
@@ -990,7 +1000,8 @@ void mg_refine_begin(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     const bool red = fuse_red0(mg);
     k_resid64<double><<<grid3((nx + 1) / 2, ny, nz), RB, 0, s>>>(mg.x64, mg.bkeep, nx, ny, nz, (double)mg.h[0], mg.b[0], mg.p[0],
-                                                                  mg.partials, mg.st, 1, &mg.st->never, mg.h[0], mg.omega, red ? 1 : 0);
+                                                                  mg.partials, mg.st, 1, mg.gate ? mg.gate : &mg.st->never, mg.h[0],
+                                                                  mg.omega, red ? 1 : 0);
 }
 
 // The correction's V-cycles iterate on e (the first from e = 0, whose first colour-0 pass the residual kernel
@@ -1004,7 +1015,7 @@ void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cy
     const int nx = mg.dims[0][0], ny = mg.dims[0][1], nz = mg.dims[0][2];
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
     if (cycles < 1) cycles = 1;
-    int* done = &mg.st->done;
+    const int* done = mg.gate ? &mg.st->skipv : &mg.st->done;   // (a step enqueued ahead: also held)
     const bool red = fuse_red0(mg);
     const FinArgs fin{mg.st, mg.partials, mg.npart, max_cycles, tol};
     if (form_rhs)   // r = b - A x as the V-cycles' right-hand side, e = 0 (first colour-0 pass written); no norms
@@ -1031,9 +1042,9 @@ void mg_refine_iter(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int cy
                                                                   cycles);
 }
 
-void mg_state_ready(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int* host) {
+void mg_state_ready(GpuMG& mg, hipStream_t s, double tol, int max_cycles, int* host, const int* gate, bool set_hold) {
     if (max_cycles > MG_MAX_REFINE) max_cycles = MG_MAX_REFINE;
-    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, host);
+    k_finalize<<<1, FIN, 0, s>>>(mg.st, mg.partials, mg.npart, tol, max_cycles, host, gate, set_hold ? 1 : 0);
 }
 
 void mg_refine(GpuMG& mg, hipStream_t s, double tol, int max_cycles) {
