@@ -10,7 +10,7 @@
 // path never links or calls it.
 //
 // Conventions fixed here where the reference leaves them to the WebGPU stack
-// (all "parity unpinned" against real WGSL):
+// (all "parity unpinned" against real WGSL: no WebGPU implementation runs here to compare with):
 //   * trilinear sampling uses exact f32 weights (hardware samplers use ~8-bit
 //     fixed-point weights; research/webgpu.md:103);
 //   * clamp/min/max return the non-NaN operand (IEEE maxNum/minNum);
