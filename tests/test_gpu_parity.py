@@ -205,6 +205,26 @@ def test_adaptive_and_async_solves_agree(solve, first, tol):
             assert np.array_equal(a, s.read(f, "native")), f
 
 
+def test_refine_steps_run_ahead_equal_one_step_per_call():
+    """The refinement's steps of one call run one ahead of the host's stopping test (a step whose predecessor
+    needs another correction skips its stores and is enqueued again); one step per call runs unpipelined.  With
+    a tolerance that makes steps take more corrections than the last step did, twelve steps as four calls of three
+    equal twelve calls of one, bit for bit, with graphs and without."""
+    print("refine run-ahead vs one step per call", flush=True)
+    for g in (True, False):
+        sims = [F.FireSim(res=TEST_RES, size=TEST_SIZE, noise_n=32, precision=F.PREC_FP16, solve=F.SOLVE_REFINE64,
+                          div_tol=1e-8, max_vcycles=10, use_graph=g) for _ in range(2)]
+        for _ in range(4):
+            sims[0].step(DT, 3)
+            for _ in range(3):
+                sims[1].step(DT, 1)
+            assert sims[0].residual() == sims[1].residual()
+        for f in (F.VEL, F.S, F.S2, F.P):
+            assert np.array_equal(sims[0].read(f, "native"), sims[1].read(f, "native")), (g, f)
+        for s in sims:
+            s.close()
+
+
 @pytest.mark.parametrize("res", [(64, 72, 80), (96, 88, 48)], ids=["32x36x40", "48x44x24"])
 def test_tol32_exact_coarse_partial_tiles(res):
     """The exact coarse solve on levels whose y and z are not multiples of 16 (32x36x40, 48x44x24): the
